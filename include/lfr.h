@@ -1,0 +1,179 @@
+/*
+ * lfr.h - C ABI of the MI355X operator-apply / column-solve library.
+ *
+ * One call here replaces one PSy-layer `invoke` of the reference (the loop
+ * over cells or colours that the generated PSy code wraps around a kernel),
+ * not one per-column kernel call: a per-cell C call would cost one GPU launch
+ * per column.  Argument vocabulary follows the reference kernels:
+ *
+ *   - dofmaps are map(ndf, ncell) in Fortran memory order, 1-based dof
+ *     numbers of the bottom cell of each column (layer k adds k);
+ *   - LMA operators are local_stencil(ncell_3d, ndf_to, ndf_from) in Fortran
+ *     memory order, ncell_3d = ncell * nlayers, ik = (cell-1)*nlayers + k + 1;
+ *   - columnwise (CMA) operators are columnwise_matrix(bandwidth, nrow,
+ *     ncell_2d) with bands (1,2,3) = (sub, diag, super) for tridiagonals;
+ *   - field data are plain device arrays of undf reals; `prec` is the real
+ *     kind in bytes: 4 (r_single / r_solver) or 8 (r_double / r_def).
+ *
+ * Every array argument named *_d is a device pointer (hipMalloc'd, e.g. by
+ * lfr_malloc or by PyTorch).  Host arrays are named *_h.  All work is queued
+ * on the context's stream; calls return when the work is queued unless they
+ * return a host value (dots, norms, solver statistics).
+ *
+ * Errors: every entry point returns LFR_OK (0) or a nonzero LFR_ERR_* code
+ * and records a message retrievable with lfr_last_error().  The reference
+ * kernels have no error channel; its infrastructure aborts through
+ * log_event(..., LOG_LEVEL_ERROR) (infrastructure/source/utilities/log_mod.F90:381-476),
+ * which is what a Fortran shim over this ABI does on a nonzero return.
+ *
+ * A context is not re-entrant: calls on one context must not overlap.
+ */
+#ifndef LFR_H
+#define LFR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LFR_OK = 0,
+  LFR_ERR_ARG = 1,           /* invalid argument / inconsistent sizes        */
+  LFR_ERR_HIP = 2,           /* HIP runtime error                            */
+  LFR_ERR_NOMEM = 3,         /* device allocation failed                     */
+  LFR_ERR_NOT_CONVERGED = 4, /* solver hit max_iter with fail_on_non_converged */
+  LFR_ERR_NAN = 5,           /* NaN detected in a solver residual            */
+  LFR_ERR_SINGULAR = 6       /* zero pivot in a column solve                  */
+};
+
+typedef struct lfr_ctx lfr_ctx;
+typedef struct lfr_space lfr_space;
+typedef struct lfr_plan lfr_plan;
+
+/* ---------------------------------------------------------------------- */
+/* Context, memory, streams                                                */
+/* ---------------------------------------------------------------------- */
+
+/* Create a context on `device` with its own default-priority stream. */
+int lfr_init(int device, lfr_ctx** ctx);
+int lfr_finalise(lfr_ctx* ctx);
+const char* lfr_last_error(const lfr_ctx* ctx);
+/* Queue subsequent work on an existing hipStream_t; NULL selects HIP's default
+ * (null) stream.  Until the first call the context uses its own stream. */
+int lfr_set_stream(lfr_ctx* ctx, void* hip_stream);
+void* lfr_get_stream(const lfr_ctx* ctx);
+int lfr_synchronize(lfr_ctx* ctx);
+int lfr_malloc(lfr_ctx* ctx, size_t bytes, void** ptr_d);
+int lfr_free(lfr_ctx* ctx, void* ptr_d);
+int lfr_memcpy_h2d(lfr_ctx* ctx, void* dst_d, const void* src_h, size_t bytes);
+int lfr_memcpy_d2h(lfr_ctx* ctx, void* dst_h, const void* src_d, size_t bytes);
+
+/* ---------------------------------------------------------------------- */
+/* Function spaces and gather plans                                        */
+/* ---------------------------------------------------------------------- */
+
+/* Upload the dofmap of one function space: map(ndf, ncell) host array as
+ * returned by function_space_type%get_whole_dofmap()
+ * (infrastructure/source/function_space/function_space_mod.F90:726-750),
+ * with nlayers and undf (= last_dof_owned on one rank, :1277-1302). */
+int lfr_space_create(lfr_ctx* ctx, const int32_t* dofmap_h, int ndf, int ncell, int nlayers,
+                     int undf, lfr_space** space);
+int lfr_space_destroy(lfr_space* space);
+
+/* Colour-ordered inverse dofmap of a continuous output space.
+ *
+ * The reference writes GH_INC outputs colour by colour
+ * (infrastructure/build/psyclone/psyclone_tools.py:71-119) using the cell
+ * colouring of infrastructure/source/mesh/mesh_colouring_mod.F90, so a dof
+ * shared by several cells receives their contributions in colour order.  The
+ * plan lists, for each column of output dofs, the (cell, local dof) pairs
+ * that touch it in exactly that order, so that one owner-computes gather pass
+ * reproduces the coloured sum without atomics or colour barriers.
+ * cells_in_colour(ncolour, maxcells) is the reference's
+ * mesh%get_colour_map() array in Fortran memory order (1-based, 0-padded).
+ * kind: 0 for matrix_vector order (df2 outer, df inner within a cell),
+ *       1 for transpose_matrix_vector order (layer outer, df inner). */
+int lfr_plan_create(lfr_ctx* ctx, const lfr_space* out_space, const int32_t* cells_in_colour_h,
+                    const int32_t* ncells_per_colour_h, int ncolour, int maxcells, int kind,
+                    lfr_plan** plan);
+int lfr_plan_destroy(lfr_plan* plan);
+/* Number of output-dof columns and of classes (for diagnostics). */
+int lfr_plan_info(const lfr_plan* plan, int* ncolumns, int* nclasses, int64_t* nbytes);
+
+/* ---------------------------------------------------------------------- */
+/* LMA operator applies (one call = one invoke over all cells)             */
+/* ---------------------------------------------------------------------- */
+
+/* invoke(matrix_vector_kernel_type(lhs, x, matrix)):
+ * lhs += matrix * x on a continuous output space, all cells, colour order.
+ * Replaces matrix_vector_code (components/science/source/kernel/algebra/
+ * matrix_vector_kernel_mod.F90:68-140) looped by the PSy layer.
+ * If overwrite != 0 the call computes lhs = matrix * x, i.e. the pair
+ * invoke(setval_c(lhs, 0.0), matrix_vector_kernel_type(lhs, x, matrix)) of
+ * mass_matrix_operator_type%apply (components/science/source/algorithm/solver/
+ * sci_mass_matrix_operator_alg_mod.x90:98-99), without reading lhs.
+ * `plan` must be a kind-0 plan for `to_space`. */
+int lfr_matrix_vector(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_d, const void* matrix_d,
+                      const lfr_space* to_space, const lfr_space* from_space,
+                      const lfr_plan* plan, int overwrite);
+
+/* invoke(dg_matrix_vector_kernel_type(lhs, x, matrix)): lhs = matrix * x on a
+ * discontinuous output space (components/science/source/kernel/algebra/
+ * dg_matrix_vector_kernel_mod.F90:68-154). */
+int lfr_dg_matrix_vector(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_d,
+                         const void* matrix_d, const lfr_space* to_space,
+                         const lfr_space* from_space);
+
+/* invoke(dg_inc_matrix_vector_kernel_type(lhs, x, matrix)): lhs += matrix * x on
+ * a discontinuous output space (components/science/source/kernel/algebra/
+ * dg_inc_matrix_vector_kernel_mod.F90:69-144). */
+int lfr_dg_inc_matrix_vector(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_d,
+                             const void* matrix_d, const lfr_space* to_space,
+                             const lfr_space* from_space);
+
+/* invoke(transpose_matrix_vector_kernel_type(lhs, x, matrix)): lhs += matrix^T x
+ * where matrix is stored local_stencil(ncell_3d, ndf_x, ndf_lhs)
+ * (components/science/source/kernel/algebra/transpose_matrix_vector_kernel_mod.F90:66-144).
+ * This is also the adjoint of lfr_matrix_vector / lfr_dg_matrix_vector.
+ * `plan` must be a kind-1 plan for `lhs_space`.  overwrite != 0: lhs = M^T x. */
+int lfr_transpose_matrix_vector(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_d,
+                                const void* matrix_d, const lfr_space* lhs_space,
+                                const lfr_space* x_space, const lfr_plan* plan, int overwrite);
+
+
+/* ---------------------------------------------------------------------- */
+/* Vector algebra (abstract_vector_type methods / PSyclone built-ins)      */
+/* ---------------------------------------------------------------------- */
+/* Each call is one pass over n = undf dofs (1..last_dof_owned on one rank).
+ * Scalars are r_def (double) and are cast to the field kind before use, as
+ * r_solver_field_vector_type does (components/science/source/algorithm/
+ * solver/sci_r_solver_field_vector_mod.x90:189-195).  Built-in semantics are
+ * those named in sci_field_vector_mod.x90:132-283. */
+int lfr_vec_setval(lfr_ctx* ctx, int prec, int64_t n, void* x_d, double c);            /* setval_c: x = c      */
+int lfr_vec_copy(lfr_ctx* ctx, int prec, int64_t n, void* y_d, const void* x_d);       /* setval_X: y = x      */
+int lfr_vec_axpy(lfr_ctx* ctx, int prec, int64_t n, double a, const void* x_d, void* y_d);  /* inc_X_plus_bY: y += a x */
+int lfr_vec_aypx(lfr_ctx* ctx, int prec, int64_t n, double a, const void* x_d, void* y_d);  /* inc_aX_plus_Y: y = a y + x */
+int lfr_vec_axpby(lfr_ctx* ctx, int prec, int64_t n, double a, const void* x_d, double b,
+                  const void* y_d, void* z_d);                                          /* aX_plus_bY: z = a x + b y */
+int lfr_vec_scale(lfr_ctx* ctx, int prec, int64_t n, double a, void* x_d);             /* inc_a_times_X: x = a x */
+int lfr_vec_divide(lfr_ctx* ctx, int prec, int64_t n, void* z_d, const void* x_d,
+                   const void* y_d);                                                    /* X_divideby_Y: z = x / y */
+int lfr_vec_multiply(lfr_ctx* ctx, int prec, int64_t n, void* z_d, const void* x_d,
+                     const void* y_d);                                                  /* X_times_Y: z = x * y */
+/* Copy between kinds (copy_32_to_64 / copy_64_to_32 of
+ * components/science/source/psy/sci_psykal_builtin_light_mod.f90). */
+int lfr_vec_convert(lfr_ctx* ctx, int prec_out, void* y_d, int prec_in, const void* x_d, int64_t n);
+/* X_innerproduct_Y: sum of real(x,r_double)*real(y,r_double), FP64
+ * accumulation (components/science/source/psy/sci_psykal_light_mod.f90:146-214).
+ * Deterministic: fixed grid for a given n, fixed-order partial combine.
+ * lfr_vec_dot returns the value to the host; lfr_vec_dot_dev leaves it in
+ * device memory (no host synchronisation). */
+int lfr_vec_dot(lfr_ctx* ctx, int prec, int64_t n, const void* x_d, const void* y_d, double* out_h);
+int lfr_vec_dot_dev(lfr_ctx* ctx, int prec, int64_t n, const void* x_d, const void* y_d, double* out_d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LFR_H */

@@ -1,0 +1,100 @@
+"""ctypes binding of liblfr.so (the C ABI declared in include/lfr.h).
+
+The library is loaded from ``lfric_core_amd/_lib/liblfr.so``.  There is no
+fallback: if the library is missing or a HIP device is absent every call
+raises ``LfrError``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "liblfr.so"
+
+LFR_OK = 0
+ERRORS = {1: "LFR_ERR_ARG", 2: "LFR_ERR_HIP", 3: "LFR_ERR_NOMEM", 4: "LFR_ERR_NOT_CONVERGED",
+          5: "LFR_ERR_NAN", 6: "LFR_ERR_SINGULAR"}
+
+
+class LfrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_P = C.c_void_p
+_I = C.c_int
+_I64 = C.c_int64
+_D = C.c_double
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+SIGNATURES = {
+    "lfr_init": [_I, C.POINTER(_P)],
+    "lfr_finalise": [_P],
+    "lfr_last_error": [_P],
+    "lfr_set_stream": [_P, _P],
+    "lfr_get_stream": [_P],
+    "lfr_synchronize": [_P],
+    "lfr_malloc": [_P, C.c_size_t, C.POINTER(_P)],
+    "lfr_free": [_P, _P],
+    "lfr_memcpy_h2d": [_P, _P, _P, C.c_size_t],
+    "lfr_memcpy_d2h": [_P, _P, _P, C.c_size_t],
+    "lfr_space_create": [_P, _P, _I, _I, _I, _I, C.POINTER(_P)],
+    "lfr_space_destroy": [_P],
+    "lfr_plan_create": [_P, _P, _P, _P, _I, _I, _I, C.POINTER(_P)],
+    "lfr_plan_destroy": [_P],
+    "lfr_plan_info": [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I64)],
+    "lfr_matrix_vector": [_P, _I, _P, _P, _P, _P, _P, _P, _I],
+    "lfr_dg_matrix_vector": [_P, _I, _P, _P, _P, _P, _P],
+    "lfr_dg_inc_matrix_vector": [_P, _I, _P, _P, _P, _P, _P],
+    "lfr_transpose_matrix_vector": [_P, _I, _P, _P, _P, _P, _P, _P, _I],
+    "lfr_vec_setval": [_P, _I, _I64, _P, _D],
+    "lfr_vec_copy": [_P, _I, _I64, _P, _P],
+    "lfr_vec_axpy": [_P, _I, _I64, _D, _P, _P],
+    "lfr_vec_aypx": [_P, _I, _I64, _D, _P, _P],
+    "lfr_vec_axpby": [_P, _I, _I64, _D, _P, _D, _P, _P],
+    "lfr_vec_scale": [_P, _I, _I64, _D, _P],
+    "lfr_vec_divide": [_P, _I, _I64, _P, _P, _P],
+    "lfr_vec_multiply": [_P, _I, _I64, _P, _P, _P],
+    "lfr_vec_convert": [_P, _I, _P, _I, _P, _I64],
+    "lfr_vec_dot": [_P, _I, _I64, _P, _P, C.POINTER(_D)],
+    "lfr_vec_dot_dev": [_P, _I, _I64, _P, _P, _P],
+}
+_RESTYPE = {"lfr_last_error": C.c_char_p, "lfr_get_stream": _P}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Load liblfr.so once; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not LIB_PATH.exists():
+                    raise LfrError(2, f"{LIB_PATH} is not built: run "
+                                      "`make -C lfric_core_amd/csrc` or __graft_entry__.build()")
+                L = C.CDLL(str(LIB_PATH))
+                for name, args in SIGNATURES.items():
+                    if not hasattr(L, name):
+                        continue
+                    fn = getattr(L, name)
+                    fn.argtypes = args
+                    fn.restype = _RESTYPE.get(name, _I)
+                _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    """Names in SIGNATURES that the loaded library exports."""
+    L = lib()
+    return [n for n in SIGNATURES if hasattr(L, n)]
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != LFR_OK:
+        msg = lib().lfr_last_error(ctx).decode() if ctx else "(no context)"
+        raise LfrError(rc, msg)

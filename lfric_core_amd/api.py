@@ -1,0 +1,306 @@
+"""Python front end over the C ABI (include/lfr.h), on PyTorch device tensors.
+
+Mirrors the reference's invoke-level vocabulary:
+
+* ``matrix_vector(lhs, x, op)``            invoke(matrix_vector_kernel_type(lhs, x, op))
+* ``dg_matrix_vector(lhs, x, op)``         invoke(dg_matrix_vector_kernel_type(lhs, x, op))
+* ``dg_inc_matrix_vector(lhs, x, op)``     invoke(dg_inc_matrix_vector_kernel_type(lhs, x, op))
+* ``transpose_matrix_vector(lhs, x, op)``  invoke(transpose_matrix_vector_kernel_type(lhs, x, op))
+
+Fields are 1-D tensors of length undf (float32 = r_solver / r_single,
+float64 = r_def / r_double) on the context's device; operators are
+``LMAOperator`` objects holding local_stencil(ncell_3d, ndf_to, ndf_from) as a
+tensor of shape (ndf_from, ndf_to, ncell_3d) (the same memory as the Fortran
+array).  Every argument is checked on the host before a kernel is queued.
+PyTorch supplies device memory and the stream; all arithmetic is in liblfr.so.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from ._ffi import LfrError, check, lib
+from .function_space import Colouring, FunctionSpace
+
+_P = C.c_void_p
+
+
+def _prec(t: torch.Tensor) -> int:
+    if t.dtype == torch.float32:
+        return 4
+    if t.dtype == torch.float64:
+        return 8
+    raise LfrError(1, f"unsupported dtype {t.dtype}; fields are float32 or float64")
+
+
+class Context:
+    """A device context (lfr_init); work is queued on PyTorch's current stream."""
+
+    def __init__(self, device: int = 0):
+        if not torch.cuda.is_available():
+            raise LfrError(2, "no GPU: liblfr needs a HIP device")
+        self.device = torch.device("cuda", device)
+        h = _P()
+        rc = lib().lfr_init(device, C.byref(h))
+        if rc != 0:
+            raise LfrError(rc, "lfr_init failed (see stderr)")
+        self.h = h
+
+    def sync_stream(self) -> None:
+        """Queue subsequent library work on PyTorch's current stream."""
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        check(lib().lfr_set_stream(self.h, _P(s)), self.h)
+
+    def synchronize(self) -> None:
+        check(lib().lfr_synchronize(self.h), self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().lfr_finalise(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Space:
+    """A function space's dofmap on the device (lfr_space_create)."""
+
+    def __init__(self, ctx: Context, fs: FunctionSpace):
+        self.ctx, self.fs = ctx, fs
+        dm = np.ascontiguousarray(fs.dofmap, dtype=np.int32)
+        h = _P()
+        check(lib().lfr_space_create(ctx.h, dm.ctypes.data_as(_P), fs.ndf, fs.ncells,
+                                     fs.nlayers, fs.undf, C.byref(h)), ctx.h)
+        self.h = h
+        self._plans = {}
+
+    @property
+    def undf(self) -> int:
+        return self.fs.undf
+
+    @property
+    def ndf(self) -> int:
+        return self.fs.ndf
+
+    def plan(self, colouring: Colouring, kind: int) -> "Plan":
+        key = (id(colouring), kind)
+        if key not in self._plans:
+            self._plans[key] = Plan(self, colouring, kind)
+        return self._plans[key]
+
+    def __del__(self):
+        try:
+            for p in self._plans.values():
+                p.close()
+            if self.h:
+                lib().lfr_space_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Plan:
+    """Colour-ordered gather plan of a continuous output space (lfr_plan_create)."""
+
+    def __init__(self, space: Space, colouring: Colouring, kind: int):
+        self.space, self.kind = space, kind
+        cic = np.asfortranarray(colouring.cells_in_colour.astype(np.int32))
+        npc = np.ascontiguousarray(colouring.ncells_per_colour, dtype=np.int32)
+        self._keep = (cic, npc)
+        h = _P()
+        check(lib().lfr_plan_create(space.ctx.h, space.h, cic.ctypes.data_as(_P),
+                                    npc.ctypes.data_as(_P), colouring.ncolours,
+                                    cic.shape[1], kind, C.byref(h)), space.ctx.h)
+        self.h = h
+
+    def info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int64()
+        check(lib().lfr_plan_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"ncolumns": a.value, "nclasses": b.value, "nbytes": c.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lfr_plan_destroy(self.h)
+            self.h = None
+
+
+@dataclass
+class LMAOperator:
+    """operator_type: local_stencil(ncell_3d, ndf_to, ndf_from) between two spaces."""
+
+    matrix: torch.Tensor          # (ndf_from, ndf_to, ncell_3d), contiguous
+    to_space: Space
+    from_space: Space
+    colouring: Colouring | None = None
+    _cache: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        nt, nf = self.to_space.ndf, self.from_space.ndf
+        n3 = self.to_space.fs.ncell_3d
+        if tuple(self.matrix.shape) != (nf, nt, n3):
+            raise LfrError(1, f"operator matrix shape {tuple(self.matrix.shape)} != {(nf, nt, n3)}")
+        if not self.matrix.is_contiguous():
+            raise LfrError(1, "operator matrix must be contiguous")
+        if self.matrix.device != self.to_space.ctx.device:
+            raise LfrError(1, "operator matrix on the wrong device")
+
+    @property
+    def ctx(self) -> Context:
+        return self.to_space.ctx
+
+    def with_matrix(self, matrix: torch.Tensor) -> "LMAOperator":
+        return LMAOperator(matrix, self.to_space, self.from_space, self.colouring)
+
+
+def _field(t: torch.Tensor, space: Space, name: str, dtype=None) -> None:
+    if not isinstance(t, torch.Tensor):
+        raise LfrError(1, f"{name} must be a torch tensor")
+    if t.device != space.ctx.device:
+        raise LfrError(1, f"{name} is on {t.device}, context is on {space.ctx.device}")
+    if t.dim() != 1 or t.numel() != space.undf:
+        raise LfrError(1, f"{name} has shape {tuple(t.shape)}, space has undf={space.undf}")
+    if not t.is_contiguous():
+        raise LfrError(1, f"{name} must be contiguous")
+    if dtype is not None and t.dtype != dtype:
+        raise LfrError(1, f"{name} dtype {t.dtype} differs from the operator's {dtype}")
+
+
+def matrix_vector(lhs: torch.Tensor, x: torch.Tensor, op: LMAOperator, overwrite: bool = False):
+    """lhs += M x (GH_INC, colour order); overwrite=True gives lhs = M x."""
+    if op.colouring is None:
+        raise LfrError(1, "matrix_vector needs the operator's colouring")
+    _field(lhs, op.to_space, "lhs", op.matrix.dtype)
+    _field(x, op.from_space, "x", op.matrix.dtype)
+    plan = op.to_space.plan(op.colouring, 0)
+    ctx = op.ctx
+    ctx.sync_stream()
+    check(lib().lfr_matrix_vector(ctx.h, _prec(lhs), _P(lhs.data_ptr()), _P(x.data_ptr()),
+                                  _P(op.matrix.data_ptr()), op.to_space.h, op.from_space.h,
+                                  plan.h, int(overwrite)), ctx.h)
+    return lhs
+
+
+def dg_matrix_vector(lhs: torch.Tensor, x: torch.Tensor, op: LMAOperator):
+    """lhs = M x on a discontinuous output space."""
+    _field(lhs, op.to_space, "lhs", op.matrix.dtype)
+    _field(x, op.from_space, "x", op.matrix.dtype)
+    ctx = op.ctx
+    ctx.sync_stream()
+    check(lib().lfr_dg_matrix_vector(ctx.h, _prec(lhs), _P(lhs.data_ptr()), _P(x.data_ptr()),
+                                     _P(op.matrix.data_ptr()), op.to_space.h, op.from_space.h),
+          ctx.h)
+    return lhs
+
+
+def dg_inc_matrix_vector(lhs: torch.Tensor, x: torch.Tensor, op: LMAOperator):
+    """lhs += M x on a discontinuous output space."""
+    _field(lhs, op.to_space, "lhs", op.matrix.dtype)
+    _field(x, op.from_space, "x", op.matrix.dtype)
+    ctx = op.ctx
+    ctx.sync_stream()
+    check(lib().lfr_dg_inc_matrix_vector(ctx.h, _prec(lhs), _P(lhs.data_ptr()),
+                                         _P(x.data_ptr()), _P(op.matrix.data_ptr()),
+                                         op.to_space.h, op.from_space.h), ctx.h)
+    return lhs
+
+
+def transpose_matrix_vector(lhs: torch.Tensor, x: torch.Tensor, op: LMAOperator,
+                            colouring: Colouring | None = None, overwrite: bool = False):
+    """lhs += M^T x with op.matrix = local_stencil(ncell_3d, ndf_x, ndf_lhs).
+
+    ``op.to_space`` is x's space and ``op.from_space`` is lhs' space, so the
+    same LMAOperator that maps from_space -> to_space in ``matrix_vector``
+    is applied transposed here (the adjoint).
+    """
+    col = colouring or op.colouring
+    if col is None:
+        raise LfrError(1, "transpose_matrix_vector needs a colouring of lhs' space")
+    lsp, xsp = op.from_space, op.to_space
+    _field(lhs, lsp, "lhs", op.matrix.dtype)
+    _field(x, xsp, "x", op.matrix.dtype)
+    plan = lsp.plan(col, 1)
+    ctx = op.ctx
+    ctx.sync_stream()
+    check(lib().lfr_transpose_matrix_vector(ctx.h, _prec(lhs), _P(lhs.data_ptr()),
+                                            _P(x.data_ptr()), _P(op.matrix.data_ptr()),
+                                            lsp.h, xsp.h, plan.h, int(overwrite)), ctx.h)
+    return lhs
+
+
+# --- vector algebra ---------------------------------------------------------
+
+def _vec(ctx: Context, *ts: torch.Tensor):
+    n = ts[0].numel()
+    for t in ts:
+        if t.device != ctx.device or not t.is_contiguous() or t.numel() != n or t.dtype != ts[0].dtype:
+            raise LfrError(1, "vector operands must be contiguous, same length/dtype/device")
+    ctx.sync_stream()
+    return _prec(ts[0]), n
+
+
+def setval(ctx, x, c):
+    p, n = _vec(ctx, x)
+    check(lib().lfr_vec_setval(ctx.h, p, n, _P(x.data_ptr()), float(c)), ctx.h)
+
+
+def copy(ctx, y, x):
+    p, n = _vec(ctx, y, x)
+    check(lib().lfr_vec_copy(ctx.h, p, n, _P(y.data_ptr()), _P(x.data_ptr())), ctx.h)
+
+
+def axpy(ctx, a, x, y):
+    """y = y + a x"""
+    p, n = _vec(ctx, x, y)
+    check(lib().lfr_vec_axpy(ctx.h, p, n, float(a), _P(x.data_ptr()), _P(y.data_ptr())), ctx.h)
+
+
+def aypx(ctx, a, x, y):
+    """y = a y + x"""
+    p, n = _vec(ctx, x, y)
+    check(lib().lfr_vec_aypx(ctx.h, p, n, float(a), _P(x.data_ptr()), _P(y.data_ptr())), ctx.h)
+
+
+def axpby(ctx, a, x, b, y, z):
+    """z = a x + b y"""
+    p, n = _vec(ctx, x, y, z)
+    check(lib().lfr_vec_axpby(ctx.h, p, n, float(a), _P(x.data_ptr()), float(b),
+                              _P(y.data_ptr()), _P(z.data_ptr())), ctx.h)
+
+
+def scale(ctx, a, x):
+    p, n = _vec(ctx, x)
+    check(lib().lfr_vec_scale(ctx.h, p, n, float(a), _P(x.data_ptr())), ctx.h)
+
+
+def divide(ctx, z, x, y):
+    p, n = _vec(ctx, z, x, y)
+    check(lib().lfr_vec_divide(ctx.h, p, n, _P(z.data_ptr()), _P(x.data_ptr()), _P(y.data_ptr())), ctx.h)
+
+
+def multiply(ctx, z, x, y):
+    p, n = _vec(ctx, z, x, y)
+    check(lib().lfr_vec_multiply(ctx.h, p, n, _P(z.data_ptr()), _P(x.data_ptr()), _P(y.data_ptr())), ctx.h)
+
+
+def convert(ctx, y, x):
+    if x.numel() != y.numel() or x.device != ctx.device or y.device != ctx.device:
+        raise LfrError(1, "convert: operands must match in length and device")
+    ctx.sync_stream()
+    check(lib().lfr_vec_convert(ctx.h, _prec(y), _P(y.data_ptr()), _prec(x), _P(x.data_ptr()),
+                                x.numel()), ctx.h)
+
+
+def dot(ctx, x, y) -> float:
+    """<x, y> accumulated in FP64 (deterministic)."""
+    p, n = _vec(ctx, x, y)
+    out = C.c_double()
+    check(lib().lfr_vec_dot(ctx.h, p, n, _P(x.data_ptr()), _P(y.data_ptr()), C.byref(out)), ctx.h)
+    return out.value

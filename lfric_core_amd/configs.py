@@ -1,0 +1,84 @@
+"""Synthetic problems of the benchmark configurations (host arrays, numpy).
+
+All random inputs come from ``rng.Stream`` (SplitMix64) so the CPU oracle
+and the GPU see identical bytes.  Draw order is part of each definition:
+
+* ``spd_lma(stream, n3, ndf, shift, std)``: for every cell-layer ik in order,
+  a ndf x ndf matrix B (row-major draws, N(0, std^2)); the local matrix is
+  B B^T + shift*I.  Stored as local_stencil(ncell_3d, ndf, ndf): numpy shape
+  (ndf, ndf, ncell_3d) (identical memory to the Fortran array).
+* ``normal_lma(stream, n3, ndf_to, ndf_from)``: entries N(0,1) drawn for
+  ik-major, then row (to), then column (from).
+
+Configuration "precision" (C96, 70 layers, quadratic extrusion to 80 km):
+W2 mass = BB^T + 6I, B ~ N(0,1), seed 1; then div (W2 -> W3, 1x6) and grad
+(W3 -> W2, 6x1) ~ N(0,1); then the right-hand side b ~ U[-1,1] on W2.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import function_space as fsm
+from .mesh import cubed_sphere, extrusion_eta
+from .rng import Stream
+
+
+def spd_lma(stream: Stream, n3: int, ndf: int, shift: float, std: float = 1.0,
+            dtype=np.float64, chunk: int = 1 << 18) -> np.ndarray:
+    out = np.empty((ndf, ndf, n3), dtype=dtype)
+    eye = shift * np.eye(ndf)
+    for s in range(0, n3, chunk):
+        m = min(chunk, n3 - s)
+        B = stream.normal(m * ndf * ndf).reshape(m, ndf, ndf) * std
+        A = np.einsum("kab,kcb->kac", B, B) + eye          # A[k, i, j]
+        out[:, :, s:s + m] = A.transpose(2, 1, 0)             # out[j, i, k] = A[k, i, j]
+    return out
+
+
+def normal_lma(stream: Stream, n3: int, ndf_to: int, ndf_from: int, dtype=np.float64,
+               chunk: int = 1 << 18) -> np.ndarray:
+    out = np.empty((ndf_from, ndf_to, n3), dtype=dtype)
+    for s in range(0, n3, chunk):
+        m = min(chunk, n3 - s)
+        A = stream.normal(m * ndf_to * ndf_from).reshape(m, ndf_to, ndf_from)
+        out[:, :, s:s + m] = A.transpose(2, 1, 0)
+    return out
+
+
+@dataclass
+class PrecisionConfig:
+    """W2/W3 lowest-order operators on a cubed sphere (the C96L70 precision case)."""
+
+    edge_cells: int
+    nlayers: int
+    mesh: object
+    w2: fsm.FunctionSpace
+    w3: fsm.FunctionSpace
+    colouring: fsm.Colouring
+    eta: np.ndarray
+    mass: np.ndarray      # (6, 6, n3) float64
+    div: np.ndarray       # (6, 1, n3): local_stencil(n3, 1, 6)
+    grad: np.ndarray      # (1, 6, n3): local_stencil(n3, 6, 1)
+    b: np.ndarray         # (undf_w2,) float64
+
+
+def precision_config(edge_cells: int = 96, nlayers: int = 70, seed: int = 1,
+                     with_div_grad: bool = True) -> PrecisionConfig:
+    mesh = cubed_sphere(edge_cells)
+    w2 = fsm.build_dofmap(mesh, "W2", nlayers)
+    w3 = fsm.build_dofmap(mesh, "W3", nlayers)
+    col = fsm.colour_cells(mesh)
+    n3 = mesh.ncells * nlayers
+    st = Stream(seed)
+    mass = spd_lma(st, n3, 6, 6.0)
+    if with_div_grad:
+        div = normal_lma(st, n3, 1, 6)
+        grad = normal_lma(st, n3, 6, 1)
+    else:
+        div = grad = None
+    b = st.uniform(w2.undf)
+    return PrecisionConfig(edge_cells, nlayers, mesh, w2, w3, col,
+                           80.0e3 * extrusion_eta(nlayers, "quadratic"), mass, div, grad, b)

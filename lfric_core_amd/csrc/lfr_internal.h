@@ -1,0 +1,89 @@
+// Internal definitions shared by the HIP translation units of liblfr.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/lfr.h"
+
+struct lfr_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // grow-only device workspace for reductions and solver scalars
+  void* work = nullptr;
+  size_t work_bytes = 0;
+  // pinned host staging for scalar read-backs
+  double* host_scalars = nullptr;
+  int cu_count = 256;
+};
+
+struct lfr_space {
+  lfr_ctx* ctx = nullptr;
+  int ndf = 0, ncell = 0, nlayers = 0, undf = 0;
+  int32_t* map0_d = nullptr;        // map(ndf, ncell) converted to 0-based, device
+  std::vector<int32_t> map_h;       // 1-based host copy (plan construction)
+};
+
+// One class of output-dof columns with identical structure: every column has
+// `height` dofs and its contributions come from `ngroup` cells, each
+// contributing `gsize` local dofs (entries), listed in summation order.
+struct lfr_plan_class {
+  int height = 0;
+  int ngroup = 0;
+  int gsize = 0;
+  int ncol = 0;
+  int32_t* col_start_d = nullptr;   // (ncol) first dof of the column, 0-based
+  int32_t* ent_d = nullptr;         // (ncol, ngroup*gsize, 2): cell (0-based), (i << 1) | shift
+};
+
+struct lfr_plan {
+  lfr_ctx* ctx = nullptr;
+  const lfr_space* space = nullptr;
+  int kind = 0;
+  int ncolumns = 0;
+  int64_t nbytes = 0;
+  std::vector<lfr_plan_class> classes;
+};
+
+int lfr_set_error(lfr_ctx* ctx, int code, const char* fmt, ...);
+
+#define LFR_HIP(ctx, call)                                                        \
+  do {                                                                            \
+    hipError_t _e = (call);                                                       \
+    if (_e != hipSuccess)                                                         \
+      return lfr_set_error((ctx), LFR_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, \
+                           #call, hipGetErrorString(_e));                         \
+  } while (0)
+
+#define LFR_CHECK_LAUNCH(ctx) LFR_HIP(ctx, hipGetLastError())
+
+#define LFR_REQUIRE(ctx, cond, ...)                       \
+  do {                                                    \
+    if (!(cond)) return lfr_set_error((ctx), LFR_ERR_ARG, __VA_ARGS__); \
+  } while (0)
+
+// Grow the context workspace to at least `bytes`.
+int lfr_work_reserve(lfr_ctx* ctx, size_t bytes);
+
+// Blocks for a grid-stride launch over n items with `threads` per block.
+static inline unsigned lfr_blocks(int64_t n, int threads, unsigned cap = 1u << 30) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > (int64_t)cap) b = cap;
+  return (unsigned)b;
+}
+
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with a private L2.  Remapping the hardware block id so that each XCD gets a
+// contiguous range of logical blocks keeps neighbouring columns (which share
+// gathered operand dofs) in one L2.  nb_padded must be a multiple of 8.
+__device__ __forceinline__ unsigned lfr_xcd_block(unsigned hw_block, unsigned nb_padded) {
+  const unsigned per = nb_padded >> 3;
+  return (hw_block & 7u) * per + (hw_block >> 3);
+}

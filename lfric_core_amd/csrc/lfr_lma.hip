@@ -1,0 +1,463 @@
+// LMA operator applies: colour-ordered gather plans and the HIP kernels for
+// matrix_vector, dg_matrix_vector, dg_inc_matrix_vector and
+// transpose_matrix_vector at invoke granularity.
+//
+// Data layout (unchanged from the reference, so a PSy caller passes its
+// arrays straight through):
+//   local_stencil(ncell_3d, ndf_to, ndf_from): entry (ik, i, j) at
+//     ik + ncell_3d*(i + ndf_to*j) with ik = cell*nl + k (0-based here).  For
+//     a fixed (i, j) the whole operator is contiguous over (cell, k), k
+//     fastest, so lanes that walk consecutive layers read consecutive
+//     addresses: every matrix load of a wavefront is one coalesced 256-byte
+//     (fp32) / 512-byte (fp64) request.
+//   fields: layer k of the column that starts at map(df) is map(df) + k.
+//
+// Summation order matches the reference statement order exactly and every
+// term is formed as a separate multiply and add (this file is compiled with
+// -ffp-contract=off), so results are bit-identical to the reference loops.
+#include <algorithm>
+#include <map>
+#include <numeric>
+#include <tuple>
+
+#include "lfr_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------------------
+// Owner-computes gather for a continuous output space.
+//
+// Thread <-> (column, level L).  The column's contributions come in ngroup
+// cell groups (colour order); each group holds GS local dofs (i, shift s) of
+// that cell; entry (i, s) adds the cell's layer k = L - s when 0 <= k < nl.
+//  KIND 0 (matrix_vector): per group, for j (df2) outer, for entries (df)
+//         inner: acc = acc + M(ik,i,j)*x(map_in(j)+k)  (matrix_vector_kernel_mod.F90:130-138)
+//  KIND 1 (transpose):     per group, for entries in layer order: the full
+//         dot over j is formed first, then added
+//         (transpose_matrix_vector_kernel_mod.F90:92-101).
+// ---------------------------------------------------------------------------
+template <typename T, int NDF_IN, int GS, int KIND>
+__global__ void __launch_bounds__(kThreads)
+k_gather(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
+         const int32_t* __restrict__ map_in, const int32_t* __restrict__ col_start,
+         const int32_t* __restrict__ ent, int ncol, int H, int ngroup, int nl, int64_t n3,
+         int ndf_out, int ndf_in_rt, int overwrite, unsigned nb_pad) {
+  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
+  const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
+  if (t >= (int64_t)ncol * H) return;
+  const int col = (int)(t / H);
+  const int L = (int)(t - (int64_t)col * H);
+  const int32_t g0 = col_start[col];
+  const int ndf_in = NDF_IN > 0 ? NDF_IN : ndf_in_rt;
+  T acc = overwrite ? T(0) : y[g0 + L];
+  const int32_t* e = ent + (int64_t)col * ngroup * GS * 2;
+  for (int g = 0; g < ngroup; ++g) {
+    const int32_t* eg = e + g * GS * 2;
+    const int c = eg[0];
+    const int32_t* mi = map_in + (int64_t)c * ndf_in;
+    int ii[GS], kk[GS];
+    bool ok[GS];
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+      const int code = eg[2 * q + 1];
+      ii[q] = code >> 1;
+      kk[q] = L - (code & 1);
+      ok[q] = code >= 0 && kk[q] >= 0 && kk[q] < nl;
+    }
+    if (KIND == 0) {
+      for (int j = 0; j < ndf_in; ++j) {
+        const int xb = mi[j];
+#pragma unroll
+        for (int q = 0; q < GS; ++q) {
+          if (ok[q]) {
+            const int64_t ik = (int64_t)c * nl + kk[q];
+            const T m = M[ik + n3 * ((int64_t)ii[q] + (int64_t)ndf_out * j)];
+            const T p = m * x[xb + kk[q]];
+            acc = acc + p;
+          }
+        }
+      }
+    } else {
+      // entries are stored in layer order (larger shift = lower layer first)
+#pragma unroll
+      for (int q = 0; q < GS; ++q) {
+        if (ok[q]) {
+          const int64_t ik = (int64_t)c * nl + kk[q];
+          T dot = T(0);
+          for (int j = 0; j < ndf_in; ++j) {
+            const T m = M[ik + n3 * ((int64_t)j + (int64_t)ndf_in * ii[q])];
+            const T p = m * x[mi[j] + kk[q]];
+            dot = dot + p;
+          }
+          acc = acc + dot;
+        }
+      }
+    }
+  }
+  y[g0 + L] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Cell-centric apply for a discontinuous output space: thread <-> (cell, k).
+// Each output dof has exactly one (cell, df) contributor, so per output the
+// sum over df2 in ascending order is the reference order
+// (dg_matrix_vector_kernel_mod.F90:94-107, dg_inc_matrix_vector_kernel_mod.F90:134-141).
+// ---------------------------------------------------------------------------
+template <typename T, int NDF_OUT, int NDF_IN>
+__global__ void __launch_bounds__(kThreads)
+k_cells(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
+        const int32_t* __restrict__ map_out, const int32_t* __restrict__ map_in, int ncell,
+        int nl, int64_t n3, int ndf_out_rt, int ndf_in_rt, int inc, unsigned nb_pad) {
+  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
+  const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
+  if (t >= (int64_t)ncell * nl) return;
+  const int c = (int)(t / nl);
+  const int k = (int)(t - (int64_t)c * nl);
+  const int64_t ik = t;
+  const int ndf_in = NDF_IN > 0 ? NDF_IN : ndf_in_rt;
+  const int ndf_out = NDF_OUT > 0 ? NDF_OUT : ndf_out_rt;
+  const int32_t* mo = map_out + (int64_t)c * ndf_out;
+  const int32_t* mi = map_in + (int64_t)c * ndf_in;
+  if constexpr (NDF_OUT > 0 && NDF_IN > 0) {
+    T acc[NDF_OUT];
+#pragma unroll
+    for (int i = 0; i < NDF_OUT; ++i) acc[i] = inc ? y[mo[i] + k] : T(0);
+#pragma unroll
+    for (int j = 0; j < NDF_IN; ++j) {
+      const T xv = x[mi[j] + k];
+#pragma unroll
+      for (int i = 0; i < NDF_OUT; ++i) {
+        const T p = M[ik + n3 * ((int64_t)i + (int64_t)NDF_OUT * j)] * xv;
+        acc[i] = acc[i] + p;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NDF_OUT; ++i) y[mo[i] + k] = acc[i];
+  } else {
+    for (int i = 0; i < ndf_out; ++i) {
+      T acc = inc ? y[mo[i] + k] : T(0);
+      for (int j = 0; j < ndf_in; ++j) {
+        const T p = M[ik + n3 * ((int64_t)i + (int64_t)ndf_out * j)] * x[mi[j] + k];
+        acc = acc + p;
+      }
+      y[mo[i] + k] = acc;
+    }
+  }
+}
+
+unsigned pad8(unsigned nb) { return (nb + 7u) & ~7u; }
+
+template <typename T, int NDF_IN, int GS>
+void launch_gather_t(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
+                     const int32_t* mi, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
+                     int ndf_in, int overwrite) {
+  const unsigned nbp = pad8(nb);
+  if (kind == 0)
+    hipLaunchKernelGGL((k_gather<T, NDF_IN, GS, 0>), dim3(nbp), dim3(kThreads), 0, s, M, x, y, mi,
+                       c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3, ndf_out, ndf_in,
+                       overwrite, nbp);
+  else
+    hipLaunchKernelGGL((k_gather<T, NDF_IN, GS, 1>), dim3(nbp), dim3(kThreads), 0, s, M, x, y, mi,
+                       c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3, ndf_out, ndf_in,
+                       overwrite, nbp);
+}
+
+template <typename T, int GS>
+void launch_gather_gs(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
+                      const int32_t* mi, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
+                      int ndf_in, int overwrite) {
+  switch (ndf_in) {
+    case 1: launch_gather_t<T, 1, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
+    case 6: launch_gather_t<T, 6, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
+    case 8: launch_gather_t<T, 8, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
+    case 27: launch_gather_t<T, 27, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
+    case 36: launch_gather_t<T, 36, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
+    default: launch_gather_t<T, 0, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite);
+  }
+}
+
+template <typename T>
+int run_gather(lfr_ctx* ctx, const lfr_plan* plan, int kind, const T* M, const T* x, T* y,
+               const lfr_space* in, int ndf_out, int64_t n3, int overwrite) {
+  const int nl = plan->space->nlayers;
+  for (const auto& c : plan->classes) {
+    if (c.ncol == 0) continue;
+    const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
+    if (c.gsize == 1)
+      launch_gather_gs<T, 1>(ctx->stream, kind, nb, M, x, y, in->map0_d, c, nl, n3, ndf_out,
+                             in->ndf, overwrite);
+    else if (c.gsize == 2)
+      launch_gather_gs<T, 2>(ctx->stream, kind, nb, M, x, y, in->map0_d, c, nl, n3, ndf_out,
+                             in->ndf, overwrite);
+    else
+      return lfr_set_error(ctx, LFR_ERR_ARG, "plan class with %d entries per cell unsupported",
+                           c.gsize);
+    LFR_CHECK_LAUNCH(ctx);
+  }
+  return LFR_OK;
+}
+
+template <typename T, int NO, int NI>
+void launch_cells_t(hipStream_t s, const T* M, const T* x, T* y, const lfr_space* out,
+                    const lfr_space* in, int64_t n3, int inc) {
+  const unsigned nb = pad8(lfr_blocks((int64_t)out->ncell * out->nlayers, kThreads));
+  hipLaunchKernelGGL((k_cells<T, NO, NI>), dim3(nb), dim3(kThreads), 0, s, M, x, y, out->map0_d,
+                     in->map0_d, out->ncell, out->nlayers, n3, out->ndf, in->ndf, inc, nb);
+}
+
+template <typename T>
+int run_cells(lfr_ctx* ctx, const T* M, const T* x, T* y, const lfr_space* out,
+              const lfr_space* in, int64_t n3, int inc) {
+  const int no = out->ndf, ni = in->ndf;
+  hipStream_t s = ctx->stream;
+  if (no == 1 && ni == 1) launch_cells_t<T, 1, 1>(s, M, x, y, out, in, n3, inc);
+  else if (no == 1 && ni == 6) launch_cells_t<T, 1, 6>(s, M, x, y, out, in, n3, inc);
+  else if (no == 1 && ni == 8) launch_cells_t<T, 1, 8>(s, M, x, y, out, in, n3, inc);
+  else if (no == 8 && ni == 8) launch_cells_t<T, 8, 8>(s, M, x, y, out, in, n3, inc);
+  else if (no == 8 && ni == 36) launch_cells_t<T, 8, 36>(s, M, x, y, out, in, n3, inc);
+  else if (no == 6 && ni == 6) launch_cells_t<T, 6, 6>(s, M, x, y, out, in, n3, inc);
+  else if (no == 6 && ni == 1) launch_cells_t<T, 6, 1>(s, M, x, y, out, in, n3, inc);
+  else launch_cells_t<T, 0, 0>(s, M, x, y, out, in, n3, inc);
+  LFR_CHECK_LAUNCH(ctx);
+  return LFR_OK;
+}
+
+int check_pair(lfr_ctx* ctx, int prec, const void* lhs, const void* x, const void* M,
+               const lfr_space* a, const lfr_space* b, const char* who) {
+  LFR_REQUIRE(ctx, prec == 4 || prec == 8, "%s: prec must be 4 or 8, got %d", who, prec);
+  LFR_REQUIRE(ctx, lhs && x && M && a && b, "%s: null argument", who);
+  LFR_REQUIRE(ctx, a->ncell == b->ncell && a->nlayers == b->nlayers,
+              "%s: spaces on different meshes (ncell %d/%d, nlayers %d/%d)", who, a->ncell,
+              b->ncell, a->nlayers, b->nlayers);
+  return LFR_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plan construction (host)
+// ---------------------------------------------------------------------------
+extern "C" int lfr_plan_create(lfr_ctx* ctx, const lfr_space* sp, const int32_t* cic,
+                               const int32_t* npc, int ncolour, int maxc, int kind,
+                               lfr_plan** out) {
+  if (!ctx || !out) return LFR_ERR_ARG;
+  *out = nullptr;
+  LFR_REQUIRE(ctx, sp && cic && npc && ncolour > 0 && maxc > 0 && (kind == 0 || kind == 1),
+              "lfr_plan_create: bad arguments");
+  const int ncell = sp->ncell, ndf = sp->ndf, nl = sp->nlayers;
+  // colour position of every cell, in the coloured PSy loop order
+  std::vector<int> pos(ncell, -1);
+  int p = 0;
+  for (int col = 0; col < ncolour; ++col) {
+    LFR_REQUIRE(ctx, npc[col] >= 0 && npc[col] <= maxc, "lfr_plan_create: ncells_per_colour(%d)=%d",
+                col + 1, npc[col]);
+    for (int idx = 0; idx < npc[col]; ++idx) {
+      const int cell = cic[col + (int64_t)ncolour * idx];
+      LFR_REQUIRE(ctx, cell >= 1 && cell <= ncell, "lfr_plan_create: cell %d out of range", cell);
+      LFR_REQUIRE(ctx, pos[cell - 1] < 0, "lfr_plan_create: cell %d coloured twice", cell);
+      pos[cell - 1] = p++;
+    }
+  }
+  LFR_REQUIRE(ctx, p == ncell, "lfr_plan_create: colouring covers %d of %d cells", p, ncell);
+
+  // Column ranges [start, start+nl) touched by every (cell, df); overlapping
+  // ranges (bottom/top copies of one entity) merge into one column.
+  struct R { int32_t start; int32_t cell; int32_t df; };
+  std::vector<R> rs((size_t)ncell * ndf);
+  for (int c = 0; c < ncell; ++c)
+    for (int i = 0; i < ndf; ++i) rs[(size_t)c * ndf + i] = {sp->map_h[(size_t)c * ndf + i] - 1, c, i};
+  std::sort(rs.begin(), rs.end(), [](const R& a, const R& b) {
+    return std::tie(a.start, a.cell, a.df) < std::tie(b.start, b.cell, b.df);
+  });
+
+  struct Col { int32_t start, height; size_t first, count; };
+  std::vector<Col> cols;
+  int32_t expect = 0;
+  for (size_t q = 0; q < rs.size();) {
+    const int32_t cs = rs[q].start;
+    LFR_REQUIRE(ctx, cs == expect,
+                "lfr_plan_create: dofs %d..%d are not touched by any cell (dofmap not column-contiguous)",
+                expect + 1, cs);
+    int32_t ce = cs + nl;
+    size_t r = q + 1;
+    while (r < rs.size() && rs[r].start < ce) {
+      ce = std::max(ce, rs[r].start + nl);
+      ++r;
+    }
+    cols.push_back({cs, ce - cs, q, r - q});
+    expect = ce;
+    q = r;
+  }
+  LFR_REQUIRE(ctx, expect == sp->undf, "lfr_plan_create: columns cover %d dofs, undf=%d", expect,
+              sp->undf);
+
+  // Order entries inside each column and group them by cell.
+  struct E { int pos, cell, df, shift; };
+  using Sig = std::tuple<int, int, int>;  // height, ngroup, gsize
+  std::map<Sig, std::vector<int>> by_sig;  // signature -> column indices
+  std::vector<std::vector<E>> col_ent(cols.size());
+  for (size_t ci = 0; ci < cols.size(); ++ci) {
+    auto& v = col_ent[ci];
+    for (size_t q = cols[ci].first; q < cols[ci].first + cols[ci].count; ++q) {
+      const R& r = rs[q];
+      const int s = r.start - cols[ci].start;
+      LFR_REQUIRE(ctx, s == 0 || s == 1,
+                  "lfr_plan_create: local dof %d of cell %d starts %d dofs into its column",
+                  r.df + 1, r.cell + 1, s);
+      v.push_back({pos[r.cell], r.cell, r.df, s});
+    }
+    // kind 0: df order inside a cell (the kernel's inner df loop);
+    // kind 1: layer order inside a cell (k outer loop): larger shift first.
+    std::sort(v.begin(), v.end(), [kind](const E& a, const E& b) {
+      if (kind == 0) return std::tie(a.pos, a.df) < std::tie(b.pos, b.df);
+      return std::make_tuple(a.pos, -a.shift, a.df) < std::make_tuple(b.pos, -b.shift, b.df);
+    });
+    int ngroup = 0, gmax = 0;
+    for (size_t q = 0; q < v.size();) {
+      size_t r = q;
+      while (r < v.size() && v[r].cell == v[q].cell) ++r;
+      // two entries of one cell in one column must differ in shift
+      for (size_t a = q; a < r; ++a)
+        for (size_t b = a + 1; b < r; ++b)
+          LFR_REQUIRE(ctx, v[a].shift != v[b].shift,
+                      "lfr_plan_create: cell %d touches a dof twice through dofs %d and %d",
+                      v[a].cell + 1, v[a].df + 1, v[b].df + 1);
+      ++ngroup;
+      gmax = std::max<int>(gmax, (int)(r - q));
+      q = r;
+    }
+    LFR_REQUIRE(ctx, gmax <= 2, "lfr_plan_create: %d entries of one cell in a column", gmax);
+    by_sig[Sig(cols[ci].height, ngroup, gmax)].push_back((int)ci);
+  }
+
+  lfr_plan* plan = new lfr_plan();
+  plan->ctx = ctx;
+  plan->space = sp;
+  plan->kind = kind;
+  plan->ncolumns = (int)cols.size();
+  for (auto& kv : by_sig) {
+    lfr_plan_class pc;
+    std::tie(pc.height, pc.ngroup, pc.gsize) = kv.first;
+    pc.ncol = (int)kv.second.size();
+    const int per = pc.ngroup * pc.gsize * 2;
+    std::vector<int32_t> starts(pc.ncol), ent((size_t)pc.ncol * per, -1);
+    for (int q = 0; q < pc.ncol; ++q) {
+      const int ci = kv.second[q];
+      starts[q] = cols[ci].start;
+      const auto& v = col_ent[ci];
+      int g = -1, slot = 0, last_cell = -1;
+      for (const E& e : v) {
+        if (e.cell != last_cell) {
+          ++g;
+          slot = 0;
+          last_cell = e.cell;
+        }
+        int32_t* d = &ent[(size_t)q * per + (size_t)(g * pc.gsize + slot) * 2];
+        d[0] = e.cell;
+        d[1] = (e.df << 1) | e.shift;
+        ++slot;
+      }
+      // padded slots keep the group's cell id and a negative code
+      for (int gg = 0; gg < pc.ngroup; ++gg)
+        for (int s = 0; s < pc.gsize; ++s) {
+          int32_t* d = &ent[(size_t)q * per + (size_t)(gg * pc.gsize + s) * 2];
+          if (d[1] < 0) d[0] = ent[(size_t)q * per + (size_t)(gg * pc.gsize) * 2];
+        }
+    }
+    const size_t b1 = starts.size() * sizeof(int32_t), b2 = ent.size() * sizeof(int32_t);
+    if (hipMalloc(&pc.col_start_d, b1) != hipSuccess || hipMalloc(&pc.ent_d, b2) != hipSuccess) {
+      lfr_plan_destroy(plan);
+      return lfr_set_error(ctx, LFR_ERR_NOMEM, "plan allocation failed");
+    }
+    if (hipMemcpy(pc.col_start_d, starts.data(), b1, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pc.ent_d, ent.data(), b2, hipMemcpyHostToDevice) != hipSuccess) {
+      plan->classes.push_back(pc);
+      lfr_plan_destroy(plan);
+      return lfr_set_error(ctx, LFR_ERR_HIP, "plan upload failed");
+    }
+    plan->nbytes += (int64_t)(b1 + b2);
+    plan->classes.push_back(pc);
+  }
+  *out = plan;
+  return LFR_OK;
+}
+
+extern "C" int lfr_plan_destroy(lfr_plan* plan) {
+  if (!plan) return LFR_OK;
+  for (auto& c : plan->classes) {
+    if (c.col_start_d) (void)hipFree(c.col_start_d);
+    if (c.ent_d) (void)hipFree(c.ent_d);
+  }
+  delete plan;
+  return LFR_OK;
+}
+
+extern "C" int lfr_plan_info(const lfr_plan* plan, int* ncolumns, int* nclasses, int64_t* nbytes) {
+  if (!plan) return LFR_ERR_ARG;
+  if (ncolumns) *ncolumns = plan->ncolumns;
+  if (nclasses) *nclasses = (int)plan->classes.size();
+  if (nbytes) *nbytes = plan->nbytes;
+  return LFR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Entry points
+// ---------------------------------------------------------------------------
+extern "C" int lfr_matrix_vector(lfr_ctx* ctx, int prec, void* lhs, const void* x, const void* M,
+                                 const lfr_space* to, const lfr_space* from, const lfr_plan* plan,
+                                 int overwrite) {
+  if (!ctx) return LFR_ERR_ARG;
+  int rc = check_pair(ctx, prec, lhs, x, M, to, from, "lfr_matrix_vector");
+  if (rc) return rc;
+  LFR_REQUIRE(ctx, lhs != x, "lfr_matrix_vector: lhs must not alias x");
+  LFR_REQUIRE(ctx, plan && plan->space == to && plan->kind == 0,
+              "lfr_matrix_vector: plan must be a matrix_vector (kind 0) plan of the output space");
+  const int64_t n3 = (int64_t)to->ncell * to->nlayers;
+  if (prec == 4)
+    return run_gather<float>(ctx, plan, 0, (const float*)M, (const float*)x, (float*)lhs, from,
+                             to->ndf, n3, overwrite);
+  return run_gather<double>(ctx, plan, 0, (const double*)M, (const double*)x, (double*)lhs, from,
+                            to->ndf, n3, overwrite);
+}
+
+extern "C" int lfr_transpose_matrix_vector(lfr_ctx* ctx, int prec, void* lhs, const void* x,
+                                           const void* M, const lfr_space* lsp,
+                                           const lfr_space* xsp, const lfr_plan* plan,
+                                           int overwrite) {
+  if (!ctx) return LFR_ERR_ARG;
+  int rc = check_pair(ctx, prec, lhs, x, M, lsp, xsp, "lfr_transpose_matrix_vector");
+  if (rc) return rc;
+  LFR_REQUIRE(ctx, lhs != x, "lfr_transpose_matrix_vector: lhs must not alias x");
+  LFR_REQUIRE(ctx, plan && plan->space == lsp && plan->kind == 1,
+              "lfr_transpose_matrix_vector: plan must be a transpose (kind 1) plan of lhs' space");
+  const int64_t n3 = (int64_t)lsp->ncell * lsp->nlayers;
+  if (prec == 4)
+    return run_gather<float>(ctx, plan, 1, (const float*)M, (const float*)x, (float*)lhs, xsp,
+                             lsp->ndf, n3, overwrite);
+  return run_gather<double>(ctx, plan, 1, (const double*)M, (const double*)x, (double*)lhs, xsp,
+                            lsp->ndf, n3, overwrite);
+}
+
+static int dg_common(lfr_ctx* ctx, int prec, void* lhs, const void* x, const void* M,
+                     const lfr_space* to, const lfr_space* from, int inc, const char* who) {
+  if (!ctx) return LFR_ERR_ARG;
+  int rc = check_pair(ctx, prec, lhs, x, M, to, from, who);
+  if (rc) return rc;
+  const int64_t n3 = (int64_t)to->ncell * to->nlayers;
+  if (prec == 4)
+    return run_cells<float>(ctx, (const float*)M, (const float*)x, (float*)lhs, to, from, n3, inc);
+  return run_cells<double>(ctx, (const double*)M, (const double*)x, (double*)lhs, to, from, n3, inc);
+}
+
+extern "C" int lfr_dg_matrix_vector(lfr_ctx* ctx, int prec, void* lhs, const void* x,
+                                    const void* M, const lfr_space* to, const lfr_space* from) {
+  return dg_common(ctx, prec, lhs, x, M, to, from, 0, "lfr_dg_matrix_vector");
+}
+
+extern "C" int lfr_dg_inc_matrix_vector(lfr_ctx* ctx, int prec, void* lhs, const void* x,
+                                        const void* M, const lfr_space* to,
+                                        const lfr_space* from) {
+  return dg_common(ctx, prec, lhs, x, M, to, from, 1, "lfr_dg_inc_matrix_vector");
+}

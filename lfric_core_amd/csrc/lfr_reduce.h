@@ -1,0 +1,56 @@
+// Deterministic FP64 block reductions (wavefront shuffles + fixed-order LDS
+// combine).  A reduction over n items always uses the same grid for the same
+// n, and partials are combined in a fixed order, so results are bitwise
+// reproducible run to run.
+#pragma once
+
+#include "lfr_internal.h"
+
+namespace lfr {
+
+constexpr int kRedThreads = 256;
+constexpr int kRedMaxBlocks = 1024;
+
+// Grid size of the first reduction stage for n items (function of n only).
+static inline int red_blocks(int64_t n) {
+  int64_t b = (n + (int64_t)kRedThreads * 16 - 1) / ((int64_t)kRedThreads * 16);
+  if (b < 1) b = 1;
+  if (b > kRedMaxBlocks) b = kRedMaxBlocks;
+  return (int)b;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Sum of one value per thread over a block of kRedThreads threads; the
+// result is valid in thread 0.  `lds` must hold kRedThreads/64 doubles.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < kRedThreads / 64; ++w) s += lds[w];
+  }
+  __syncthreads();
+  return s;
+}
+
+// Final stage: out[q] = sum over b of part[q*stride + b], b < nb, for
+// q < nq (one block per q).
+static __global__ void __launch_bounds__(kRedThreads)
+k_reduce_final(const double* __restrict__ part, int nb, int stride, double* __restrict__ out) {
+  __shared__ double lds[kRedThreads / 64];
+  const double* p = part + (int64_t)blockIdx.x * stride;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kRedThreads) s += p[b];
+  s = block_sum(s, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+}  // namespace lfr

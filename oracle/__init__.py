@@ -1,0 +1,176 @@
+"""CPU oracle for the operator-apply / column-solve path.
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke``
+and the CPU-baseline leg of ``bench.py``, always as the checker.  The
+library in ``lfric_core_amd`` never imports it.
+
+* ``liboracle.so`` (oracle.c) - plain-C restatement of the reference kernels
+  and PSy loop shapes.
+* ``libref.so`` (oracle/_ref, optional) - the reference's own Fortran kernels
+  compiled by flang from the reference tree, driven by ref_psy.c.
+* ``solvers`` - CG / GMRES restated from
+  components/science/source/solver/sci_iterative_solver_mod.f90 in numpy.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_ORACLE = HERE / "_build" / "liboracle.so"
+LIB_REF = HERE / "_ref" / "libref.so"
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_I = C.c_int
+
+
+def _ptr(dtype):
+    return _f32p if np.dtype(dtype) == np.float32 else _f64p
+
+
+def _suf(dtype):
+    return "r32" if np.dtype(dtype) == np.float32 else "r64"
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_ORACLE.exists():
+            raise RuntimeError(f"oracle not built: run `make -C {HERE}`")
+        _lib = C.CDLL(str(LIB_ORACLE))
+        _lib.orc_dot_r32.restype = C.c_double
+        _lib.orc_dot_r64.restype = C.c_double
+    return _lib
+
+
+def ref():
+    """The flang-built reference kernels, or None when not built here."""
+    global _ref
+    if _ref is None and LIB_REF.exists():
+        _ref = C.CDLL(str(LIB_REF))
+    return _ref
+
+
+def nthreads_default() -> int:
+    return int(os.environ.get("ORACLE_THREADS", "1"))
+
+
+# --- invoke-level kernels ---------------------------------------------------
+
+def invoke_coloured(mode, colouring, nlayers, lhs, x, matrix, dm1, dm2, nthreads=None, use_ref=False):
+    """GH_INC loop.  mode 0: matrix_vector (lhs += M x); 1: transpose (lhs += M^T x).
+
+    ``matrix`` is the Fortran local_stencil as a C array of shape
+    (ndf_from, ndf_to, ncell_3d) (same memory); ``dm1``/``dm2`` are (ncell, ndf)
+    int32 dofmaps (Fortran map(ndf, ncell) memory).
+    """
+    nt = nthreads_default() if nthreads is None else nthreads
+    dt = lhs.dtype
+    n3 = matrix.shape[2]
+    ndf1, ndf2 = dm1.shape[1], dm2.shape[1]
+    cic = np.asfortranarray(colouring.cells_in_colour).ravel(order="F").astype(np.int32)
+    npc = colouring.ncells_per_colour.astype(np.int32)
+    if use_ref:
+        r = ref()
+        fn = getattr(r, f"ref_invoke_coloured_{_suf(dt)}")
+        kern = 0 if mode == 0 else 3
+        fn(_I(kern), _I(colouring.ncolours), npc.ctypes.data_as(C.c_void_p),
+           cic.ctypes.data_as(C.c_void_p), _I(nlayers), lhs.ctypes.data_as(C.c_void_p),
+           x.ctypes.data_as(C.c_void_p), _I(n3), matrix.ctypes.data_as(C.c_void_p),
+           _I(ndf1), _I(lhs.size), dm1.ctypes.data_as(C.c_void_p), _I(ndf2), _I(x.size),
+           dm2.ctypes.data_as(C.c_void_p), _I(nt))
+        return lhs
+    fn = getattr(lib(), f"orc_invoke_coloured_{_suf(dt)}")
+    fn(_I(mode), _I(colouring.ncolours), npc.ctypes.data_as(C.c_void_p),
+       cic.ctypes.data_as(C.c_void_p), _I(cic.size // max(colouring.ncolours, 1)),
+       _I(nlayers), lhs.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p), _I(n3),
+       matrix.ctypes.data_as(C.c_void_p), _I(ndf1), dm1.ctypes.data_as(C.c_void_p),
+       _I(ndf2), dm2.ctypes.data_as(C.c_void_p), _I(nt))
+    return lhs
+
+
+def invoke_dg(mode, nlayers, lhs, x, matrix, dm1, dm2, nthreads=None, use_ref=False):
+    """Discontinuous output loop.  mode 0: dg_matrix_vector (overwrite);
+    1: dg_inc_matrix_vector (increment)."""
+    nt = nthreads_default() if nthreads is None else nthreads
+    dt = lhs.dtype
+    n3 = matrix.shape[2]
+    ncell = dm1.shape[0]
+    ndf1, ndf2 = dm1.shape[1], dm2.shape[1]
+    if use_ref:
+        fn = getattr(ref(), f"ref_invoke_cells_{_suf(dt)}")
+        fn(_I(1 if mode == 0 else 2), _I(ncell), _I(nlayers), lhs.ctypes.data_as(C.c_void_p),
+           x.ctypes.data_as(C.c_void_p), _I(n3), matrix.ctypes.data_as(C.c_void_p), _I(ndf1),
+           _I(lhs.size), dm1.ctypes.data_as(C.c_void_p), _I(ndf2), _I(x.size),
+           dm2.ctypes.data_as(C.c_void_p), _I(nt))
+        return lhs
+    fn = getattr(lib(), f"orc_invoke_dg_{_suf(dt)}")
+    fn(_I(mode), _I(ncell), _I(nlayers), lhs.ctypes.data_as(C.c_void_p),
+       x.ctypes.data_as(C.c_void_p), _I(n3), matrix.ctypes.data_as(C.c_void_p), _I(ndf1),
+       dm1.ctypes.data_as(C.c_void_p), _I(ndf2), dm2.ctypes.data_as(C.c_void_p), _I(nt))
+    return lhs
+
+
+def kernel(name, cell, nlayers, lhs, x, matrix, map1, map2, use_ref=False):
+    """One per-column kernel call (cell 1-based).  name in matrix_vector,
+    dg_matrix_vector, dg_inc_matrix_vector, transpose_matrix_vector."""
+    dt = lhs.dtype
+    n3 = matrix.shape[2]
+    map1 = np.ascontiguousarray(map1, dtype=np.int32)
+    map2 = np.ascontiguousarray(map2, dtype=np.int32)
+    if use_ref:
+        kid = {"matrix_vector": 0, "dg_matrix_vector": 1, "dg_inc_matrix_vector": 2,
+               "transpose_matrix_vector": 3}[name]
+        fn = getattr(ref(), f"ref_kernel_{_suf(dt)}")
+        fn(_I(kid), _I(cell), _I(nlayers), lhs.ctypes.data_as(C.c_void_p),
+           x.ctypes.data_as(C.c_void_p), _I(n3), matrix.ctypes.data_as(C.c_void_p),
+           _I(map1.size), _I(lhs.size), map1.ctypes.data_as(C.c_void_p), _I(map2.size),
+           _I(x.size), map2.ctypes.data_as(C.c_void_p))
+        return lhs
+    nm = {"matrix_vector": "orc_matrix_vector_code", "dg_matrix_vector": "orc_dg_matrix_vector_code",
+          "dg_inc_matrix_vector": "orc_matrix_vector_code",
+          "transpose_matrix_vector": "orc_transpose_matrix_vector_code"}[name]
+    fn = getattr(lib(), f"{nm}_{_suf(dt)}")
+    fn(_I(cell), _I(nlayers), lhs.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+       _I(n3), matrix.ctypes.data_as(C.c_void_p), _I(map1.size), map1.ctypes.data_as(C.c_void_p),
+       _I(map2.size), map2.ctypes.data_as(C.c_void_p))
+    return lhs
+
+
+def appinv(lhs, x, cm, dofmap, indirection, nthreads=None, use_ref=False):
+    """columnwise_op_appinv over all columns.  cm is columnwise_matrix(bw, nrow,
+    ncell_2d) as a C array (ncell_2d, nrow, bw)."""
+    nt = nthreads_default() if nthreads is None else nthreads
+    ncell, nrow, bw = cm.shape
+    ind = np.ascontiguousarray(indirection, dtype=np.int32)
+    dm = np.ascontiguousarray(dofmap, dtype=np.int32)
+    if use_ref:
+        assert lhs.dtype == np.float32
+        ref().ref_invoke_appinv_r32(_I(ncell), lhs.ctypes.data_as(C.c_void_p),
+                                    x.ctypes.data_as(C.c_void_p), cm.ctypes.data_as(C.c_void_p),
+                                    _I(nrow), _I(bw), _I(dm.shape[1]), _I(lhs.size),
+                                    dm.ctypes.data_as(C.c_void_p), ind.ctypes.data_as(C.c_void_p),
+                                    _I(nt))
+        return lhs
+    fn = getattr(lib(), f"orc_invoke_appinv_{_suf(lhs.dtype)}")
+    fn(_I(ncell), lhs.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+       cm.ctypes.data_as(C.c_void_p), _I(nrow), _I(bw), _I(dm.shape[1]),
+       dm.ctypes.data_as(C.c_void_p), ind.ctypes.data_as(C.c_void_p), _I(nt))
+    return lhs
+
+
+def dot(x, y, nthreads=None) -> float:
+    nt = nthreads_default() if nthreads is None else nthreads
+    fn = getattr(lib(), f"orc_dot_{_suf(x.dtype)}")
+    return float(fn(C.c_int64(x.size), x.ctypes.data_as(C.c_void_p),
+                    y.ctypes.data_as(C.c_void_p), _I(nt)))

@@ -1,0 +1,328 @@
+/*
+ * CPU oracle for the LFRic Core operator-apply / column-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates the reference's Fortran
+ * kernels and PSy-layer loop shapes in plain C so that tests, the smoke check
+ * and the CPU baseline leg of the benchmark can check the HIP library
+ * against them.  Nothing in the shipped library calls it.
+ *
+ * Conventions kept from the reference so that arrays pass through unchanged:
+ *   - dofmaps are map(ndf, ncell) column-major (1-based dof numbers of the
+ *     bottom cell of each column; layer k adds k);
+ *   - LMA matrices are local_stencil(ncell_3d, ndf_to, ndf_from) column-major,
+ *     element (ik, i, j) at ik + ncell_3d*(i + ndf_to*j), ik = cell*nl + k;
+ *   - columnwise matrices are columnwise_matrix(bw, nrow, ncell_2d).
+ *   - Cell numbers passed to the per-cell kernels are 1-based like the
+ *     Fortran `cell` argument.
+ *
+ * Floating-point: every accumulation is written as separate multiply and add
+ * in the reference's statement order; build with -ffp-contract=off so the
+ * compiler does not fuse them.
+ *
+ * Parity pins (see tests/): the per-cell kernels are compared bit-for-bit
+ * against the reference Fortran compiled by flang (oracle/_ref) and against
+ * the numbers in the reference's pFUnit tests; the solvers are pinned by
+ * mathematical invariants (exact convergence on small SPD systems,
+ * A-orthogonality, agreement with a dense direct solve).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define IDX3(ik, i, j, n3, nto) ((int64_t)(ik) + (int64_t)(n3) * ((int64_t)(i) + (int64_t)(nto) * (int64_t)(j)))
+
+/* ------------------------------------------------------------------------ */
+/* Per-column kernels                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* matrix_vector_code: lhs += M x on one column
+ * (components/science/source/kernel/algebra/matrix_vector_kernel_mod.F90:130-138). */
+#define DEF_MATVEC(T, SUF)                                                              \
+  void orc_matrix_vector_code_##SUF(int cell, int nlayers, T* lhs, const T* x,        \
+                                    int ncell_3d, const T* matrix, int ndf1,           \
+                                    const int* map1, int ndf2, const int* map2) {      \
+    int64_t ik = (int64_t)(cell - 1) * nlayers;                                        \
+    for (int df2 = 0; df2 < ndf2; ++df2) {                                             \
+      int64_t i2 = map2[df2] - 1;                                                      \
+      for (int df = 0; df < ndf1; ++df) {                                              \
+        int64_t i1 = map1[df] - 1;                                                     \
+        const T* m = matrix + IDX3(ik, df, df2, ncell_3d, ndf1);                       \
+        /* Fortran array statement: the whole right-hand side is formed before    \
+           the assignment, which matters when map1(df) columns overlap. */          \
+        T tmp[512];                                                                    \
+        for (int k = 0; k < nlayers; ++k) {                                            \
+          T prod = m[k] * x[i2 + k];                                                   \
+          tmp[k] = lhs[i1 + k] + prod;                                                 \
+        }                                                                              \
+        for (int k = 0; k < nlayers; ++k) lhs[i1 + k] = tmp[k];                        \
+      }                                                                                \
+    }                                                                                  \
+  }
+DEF_MATVEC(float, r32)
+DEF_MATVEC(double, r64)
+
+/* dg_matrix_vector_code: lhs = M x (zero, then accumulate)
+ * (components/science/source/kernel/algebra/dg_matrix_vector_kernel_mod.F90:94-107). */
+#define DEF_DGMATVEC(T, SUF)                                                            \
+  void orc_dg_matrix_vector_code_##SUF(int cell, int nlayers, T* lhs, const T* x,     \
+                                       int ncell_3d, const T* matrix, int ndf1,        \
+                                       const int* map1, int ndf2, const int* map2) {   \
+    for (int df1 = 0; df1 < ndf1; ++df1)                                               \
+      for (int k = 0; k < nlayers; ++k) lhs[map1[df1] - 1 + k] = (T)0;                 \
+    orc_matrix_vector_code_##SUF(cell, nlayers, lhs, x, ncell_3d, matrix, ndf1, map1, \
+                                 ndf2, map2);                                          \
+  }
+DEF_DGMATVEC(float, r32)
+DEF_DGMATVEC(double, r64)
+
+/* transpose_matrix_vector_code: lhs += M^T x, M stored (ncell_3d, ndf2, ndf1)
+ * (components/science/source/kernel/algebra/transpose_matrix_vector_kernel_mod.F90:92-101):
+ * per layer, x_e gathered, lhs_e = matmul(transpose(M(ik,:,:)), x_e) summed
+ * over df2 in ascending order, then lhs(map1(df)+k) += lhs_e(df). */
+#define DEF_TMATVEC(T, SUF)                                                             \
+  void orc_transpose_matrix_vector_code_##SUF(int cell, int nlayers, T* lhs,          \
+                                              const T* x, int ncell_3d,                \
+                                              const T* matrix, int ndf1,               \
+                                              const int* map1, int ndf2,               \
+                                              const int* map2) {                       \
+    T x_e[512], lhs_e[512];                                                            \
+    for (int k = 0; k < nlayers; ++k) {                                                \
+      for (int df = 0; df < ndf2; ++df) x_e[df] = x[map2[df] - 1 + k];                 \
+      int64_t ik = (int64_t)(cell - 1) * nlayers + k;                                  \
+      for (int df = 0; df < ndf1; ++df) {                                              \
+        T acc = (T)0;                                                                  \
+        for (int df2 = 0; df2 < ndf2; ++df2) {                                         \
+          T prod = matrix[IDX3(ik, df2, df, ncell_3d, ndf2)] * x_e[df2];               \
+          acc = acc + prod;                                                            \
+        }                                                                              \
+        lhs_e[df] = acc;                                                               \
+      }                                                                                \
+      for (int df = 0; df < ndf1; ++df) lhs[map1[df] - 1 + k] += lhs_e[df];            \
+    }                                                                                  \
+  }
+DEF_TMATVEC(float, r32)
+DEF_TMATVEC(double, r64)
+
+/* columnwise_op_appinv_kernel_code: Thomas solve of one tridiagonal column,
+ * overwriting lhs (components/science/source/kernel/columnwise_operators/
+ * sci_columnwise_op_appinv_kernel_mod.F90:116-143).  Bands (1,2,3) = (sub,
+ * diag, super); one reciprocal of the pivot per row. */
+#define DEF_APPINV(T, SUF)                                                              \
+  void orc_columnwise_op_appinv_code_##SUF(int cell, T* lhs, const T* x,              \
+                                           const T* cm, int nrow, int bw,              \
+                                           const int* map, const int* ind) {           \
+    T cp[4096], dp[4096];                                                              \
+    const T* c = cm + (int64_t)(cell - 1) * bw * nrow;                                 \
+    for (int i = 0; i < nrow; ++i) {                                                   \
+      int64_t mu = (int64_t)map[0] + ind[i] - 1 - 1;                                   \
+      const T* r = c + (int64_t)bw * i;                                                \
+      if (i == 0) {                                                                    \
+        T inv = (T)1 / r[1];                                                           \
+        cp[i] = inv * r[2];                                                            \
+        dp[i] = inv * x[mu];                                                           \
+      } else {                                                                         \
+        T den = r[1] - r[0] * cp[i - 1];                                               \
+        T inv = (T)1 / den;                                                            \
+        if (i < nrow - 1) cp[i] = inv * r[2];                                          \
+        T num = x[mu] - r[0] * dp[i - 1];                                              \
+        dp[i] = inv * num;                                                             \
+      }                                                                                \
+    }                                                                                  \
+    for (int i = nrow - 1; i >= 0; --i) {                                              \
+      int64_t mu = (int64_t)map[0] + ind[i] - 1 - 1;                                   \
+      if (i < nrow - 1) {                                                              \
+        T prod = cp[i] * dp[i + 1];                                                    \
+        dp[i] = dp[i] - prod;                                                          \
+      }                                                                                \
+      lhs[mu] = dp[i];                                                                 \
+    }                                                                                  \
+  }
+DEF_APPINV(float, r32)
+DEF_APPINV(double, r64)
+
+/* columnwise_op_app_kernel_code: lhs += A x for a banded column operator
+ * (components/science/source/kernel/columnwise_operators/
+ * sci_columnwise_op_app_kernel_mod.F90:116-125).  The band limits are formed
+ * in the operator's real kind, as the reference does:
+ * j_minus = ceiling((alpha*i-gamma_p)/beta), j_plus = floor((alpha*i+gamma_m)/beta). */
+#define DEF_CMAAPP(T, SUF)                                                              \
+  void orc_columnwise_op_app_code_##SUF(int cell, T* lhs, const T* x, const T* cm,    \
+                                        int nrow, int ncol, int bw, int alpha,         \
+                                        int beta, int gamma_m, int gamma_p,            \
+                                        const int* map1, const int* ind_to,            \
+                                        const int* map2, const int* ind_from) {        \
+    const T* c = cm + (int64_t)(cell - 1) * bw * nrow;                                 \
+    for (int i = 1; i <= nrow; ++i) {                                                  \
+      int64_t mu_i = (int64_t)map1[0] + ind_to[i - 1] - 1 - 1;                         \
+      int j_minus = (int)ceil((T)(alpha * i - gamma_p) / ((T)1 * beta));               \
+      int j_plus = (int)floor((T)(alpha * i + gamma_m) / ((T)1 * beta));               \
+      int lo = j_minus > 1 ? j_minus : 1;                                              \
+      int hi = j_plus < ncol ? j_plus : ncol;                                          \
+      for (int j = lo; j <= hi; ++j) {                                                 \
+        int64_t mu_j = (int64_t)map2[0] + ind_from[j - 1] - 1 - 1;                     \
+        T prod = c[(j - j_minus) + (int64_t)bw * (i - 1)] * x[mu_j];                   \
+        lhs[mu_i] = lhs[mu_i] + prod;                                                  \
+      }                                                                                \
+    }                                                                                  \
+  }
+DEF_CMAAPP(float, r32)
+DEF_CMAAPP(double, r64)
+
+/* columnwise_op_asm_kernel_code: assemble one column's banded matrix from
+ * the LMA local stencil (components/science/source/kernel/columnwise_operators/
+ * sci_columnwise_op_asm_kernel_mod.F90:110-126).  column_banded_dofmap(ndf, nl). */
+#define DEF_CMAASM(T, SUF)                                                              \
+  void orc_columnwise_op_asm_code_##SUF(int cell, int nlayers, int ncell_3d,           \
+                                        const T* local_stencil, T* cm, int nrow,       \
+                                        int bw, int alpha, int beta, int gamma_p,      \
+                                        int ndf_to, const int* cbd_to, int ndf_from,   \
+                                        const int* cbd_from) {                         \
+    T* c = cm + (int64_t)(cell - 1) * bw * nrow;                                       \
+    for (int64_t q = 0; q < (int64_t)bw * nrow; ++q) c[q] = (T)0;                      \
+    for (int k = 1; k <= nlayers; ++k) {                                               \
+      int64_t ik = (int64_t)(cell - 1) * nlayers + k - 1;                              \
+      for (int df1 = 0; df1 < ndf_to; ++df1) {                                         \
+        int i = cbd_to[df1 + (int64_t)ndf_to * (k - 1)];                               \
+        int j_minus = (int)ceil((T)(alpha * i - gamma_p) / ((T)1 * beta));             \
+        for (int df2 = 0; df2 < ndf_from; ++df2) {                                     \
+          int j = cbd_from[df2 + (int64_t)ndf_from * (k - 1)];                         \
+          T* e = c + (j - j_minus) + (int64_t)bw * (i - 1);                            \
+          *e = *e + local_stencil[IDX3(ik, df1, df2, ncell_3d, ndf_to)];               \
+        }                                                                              \
+      }                                                                                \
+    }                                                                                  \
+  }
+DEF_CMAASM(float, r32)
+DEF_CMAASM(double, r64)
+
+/* ------------------------------------------------------------------------ */
+/* Invoke-level loops (what the generated PSy layer does around a kernel)   */
+/* ------------------------------------------------------------------------ */
+
+/* Coloured loop for a continuous (GH_INC) output: colour by colour, cells of
+ * one colour in parallel (infrastructure/build/psyclone/psyclone_tools.py:71-119).
+ * cells_in_colour(ncolour, maxc) column-major, 1-based.  mode: 0 forward
+ * matrix_vector, 1 transpose_matrix_vector. */
+#define DEF_INVOKE_COLOURED(T, SUF)                                                     \
+  void orc_invoke_coloured_##SUF(int mode, int ncolour, const int* ncells_per_colour, \
+                                 const int* cells_in_colour, int maxc, int nlayers,    \
+                                 T* lhs, const T* x, int ncell_3d, const T* matrix,    \
+                                 int ndf1, const int* dofmap1, int ndf2,               \
+                                 const int* dofmap2, int nthreads) {                   \
+    for (int col = 0; col < ncolour; ++col) {                                          \
+      int nc = ncells_per_colour[col];                                                 \
+      _Pragma("omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)") \
+      for (int idx = 0; idx < nc; ++idx) {                                             \
+        int cell = cells_in_colour[col + (int64_t)ncolour * idx];                      \
+        const int* m1 = dofmap1 + (int64_t)(cell - 1) * ndf1;                          \
+        const int* m2 = dofmap2 + (int64_t)(cell - 1) * ndf2;                          \
+        if (mode == 0)                                                                 \
+          orc_matrix_vector_code_##SUF(cell, nlayers, lhs, x, ncell_3d, matrix, ndf1,  \
+                                       m1, ndf2, m2);                                  \
+        else                                                                           \
+          orc_transpose_matrix_vector_code_##SUF(cell, nlayers, lhs, x, ncell_3d,      \
+                                                 matrix, ndf1, m1, ndf2, m2);          \
+      }                                                                                \
+    }                                                                                  \
+  }
+DEF_INVOKE_COLOURED(float, r32)
+DEF_INVOKE_COLOURED(double, r64)
+
+/* Plain cell loop for a discontinuous output (psyclone_tools.py:89-95).
+ * mode: 0 dg_matrix_vector (overwrite), 1 dg_inc_matrix_vector (increment). */
+#define DEF_INVOKE_DG(T, SUF)                                                           \
+  void orc_invoke_dg_##SUF(int mode, int ncell, int nlayers, T* lhs, const T* x,      \
+                           int ncell_3d, const T* matrix, int ndf1,                    \
+                           const int* dofmap1, int ndf2, const int* dofmap2,           \
+                           int nthreads) {                                             \
+    _Pragma("omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)") \
+    for (int c = 0; c < ncell; ++c) {                                                  \
+      const int* m1 = dofmap1 + (int64_t)c * ndf1;                                     \
+      const int* m2 = dofmap2 + (int64_t)c * ndf2;                                     \
+      if (mode == 0)                                                                   \
+        orc_dg_matrix_vector_code_##SUF(c + 1, nlayers, lhs, x, ncell_3d, matrix,      \
+                                        ndf1, m1, ndf2, m2);                           \
+      else                                                                             \
+        orc_matrix_vector_code_##SUF(c + 1, nlayers, lhs, x, ncell_3d, matrix, ndf1,   \
+                                     m1, ndf2, m2);                                    \
+    }                                                                                  \
+  }
+DEF_INVOKE_DG(float, r32)
+DEF_INVOKE_DG(double, r64)
+
+/* CMA inverse over all columns (no colouring: the input space is
+ * horizontally discontinuous, columnwise_operator_mod.f90:427-469). */
+#define DEF_INVOKE_APPINV(T, SUF)                                                       \
+  void orc_invoke_appinv_##SUF(int ncell_2d, T* lhs, const T* x, const T* cm,          \
+                               int nrow, int bw, int ndf, const int* dofmap,           \
+                               const int* ind, int nthreads) {                         \
+    _Pragma("omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)") \
+    for (int c = 0; c < ncell_2d; ++c)                                                 \
+      orc_columnwise_op_appinv_code_##SUF(c + 1, lhs, x, cm, nrow, bw,                 \
+                                          dofmap + (int64_t)c * ndf, ind);             \
+  }
+DEF_INVOKE_APPINV(float, r32)
+DEF_INVOKE_APPINV(double, r64)
+
+/* ------------------------------------------------------------------------ */
+/* Vector algebra (PSyclone built-ins / abstract_vector_type methods)        */
+/* ------------------------------------------------------------------------ */
+
+/* X_innerproduct_Y with FP64 accumulation of real(x,r_double)*real(y,r_double):
+ * static per-thread partials summed in thread order
+ * (components/science/source/psy/sci_psykal_light_mod.f90:146-214). */
+double orc_dot_r32(int64_t n, const float* x, const float* y, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  double part[256];
+  if (nthreads > 256) nthreads = 256;
+  for (int t = 0; t < nthreads; ++t) part[t] = 0.0;
+#pragma omp parallel num_threads(nthreads)
+  {
+    int t = 0, nt = 1;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+    nt = omp_get_num_threads();
+#endif
+    int64_t chunk = (n + nt - 1) / nt, lo = t * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0.0;
+    for (int64_t i = lo; i < hi; ++i) s = s + (double)x[i] * (double)y[i];
+    part[t] = s;
+  }
+  double s = 0.0;
+  for (int t = 0; t < nthreads; ++t) s += part[t];
+  return s;
+}
+
+double orc_dot_r64(int64_t n, const double* x, const double* y, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  double part[256];
+  if (nthreads > 256) nthreads = 256;
+  for (int t = 0; t < nthreads; ++t) part[t] = 0.0;
+#pragma omp parallel num_threads(nthreads)
+  {
+    int t = 0, nt = 1;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+    nt = omp_get_num_threads();
+#endif
+    int64_t chunk = (n + nt - 1) / nt, lo = t * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0.0;
+    for (int64_t i = lo; i < hi; ++i) s = s + x[i] * y[i];
+    part[t] = s;
+  }
+  double s = 0.0;
+  for (int t = 0; t < nthreads; ++t) s += part[t];
+  return s;
+}
+
+int orc_omp_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,148 @@
+/*
+ * Loop driver for the reference Fortran kernels compiled by flang.
+ *
+ * TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles the reference's own
+ * kernel sources (matrix_vector, dg_matrix_vector, dg_inc_matrix_vector,
+ * transpose_matrix_vector and the columnwise app/appinv/asm kernels, plus the
+ * constants/argument/kernel metadata modules they use) straight from the
+ * reference tree into oracle/_ref/libref.so, and links this file with them.
+ * This file is the caller: it walks cells the way the generated PSy layer
+ * does (colour by colour with OpenMP over the cells of one colour for
+ * continuous outputs, a plain cell loop for discontinuous ones), calling one
+ * Fortran kernel per column.  It is used to pin the C oracle bit-for-bit and
+ * as the CPU baseline.
+ *
+ * Fortran dummy arguments are passed by reference; explicit-shape arrays are
+ * plain pointers.  Symbol names are flang's module-procedure mangling
+ * _QM<module>P<procedure>.
+ */
+#include <stdint.h>
+
+#define FK(mod, proc) _QM##mod##P##proc
+
+#define DECL_LMA(mod, proc, T)                                                          \
+  extern void FK(mod, proc)(const int* cell, const int* nlayers, T* lhs, const T* x,  \
+                            const int* ncell_3d, const T* matrix, const int* ndf1,     \
+                            const int* undf1, const int* map1, const int* ndf2,        \
+                            const int* undf2, const int* map2);
+
+DECL_LMA(matrix_vector_kernel_mod, matrix_vector_code_r_single, float)
+DECL_LMA(matrix_vector_kernel_mod, matrix_vector_code_r_double, double)
+DECL_LMA(dg_matrix_vector_kernel_mod, dg_matrix_vector_code_r_single, float)
+DECL_LMA(dg_matrix_vector_kernel_mod, dg_matrix_vector_code_r_double, double)
+DECL_LMA(dg_inc_matrix_vector_kernel_mod, dg_inc_matrix_vector_code_r_single, float)
+DECL_LMA(dg_inc_matrix_vector_kernel_mod, dg_inc_matrix_vector_code_r_double, double)
+DECL_LMA(transpose_matrix_vector_kernel_mod, transpose_matrix_vector_code_r_single, float)
+DECL_LMA(transpose_matrix_vector_kernel_mod, transpose_matrix_vector_code_r_double, double)
+
+/* r_solver is real32 in this build (R_SOLVER_PRECISION=32, the reference default). */
+extern void FK(sci_columnwise_op_appinv_kernel_mod, columnwise_op_appinv_kernel_code)(
+    const int* cell, const int* ncell_2d, float* lhs, const float* x, const float* cm,
+    const int* nrow, const int* bw, const int* alpha, const int* beta, const int* gamma_m,
+    const int* gamma_p, const int* ndf, const int* undf, const int* map, const int* ind);
+
+extern void FK(sci_columnwise_op_app_kernel_mod, columnwise_op_app_kernel_code)(
+    const int* cell, const int* ncell_2d, float* lhs, const float* x, const float* cm,
+    const int* nrow, const int* ncol, const int* bw, const int* alpha, const int* beta,
+    const int* gamma_m, const int* gamma_p, const int* ndf1, const int* undf1,
+    const int* map1, const int* ind_to, const int* ndf2, const int* undf2, const int* map2,
+    const int* ind_from);
+
+extern void FK(sci_columnwise_op_asm_kernel_mod, columnwise_op_asm_kernel_code)(
+    const int* cell, const int* nlayers, const int* ncell_2d, const int* ncell_3d,
+    const float* local_stencil, float* cm, const int* nrow, const int* ncol, const int* bw,
+    const int* alpha, const int* beta, const int* gamma_m, const int* gamma_p,
+    const int* ndf_to, const int* cbd_to, const int* ndf_from, const int* cbd_from);
+
+/* kernel ids: 0 matrix_vector, 1 dg_matrix_vector, 2 dg_inc_matrix_vector,
+ * 3 transpose_matrix_vector */
+#define DEF_CALL(T, SUF, S1, S2)                                                         \
+  static void call_##SUF(int kern, int cell, int nl, T* lhs, const T* x, int n3,       \
+                         const T* m, int ndf1, int undf1, const int* m1, int ndf2,     \
+                         int undf2, const int* m2) {                                   \
+    switch (kern) {                                                                     \
+      case 0: FK(matrix_vector_kernel_mod, matrix_vector_code_##S1)(                   \
+                  &cell, &nl, lhs, x, &n3, m, &ndf1, &undf1, m1, &ndf2, &undf2, m2);    \
+        break;                                                                          \
+      case 1: FK(dg_matrix_vector_kernel_mod, dg_matrix_vector_code_##S1)(             \
+                  &cell, &nl, lhs, x, &n3, m, &ndf1, &undf1, m1, &ndf2, &undf2, m2);    \
+        break;                                                                          \
+      case 2: FK(dg_inc_matrix_vector_kernel_mod, dg_inc_matrix_vector_code_##S1)(     \
+                  &cell, &nl, lhs, x, &n3, m, &ndf1, &undf1, m1, &ndf2, &undf2, m2);    \
+        break;                                                                          \
+      default: FK(transpose_matrix_vector_kernel_mod, transpose_matrix_vector_code_##S1)( \
+                  &cell, &nl, lhs, x, &n3, m, &ndf1, &undf1, m1, &ndf2, &undf2, m2);    \
+    }                                                                                   \
+  }                                                                                     \
+  /* one kernel call on one column (cell is 1-based) */                                \
+  void ref_kernel_##SUF(int kern, int cell, int nl, T* lhs, const T* x, int n3,        \
+                        const T* m, int ndf1, int undf1, const int* m1, int ndf2,      \
+                        int undf2, const int* m2) {                                    \
+    call_##SUF(kern, cell, nl, lhs, x, n3, m, ndf1, undf1, m1, ndf2, undf2, m2);       \
+  }                                                                                     \
+  /* coloured PSy loop: cells_in_colour(ncolour, maxc) column-major, 1-based */         \
+  void ref_invoke_coloured_##SUF(int kern, int ncolour, const int* npc, const int* cic, \
+                                 int nl, T* lhs, const T* x, int n3, const T* m,        \
+                                 int ndf1, int undf1, const int* dm1, int ndf2,         \
+                                 int undf2, const int* dm2, int nthreads) {             \
+    for (int col = 0; col < ncolour; ++col) {                                           \
+      int nc = npc[col];                                                                \
+      _Pragma("omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)") \
+      for (int idx = 0; idx < nc; ++idx) {                                              \
+        int cell = cic[col + (int64_t)ncolour * idx];                                   \
+        call_##SUF(kern, cell, nl, lhs, x, n3, m, ndf1, undf1,                          \
+                   dm1 + (int64_t)(cell - 1) * ndf1, ndf2, undf2,                       \
+                   dm2 + (int64_t)(cell - 1) * ndf2);                                   \
+      }                                                                                 \
+    }                                                                                   \
+  }                                                                                     \
+  /* plain PSy loop over cells (discontinuous output) */                                \
+  void ref_invoke_cells_##SUF(int kern, int ncell, int nl, T* lhs, const T* x, int n3, \
+                              const T* m, int ndf1, int undf1, const int* dm1,          \
+                              int ndf2, int undf2, const int* dm2, int nthreads) {      \
+    _Pragma("omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)") \
+    for (int c = 0; c < ncell; ++c)                                                     \
+      call_##SUF(kern, c + 1, nl, lhs, x, n3, m, ndf1, undf1, dm1 + (int64_t)c * ndf1,  \
+                 ndf2, undf2, dm2 + (int64_t)c * ndf2);                                 \
+  }
+
+DEF_CALL(float, r32, r_single, x)
+DEF_CALL(double, r64, r_double, x)
+
+void ref_appinv_r32(int cell, int ncell_2d, float* lhs, const float* x, const float* cm,
+                    int nrow, int bw, int alpha, int beta, int gamma_m, int gamma_p, int ndf,
+                    int undf, const int* map, const int* ind) {
+  FK(sci_columnwise_op_appinv_kernel_mod, columnwise_op_appinv_kernel_code)(
+      &cell, &ncell_2d, lhs, x, cm, &nrow, &bw, &alpha, &beta, &gamma_m, &gamma_p, &ndf,
+      &undf, map, ind);
+}
+
+void ref_invoke_appinv_r32(int ncell_2d, float* lhs, const float* x, const float* cm,
+                           int nrow, int bw, int ndf, int undf, const int* dofmap,
+                           const int* ind, int nthreads) {
+  int one = 1, zero = 0;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int c = 0; c < ncell_2d; ++c) {
+    int cell = c + 1;
+    FK(sci_columnwise_op_appinv_kernel_mod, columnwise_op_appinv_kernel_code)(
+        &cell, &ncell_2d, lhs, x, cm, &nrow, &bw, &one, &one, &zero, &zero, &ndf, &undf,
+        dofmap + (int64_t)c * ndf, ind);
+  }
+}
+
+void ref_app_r32(int cell, int ncell_2d, float* lhs, const float* x, const float* cm,
+                 int nrow, int ncol, int bw, int alpha, int beta, int gamma_m, int gamma_p,
+                 int ndf1, int undf1, const int* map1, const int* ind_to, int ndf2, int undf2,
+                 const int* map2, const int* ind_from) {
+  FK(sci_columnwise_op_app_kernel_mod, columnwise_op_app_kernel_code)(
+      &cell, &ncell_2d, lhs, x, cm, &nrow, &ncol, &bw, &alpha, &beta, &gamma_m, &gamma_p,
+      &ndf1, &undf1, map1, ind_to, &ndf2, &undf2, map2, ind_from);
+}
+
+void ref_asm_r32(int cell, int nlayers, int ncell_2d, int ncell_3d, const float* ls, float* cm,
+                 int nrow, int ncol, int bw, int alpha, int beta, int gamma_m, int gamma_p,
+                 int ndf_to, const int* cbd_to, int ndf_from, const int* cbd_from) {
+  FK(sci_columnwise_op_asm_kernel_mod, columnwise_op_asm_kernel_code)(
+      &cell, &nlayers, &ncell_2d, &ncell_3d, ls, cm, &nrow, &ncol, &bw, &alpha, &beta,
+      &gamma_m, &gamma_p, &ndf_to, cbd_to, &ndf_from, cbd_from);
+}

@@ -1,0 +1,168 @@
+"""The CPU oracle pinned against the reference's own test numbers, the
+reference Fortran compiled by flang (when built here), dense linear algebra
+and closed forms."""
+import numpy as np
+import pytest
+
+import oracle
+from lfric_core_amd.configs import normal_lma
+from lfric_core_amd.function_space import build_dofmap, colour_cells
+from lfric_core_amd.mesh import cubed_sphere
+from lfric_core_amd.rng import Stream
+
+# components/science/unit-test/kernel/algebra/matrix_vector_mod_test.pf:66-72
+D = np.array([[2.0, -2.0], [3.0, -1.0], [0.4, 1.3], [-4.5, 7.9]])
+X2 = np.array([0.25, 1.5])
+X4 = np.array([0.25, 1.5, -0.35, 5.0])   # transpose_matrix_vector_mod_test.pf:54
+
+
+def _lma(d, dtype):
+    """local_stencil(1, nto, nfrom) from a dense (nto, nfrom) matrix."""
+    return np.ascontiguousarray(d.T.reshape(d.shape[1], d.shape[0], 1)).astype(dtype)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-14), (np.float32, 1e-6)])
+@pytest.mark.parametrize("use_ref", [False, True])
+def test_pfunit_numbers(dtype, tol, use_ref):
+    if use_ref and oracle.ref() is None:
+        pytest.skip("reference kernels not built here (oracle/_ref)")
+    m = _lma(D, dtype)
+    ans = D @ X2
+    for name in ("matrix_vector", "dg_matrix_vector", "dg_inc_matrix_vector"):
+        y = np.full(4, 0.0 if name != "dg_matrix_vector" else 99.0, dtype=dtype)
+        oracle.kernel(name, 1, 1, y, X2.astype(dtype), m, [1, 2, 3, 4], [1, 2], use_ref=use_ref)
+        np.testing.assert_allclose(y, ans, rtol=0, atol=tol * 10)
+    y = np.zeros(2, dtype=dtype)
+    oracle.kernel("transpose_matrix_vector", 1, 1, y, X4.astype(dtype), m, [1, 2], [1, 2, 3, 4],
+                  use_ref=use_ref)
+    np.testing.assert_allclose(y, D.T @ X4, rtol=0, atol=tol * 10)
+
+
+def _dense(fo, fi, M):
+    """Assemble the global matrix of an LMA operator by brute force."""
+    nl = fo.nlayers
+    A = np.zeros((fo.undf, fi.undf))
+    for c in range(fo.ncells):
+        for k in range(nl):
+            ik = c * nl + k
+            for i in range(fo.ndf):
+                for j in range(fi.ndf):
+                    A[fo.dofmap[c, i] - 1 + k, fi.dofmap[c, j] - 1 + k] += M[j, i, ik]
+    return A
+
+
+@pytest.mark.parametrize("to,frm,k", [("W2", "W2", 0), ("W0", "W0", 0), ("W3", "W2", 0),
+                                      ("W2", "W3", 0), ("W2", "W2", 1)])
+def test_coloured_loop_equals_dense_operator(to, frm, k):
+    m = cubed_sphere(2)
+    nl = 3
+    col = colour_cells(m)
+    fo, fi = build_dofmap(m, to, nl, k), build_dofmap(m, frm, nl, k)
+    st = Stream(21)
+    M = normal_lma(st, m.ncells * nl, fo.ndf, fi.ndf)
+    x = st.uniform(fi.undf)
+    A = _dense(fo, fi, M)
+    if fo.is_discontinuous:
+        y = oracle.invoke_dg(0, nl, np.zeros(fo.undf), x, M, fo.dofmap, fi.dofmap)
+    else:
+        y = oracle.invoke_coloured(0, col, nl, np.zeros(fo.undf), x, M, fo.dofmap, fi.dofmap)
+    np.testing.assert_allclose(y, A @ x, rtol=1e-13, atol=1e-13)
+    # adjoint: transpose kernel applies A^T
+    if not fi.is_discontinuous:
+        yt = st.uniform(fo.undf)
+        z = oracle.invoke_coloured(1, col, nl, np.zeros(fi.undf), yt, M, fi.dofmap, fo.dofmap)
+        np.testing.assert_allclose(z, A.T @ yt, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_oracle_bitexact_with_reference_fortran(dtype):
+    if oracle.ref() is None:
+        pytest.skip("reference kernels not built here (oracle/_ref)")
+    m = cubed_sphere(6)
+    nl = 8
+    col = colour_cells(m)
+    st = Stream(22)
+    for (to, frm, k) in (("W2", "W2", 0), ("W0", "W0", 0), ("W3", "W2", 0), ("W2", "W3", 0),
+                         ("W2", "W2", 1)):
+        fo, fi = build_dofmap(m, to, nl, k), build_dofmap(m, frm, nl, k)
+        M = normal_lma(st, m.ncells * nl, fo.ndf, fi.ndf).astype(dtype)
+        x = st.uniform(fi.undf).astype(dtype)
+        y0 = st.uniform(fo.undf).astype(dtype)
+        if fo.is_discontinuous:
+            for mode in (0, 1):
+                a = oracle.invoke_dg(mode, nl, y0.copy(), x, M, fo.dofmap, fi.dofmap)
+                b = oracle.invoke_dg(mode, nl, y0.copy(), x, M, fo.dofmap, fi.dofmap, use_ref=True)
+                np.testing.assert_array_equal(a, b)
+        else:
+            a = oracle.invoke_coloured(0, col, nl, y0.copy(), x, M, fo.dofmap, fi.dofmap)
+            b = oracle.invoke_coloured(0, col, nl, y0.copy(), x, M, fo.dofmap, fi.dofmap, use_ref=True)
+            np.testing.assert_array_equal(a, b)
+        if not fi.is_discontinuous:
+            yt = st.uniform(fo.undf).astype(dtype)
+            l0 = st.uniform(fi.undf).astype(dtype)
+            a = oracle.invoke_coloured(1, col, nl, l0.copy(), yt, M, fi.dofmap, fo.dofmap)
+            b = oracle.invoke_coloured(1, col, nl, l0.copy(), yt, M, fi.dofmap, fo.dofmap, use_ref=True)
+            np.testing.assert_array_equal(a, b)
+
+
+def _wtheta_column(nrow, dtype):
+    """columnwise_op_appinv_kernel_mod_test.pf:107-131: 1/3, 2/3, 1/6 bands."""
+    cm = np.zeros((1, nrow, 3), dtype=dtype)
+    for i in range(nrow):
+        cm[0, i, 1] = 2.0 / 3.0 if 0 < i < nrow - 1 else 1.0 / 3.0
+        if i > 0:
+            cm[0, i, 0] = 1.0 / 6.0
+        if i < nrow - 1:
+            cm[0, i, 2] = 1.0 / 6.0
+    dense = np.diag(cm[0, :, 1].astype(np.float64)) + np.diag(cm[0, 1:, 0].astype(np.float64), -1) \
+        + np.diag(cm[0, :-1, 2].astype(np.float64), 1)
+    return cm, dense
+
+
+@pytest.mark.parametrize("use_ref", [False, True])
+def test_appinv_round_trip(use_ref):
+    """dense * appinv(x) == x to 1e-6 (the reference test's tolerance), nlayers=3."""
+    if use_ref and oracle.ref() is None:
+        pytest.skip("reference kernels not built here (oracle/_ref)")
+    nrow = 4
+    cm, dense = _wtheta_column(nrow, np.float32)
+    x = Stream(23).uniform01(nrow).astype(np.float32)
+    lhs = np.zeros(nrow, np.float32)
+    oracle.appinv(lhs, x, cm, np.array([[1, 2]]), np.arange(1, nrow + 1), use_ref=use_ref)
+    np.testing.assert_allclose(dense @ lhs.astype(np.float64), x, atol=1e-6)
+
+
+def test_appinv_matches_banded_solve_and_reference():
+    from scipy.linalg import solve_banded
+    st = Stream(24)
+    ncell, nrow = 50, 70
+    cm = np.empty((ncell, nrow, 3))
+    cm[:, :, 1] = 2.0 + st.uniform(ncell * nrow, 0.1, 10.0).reshape(ncell, nrow)
+    cm[:, :, 0] = -1.0 + 0.1 * st.uniform(ncell * nrow).reshape(ncell, nrow)
+    cm[:, :, 2] = -1.0 + 0.1 * st.uniform(ncell * nrow).reshape(ncell, nrow)
+    x = st.uniform(ncell * nrow)
+    dm = (np.arange(ncell) * nrow + 1).reshape(ncell, 1).astype(np.int32)
+    ind = np.arange(1, nrow + 1)
+    y = oracle.appinv(np.zeros_like(x), x, cm, dm, ind)
+    for c in range(ncell):
+        ab = np.zeros((3, nrow))
+        ab[0, 1:] = cm[c, :-1, 2]
+        ab[1] = cm[c, :, 1]
+        ab[2, :-1] = cm[c, 1:, 0]
+        np.testing.assert_allclose(y[c * nrow:(c + 1) * nrow],
+                                   solve_banded((1, 1), ab, x[c * nrow:(c + 1) * nrow]),
+                                   rtol=1e-12, atol=1e-12)
+    if oracle.ref() is not None:
+        c32, x32 = cm.astype(np.float32), x.astype(np.float32)
+        a = oracle.appinv(np.zeros_like(x32), x32, c32, dm, ind)
+        b = oracle.appinv(np.zeros_like(x32), x32, c32, dm, ind, use_ref=True)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_dot_fp64_accumulation():
+    st = Stream(25)
+    x = st.uniform(100_001).astype(np.float32)
+    y = st.uniform(100_001).astype(np.float32)
+    want = float(np.dot(x.astype(np.float64), y.astype(np.float64)))
+    for nt in (1, 4):
+        assert abs(oracle.dot(x, y, nthreads=nt) - want) <= 1e-12 * abs(want)
