@@ -173,6 +173,84 @@ int lfr_vec_convert(lfr_ctx* ctx, int prec_out, void* y_d, int prec_in, const vo
 int lfr_vec_dot(lfr_ctx* ctx, int prec, int64_t n, const void* x_d, const void* y_d, double* out_h);
 int lfr_vec_dot_dev(lfr_ctx* ctx, int prec, int64_t n, const void* x_d, const void* y_d, double* out_d);
 
+/* ---------------------------------------------------------------------- */
+/* Solvers                                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* A square LMA operator y = M x (mass_matrix_operator_type%apply without
+ * boundary mask: components/science/source/algorithm/solver/
+ * sci_mass_matrix_operator_alg_mod.x90:72-119).  plan: kind-0 plan of space. */
+typedef struct lfr_lma_op {
+  int prec;                 /* 4 or 8: kind of matrix and vectors          */
+  const void* matrix_d;     /* local_stencil(ncell_3d, ndf, ndf)            */
+  const lfr_space* space;
+  const lfr_plan* plan;
+} lfr_lma_op;
+
+/* Solver settings as in the reference constructors
+ * (components/science/source/solver/sci_iterative_solver_mod.f90:465-490). */
+typedef struct lfr_solver_params {
+  double r_tol;             /* stop when ||r||/||r0|| <= r_tol ...            */
+  double a_tol;             /* ... or ||r|| <= a_tol (only if monitoring)     */
+  int max_iter;
+  int monitor_convergence;  /* 0: run exactly max_iter iterations             */
+  int fail_on_non_converged;/* return LFR_ERR_NOT_CONVERGED when not converged */
+  int preconditioner;       /* 0 null (y = x), 1 diagonal (y = x / diag)      */
+  const void* diag_d;       /* diagonal for preconditioner 1                  */
+  int gcrk;                 /* Krylov space size (GMRES)                      */
+  int x_is_zero;            /* 1: ignore x on entry and start from x = 0      */
+} lfr_solver_params;
+
+typedef struct lfr_solver_stats {
+  int iterations;
+  int converged;
+  double r_nrm0;            /* ||b - A x0||                                   */
+  double r_nrm;             /* final ||r|| as the solver last computed it      */
+} lfr_solver_stats;
+
+/* conjugate_gradient_type%cg_solve (sci_iterative_solver_mod.f90:495-592):
+ * solves A x = b; x holds the initial guess on entry.  FP64 scalars; for
+ * prec 4 vectors the scalars are cast to FP32 before each update and inner
+ * products accumulate in FP64. */
+int lfr_cg(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* params, void* x_d,
+           const void* b_d, lfr_solver_stats* stats);
+
+typedef struct lfr_refine_stats {
+  int outer;                /* FP64 residual corrections applied              */
+  int inner_total;          /* FP32 CG iterations over all corrections        */
+  int converged;
+  double b_nrm;
+  double r_nrm;             /* final FP64 ||b - A x||                        */
+} lfr_refine_stats;
+
+/* Iterative refinement: FP64 residual r = b - A64 x, FP32 CG on A32 d = r
+ * (inner settings; started from 0), x += d, until ||r|| <= r_tol ||b||.
+ * A32 is the same operator stored in FP32 (r_solver).  No counterpart
+ * routine in the reference; it combines cg_solve with the r_def / r_solver
+ * split of components/science/source/algorithm/solver/
+ * sci_r_solver_field_vector_mod.x90. */
+int lfr_refined_cg(lfr_ctx* ctx, const lfr_lma_op* A32, const lfr_lma_op* A64, double r_tol,
+                   int max_outer, const lfr_solver_params* inner, void* x64_d, const void* b64_d,
+                   lfr_refine_stats* stats);
+
+/* ---------------------------------------------------------------------- */
+/* Column-wise (CMA) operators                                             */
+/* ---------------------------------------------------------------------- */
+
+/* invoke(columnwise_op_appinv_kernel_type(lhs, x, cma_op)) over all columns:
+ * Thomas solve of each tridiagonal column, OVERWRITING
+ * lhs(map(1) + indirection_dofmap(i) - 1) (components/science/source/kernel/
+ * columnwise_operators/sci_columnwise_op_appinv_kernel_mod.F90:77-145; the
+ * kernel's doc says lhs += A^{-1} x but the code assigns).  cm is
+ * columnwise_matrix(bandwidth=3, nrow, ncell_2d); indirection_d is the
+ * device copy of indirection_dofmap(nrow); space gives map(1) of each column.
+ * fp64_pivots != 0 runs the FP32 recurrence in FP64 (pivots, c', d') and
+ * rounds the solution to FP32 (for ill-conditioned columns). */
+int lfr_columnwise_op_appinv(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_d,
+                             const void* cm_d, int bandwidth, int nrow, int ncell_2d,
+                             const lfr_space* space, const int32_t* indirection_d,
+                             int fp64_pivots);
+
 #ifdef __cplusplus
 }
 #endif

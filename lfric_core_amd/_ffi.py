@@ -61,6 +61,11 @@ SIGNATURES = {
     "lfr_vec_convert": [_P, _I, _P, _I, _P, _I64],
     "lfr_vec_dot": [_P, _I, _I64, _P, _P, C.POINTER(_D)],
     "lfr_vec_dot_dev": [_P, _I, _I64, _P, _P, _P],
+    # solver entry points take structure pointers; lfric_core_amd.solvers
+    # installs the typed signatures before calling them
+    "lfr_columnwise_op_appinv": [_P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I],
+    "lfr_cg": None,
+    "lfr_refined_cg": None,
 }
 _RESTYPE = {"lfr_last_error": C.c_char_p, "lfr_get_stream": _P}
 
@@ -79,7 +84,7 @@ def lib():
                                       "`make -C lfric_core_amd/csrc` or __graft_entry__.build()")
                 L = C.CDLL(str(LIB_PATH))
                 for name, args in SIGNATURES.items():
-                    if not hasattr(L, name):
+                    if args is None or not hasattr(L, name):
                         continue
                     fn = getattr(L, name)
                     fn.argtypes = args

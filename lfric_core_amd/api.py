@@ -304,3 +304,40 @@ def dot(ctx, x, y) -> float:
     out = C.c_double()
     check(lib().lfr_vec_dot(ctx.h, p, n, _P(x.data_ptr()), _P(y.data_ptr()), C.byref(out)), ctx.h)
     return out.value
+
+
+# --- column-wise operators --------------------------------------------------
+
+class CMAOperator:
+    """columnwise_operator_type on one space: columnwise_matrix(bw, nrow, ncell_2d)
+    as a tensor of shape (ncell_2d, nrow, bw) plus indirection_dofmap(nrow)."""
+
+    def __init__(self, matrix: torch.Tensor, space: Space, indirection: np.ndarray):
+        nc = space.fs.ncells
+        if matrix.dim() != 3 or matrix.shape[0] != nc:
+            raise LfrError(1, f"CMA matrix shape {tuple(matrix.shape)} does not match ncell={nc}")
+        if not matrix.is_contiguous() or matrix.device != space.ctx.device:
+            raise LfrError(1, "CMA matrix must be contiguous and on the context's device")
+        self.matrix, self.space = matrix, space
+        self.nrow, self.bandwidth = int(matrix.shape[1]), int(matrix.shape[2])
+        ind = np.ascontiguousarray(indirection, dtype=np.int32)
+        if ind.shape != (self.nrow,):
+            raise LfrError(1, "indirection_dofmap must have nrow entries")
+        base = space.fs.dofmap[:, 0].astype(np.int64)
+        if ind.min() < 1 or base.min() + ind.max() - 1 > space.undf:
+            raise LfrError(1, "indirection_dofmap addresses dofs outside the space")
+        self.indirection = torch.from_numpy(ind).to(space.ctx.device)
+
+
+def columnwise_op_appinv(lhs: torch.Tensor, x: torch.Tensor, op: CMAOperator,
+                         fp64_pivots: bool = False):
+    """lhs = A^{-1} x column by column (tridiagonal Thomas solve; overwrites lhs)."""
+    _field(lhs, op.space, "lhs", op.matrix.dtype)
+    _field(x, op.space, "x", op.matrix.dtype)
+    ctx = op.space.ctx
+    ctx.sync_stream()
+    check(lib().lfr_columnwise_op_appinv(ctx.h, _prec(lhs), _P(lhs.data_ptr()), _P(x.data_ptr()),
+                                         _P(op.matrix.data_ptr()), op.bandwidth, op.nrow,
+                                         op.space.fs.ncells, op.space.h,
+                                         _P(op.indirection.data_ptr()), int(fp64_pivots)), ctx.h)
+    return lhs

@@ -21,131 +21,13 @@
 #include <tuple>
 
 #include "lfr_internal.h"
+#include "lfr_lma_kernels.h"
 
 namespace {
 
-constexpr int kThreads = 256;
-
-// ---------------------------------------------------------------------------
-// Owner-computes gather for a continuous output space.
-//
-// Thread <-> (column, level L).  The column's contributions come in ngroup
-// cell groups (colour order); each group holds GS local dofs (i, shift s) of
-// that cell; entry (i, s) adds the cell's layer k = L - s when 0 <= k < nl.
-//  KIND 0 (matrix_vector): per group, for j (df2) outer, for entries (df)
-//         inner: acc = acc + M(ik,i,j)*x(map_in(j)+k)  (matrix_vector_kernel_mod.F90:130-138)
-//  KIND 1 (transpose):     per group, for entries in layer order: the full
-//         dot over j is formed first, then added
-//         (transpose_matrix_vector_kernel_mod.F90:92-101).
-// ---------------------------------------------------------------------------
-template <typename T, int NDF_IN, int GS, int KIND>
-__global__ void __launch_bounds__(kThreads)
-k_gather(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
-         const int32_t* __restrict__ map_in, const int32_t* __restrict__ col_start,
-         const int32_t* __restrict__ ent, int ncol, int H, int ngroup, int nl, int64_t n3,
-         int ndf_out, int ndf_in_rt, int overwrite, unsigned nb_pad) {
-  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
-  const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
-  if (t >= (int64_t)ncol * H) return;
-  const int col = (int)(t / H);
-  const int L = (int)(t - (int64_t)col * H);
-  const int32_t g0 = col_start[col];
-  const int ndf_in = NDF_IN > 0 ? NDF_IN : ndf_in_rt;
-  T acc = overwrite ? T(0) : y[g0 + L];
-  const int32_t* e = ent + (int64_t)col * ngroup * GS * 2;
-  for (int g = 0; g < ngroup; ++g) {
-    const int32_t* eg = e + g * GS * 2;
-    const int c = eg[0];
-    const int32_t* mi = map_in + (int64_t)c * ndf_in;
-    int ii[GS], kk[GS];
-    bool ok[GS];
-#pragma unroll
-    for (int q = 0; q < GS; ++q) {
-      const int code = eg[2 * q + 1];
-      ii[q] = code >> 1;
-      kk[q] = L - (code & 1);
-      ok[q] = code >= 0 && kk[q] >= 0 && kk[q] < nl;
-    }
-    if (KIND == 0) {
-      for (int j = 0; j < ndf_in; ++j) {
-        const int xb = mi[j];
-#pragma unroll
-        for (int q = 0; q < GS; ++q) {
-          if (ok[q]) {
-            const int64_t ik = (int64_t)c * nl + kk[q];
-            const T m = M[ik + n3 * ((int64_t)ii[q] + (int64_t)ndf_out * j)];
-            const T p = m * x[xb + kk[q]];
-            acc = acc + p;
-          }
-        }
-      }
-    } else {
-      // entries are stored in layer order (larger shift = lower layer first)
-#pragma unroll
-      for (int q = 0; q < GS; ++q) {
-        if (ok[q]) {
-          const int64_t ik = (int64_t)c * nl + kk[q];
-          T dot = T(0);
-          for (int j = 0; j < ndf_in; ++j) {
-            const T m = M[ik + n3 * ((int64_t)j + (int64_t)ndf_in * ii[q])];
-            const T p = m * x[mi[j] + kk[q]];
-            dot = dot + p;
-          }
-          acc = acc + dot;
-        }
-      }
-    }
-  }
-  y[g0 + L] = acc;
-}
-
-// ---------------------------------------------------------------------------
-// Cell-centric apply for a discontinuous output space: thread <-> (cell, k).
-// Each output dof has exactly one (cell, df) contributor, so per output the
-// sum over df2 in ascending order is the reference order
-// (dg_matrix_vector_kernel_mod.F90:94-107, dg_inc_matrix_vector_kernel_mod.F90:134-141).
-// ---------------------------------------------------------------------------
-template <typename T, int NDF_OUT, int NDF_IN>
-__global__ void __launch_bounds__(kThreads)
-k_cells(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
-        const int32_t* __restrict__ map_out, const int32_t* __restrict__ map_in, int ncell,
-        int nl, int64_t n3, int ndf_out_rt, int ndf_in_rt, int inc, unsigned nb_pad) {
-  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
-  const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
-  if (t >= (int64_t)ncell * nl) return;
-  const int c = (int)(t / nl);
-  const int k = (int)(t - (int64_t)c * nl);
-  const int64_t ik = t;
-  const int ndf_in = NDF_IN > 0 ? NDF_IN : ndf_in_rt;
-  const int ndf_out = NDF_OUT > 0 ? NDF_OUT : ndf_out_rt;
-  const int32_t* mo = map_out + (int64_t)c * ndf_out;
-  const int32_t* mi = map_in + (int64_t)c * ndf_in;
-  if constexpr (NDF_OUT > 0 && NDF_IN > 0) {
-    T acc[NDF_OUT];
-#pragma unroll
-    for (int i = 0; i < NDF_OUT; ++i) acc[i] = inc ? y[mo[i] + k] : T(0);
-#pragma unroll
-    for (int j = 0; j < NDF_IN; ++j) {
-      const T xv = x[mi[j] + k];
-#pragma unroll
-      for (int i = 0; i < NDF_OUT; ++i) {
-        const T p = M[ik + n3 * ((int64_t)i + (int64_t)NDF_OUT * j)] * xv;
-        acc[i] = acc[i] + p;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < NDF_OUT; ++i) y[mo[i] + k] = acc[i];
-  } else {
-    for (int i = 0; i < ndf_out; ++i) {
-      T acc = inc ? y[mo[i] + k] : T(0);
-      for (int j = 0; j < ndf_in; ++j) {
-        const T p = M[ik + n3 * ((int64_t)i + (int64_t)ndf_out * j)] * x[mi[j] + k];
-        acc = acc + p;
-      }
-      y[mo[i] + k] = acc;
-    }
-  }
-}
+using lfr::kThreads;
+using lfr::k_gather;
+using lfr::k_cells;
 
 unsigned pad8(unsigned nb) { return (nb + 7u) & ~7u; }
 

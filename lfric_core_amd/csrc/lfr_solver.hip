@@ -1,0 +1,547 @@
+// Krylov solvers over LMA operators, device resident.
+//
+// conjugate_gradient_type%cg_solve
+// (components/science/source/solver/sci_iterative_solver_mod.f90:495-592) with
+// the same statement sequence, FP64 (r_def) scalars and the same convergence
+// test, restated as a fixed chain of kernels per iteration:
+//
+//   apply    z = A p            gather pass, fused partial sums of <p, z>
+//   alpha    alpha = rz / <p,z> one block, fixed-order combine of partials
+//   update   x += alpha p, r -= alpha z (, z = r / diag), partials of <r,r>, <r,z>
+//   beta     ||r||, convergence test, beta = rz_new / rz
+//   direction p = z + beta p
+//
+// The scalars never leave the device.  The convergence predicate is
+// evaluated on the device; once it holds every later kernel returns at
+// entry, so the host launches iterations in captured chunks (a hipGraph of
+// `chunk` iterations) and reads the state back once per chunk.  The iterate
+// sequence is exactly the reference's: the predicate only stops work that
+// the reference would not have done.
+//
+// With the null preconditioner z = P^{-1} r is a copy of r
+// (sci_null_preconditioner_alg_mod.x90:84); r is used in its place.
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+#include "lfr_internal.h"
+#include "lfr_lma_kernels.h"
+#include "lfr_reduce.h"
+
+namespace {
+
+using lfr::kThreads;
+
+enum Slot { S_STOP = 0, S_RZ, S_ALPHA, S_BETA, S_RNRM0, S_RNRM, S_ITER, S_NAN, S_RR, S_NSLOT = 16 };
+// S_STOP: 0 running, 1 converged, 2 max_iter reached, 3 NaN
+
+constexpr double kEps = 2.220446049250313e-16;  // epsilon(1.0_r_def), constants_mod.F90:167
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_cg_alpha(const double* __restrict__ part, int np, double* __restrict__ S, int max_iter) {
+  if (S[S_STOP] != 0.0) return;
+  if (S[S_ITER] >= (double)max_iter) {
+    if (threadIdx.x == 0) S[S_STOP] = 2.0;
+    return;
+  }
+  __shared__ double lds[kThreads / 64];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < np; b += kThreads) s += part[b];
+  s = lfr::block_sum(s, lds);
+  if (threadIdx.x == 0) {
+    S[S_ALPHA] = S[S_RZ] / s;
+    S[S_ITER] = S[S_ITER] + 1.0;
+  }
+}
+
+template <typename T, bool DIAG>
+__global__ void __launch_bounds__(kThreads)
+k_cg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, T* __restrict__ z,
+            const T* __restrict__ diag, int64_t n, const double* __restrict__ S,
+            double* __restrict__ part_rr, double* __restrict__ part_rz) {
+  if (S[S_STOP] != 0.0) return;
+  __shared__ double lds[kThreads / 64];
+  const double alpha = S[S_ALPHA];
+  const T a = (T)alpha, ma = (T)(-alpha);
+  double rr = 0.0, rz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    const T xp = a * p[i];
+    x[i] = x[i] + xp;
+    const T rq = ma * z[i];
+    const T ri = r[i] + rq;
+    r[i] = ri;
+    const double rd = (double)ri;
+    rr = rr + rd * rd;
+    if (DIAG) {
+      const T zi = ri / diag[i];
+      z[i] = zi;
+      rz = rz + rd * (double)zi;
+    }
+  }
+  rr = lfr::block_sum(rr, lds);
+  if (DIAG) rz = lfr::block_sum(rz, lds);
+  if (threadIdx.x == 0) {
+    part_rr[blockIdx.x] = rr;
+    if (DIAG) part_rz[blockIdx.x] = rz;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_cg_beta(const double* __restrict__ part_rr, const double* __restrict__ part_rz, int np,
+          double* __restrict__ S, int monitor, double r_tol, double a_tol, int diag) {
+  if (S[S_STOP] != 0.0) return;
+  __shared__ double lds[kThreads / 64];
+  double rr = 0.0, rz = 0.0;
+  for (int b = threadIdx.x; b < np; b += kThreads) {
+    rr += part_rr[b];
+    if (diag) rz += part_rz[b];
+  }
+  rr = lfr::block_sum(rr, lds);
+  if (diag) rz = lfr::block_sum(rz, lds);
+  if (threadIdx.x == 0) {
+    if (!diag) rz = rr;
+    const double rn = sqrt(rr);
+    S[S_RNRM] = rn;
+    S[S_RR] = rr;
+    if (rn != rn) {
+      S[S_NAN] = 1.0;
+      S[S_STOP] = 3.0;
+      return;
+    }
+    if (monitor && ((rn / S[S_RNRM0] <= r_tol) || (rn <= a_tol))) {
+      S[S_STOP] = 1.0;
+      return;
+    }
+    S[S_BETA] = rz / S[S_RZ];
+    S[S_RZ] = rz;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_cg_direction(T* __restrict__ p, const T* __restrict__ z, int64_t n, const double* __restrict__ S) {
+  if (S[S_STOP] != 0.0) return;
+  const T b = (T)S[S_BETA];
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    const T bp = b * p[i];
+    p[i] = bp + z[i];
+  }
+}
+
+// r = b - r  (r holds A x): the reference's r%scale(-1); r%axpy(1, b)
+// (sci_iterative_solver_mod.f90:518-520), with FP64 partials of <r, r>.
+// If out32 is given, also store (float) r there (refinement residual).
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_residual(T* __restrict__ r, const T* __restrict__ b, int64_t n, double* __restrict__ part,
+           float* __restrict__ out32) {
+  __shared__ double lds[kThreads / 64];
+  double rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    const T m = (T)(-1) * r[i];
+    const T v = m + b[i];
+    r[i] = v;
+    if (out32) out32[i] = (float)v;
+    rr = rr + (double)v * (double)v;
+  }
+  rr = lfr::block_sum(rr, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// S[slot] = (sqrt?) sum(part)
+__global__ void __launch_bounds__(kThreads)
+k_store_sum(const double* __restrict__ part, int np, double* __restrict__ S, int slot, int take_sqrt) {
+  __shared__ double lds[kThreads / 64];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < np; b += kThreads) s += part[b];
+  s = lfr::block_sum(s, lds);
+  if (threadIdx.x == 0) S[slot] = take_sqrt ? sqrt(s) : s;
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ x, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    x[i] = v;
+}
+
+template <typename T>
+__global__ void k_copy_v(T* __restrict__ y, const T* __restrict__ x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    y[i] = x[i];
+}
+
+// z = r / diag with partials of <r, z>
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_prec_diag(T* __restrict__ z, const T* __restrict__ r, const T* __restrict__ diag, int64_t n,
+            double* __restrict__ part) {
+  __shared__ double lds[kThreads / 64];
+  double rz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    const T zi = r[i] / diag[i];
+    z[i] = zi;
+    rz = rz + (double)r[i] * (double)zi;
+  }
+  rz = lfr::block_sum(rz, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = rz;
+}
+
+// x64 += (double) d32   (refinement correction)
+__global__ void k_add_correction(double* __restrict__ x, const float* __restrict__ d, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    x[i] = x[i] + (double)d[i];
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+struct Ws {
+  void* r = nullptr;
+  void* p = nullptr;
+  void* z = nullptr;
+  void* aux = nullptr;       // refinement: fp64 residual, fp32 correction, fp32 rhs
+  void* aux2 = nullptr;
+  void* aux3 = nullptr;
+  double* part_apply = nullptr;
+  double* part_rr = nullptr;
+  double* part_rz = nullptr;
+  double* S = nullptr;
+  int napply = 0;
+};
+
+size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int apply_partials(const lfr_plan* plan) {
+  int np = 0;
+  for (const auto& c : plan->classes) {
+    const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
+    np += (int)((nb + 7u) & ~7u);
+  }
+  return np;
+}
+
+// Carve the solver workspace out of one grow-only device buffer.
+int ws_get(lfr_ctx* ctx, int64_t n, int prec, const lfr_plan* plan, int naux, Ws& w) {
+  const size_t vb = align_up((size_t)n * prec);
+  const int napply = apply_partials(plan);
+  const size_t need = 3 * vb + (size_t)naux * vb + align_up(napply * sizeof(double)) +
+                      2 * align_up(lfr::kRedMaxBlocks * sizeof(double)) +
+                      align_up(S_NSLOT * sizeof(double)) + align_up(8 * sizeof(double));
+  // the first 8 doubles stay free for lfr_vec_dot's result slot users
+  int rc = lfr_work_reserve(ctx, need + align_up((lfr::kRedMaxBlocks + 8) * sizeof(double)));
+  if (rc) return rc;
+  char* b = (char*)ctx->work + align_up((lfr::kRedMaxBlocks + 8) * sizeof(double));
+  w.r = b; b += vb;
+  w.p = b; b += vb;
+  w.z = b; b += vb;
+  if (naux > 0) { w.aux = b; b += vb; }
+  if (naux > 1) { w.aux2 = b; b += vb; }
+  if (naux > 2) { w.aux3 = b; b += vb; }
+  w.part_apply = (double*)b; b += align_up(napply * sizeof(double));
+  w.part_rr = (double*)b; b += align_up(lfr::kRedMaxBlocks * sizeof(double));
+  w.part_rz = (double*)b; b += align_up(lfr::kRedMaxBlocks * sizeof(double));
+  w.S = (double*)b;
+  w.napply = napply;
+  return LFR_OK;
+}
+
+template <typename T, int NDF_IN, int GS>
+void launch_apply_t(hipStream_t s, const lfr_plan_class& c, const T* M, const T* x, T* y,
+                    const int32_t* mi, int nl, int64_t n3, int ndf_out, int ndf_in,
+                    const T* dotv, double* part, const double* stop) {
+  const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
+  const unsigned nbp = (nb + 7u) & ~7u;
+  if (dotv)
+    hipLaunchKernelGGL((lfr::k_gather<T, NDF_IN, GS, 0, true>), dim3(nbp), dim3(kThreads), 0, s, M,
+                       x, y, mi, c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3,
+                       ndf_out, ndf_in, 1, nbp, dotv, part, stop);
+  else
+    hipLaunchKernelGGL((lfr::k_gather<T, NDF_IN, GS, 0, false>), dim3(nbp), dim3(kThreads), 0, s,
+                       M, x, y, mi, c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3,
+                       ndf_out, ndf_in, 1, nbp, (const T*)nullptr, (double*)nullptr,
+                       (const double*)nullptr);
+}
+
+// y = A x (overwrite); with dotv: partials of <y, dotv> into part[0..napply)
+template <typename T>
+int apply_op(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv, double* part,
+             const double* stop) {
+  const lfr_space* sp = A->space;
+  const int nl = sp->nlayers;
+  const int64_t n3 = (int64_t)sp->ncell * nl;
+  int off = 0;
+  for (const auto& c : A->plan->classes) {
+    const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
+    const unsigned nbp = (nb + 7u) & ~7u;
+    double* pp = part ? part + off : nullptr;
+    const T* M = (const T*)A->matrix_d;
+#define LFR_APPLY_CASE(ND, GSV) \
+  launch_apply_t<T, ND, GSV>(ctx->stream, c, M, x, y, sp->map0_d, nl, n3, sp->ndf, sp->ndf, dotv, pp, stop)
+    if (c.gsize == 1) {
+      switch (sp->ndf) {
+        case 6: LFR_APPLY_CASE(6, 1); break;
+        case 8: LFR_APPLY_CASE(8, 1); break;
+        case 27: LFR_APPLY_CASE(27, 1); break;
+        case 36: LFR_APPLY_CASE(36, 1); break;
+        default: LFR_APPLY_CASE(0, 1);
+      }
+    } else {
+      switch (sp->ndf) {
+        case 6: LFR_APPLY_CASE(6, 2); break;
+        case 8: LFR_APPLY_CASE(8, 2); break;
+        case 27: LFR_APPLY_CASE(27, 2); break;
+        case 36: LFR_APPLY_CASE(36, 2); break;
+        default: LFR_APPLY_CASE(0, 2);
+      }
+    }
+#undef LFR_APPLY_CASE
+    LFR_CHECK_LAUNCH(ctx);
+    off += (int)nbp;
+  }
+  return LFR_OK;
+}
+
+int check_op(lfr_ctx* ctx, const lfr_lma_op* A, const char* who) {
+  LFR_REQUIRE(ctx, A && A->matrix_d && A->space && A->plan, "%s: incomplete operator", who);
+  LFR_REQUIRE(ctx, A->prec == 4 || A->prec == 8, "%s: prec %d", who, A->prec);
+  LFR_REQUIRE(ctx, A->plan->space == A->space && A->plan->kind == 0,
+              "%s: operator plan must be a matrix_vector plan of its space", who);
+  return LFR_OK;
+}
+
+bool use_graphs() {
+  const char* e = getenv("LFR_NO_GRAPH");
+  return !(e && e[0] == '1');
+}
+
+template <typename T>
+int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x, Ws& w,
+                  int64_t n, int chunk) {
+  const bool diag = P->preconditioner == 1;
+  T* r = (T*)w.r;
+  T* p = (T*)w.p;
+  T* z = (T*)w.z;
+  const T* dg = (const T*)P->diag_d;
+  const int G = lfr::red_blocks(n);
+  auto one_iteration = [&]() -> int {
+    int rc = apply_op<T>(ctx, A, p, z, p, w.part_apply, w.S);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_apply,
+                       w.napply, w.S, P->max_iter);
+    if (diag)
+      hipLaunchKernelGGL((k_cg_update<T, true>), dim3(G), dim3(kThreads), 0, ctx->stream, x, r, p,
+                         z, dg, n, w.S, w.part_rr, w.part_rz);
+    else
+      hipLaunchKernelGGL((k_cg_update<T, false>), dim3(G), dim3(kThreads), 0, ctx->stream, x, r,
+                         p, z, dg, n, w.S, w.part_rr, w.part_rz);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rr, w.part_rz,
+                       G, w.S, P->monitor_convergence, P->r_tol, P->a_tol, (int)diag);
+    hipLaunchKernelGGL(k_cg_direction<T>, dim3(G), dim3(kThreads), 0, ctx->stream, p,
+                       diag ? (const T*)z : (const T*)r, n, w.S);
+    LFR_CHECK_LAUNCH(ctx);
+    return LFR_OK;
+  };
+
+  hipGraphExec_t exec = nullptr;
+  if (use_graphs()) {
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed) == hipSuccess) {
+      int rc = LFR_OK;
+      for (int i = 0; i < chunk && rc == LFR_OK; ++i) rc = one_iteration();
+      hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+      if (rc != LFR_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e == hipSuccess && graph) {
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
+        (void)hipGraphDestroy(graph);
+      }
+    }
+    (void)hipGetLastError();
+  }
+  double* Sh = ctx->host_scalars;
+  for (;;) {
+    if (exec) {
+      LFR_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+    } else {
+      for (int i = 0; i < chunk; ++i) {
+        int rc = one_iteration();
+        if (rc) return rc;
+      }
+    }
+    LFR_HIP(ctx, hipMemcpyAsync(Sh, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (Sh[S_STOP] != 0.0) break;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  return LFR_OK;
+}
+
+template <typename T>
+int cg_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x, const T* b,
+             lfr_solver_stats* st, Ws& w) {
+  const int64_t n = A->space->undf;
+  const int G = lfr::red_blocks(n);
+  T* r = (T*)w.r;
+  T* p = (T*)w.p;
+  T* z = (T*)w.z;
+  const bool diag = P->preconditioner == 1;
+  // S = 0
+  hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(kThreads), 0, ctx->stream, w.S, (int64_t)S_NSLOT, 0.0);
+  // r = b - A x
+  if (P->x_is_zero) {
+    // x = 0, so A x = 0 exactly: skip the apply
+    hipLaunchKernelGGL(k_fill<T>, dim3(G), dim3(kThreads), 0, ctx->stream, x, n, T(0));
+    hipLaunchKernelGGL(k_fill<T>, dim3(G), dim3(kThreads), 0, ctx->stream, r, n, T(0));
+    // r = b - 0 evaluated as the reference does: (-1)*0 + b == b exactly
+    hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kThreads), 0, ctx->stream, r, b, n,
+                       w.part_rr, (float*)nullptr);
+  } else {
+    int rc = apply_op<T>(ctx, A, x, r, (const T*)nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kThreads), 0, ctx->stream, r, b, n,
+                       w.part_rr, (float*)nullptr);
+  }
+  hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rr, G, w.S,
+                     (int)S_RNRM0, 1);
+  LFR_CHECK_LAUNCH(ctx);
+  double* Sh = ctx->host_scalars;
+  LFR_HIP(ctx, hipMemcpyAsync(Sh, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  st->r_nrm0 = Sh[S_RNRM0];
+  st->r_nrm = Sh[S_RNRM0];
+  st->iterations = 0;
+  st->converged = 0;
+  if (Sh[S_RNRM0] < kEps) {  // cg converged in 0 iterations (:523-529)
+    st->converged = 1;
+    return LFR_OK;
+  }
+  // z = P^{-1} r ; rz = <r, z> ; p = z
+  if (diag) {
+    hipLaunchKernelGGL(k_prec_diag<T>, dim3(G), dim3(kThreads), 0, ctx->stream, z, r,
+                       (const T*)P->diag_d, n, w.part_rz);
+    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rz, G, w.S,
+                       (int)S_RZ, 0);
+    hipLaunchKernelGGL(k_copy_v<T>, dim3(G), dim3(kThreads), 0, ctx->stream, p, (const T*)z, n);
+  } else {
+    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rr, G, w.S,
+                       (int)S_RZ, 0);
+    hipLaunchKernelGGL(k_copy_v<T>, dim3(G), dim3(kThreads), 0, ctx->stream, p, (const T*)r, n);
+  }
+  LFR_CHECK_LAUNCH(ctx);
+  if (P->max_iter <= 0) return LFR_OK;
+  const int chunk = P->max_iter < 10 ? P->max_iter : 10;
+  int rc = cg_iterations<T>(ctx, A, P, x, w, n, chunk);
+  if (rc) return rc;
+  st->iterations = (int)Sh[S_ITER];
+  st->r_nrm = Sh[S_RNRM];
+  st->converged = Sh[S_STOP] == 1.0 ? 1 : 0;
+  if (Sh[S_STOP] == 3.0)
+    return lfr_set_error(ctx, LFR_ERR_NAN, "cg: NaN residual after %d iterations", st->iterations);
+  if (P->monitor_convergence && !st->converged && P->fail_on_non_converged)
+    return lfr_set_error(ctx, LFR_ERR_NOT_CONVERGED,
+                         "cg failed to converge after %d iterations (||r||/||r0|| = %g)",
+                         st->iterations, st->r_nrm / st->r_nrm0);
+  return LFR_OK;
+}
+
+}  // namespace
+
+extern "C" int lfr_cg(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, void* x,
+                      const void* b, lfr_solver_stats* st) {
+  if (!ctx) return LFR_ERR_ARG;
+  int rc = check_op(ctx, A, "lfr_cg");
+  if (rc) return rc;
+  LFR_REQUIRE(ctx, P && x && b && st, "lfr_cg: null argument");
+  LFR_REQUIRE(ctx, P->preconditioner == 0 || (P->preconditioner == 1 && P->diag_d),
+              "lfr_cg: preconditioner must be 0 (null) or 1 (diagonal, with diag_d)");
+  Ws w;
+  rc = ws_get(ctx, A->space->undf, A->prec, A->plan, 0, w);
+  if (rc) return rc;
+  if (A->prec == 4) return cg_solve<float>(ctx, A, P, (float*)x, (const float*)b, st, w);
+  return cg_solve<double>(ctx, A, P, (double*)x, (const double*)b, st, w);
+}
+
+// Mixed-precision iterative refinement: FP64 residual and solution, FP32
+// inner CG (the r_solver split of the reference: r_solver operators and
+// vectors inside an r_def outer loop).
+extern "C" int lfr_refined_cg(lfr_ctx* ctx, const lfr_lma_op* A32, const lfr_lma_op* A64,
+                              double r_tol, int max_outer, const lfr_solver_params* inner,
+                              void* x64_v, const void* b64_v, lfr_refine_stats* st) {
+  if (!ctx) return LFR_ERR_ARG;
+  int rc = check_op(ctx, A32, "lfr_refined_cg(A32)");
+  if (rc) return rc;
+  rc = check_op(ctx, A64, "lfr_refined_cg(A64)");
+  if (rc) return rc;
+  LFR_REQUIRE(ctx, A32->prec == 4 && A64->prec == 8, "lfr_refined_cg: need fp32 and fp64 operators");
+  LFR_REQUIRE(ctx, A32->space->undf == A64->space->undf, "lfr_refined_cg: operator sizes differ");
+  LFR_REQUIRE(ctx, inner && x64_v && b64_v && st && max_outer > 0, "lfr_refined_cg: bad arguments");
+  LFR_REQUIRE(ctx, inner->preconditioner == 0, "lfr_refined_cg: inner preconditioner must be null");
+  const int64_t n = A64->space->undf;
+  double* x = (double*)x64_v;
+  const double* b = (const double*)b64_v;
+  // workspace: fp64-sized slots; the inner fp32 CG uses r, p, z as fp32 views
+  Ws w;
+  rc = ws_get(ctx, n, 8, A32->plan, 3, w);
+  if (rc) return rc;
+  double* r64 = (double*)w.aux;    // fp64 residual b - A x
+  float* d32 = (float*)w.aux2;     // fp32 correction
+  float* rhs32 = (float*)w.aux3;   // fp32 right-hand side of the inner solve
+  const int G = lfr::red_blocks(n);
+  double* Sh = ctx->host_scalars;
+  st->outer = 0;
+  st->inner_total = 0;
+  st->converged = 0;
+  // ||b||
+  hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(kThreads), 0, ctx->stream, w.S, (int64_t)S_NSLOT, 0.0);
+  {
+    double bn = 0.0;
+    rc = lfr_vec_dot(ctx, 8, n, b, b, &bn);
+    if (rc) return rc;
+    st->b_nrm = sqrt(bn);
+  }
+  for (int it = 0; it <= max_outer; ++it) {
+    // r64 = b - A64 x ; rhs32 = (float) r64 ; ||r||
+    rc = apply_op<double>(ctx, A64, x, r64, (const double*)nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_residual<double>, dim3(G), dim3(kThreads), 0, ctx->stream, r64, b, n,
+                       w.part_rr, rhs32);
+    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rr, G, w.S,
+                       (int)S_RNRM, 1);
+    LFR_CHECK_LAUNCH(ctx);
+    LFR_HIP(ctx, hipMemcpyAsync(Sh, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    st->r_nrm = Sh[S_RNRM];
+    if (st->r_nrm != st->r_nrm)
+      return lfr_set_error(ctx, LFR_ERR_NAN, "refined cg: NaN residual at outer iteration %d", it);
+    if (st->r_nrm <= r_tol * st->b_nrm) {
+      st->converged = 1;
+      break;
+    }
+    if (it == max_outer) break;
+    // inner FP32 CG from zero
+    lfr_solver_stats ist;
+    lfr_solver_params ip = *inner;
+    ip.x_is_zero = 1;
+    ip.fail_on_non_converged = 0;
+    rc = cg_solve<float>(ctx, A32, &ip, d32, rhs32, &ist, w);
+    if (rc) return rc;
+    st->inner_total += ist.iterations;
+    st->outer = it + 1;
+    hipLaunchKernelGGL(k_add_correction, dim3(G), dim3(kThreads), 0, ctx->stream, x, d32, n);
+    LFR_CHECK_LAUNCH(ctx);
+  }
+  if (!st->converged)
+    return lfr_set_error(ctx, LFR_ERR_NOT_CONVERGED,
+                         "refined cg: ||r||/||b|| = %g after %d outer iterations",
+                         st->r_nrm / st->b_nrm, st->outer);
+  return LFR_OK;
+}
