@@ -33,10 +33,12 @@ def log(*a):
 
 
 def apply_bytes(n3: int, ndf_to: int, ndf_from: int, undf_to: int, undf_from: int, ncell: int,
-                s: int, inc: bool = False) -> int:
+                s: int, inc: bool = False, same_space: bool = True) -> int:
     """Algorithmic HBM bytes of one LMA apply: matrix + x read + y write (+ y read
-    for the increment form) + both dofmaps read once."""
-    b = n3 * ndf_to * ndf_from * s + undf_from * s + undf_to * s + ncell * (ndf_to + ndf_from) * 4
+    for the increment form) + the dofmap(s) read once (one map when the
+    operator maps a space to itself)."""
+    maps = ncell * ndf_to * 4 if same_space else ncell * (ndf_to + ndf_from) * 4
+    b = n3 * ndf_to * ndf_from * s + undf_from * s + undf_to * s + maps
     if inc:
         b += undf_to * s
     return b
@@ -85,6 +87,67 @@ def cpu_baseline(cfg, seconds: float = 15.0):
                       f"loop, {nt} OpenMP threads), {dt*1e3:.1f} ms each"}
 
 
+def bench_solvers(ctx, cfg, w2, torch, steps_cg: int = 50):
+    """FP64 CG vs FP32-inner/FP64-refined CG to 1e-10 on the W2 mass matrix, and
+    the time of fixed 50-iteration FP32 and FP64 CG runs (per-iteration roofline)."""
+    from lfric_core_amd import api
+    from lfric_core_amd.solvers import ConjugateGradient, RefinedCG
+    dev = ctx.device
+    M64 = torch.from_numpy(cfg.mass).to(dev)
+    M32 = M64.float().contiguous()
+    op64 = api.LMAOperator(M64, w2, w2, cfg.colouring)
+    op32 = api.LMAOperator(M32, w2, w2, cfg.colouring)
+    b = torch.from_numpy(cfg.b).to(dev)
+    n = cfg.w2.undf
+    n3 = cfg.mesh.ncells * cfg.nlayers
+    out = {}
+
+    def true_rel_residual(x):
+        r = torch.empty_like(x)
+        api.matrix_vector(r, x, op64, overwrite=True)
+        r = b - r
+        return float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b))
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = fn()
+        torch.cuda.synchronize()
+        return st, time.perf_counter() - t0
+
+    # warm up (graph capture paths, workspace growth)
+    x = torch.zeros_like(b)
+    ConjugateGradient(op64, r_tol=1e-10, max_iter=20).apply(x, b)
+    x64 = torch.zeros_like(b)
+    st64, t64 = timed(lambda: ConjugateGradient(op64, r_tol=1e-10, max_iter=5000).apply(x64, b))
+    xr = torch.zeros_like(b)
+    rcg = RefinedCG(op32, op64, r_tol=1e-10, max_outer=30, inner_r_tol=1e-5, inner_max_iter=500)
+    str_, tr = timed(lambda: rcg.apply(xr, b))
+    err = float(torch.linalg.vector_norm(xr - x64) / torch.linalg.vector_norm(x64))
+    out["fp64_cg"] = {"seconds": round(t64, 6), "iterations": st64.iterations,
+                      "converged": bool(st64.converged), "rel_residual": true_rel_residual(x64)}
+    out["refined_cg"] = {"seconds": round(tr, 6), "outer": str_.outer,
+                         "inner_iterations": str_.inner_total, "converged": bool(str_.converged),
+                         "rel_residual": true_rel_residual(xr), "rel_err_vs_fp64_cg": err}
+    out["refined_speedup_vs_fp64"] = round(t64 / tr, 3)
+    log(f"FP64 CG to 1e-10: {t64*1e3:.1f} ms, {st64.iterations} its; refined: {tr*1e3:.1f} ms, "
+        f"{str_.outer} outer / {str_.inner_total} inner its; x err vs FP64 {err:.2e}")
+    # fixed 50-iteration CG, per-iteration time against the fused-iteration bytes
+    for name, op, s in (("fp32", op32, 4), ("fp64", op64, 8)):
+        xb = torch.zeros(n, dtype=op.matrix.dtype, device=dev)
+        bb = b.to(op.matrix.dtype)
+        cg = ConjugateGradient(op, max_iter=steps_cg, monitor_convergence=False)
+        cg.apply(xb, bb, x_is_zero=True)
+        _, t = timed(lambda: cg.apply(xb, bb, x_is_zero=True))
+        per_it = t / steps_cg
+        nbytes = n3 * 36 * s + 10 * n * s          # matrix + 10 vector passes per iteration
+        out[f"cg50_{name}"] = {"seconds": round(t, 6), "us_per_iteration": round(per_it * 1e6, 2),
+                               "gbs": round(nbytes / per_it / 1e9, 1),
+                               "roofline_frac": round(nbytes / per_it / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"CG {name}: {per_it*1e6:.1f} us/iteration, {nbytes/per_it/1e9:.0f} GB/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +156,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="C24L70 instead of C96L70 (debug)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-solvers", action="store_true")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="also run a plain copy of 1 GiB (FP32) 5 times, for scaling "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -138,12 +205,16 @@ def main():
         del M, op, x, y, xb
         torch.cuda.empty_cache()
 
-    try:
-        from lfric_core_amd import solvers  # noqa: F401
-        if hasattr(solvers, "bench_refined_cg"):
-            extra["refined_cg"] = solvers.bench_refined_cg(ctx, cfg, w2, log)
-    except ImportError:
-        pass
+    if not args.no_solvers:
+        extra["solvers"] = bench_solvers(ctx, cfg, w2, torch)
+
+    if args.copy_calibrate:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            api.copy(ctx, dst, src)
+        torch.cuda.synchronize()
+        del src, dst
 
     r32 = res["fp32"]
     out = {

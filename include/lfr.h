@@ -143,6 +143,16 @@ int lfr_transpose_matrix_vector(lfr_ctx* ctx, int prec, void* lhs_d, const void*
                                 const lfr_space* x_space, const lfr_plan* plan, int overwrite);
 
 
+/* Gradient of an LMA apply with respect to the local matrices:
+ * Mbar(ik,i,j) (+)= alpha * ybar(map_to(i)+k) * x(map_from(j)+k), for the
+ * operator local_stencil(ncell_3d, ndf_to, ndf_from) of
+ * lfr_matrix_vector/lfr_dg_matrix_vector.  No counterpart in the reference
+ * (it has no adjoint with respect to operator coefficients); used by the
+ * autograd front end. */
+int lfr_lma_outer(lfr_ctx* ctx, int prec, void* Mbar_d, const void* ybar_d, const void* x_d,
+                  const lfr_space* to_space, const lfr_space* from_space, double alpha,
+                  int accumulate);
+
 /* ---------------------------------------------------------------------- */
 /* Vector algebra (abstract_vector_type methods / PSyclone built-ins)      */
 /* ---------------------------------------------------------------------- */
@@ -214,6 +224,13 @@ typedef struct lfr_solver_stats {
  * products accumulate in FP64. */
 int lfr_cg(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* params, void* x_d,
            const void* b_d, lfr_solver_stats* stats);
+
+/* gmres_type%gmres_solve (sci_iterative_solver_mod.f90:830-1003):
+ * left-preconditioned restarted GMRES(gcrk) with modified Gram-Schmidt;
+ * max_iter counts restarts; convergence is ||b - A x|| / max(||r0||, a_tol)
+ * < r_tol, tested once per restart when monitoring. */
+int lfr_gmres(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* params, void* x_d,
+              const void* b_d, lfr_solver_stats* stats);
 
 typedef struct lfr_refine_stats {
   int outer;                /* FP64 residual corrections applied              */

@@ -50,6 +50,7 @@ SIGNATURES = {
     "lfr_dg_matrix_vector": [_P, _I, _P, _P, _P, _P, _P],
     "lfr_dg_inc_matrix_vector": [_P, _I, _P, _P, _P, _P, _P],
     "lfr_transpose_matrix_vector": [_P, _I, _P, _P, _P, _P, _P, _P, _I],
+    "lfr_lma_outer": [_P, _I, _P, _P, _P, _P, _P, _D, _I],
     "lfr_vec_setval": [_P, _I, _I64, _P, _D],
     "lfr_vec_copy": [_P, _I, _I64, _P, _P],
     "lfr_vec_axpy": [_P, _I, _I64, _D, _P, _P],
@@ -66,6 +67,7 @@ SIGNATURES = {
     "lfr_columnwise_op_appinv": [_P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I],
     "lfr_cg": None,
     "lfr_refined_cg": None,
+    "lfr_gmres": None,
 }
 _RESTYPE = {"lfr_last_error": C.c_char_p, "lfr_get_stream": _P}
 

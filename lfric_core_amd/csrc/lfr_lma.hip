@@ -31,32 +31,55 @@ using lfr::k_cells;
 
 unsigned pad8(unsigned nb) { return (nb + 7u) & ~7u; }
 
-template <typename T, int NDF_IN, int GS>
+template <typename T, int NDF_IN, int NG, int GS>
 void launch_gather_t(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
                      const int32_t* mi, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
                      int ndf_in, int overwrite) {
   const unsigned nbp = pad8(nb);
   if (kind == 0)
-    hipLaunchKernelGGL((k_gather<T, NDF_IN, GS, 0>), dim3(nbp), dim3(kThreads), 0, s, M, x, y, mi,
-                       c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3, ndf_out, ndf_in,
-                       overwrite, nbp);
+    hipLaunchKernelGGL((k_gather<T, NDF_IN, NG, GS, 0>), dim3(nbp), dim3(kThreads), 0, s, M, x, y,
+                       mi, c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3, ndf_out,
+                       ndf_in, overwrite, nbp, (const T*)nullptr, (double*)nullptr,
+                       (const double*)nullptr);
   else
-    hipLaunchKernelGGL((k_gather<T, NDF_IN, GS, 1>), dim3(nbp), dim3(kThreads), 0, s, M, x, y, mi,
-                       c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3, ndf_out, ndf_in,
-                       overwrite, nbp);
+    hipLaunchKernelGGL((k_gather<T, NDF_IN, NG, GS, 1>), dim3(nbp), dim3(kThreads), 0, s, M, x, y,
+                       mi, c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3, ndf_out,
+                       ndf_in, overwrite, nbp, (const T*)nullptr, (double*)nullptr,
+                       (const double*)nullptr);
 }
 
-template <typename T, int GS>
-void launch_gather_gs(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
-                      const int32_t* mi, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
-                      int ndf_in, int overwrite) {
+// Compile-time shapes for the common column structures (groups x entries):
+// (2,1) shared side faces, (1,2) bottom/top stacks, (1,1) single-cell
+// columns, (4,2)/(3,2) vertex columns (3 at cube corners), (2,2)
+// horizontal edges and (4,1)/(3,1) vertical edges at element order 1.
+template <typename T, int NDF_IN>
+void launch_gather_shape(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
+                         const int32_t* mi, const lfr_plan_class& c, int nl, int64_t n3,
+                         int ndf_out, int ndf_in, int overwrite) {
+#define SHAPE(NGV, GSV) \
+  if (c.ngroup == NGV && c.gsize == GSV) \
+    return launch_gather_t<T, NDF_IN, NGV, GSV>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite);
+  SHAPE(2, 1) SHAPE(1, 2) SHAPE(1, 1) SHAPE(4, 2) SHAPE(3, 2) SHAPE(2, 2) SHAPE(4, 1) SHAPE(3, 1)
+#undef SHAPE
+  if (c.gsize == 1)
+    launch_gather_t<T, NDF_IN, 0, 1>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite);
+  else
+    launch_gather_t<T, NDF_IN, 0, 2>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite);
+}
+
+template <typename T>
+void launch_gather_any(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
+                       const int32_t* mi, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
+                       int ndf_in, int overwrite) {
   switch (ndf_in) {
-    case 1: launch_gather_t<T, 1, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
-    case 6: launch_gather_t<T, 6, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
-    case 8: launch_gather_t<T, 8, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
-    case 27: launch_gather_t<T, 27, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
-    case 36: launch_gather_t<T, 36, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
-    default: launch_gather_t<T, 0, GS>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite);
+    case 1: launch_gather_shape<T, 1>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
+    case 6: launch_gather_shape<T, 6>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
+    case 8: launch_gather_shape<T, 8>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite); break;
+    default:
+      if (c.gsize == 1)
+        launch_gather_t<T, 0, 0, 1>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite);
+      else
+        launch_gather_t<T, 0, 0, 2>(s, kind, nb, M, x, y, mi, c, nl, n3, ndf_out, ndf_in, overwrite);
   }
 }
 
@@ -67,15 +90,11 @@ int run_gather(lfr_ctx* ctx, const lfr_plan* plan, int kind, const T* M, const T
   for (const auto& c : plan->classes) {
     if (c.ncol == 0) continue;
     const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
-    if (c.gsize == 1)
-      launch_gather_gs<T, 1>(ctx->stream, kind, nb, M, x, y, in->map0_d, c, nl, n3, ndf_out,
-                             in->ndf, overwrite);
-    else if (c.gsize == 2)
-      launch_gather_gs<T, 2>(ctx->stream, kind, nb, M, x, y, in->map0_d, c, nl, n3, ndf_out,
-                             in->ndf, overwrite);
-    else
+    if (c.gsize != 1 && c.gsize != 2)
       return lfr_set_error(ctx, LFR_ERR_ARG, "plan class with %d entries per cell unsupported",
                            c.gsize);
+    launch_gather_any<T>(ctx->stream, kind, nb, M, x, y, in->map0_d, c, nl, n3, ndf_out, in->ndf,
+                         overwrite);
     LFR_CHECK_LAUNCH(ctx);
   }
   return LFR_OK;
@@ -342,4 +361,56 @@ extern "C" int lfr_dg_inc_matrix_vector(lfr_ctx* ctx, int prec, void* lhs, const
                                         const void* M, const lfr_space* to,
                                         const lfr_space* from) {
   return dg_common(ctx, prec, lhs, x, M, to, from, 1, "lfr_dg_inc_matrix_vector");
+}
+
+// ---------------------------------------------------------------------------
+// Gradient of an LMA apply with respect to its local matrices:
+//   y(map_to(i)+k) += M(ik,i,j) x(map_from(j)+k)   =>
+//   dL/dM(ik,i,j) = ybar(map_to(i)+k) * x(map_from(j)+k)
+// Mbar = alpha * ybar (x) x  (+ Mbar when accumulate).  One thread per
+// (cell, layer); every matrix plane is written coalesced.  No counterpart in
+// the reference (it has no adjoint with respect to operator coefficients).
+// ---------------------------------------------------------------------------
+namespace {
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_lma_outer(T* __restrict__ Mbar, const T* __restrict__ ybar, const T* __restrict__ x,
+            const int32_t* __restrict__ map_to, const int32_t* __restrict__ map_from, int ncell,
+            int nl, int ndf_to, int ndf_from, T alpha, int accumulate) {
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t n3 = (int64_t)ncell * nl;
+  if (t >= n3) return;
+  const int c = (int)(t / nl);
+  const int k = (int)(t - (int64_t)c * nl);
+  const int32_t* mt = map_to + (int64_t)c * ndf_to;
+  const int32_t* mf = map_from + (int64_t)c * ndf_from;
+  for (int j = 0; j < ndf_from; ++j) {
+    const T xv = x[mf[j] + k];
+    for (int i = 0; i < ndf_to; ++i) {
+      const T g = alpha * (ybar[mt[i] + k] * xv);
+      T* dst = Mbar + t + n3 * ((int64_t)i + (int64_t)ndf_to * j);
+      *dst = accumulate ? *dst + g : g;
+    }
+  }
+}
+}  // namespace
+
+extern "C" int lfr_lma_outer(lfr_ctx* ctx, int prec, void* Mbar, const void* ybar, const void* x,
+                             const lfr_space* to, const lfr_space* from, double alpha,
+                             int accumulate) {
+  if (!ctx) return LFR_ERR_ARG;
+  int rc = check_pair(ctx, prec, Mbar, ybar, x, to, from, "lfr_lma_outer");
+  if (rc) return rc;
+  const int64_t n3 = (int64_t)to->ncell * to->nlayers;
+  const unsigned nb = lfr_blocks(n3, kThreads);
+  if (prec == 4)
+    hipLaunchKernelGGL(k_lma_outer<float>, dim3(nb), dim3(kThreads), 0, ctx->stream, (float*)Mbar,
+                       (const float*)ybar, (const float*)x, to->map0_d, from->map0_d, to->ncell,
+                       to->nlayers, to->ndf, from->ndf, (float)alpha, accumulate);
+  else
+    hipLaunchKernelGGL(k_lma_outer<double>, dim3(nb), dim3(kThreads), 0, ctx->stream, (double*)Mbar,
+                       (const double*)ybar, (const double*)x, to->map0_d, from->map0_d, to->ncell,
+                       to->nlayers, to->ndf, from->ndf, alpha, accumulate);
+  LFR_CHECK_LAUNCH(ctx);
+  return LFR_OK;
 }

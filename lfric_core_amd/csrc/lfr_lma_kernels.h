@@ -21,73 +21,128 @@ static_assert(kThreads == kRedThreads, "block reductions assume 256 threads");
 //         dot over j is formed first, then added
 //         (transpose_matrix_vector_kernel_mod.F90:92-101).
 // ---------------------------------------------------------------------------
-template <typename T, int NDF_IN, int GS, int KIND, bool DOT = false>
+template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT = false>
 __global__ void __launch_bounds__(kThreads)
 k_gather(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
          const int32_t* __restrict__ map_in, const int32_t* __restrict__ col_start,
-         const int32_t* __restrict__ ent, int ncol, int H, int ngroup, int nl, int64_t n3,
+         const int32_t* __restrict__ ent, int ncol, int H, int ngroup_rt, int nl, int64_t n3,
          int ndf_out, int ndf_in_rt, int overwrite, unsigned nb_pad,
          const T* __restrict__ dotv = nullptr, double* __restrict__ part = nullptr,
          const double* __restrict__ stop = nullptr) {
+  // NG > 0: compile-time group count (all loads issued before the ordered
+  // accumulation); NG == 0: runtime group count.
   if (DOT && stop && stop[0] != 0.0) return;   // solver already converged (uniform)
   const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
   const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
   const bool valid = t < (int64_t)ncol * H;
-  T acc = T(0);
-  int32_t g0 = 0;
-  int L = 0;
-  if (valid) {
-    const int col = (int)(t / H);
-    L = (int)(t - (int64_t)col * H);
-    g0 = col_start[col];
-    const int ndf_in = NDF_IN > 0 ? NDF_IN : ndf_in_rt;
-    if (!overwrite) acc = y[g0 + L];
-    const int32_t* e = ent + (int64_t)col * ngroup * GS * 2;
-    for (int g = 0; g < ngroup; ++g) {
-      const int32_t* eg = e + g * GS * 2;
-      const int c = eg[0];
-      const int32_t* mi = map_in + (int64_t)c * ndf_in;
-      int ii[GS], kk[GS];
-      bool ok[GS];
+  const int ngroup = NG > 0 ? NG : ngroup_rt;
+  const int ndf_in = NDF_IN > 0 ? NDF_IN : ndf_in_rt;
+  // Invalid (padding) lanes compute column 0 level 0 and discard the result,
+  // so every address below stays inside the operands.
+  const int64_t tt = valid ? t : 0;
+  const int col = (int)(tt / H);
+  const int L = (int)(tt - (int64_t)col * H);
+  const int32_t g0 = col_start[col];
+  T acc = (overwrite || !valid) ? T(0) : y[g0 + L];
+  const int32_t* e = ent + (int64_t)col * ngroup * GS * 2;
+  constexpr int NGU = NG > 0 ? NG : 1;          // unrolled groups per pass
+  for (int gb = 0; gb < ngroup; gb += NGU) {
+    int cc[NGU], ii[NGU][GS], kk[NGU][GS];
+    bool ok[NGU][GS];
+#pragma unroll
+    for (int g = 0; g < NGU; ++g) {
+      const int32_t* eg = e + (gb + g) * GS * 2;
+      cc[g] = eg[0];
 #pragma unroll
       for (int q = 0; q < GS; ++q) {
         const int code = eg[2 * q + 1];
-        ii[q] = code >> 1;
-        kk[q] = L - (code & 1);
-        ok[q] = code >= 0 && kk[q] >= 0 && kk[q] < nl;
+        const int k = L - (code & 1);
+        ok[g][q] = code >= 0 && k >= 0 && k < nl;
+        ii[g][q] = code >= 0 ? (code >> 1) : 0;
+        kk[g][q] = k < 0 ? 0 : (k >= nl ? nl - 1 : k);   // clamped: always in bounds
       }
-      if (KIND == 0) {
-        for (int j = 0; j < ndf_in; ++j) {
-          const int xb = mi[j];
+    }
+    if constexpr (NDF_IN > 0) {
+      T mv[NGU][GS][NDF_IN], xv[NGU][GS][NDF_IN];
 #pragma unroll
-          for (int q = 0; q < GS; ++q) {
-            if (ok[q]) {
-              const int64_t ik = (int64_t)c * nl + kk[q];
-              const T m = M[ik + n3 * ((int64_t)ii[q] + (int64_t)ndf_out * j)];
-              const T p = m * x[xb + kk[q]];
-              acc = acc + p;
-            }
-          }
-        }
-      } else {
-        // entries are stored in layer order (larger shift = lower layer first)
+      for (int g = 0; g < NGU; ++g) {
+        const int32_t* mi = map_in + (int64_t)cc[g] * NDF_IN;
+        int xb[NDF_IN];
+#pragma unroll
+        for (int j = 0; j < NDF_IN; ++j) xb[j] = mi[j];
 #pragma unroll
         for (int q = 0; q < GS; ++q) {
-          if (ok[q]) {
-            const int64_t ik = (int64_t)c * nl + kk[q];
+          const int64_t ik = (int64_t)cc[g] * nl + kk[g][q];
+#pragma unroll
+          for (int j = 0; j < NDF_IN; ++j) {
+            const int64_t mo = KIND == 0 ? (int64_t)ii[g][q] + (int64_t)ndf_out * j
+                                         : (int64_t)j + (int64_t)NDF_IN * ii[g][q];
+            mv[g][q][j] = M[ik + n3 * mo];
+            xv[g][q][j] = x[xb[j] + kk[g][q]];
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < NGU; ++g) {
+        if (KIND == 0) {
+          // matrix_vector order inside a cell: df2 outer, df inner
+#pragma unroll
+          for (int j = 0; j < NDF_IN; ++j)
+#pragma unroll
+            for (int q = 0; q < GS; ++q) {
+              const T p = mv[g][q][j] * xv[g][q][j];
+              const T s = acc + p;
+              acc = ok[g][q] ? s : acc;
+            }
+        } else {
+          // transpose order: layer outer (entries stored lower layer first),
+          // the dot over df2 formed before it is added
+#pragma unroll
+          for (int q = 0; q < GS; ++q) {
             T dot = T(0);
-            for (int j = 0; j < ndf_in; ++j) {
-              const T m = M[ik + n3 * ((int64_t)j + (int64_t)ndf_in * ii[q])];
-              const T p = m * x[mi[j] + kk[q]];
+#pragma unroll
+            for (int j = 0; j < NDF_IN; ++j) {
+              const T p = mv[g][q][j] * xv[g][q][j];
               dot = dot + p;
             }
-            acc = acc + dot;
+            const T s = acc + dot;
+            acc = ok[g][q] ? s : acc;
+          }
+        }
+      }
+    } else {
+      // runtime ndf_in: same order, loads interleaved with the accumulation
+#pragma unroll
+      for (int g = 0; g < NGU; ++g) {
+        const int32_t* mi = map_in + (int64_t)cc[g] * ndf_in;
+        if (KIND == 0) {
+          for (int j = 0; j < ndf_in; ++j) {
+            const int xbj = mi[j];
+#pragma unroll
+            for (int q = 0; q < GS; ++q) {
+              const int64_t ik = (int64_t)cc[g] * nl + kk[g][q];
+              const T p = M[ik + n3 * ((int64_t)ii[g][q] + (int64_t)ndf_out * j)] * x[xbj + kk[g][q]];
+              const T s = acc + p;
+              acc = ok[g][q] ? s : acc;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < GS; ++q) {
+            const int64_t ik = (int64_t)cc[g] * nl + kk[g][q];
+            T dot = T(0);
+            for (int j = 0; j < ndf_in; ++j) {
+              const T p = M[ik + n3 * ((int64_t)j + (int64_t)ndf_in * ii[g][q])] * x[mi[j] + kk[g][q]];
+              dot = dot + p;
+            }
+            const T s = acc + dot;
+            acc = ok[g][q] ? s : acc;
           }
         }
       }
     }
-    y[g0 + L] = acc;
   }
+  if (valid) y[g0 + L] = acc;
   if (DOT) {
     __shared__ double lds[kThreads / 64];
     double v = valid ? (double)acc * (double)dotv[g0 + L] : 0.0;

@@ -252,21 +252,36 @@ int ws_get(lfr_ctx* ctx, int64_t n, int prec, const lfr_plan* plan, int naux, Ws
   return LFR_OK;
 }
 
-template <typename T, int NDF_IN, int GS>
+template <typename T, int NDF_IN, int NG, int GS>
 void launch_apply_t(hipStream_t s, const lfr_plan_class& c, const T* M, const T* x, T* y,
                     const int32_t* mi, int nl, int64_t n3, int ndf_out, int ndf_in,
                     const T* dotv, double* part, const double* stop) {
   const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
   const unsigned nbp = (nb + 7u) & ~7u;
   if (dotv)
-    hipLaunchKernelGGL((lfr::k_gather<T, NDF_IN, GS, 0, true>), dim3(nbp), dim3(kThreads), 0, s, M,
-                       x, y, mi, c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3,
+    hipLaunchKernelGGL((lfr::k_gather<T, NDF_IN, NG, GS, 0, true>), dim3(nbp), dim3(kThreads), 0, s,
+                       M, x, y, mi, c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3,
                        ndf_out, ndf_in, 1, nbp, dotv, part, stop);
   else
-    hipLaunchKernelGGL((lfr::k_gather<T, NDF_IN, GS, 0, false>), dim3(nbp), dim3(kThreads), 0, s,
-                       M, x, y, mi, c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3,
+    hipLaunchKernelGGL((lfr::k_gather<T, NDF_IN, NG, GS, 0, false>), dim3(nbp), dim3(kThreads), 0,
+                       s, M, x, y, mi, c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3,
                        ndf_out, ndf_in, 1, nbp, (const T*)nullptr, (double*)nullptr,
                        (const double*)nullptr);
+}
+
+template <typename T, int NDF_IN>
+void launch_apply_shape(hipStream_t s, const lfr_plan_class& c, const T* M, const T* x, T* y,
+                        const int32_t* mi, int nl, int64_t n3, int ndf, const T* dotv,
+                        double* part, const double* stop) {
+#define SHAPE(NGV, GSV) \
+  if (c.ngroup == NGV && c.gsize == GSV) \
+    return launch_apply_t<T, NDF_IN, NGV, GSV>(s, c, M, x, y, mi, nl, n3, ndf, ndf, dotv, part, stop);
+  SHAPE(2, 1) SHAPE(1, 2) SHAPE(1, 1) SHAPE(4, 2) SHAPE(3, 2) SHAPE(2, 2) SHAPE(4, 1) SHAPE(3, 1)
+#undef SHAPE
+  if (c.gsize == 1)
+    launch_apply_t<T, NDF_IN, 0, 1>(s, c, M, x, y, mi, nl, n3, ndf, ndf, dotv, part, stop);
+  else
+    launch_apply_t<T, NDF_IN, 0, 2>(s, c, M, x, y, mi, nl, n3, ndf, ndf, dotv, part, stop);
 }
 
 // y = A x (overwrite); with dotv: partials of <y, dotv> into part[0..napply)
@@ -282,26 +297,15 @@ int apply_op(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv,
     const unsigned nbp = (nb + 7u) & ~7u;
     double* pp = part ? part + off : nullptr;
     const T* M = (const T*)A->matrix_d;
-#define LFR_APPLY_CASE(ND, GSV) \
-  launch_apply_t<T, ND, GSV>(ctx->stream, c, M, x, y, sp->map0_d, nl, n3, sp->ndf, sp->ndf, dotv, pp, stop)
-    if (c.gsize == 1) {
-      switch (sp->ndf) {
-        case 6: LFR_APPLY_CASE(6, 1); break;
-        case 8: LFR_APPLY_CASE(8, 1); break;
-        case 27: LFR_APPLY_CASE(27, 1); break;
-        case 36: LFR_APPLY_CASE(36, 1); break;
-        default: LFR_APPLY_CASE(0, 1);
-      }
-    } else {
-      switch (sp->ndf) {
-        case 6: LFR_APPLY_CASE(6, 2); break;
-        case 8: LFR_APPLY_CASE(8, 2); break;
-        case 27: LFR_APPLY_CASE(27, 2); break;
-        case 36: LFR_APPLY_CASE(36, 2); break;
-        default: LFR_APPLY_CASE(0, 2);
-      }
+switch (sp->ndf) {
+      case 6: launch_apply_shape<T, 6>(ctx->stream, c, M, x, y, sp->map0_d, nl, n3, sp->ndf, dotv, pp, stop); break;
+      case 8: launch_apply_shape<T, 8>(ctx->stream, c, M, x, y, sp->map0_d, nl, n3, sp->ndf, dotv, pp, stop); break;
+      default:
+        if (c.gsize == 1)
+          launch_apply_t<T, 0, 0, 1>(ctx->stream, c, M, x, y, sp->map0_d, nl, n3, sp->ndf, sp->ndf, dotv, pp, stop);
+        else
+          launch_apply_t<T, 0, 0, 2>(ctx->stream, c, M, x, y, sp->map0_d, nl, n3, sp->ndf, sp->ndf, dotv, pp, stop);
     }
-#undef LFR_APPLY_CASE
     LFR_CHECK_LAUNCH(ctx);
     off += (int)nbp;
   }
@@ -544,4 +548,207 @@ extern "C" int lfr_refined_cg(lfr_ctx* ctx, const lfr_lma_op* A32, const lfr_lma
                          "refined cg: ||r||/||b|| = %g after %d outer iterations",
                          st->r_nrm / st->b_nrm, st->outer);
   return LFR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GMRES(gcrk): gmres_type%gmres_solve
+// (components/science/source/solver/sci_iterative_solver_mod.f90:830-1003).
+// Left preconditioned, modified Gram-Schmidt Arnoldi.  Inner products are
+// written straight into a device copy of the Hessenberg matrix and the
+// orthogonalisation axpys read their coefficient from there, so an Arnoldi
+// cycle never waits on the host; the (gcrk+1) x gcrk Hessenberg comes back
+// once per restart for the Givens rotations and back substitution, done on
+// the host in FP64 exactly as the reference writes them.
+// ---------------------------------------------------------------------------
+int lfr_dot_queue(lfr_ctx* ctx, int prec, int64_t n, const void* x, const void* y, double* part,
+                  double* out_d);
+
+namespace {
+
+// y += (T)(sign * a[0]) * x
+template <typename T>
+__global__ void k_axpy_dev(T* __restrict__ y, const T* __restrict__ x, int64_t n,
+                           const double* __restrict__ a, double sign) {
+  const T c = (T)(sign * a[0]);
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const T p = c * x[i];
+    y[i] = y[i] + p;
+  }
+}
+
+// v = (T)(1 / a[0]) * s   (copy then scale(1/beta))
+template <typename T>
+__global__ void k_scale_inv_dev(T* __restrict__ v, const T* __restrict__ s, int64_t n,
+                                const double* __restrict__ a) {
+  const T c = (T)(1.0 / a[0]);
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    v[i] = c * s[i];
+}
+
+__global__ void k_sqrt_slot(double* __restrict__ v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) v[0] = sqrt(v[0]);
+}
+
+template <typename T>
+__global__ void k_divide_v(T* __restrict__ z, const T* __restrict__ x, const T* __restrict__ d, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    z[i] = x[i] / d[i];
+}
+
+template <typename T>
+__global__ void k_axpy_val(T* __restrict__ y, const T* __restrict__ x, int64_t n, T a) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const T p = a * x[i];
+    y[i] = y[i] + p;
+  }
+}
+
+template <typename T>
+int gmres_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x, const T* b,
+                lfr_solver_stats* st) {
+  const int64_t n = A->space->undf;
+  const int m = P->gcrk;
+  const int G = lfr::red_blocks(n);
+  const size_t vb = align_up((size_t)n * sizeof(T));
+  const size_t hb = align_up((size_t)((m + 1) * m + 4) * sizeof(double));
+  const size_t head = align_up((lfr::kRedMaxBlocks + 8) * sizeof(double));
+  const size_t part_b = align_up(lfr::kRedMaxBlocks * sizeof(double));
+  int rc = lfr_work_reserve(ctx, head + (size_t)(m + 3) * vb + hb + part_b);
+  if (rc) return rc;
+  char* base = (char*)ctx->work + head;
+  T* V = (T*)base;                                   // m vectors
+  T* w = (T*)(base + (size_t)m * vb);
+  T* s = (T*)(base + (size_t)(m + 1) * vb);
+  T* res = (T*)(base + (size_t)(m + 2) * vb);
+  double* H = (double*)(base + (size_t)(m + 3) * vb);   // h(i,j) at H[i + (m+1)*j]; H[(m+1)*m] = beta
+  double* part = (double*)((char*)H + hb);
+  double* beta_d = H + (m + 1) * m;
+  const bool diag = P->preconditioner == 1;
+  const T* dg = (const T*)P->diag_d;
+  auto Vv = [&](int i) { return V + (size_t)i * (vb / sizeof(T)); };
+  auto prec = [&](T* out, const T* in) {
+    if (diag)
+      hipLaunchKernelGGL(k_divide_v<T>, dim3(G), dim3(kThreads), 0, ctx->stream, out, in, dg, n);
+    else
+      hipLaunchKernelGGL(k_copy_v<T>, dim3(G), dim3(kThreads), 0, ctx->stream, out, in, n);
+  };
+  auto norm_host = [&](const T* v, double* out) -> int {
+    int r = lfr_dot_queue(ctx, sizeof(T), n, v, v, part, beta_d + 1);
+    if (r) return r;
+    LFR_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, beta_d + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = sqrt(ctx->host_scalars[0]);
+    return LFR_OK;
+  };
+  // res = b - A x  (res%copy(b); res%axpy(-1, Ax))
+  rc = apply_op<T>(ctx, A, x, res, (const T*)nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kThreads), 0, ctx->stream, res, b, n, part, (float*)nullptr);
+  double sc_err = 0;
+  rc = norm_host(res, &sc_err);
+  if (rc) return rc;
+  st->r_nrm0 = sc_err;
+  st->r_nrm = sc_err;
+  st->iterations = 0;
+  st->converged = 0;
+  if (sc_err < kEps) {
+    st->converged = 1;
+    return LFR_OK;
+  }
+  if (P->monitor_convergence) sc_err = sc_err > P->a_tol ? sc_err : P->a_tol;
+  std::vector<double> h((size_t)(m + 1) * m), g(m + 1), u(m);
+  double err = 0;
+  int it;
+  for (it = 1; it <= P->max_iter; ++it) {
+    prec(s, res);
+    rc = lfr_dot_queue(ctx, sizeof(T), n, s, s, part, beta_d);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sqrt_slot, dim3(1), dim3(1), 0, ctx->stream, beta_d);
+    hipLaunchKernelGGL(k_scale_inv_dev<T>, dim3(G), dim3(kThreads), 0, ctx->stream, Vv(0), s, n, beta_d);
+    for (int iv = 0; iv < m; ++iv) {
+      rc = apply_op<T>(ctx, A, Vv(iv), s, (const T*)nullptr, nullptr, nullptr);
+      if (rc) return rc;
+      prec(w, s);
+      for (int j = 0; j <= iv; ++j) {
+        double* hij = H + j + (size_t)(m + 1) * iv;
+        rc = lfr_dot_queue(ctx, sizeof(T), n, Vv(j), w, part, hij);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_axpy_dev<T>, dim3(G), dim3(kThreads), 0, ctx->stream, w, Vv(j), n, hij, -1.0);
+      }
+      double* hn = H + (iv + 1) + (size_t)(m + 1) * iv;
+      rc = lfr_dot_queue(ctx, sizeof(T), n, w, w, part, hn);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_sqrt_slot, dim3(1), dim3(1), 0, ctx->stream, hn);
+      if (iv < m - 1)
+        hipLaunchKernelGGL(k_scale_inv_dev<T>, dim3(G), dim3(kThreads), 0, ctx->stream, Vv(iv + 1), w, n, hn);
+      LFR_CHECK_LAUNCH(ctx);
+    }
+    // Hessenberg and beta to the host
+    std::vector<double> hb_((size_t)(m + 1) * m + 1);
+    LFR_HIP(ctx, hipMemcpyAsync(hb_.data(), H, hb_.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    auto hh = [&](int i, int j) -> double& { return hb_[(size_t)i + (size_t)(m + 1) * j]; };
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = hb_[(size_t)(m + 1) * m];
+    // Givens rotations (:930-944)
+    for (int iv = 0; iv < m; ++iv) {
+      const double nrm = sqrt(hh(iv, iv) * hh(iv, iv) + hh(iv + 1, iv) * hh(iv + 1, iv));
+      const double si = hh(iv + 1, iv) / nrm, ci = hh(iv, iv) / nrm;
+      const double p_ = ci * g[iv] + si * g[iv + 1];
+      const double q_ = -si * g[iv] + ci * g[iv + 1];
+      g[iv] = p_;
+      g[iv + 1] = q_;
+      for (int j = iv; j < m; ++j) {
+        const double h1 = ci * hh(iv, j) + si * hh(iv + 1, j);
+        const double h2 = -si * hh(iv, j) + ci * hh(iv + 1, j);
+        hh(iv, j) = h1;
+        hh(iv + 1, j) = h2;
+      }
+    }
+    // back substitution (:946-953)
+    u[m - 1] = g[m - 1] / hh(m - 1, m - 1);
+    for (int iv = m - 2; iv >= 0; --iv) {
+      u[iv] = g[iv];
+      for (int j = iv + 1; j < m; ++j) u[iv] = u[iv] - hh(iv, j) * u[j];
+      u[iv] = u[iv] / hh(iv, iv);
+    }
+    for (int iv = 0; iv < m; ++iv)
+      hipLaunchKernelGGL(k_axpy_val<T>, dim3(G), dim3(kThreads), 0, ctx->stream, x, Vv(iv), n, (T)u[iv]);
+    // res = -1 * (A x) + b   (res%copy(Ax); res%aypx(-1, b))
+    rc = apply_op<T>(ctx, A, x, res, (const T*)nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kThreads), 0, ctx->stream, res, b, n, part, (float*)nullptr);
+    LFR_CHECK_LAUNCH(ctx);
+    st->iterations = it;
+    if (P->monitor_convergence) {
+      double rn = 0;
+      rc = norm_host(res, &rn);
+      if (rc) return rc;
+      st->r_nrm = rn;
+      err = rn / sc_err;
+      if (err != err) return lfr_set_error(ctx, LFR_ERR_NAN, "gmres: NaN residual at restart %d", it);
+      if (err < P->r_tol) {
+        st->converged = 1;
+        return LFR_OK;
+      }
+    }
+  }
+  if (P->monitor_convergence && P->fail_on_non_converged)
+    return lfr_set_error(ctx, LFR_ERR_NOT_CONVERGED, "GMRES solver_algorithm: NOT converged in %d iters, Res=%g",
+                         P->max_iter, err);
+  return LFR_OK;
+}
+
+}  // namespace
+
+extern "C" int lfr_gmres(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, void* x,
+                         const void* b, lfr_solver_stats* st) {
+  if (!ctx) return LFR_ERR_ARG;
+  int rc = check_op(ctx, A, "lfr_gmres");
+  if (rc) return rc;
+  LFR_REQUIRE(ctx, P && x && b && st && P->gcrk > 0 && P->gcrk <= 512, "lfr_gmres: bad arguments");
+  LFR_REQUIRE(ctx, P->preconditioner == 0 || (P->preconditioner == 1 && P->diag_d),
+              "lfr_gmres: preconditioner must be 0 (null) or 1 (diagonal, with diag_d)");
+  if (A->prec == 4) return gmres_solve<float>(ctx, A, P, (float*)x, (const float*)b, st);
+  return gmres_solve<double>(ctx, A, P, (double*)x, (const double*)b, st);
 }

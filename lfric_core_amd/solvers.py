@@ -50,6 +50,8 @@ def _sig():
     L.lfr_refined_cg.argtypes = [_P, C.POINTER(LmaOp), C.POINTER(LmaOp), C.c_double, C.c_int,
                                  C.POINTER(SolverParams), _P, _P, C.POINTER(RefineStats)]
     L.lfr_refined_cg.restype = C.c_int
+    L.lfr_gmres.argtypes = L.lfr_cg.argtypes
+    L.lfr_gmres.restype = C.c_int
     return L
 
 
@@ -131,5 +133,35 @@ class RefinedCG:
         check(L.lfr_refined_cg(ctx.h, C.byref(A32), C.byref(A64), self.r_tol, self.max_outer,
                                C.byref(self.inner), _P(x64.data_ptr()), _P(b64.data_ptr()),
                                C.byref(st)), ctx.h)
+        self.stats = st
+        return st
+
+
+class GMRES(ConjugateGradient):
+    """gmres_type (sci_iterative_solver_mod.f90:795-1003): restarted GMRES(gcrk);
+    max_iter counts restarts."""
+
+    def __init__(self, lin_op, prec=None, gcrk: int = 10, r_tol: float = 1e-8, a_tol: float = 0.0,
+                 max_iter: int = 10, monitor_convergence: bool = True,
+                 fail_on_non_converged: bool = False):
+        super().__init__(lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence,
+                         fail_on_non_converged)
+        self.gcrk = gcrk
+
+    def apply(self, x, b, x_is_zero: bool = False):
+        op = self.lin_op
+        api._field(x, op.to_space, "x", op.matrix.dtype)
+        api._field(b, op.to_space, "b", op.matrix.dtype)
+        if x_is_zero:
+            x.zero_()
+        L = _sig()
+        ctx = op.ctx
+        ctx.sync_stream()
+        A = lma_op(op)
+        P = self.params(False)
+        P.gcrk = self.gcrk
+        st = SolverStats()
+        check(L.lfr_gmres(ctx.h, C.byref(A), C.byref(P), _P(x.data_ptr()), _P(b.data_ptr()),
+                          C.byref(st)), ctx.h)
         self.stats = st
         return st

@@ -113,3 +113,28 @@ def test_refined_cg_reaches_1e10_and_matches_fp64(ctx, prob):
     ConjugateGradient(op64, r_tol=1e-10, max_iter=1000).apply(x64, b)
     x64 = x64.cpu().numpy()
     assert np.linalg.norm(xr - x64) <= 1e-9 * np.linalg.norm(x64)
+
+
+def test_gmres_fp64_matches_oracle(ctx, prob):
+    """GMRES(5) on a nonsymmetric W2 operator (random local matrices + 8 I)."""
+    from oracle.solvers import gmres_solve
+    from lfric_core_amd import api
+    from lfric_core_amd.configs import normal_lma
+    from lfric_core_amd.rng import Stream
+    from lfric_core_amd.solvers import GMRES
+    cfg, w2 = prob
+    st = Stream(51)
+    n3 = cfg.mesh.ncells * cfg.nlayers
+    M = normal_lma(st, n3, 6, 6) * 0.3
+    for i in range(6):
+        M[i, i, :] += 8.0
+    A = _oracle_apply(cfg, M)
+    xo = np.zeros(cfg.w2.undf)
+    it_o, conv_o, err_o = gmres_solve(A, xo, cfg.b, gcrk=5, r_tol=1e-9, max_iter=60)
+    op = api.LMAOperator(torch.from_numpy(M).to(ctx.device), w2, w2, cfg.colouring)
+    x = torch.zeros(cfg.w2.undf, dtype=torch.float64, device=ctx.device)
+    s = GMRES(op, gcrk=5, r_tol=1e-9, max_iter=60).apply(x, torch.from_numpy(cfg.b).to(ctx.device))
+    assert conv_o and s.converged and s.iterations == it_o
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-10 * np.linalg.norm(xo)
+    res = cfg.b - A(x.cpu().numpy())
+    assert np.linalg.norm(res) < 1e-9 * np.linalg.norm(cfg.b)

@@ -86,3 +86,24 @@ def test_gmres_restarts_monotone_residual():
                                 max_iter=60, record=rec)
     assert conv
     assert all(b_ <= a_ * (1 + 1e-12) for a_, b_ in zip(rec, rec[1:]))
+
+
+def test_oracle_cg_adjoint_matches_finite_differences():
+    """The CPU reverse-mode gradient of L = <w, CG_K(M, D^T p)> (reference
+    kernels when built) against central differences, for p and for M."""
+    from oracle.adjoint import loss_and_grad
+    from lfric_core_amd.configs import precision_config
+    cfg = precision_config(2, 3)
+    st = Stream(2)
+    p = st.uniform(cfg.w3.undf)
+    w = st.normal(cfg.w2.undf)
+    L, gp, gM = loss_and_grad(cfg, p, w, 10)
+    eps = 1e-6
+    for _ in range(4):
+        d = st.uniform(cfg.w3.undf)
+        fd = (loss_and_grad(cfg, p + eps * d, w, 10)[0] - loss_and_grad(cfg, p - eps * d, w, 10)[0]) / (2 * eps)
+        assert abs(fd - gp @ d) <= 1e-7 * abs(gp @ d)
+        dM = st.uniform(cfg.mass.size).reshape(cfg.mass.shape)
+        fd = (loss_and_grad(cfg, p, w, 10, M=cfg.mass + eps * dM)[0]
+              - loss_and_grad(cfg, p, w, 10, M=cfg.mass - eps * dM)[0]) / (2 * eps)
+        assert abs(fd - (gM * dM).sum()) <= 1e-7 * abs((gM * dM).sum())
