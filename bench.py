@@ -208,6 +208,21 @@ def main():
     if not args.no_solvers:
         extra["solvers"] = bench_solvers(ctx, cfg, w2, torch)
 
+    # achievable-bandwidth reference points on this GPU (PyTorch's own kernels)
+    src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    ms_copy = time_events(lambda: dst.copy_(src), 20, torch)
+    ms_sum = time_events(lambda: src.sum(), 20, torch)
+    extra["hbm_reference"] = {"torch_copy_gbs": round(2 * src.numel() * 4 / (ms_copy * 1e-3) / 1e9, 1),
+                              "torch_sum_read_gbs": round(src.numel() * 4 / (ms_sum * 1e-3) / 1e9, 1),
+                              "bytes": src.numel() * 4}
+    log(f"reference: torch copy {extra['hbm_reference']['torch_copy_gbs']} GB/s, "
+        f"sum (read) {extra['hbm_reference']['torch_sum_read_gbs']} GB/s")
+    del src, dst
+    torch.cuda.empty_cache()
+
     if args.copy_calibrate:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)

@@ -142,8 +142,10 @@ int lfr_space_create(lfr_ctx* ctx, const int32_t* map, int ndf, int ncell, int n
                            (int)(q % ndf) + 1, (int)(q / ndf) + 1, v, undf);
     m0[q] = v - 1;
   }
+  static uint64_t next_uid = 0;
   lfr_space* s = new lfr_space();
   s->ctx = ctx;
+  s->uid = ++next_uid;
   s->ndf = ndf;
   s->ncell = ncell;
   s->nlayers = nlayers;

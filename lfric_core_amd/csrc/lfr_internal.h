@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/lfr.h"
@@ -25,6 +26,7 @@ struct lfr_ctx {
 
 struct lfr_space {
   lfr_ctx* ctx = nullptr;
+  uint64_t uid = 0;                 // unique per created space (never reused)
   int ndf = 0, ncell = 0, nlayers = 0, undf = 0;
   int32_t* map0_d = nullptr;        // map(ndf, ncell) converted to 0-based, device
   std::vector<int32_t> map_h;       // 1-based host copy (plan construction)
@@ -40,6 +42,8 @@ struct lfr_plan_class {
   int ncol = 0;
   int32_t* col_start_d = nullptr;   // (ncol) first dof of the column, 0-based
   int32_t* ent_d = nullptr;         // (ncol, ngroup*gsize, 2): cell (0-based), (i << 1) | shift
+  std::vector<int32_t> starts_h;    // host copies, for compiling column records
+  std::vector<int32_t> ent_h;
 };
 
 struct lfr_plan {
@@ -49,7 +53,19 @@ struct lfr_plan {
   int ncolumns = 0;
   int64_t nbytes = 0;
   std::vector<lfr_plan_class> classes;
+  // compiled column records per input space (one device array per class,
+  // nullptr where the class shape has no record kernel)
+  // (keyed by the input space's uid, which is never reused)
+  mutable std::vector<std::pair<uint64_t, std::vector<int32_t*>>> records;
 };
+
+// Column records of `plan` class `ci` for input space `in`, built on first
+// use; nullptr when the shape has no record kernel.  Returns the int stride.
+const int32_t* lfr_plan_records(lfr_ctx* ctx, const lfr_plan* plan, size_t ci, const lfr_space* in,
+                                int* stride);
+bool lfr_record_shape(int ndf_in, int ngroup, int gsize);
+bool lfr_records_enabled();
+bool lfr_gather_nt();
 
 int lfr_set_error(lfr_ctx* ctx, int code, const char* fmt, ...);
 

@@ -16,6 +16,7 @@
 // term is formed as a separate multiply and add (this file is compiled with
 // -ffp-contract=off), so results are bit-identical to the reference loops.
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <numeric>
 #include <tuple>
@@ -30,6 +31,29 @@ using lfr::k_gather;
 using lfr::k_cells;
 
 unsigned pad8(unsigned nb) { return (nb + 7u) & ~7u; }
+
+}  // namespace
+
+// Non-temporal (streaming) loads of the matrix in the record kernels;
+// LFR_GATHER_NT=0 uses ordinary loads.
+bool lfr_gather_nt() {
+  static const bool on = [] {
+    const char* e = getenv("LFR_GATHER_NT");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// LFR_NO_RECORDS=1 switches the record kernels off (plan-entry kernels only).
+bool lfr_records_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("LFR_NO_RECORDS");
+    return !(e && e[0] == '1');
+  }();
+  return on;
+}
+
+namespace {
 
 template <typename T, int NDF_IN, int NG, int GS>
 void launch_gather_t(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
@@ -83,13 +107,54 @@ void launch_gather_any(hipStream_t s, int kind, unsigned nb, const T* M, const T
   }
 }
 
+template <typename T, int NDF_IN, int NG, int GS>
+void launch_rec_t(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
+                  const int32_t* rec, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
+                  int overwrite) {
+  const unsigned nbp = pad8(nb);
+  const bool nt = lfr_gather_nt();
+#define LREC(KD, NTV)                                                                             \
+  hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, KD, false, NTV>), dim3(nbp), dim3(kThreads), \
+                     0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, overwrite, nbp,       \
+                     (const T*)nullptr, (double*)nullptr, (const double*)nullptr)
+  if (kind == 0) {
+    if (nt) LREC(0, true); else LREC(0, false);
+  } else {
+    if (nt) LREC(1, true); else LREC(1, false);
+  }
+#undef LREC
+}
+
+template <typename T, int NDF_IN>
+void launch_rec_shape(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
+                      const int32_t* rec, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
+                      int overwrite) {
+#define SHAPE(NGV, GSV) \
+  if (c.ngroup == NGV && c.gsize == GSV) \
+    return launch_rec_t<T, NDF_IN, NGV, GSV>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite);
+  SHAPE(2, 1) SHAPE(1, 2) SHAPE(1, 1) SHAPE(4, 2) SHAPE(3, 2) SHAPE(2, 2) SHAPE(4, 1) SHAPE(3, 1)
+#undef SHAPE
+}
+
 template <typename T>
 int run_gather(lfr_ctx* ctx, const lfr_plan* plan, int kind, const T* M, const T* x, T* y,
                const lfr_space* in, int ndf_out, int64_t n3, int overwrite) {
   const int nl = plan->space->nlayers;
-  for (const auto& c : plan->classes) {
+  for (size_t ci = 0; ci < plan->classes.size(); ++ci) {
+    const auto& c = plan->classes[ci];
     if (c.ncol == 0) continue;
     const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
+    int stride = 0;
+    const int32_t* rec = lfr_records_enabled() ? lfr_plan_records(ctx, plan, ci, in, &stride) : nullptr;
+    if (rec) {
+      switch (in->ndf) {
+        case 1: launch_rec_shape<T, 1>(ctx->stream, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
+        case 6: launch_rec_shape<T, 6>(ctx->stream, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
+        case 8: launch_rec_shape<T, 8>(ctx->stream, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
+      }
+      LFR_CHECK_LAUNCH(ctx);
+      continue;
+    }
     if (c.gsize != 1 && c.gsize != 2)
       return lfr_set_error(ctx, LFR_ERR_ARG, "plan class with %d entries per cell unsupported",
                            c.gsize);
@@ -267,6 +332,8 @@ extern "C" int lfr_plan_create(lfr_ctx* ctx, const lfr_space* sp, const int32_t*
           if (d[1] < 0) d[0] = ent[(size_t)q * per + (size_t)(gg * pc.gsize) * 2];
         }
     }
+    pc.starts_h = starts;
+    pc.ent_h = ent;
     const size_t b1 = starts.size() * sizeof(int32_t), b2 = ent.size() * sizeof(int32_t);
     if (hipMalloc(&pc.col_start_d, b1) != hipSuccess || hipMalloc(&pc.ent_d, b2) != hipSuccess) {
       lfr_plan_destroy(plan);
@@ -285,8 +352,62 @@ extern "C" int lfr_plan_create(lfr_ctx* ctx, const lfr_space* sp, const int32_t*
   return LFR_OK;
 }
 
+bool lfr_record_shape(int ndf_in, int ng, int gs) {
+  if (ndf_in != 1 && ndf_in != 6 && ndf_in != 8) return false;
+  return (ng == 2 && gs == 1) || (ng == 1 && gs == 2) || (ng == 1 && gs == 1) ||
+         (ng == 4 && gs == 2) || (ng == 3 && gs == 2) || (ng == 2 && gs == 2) ||
+         (ng == 4 && gs == 1) || (ng == 3 && gs == 1);
+}
+
+const int32_t* lfr_plan_records(lfr_ctx* ctx, const lfr_plan* plan, size_t ci, const lfr_space* in,
+                                int* stride) {
+  const lfr_plan_class& c = plan->classes[ci];
+  const int ndf_in = in->ndf;
+  if (!lfr_record_shape(ndf_in, c.ngroup, c.gsize)) return nullptr;
+  const int group = 1 + c.gsize + ndf_in;
+  const int st = (1 + c.ngroup * group + 3) & ~3;
+  *stride = st;
+  for (auto& kv : plan->records)
+    if (kv.first == in->uid) return kv.second[ci];
+  // build every class's records for this input space
+  std::vector<int32_t*> recs(plan->classes.size(), nullptr);
+  const int nl = plan->space->nlayers;
+  for (size_t k = 0; k < plan->classes.size(); ++k) {
+    const lfr_plan_class& pc = plan->classes[k];
+    if (!lfr_record_shape(ndf_in, pc.ngroup, pc.gsize)) continue;
+    const int grp = 1 + pc.gsize + ndf_in;
+    const int str = (1 + pc.ngroup * grp + 3) & ~3;
+    const int per = pc.ngroup * pc.gsize * 2;
+    std::vector<int32_t> rec((size_t)pc.ncol * str, 0);
+    for (int q = 0; q < pc.ncol; ++q) {
+      int32_t* r = &rec[(size_t)q * str];
+      r[0] = pc.starts_h[q];
+      for (int g = 0; g < pc.ngroup; ++g) {
+        const int32_t* e = &pc.ent_h[(size_t)q * per + (size_t)g * pc.gsize * 2];
+        const int cell = e[0];
+        int32_t* rg = r + 1 + g * grp;
+        rg[0] = cell * nl;
+        for (int s = 0; s < pc.gsize; ++s) rg[1 + s] = e[2 * s + 1];
+        for (int j = 0; j < ndf_in; ++j) rg[1 + pc.gsize + j] = in->map_h[(size_t)cell * ndf_in + j] - 1;
+      }
+    }
+    if (hipMalloc(&recs[k], rec.size() * sizeof(int32_t)) != hipSuccess ||
+        hipMemcpy(recs[k], rec.data(), rec.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+      for (int32_t* p : recs)
+        if (p) (void)hipFree(p);
+      lfr_set_error(ctx, LFR_ERR_NOMEM, "column record allocation failed");
+      return nullptr;
+    }
+  }
+  plan->records.emplace_back(in->uid, recs);
+  return recs[ci];
+}
+
 extern "C" int lfr_plan_destroy(lfr_plan* plan) {
   if (!plan) return LFR_OK;
+  for (auto& kv : plan->records)
+    for (int32_t* p : kv.second)
+      if (p) (void)hipFree(p);
   for (auto& c : plan->classes) {
     if (c.col_start_d) (void)hipFree(c.col_start_d);
     if (c.ent_d) (void)hipFree(c.ent_d);

@@ -152,6 +152,108 @@ k_gather(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
 }
 
 // ---------------------------------------------------------------------------
+// Gather with compiled column records.  For one (output space, input space)
+// pair the plan is flattened into one record per column:
+//   [ g0 | per group: c*nl, code[GS], map_in(c, 0..NDF_IN-1) ] padded to x4
+// so a thread fetches its whole column description with STRIDE/4 aligned
+// 16-byte loads and issues every matrix and operand load one memory latency
+// later (no dependent dofmap lookup).  Summation order is identical to
+// k_gather.  The matrix is streamed exactly once, so it is read with
+// non-temporal loads that do not displace the gathered operand from cache.
+// ---------------------------------------------------------------------------
+template <int NDF_IN, int NG, int GS>
+struct RecShape {
+  static constexpr int kGroup = 1 + GS + NDF_IN;
+  static constexpr int kInts = 1 + NG * kGroup;
+  static constexpr int kStride = (kInts + 3) & ~3;
+};
+
+template <typename T>
+__device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
+
+template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT = false, bool NT = true>
+__global__ void __launch_bounds__(kThreads)
+k_gather_rec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
+             const int32_t* __restrict__ rec, int ncol, int H, int nl, int64_t n3, int ndf_out,
+             int overwrite, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
+             double* __restrict__ part = nullptr, const double* __restrict__ stop = nullptr) {
+  using S = RecShape<NDF_IN, NG, GS>;
+  if (DOT && stop && stop[0] != 0.0) return;
+  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
+  const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
+  const bool valid = t < (int64_t)ncol * H;
+  const int64_t tt = valid ? t : 0;
+  const int col = (int)(tt / H);
+  const int L = (int)(tt - (int64_t)col * H);
+  int r[S::kStride];
+  const int4* r4 = reinterpret_cast<const int4*>(rec + (int64_t)col * S::kStride);
+#pragma unroll
+  for (int q = 0; q < S::kStride / 4; ++q) {
+    const int4 v = r4[q];
+    r[4 * q + 0] = v.x;
+    r[4 * q + 1] = v.y;
+    r[4 * q + 2] = v.z;
+    r[4 * q + 3] = v.w;
+  }
+  const int32_t g0 = r[0];
+  T acc = (overwrite || !valid) ? T(0) : y[g0 + L];
+  T mv[NG][GS][NDF_IN], xv[NG][GS][NDF_IN];
+  bool ok[NG][GS];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const int* rg = r + 1 + g * S::kGroup;
+    const int64_t mbase = (int64_t)rg[0];
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+      const int code = rg[1 + q];
+      const int k = L - (code & 1);
+      ok[g][q] = code >= 0 && k >= 0 && k < nl;
+      const int ii = code >= 0 ? (code >> 1) : 0;
+      const int kk = k < 0 ? 0 : (k >= nl ? nl - 1 : k);
+#pragma unroll
+      for (int j = 0; j < NDF_IN; ++j) {
+        const int64_t mo = KIND == 0 ? (int64_t)ii + (int64_t)ndf_out * j
+                                     : (int64_t)j + (int64_t)NDF_IN * ii;
+        mv[g][q][j] = NT ? ldnt(M + mbase + kk + n3 * mo) : M[mbase + kk + n3 * mo];
+        xv[g][q][j] = x[rg[1 + GS + j] + kk];
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    if (KIND == 0) {
+#pragma unroll
+      for (int j = 0; j < NDF_IN; ++j)
+#pragma unroll
+        for (int q = 0; q < GS; ++q) {
+          const T p = mv[g][q][j] * xv[g][q][j];
+          const T s = acc + p;
+          acc = ok[g][q] ? s : acc;
+        }
+    } else {
+#pragma unroll
+      for (int q = 0; q < GS; ++q) {
+        T dot = T(0);
+#pragma unroll
+        for (int j = 0; j < NDF_IN; ++j) {
+          const T p = mv[g][q][j] * xv[g][q][j];
+          dot = dot + p;
+        }
+        const T s = acc + dot;
+        acc = ok[g][q] ? s : acc;
+      }
+    }
+  }
+  if (valid) y[g0 + L] = acc;
+  if (DOT) {
+    __shared__ double lds[kThreads / 64];
+    double v = valid ? (double)acc * (double)dotv[g0 + L] : 0.0;
+    v = block_sum(v, lds);
+    if (threadIdx.x == 0) part[lb] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Cell-centric apply for a discontinuous output space: thread <-> (cell, k).
 // Each output dof has exactly one (cell, df) contributor, so per output the
 // sum over df2 in ascending order is the reference order

@@ -269,6 +269,45 @@ void launch_apply_t(hipStream_t s, const lfr_plan_class& c, const T* M, const T*
                        (const double*)nullptr);
 }
 
+template <typename T, int NDF_IN, int NG, int GS>
+void launch_apply_rec_t(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M,
+                        const T* x, T* y, int nl, int64_t n3, int ndf_out, const T* dotv,
+                        double* part, const double* stop) {
+  const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
+  const unsigned nbp = (nb + 7u) & ~7u;
+  const bool nt = lfr_gather_nt();
+  if (dotv) {
+    if (nt)
+      hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, 0, true, true>), dim3(nbp), dim3(kThreads),
+                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, nbp, dotv, part, stop);
+    else
+      hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, 0, true, false>), dim3(nbp), dim3(kThreads),
+                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, nbp, dotv, part, stop);
+  } else {
+    if (nt)
+      hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, 0, false, true>), dim3(nbp), dim3(kThreads),
+                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, nbp,
+                         (const T*)nullptr, (double*)nullptr, (const double*)nullptr);
+    else
+      hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, 0, false, false>), dim3(nbp), dim3(kThreads),
+                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, nbp,
+                         (const T*)nullptr, (double*)nullptr, (const double*)nullptr);
+  }
+}
+
+template <typename T, int NDF_IN>
+bool launch_apply_rec(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M,
+                      const T* x, T* y, int nl, int64_t n3, int ndf, const T* dotv, double* part,
+                      const double* stop) {
+#define SHAPE(NGV, GSV) \
+  if (c.ngroup == NGV && c.gsize == GSV) { \
+    launch_apply_rec_t<T, NDF_IN, NGV, GSV>(s, c, rec, M, x, y, nl, n3, ndf, dotv, part, stop); \
+    return true; }
+  SHAPE(2, 1) SHAPE(1, 2) SHAPE(1, 1) SHAPE(4, 2) SHAPE(3, 2) SHAPE(2, 2) SHAPE(4, 1) SHAPE(3, 1)
+#undef SHAPE
+  return false;
+}
+
 template <typename T, int NDF_IN>
 void launch_apply_shape(hipStream_t s, const lfr_plan_class& c, const T* M, const T* x, T* y,
                         const int32_t* mi, int nl, int64_t n3, int ndf, const T* dotv,
@@ -292,11 +331,24 @@ int apply_op(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv,
   const int nl = sp->nlayers;
   const int64_t n3 = (int64_t)sp->ncell * nl;
   int off = 0;
-  for (const auto& c : A->plan->classes) {
+  for (size_t ci = 0; ci < A->plan->classes.size(); ++ci) {
+    const auto& c = A->plan->classes[ci];
     const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
     const unsigned nbp = (nb + 7u) & ~7u;
     double* pp = part ? part + off : nullptr;
     const T* M = (const T*)A->matrix_d;
+    int stride = 0;
+    const int32_t* rec = lfr_records_enabled() ? lfr_plan_records(ctx, A->plan, ci, sp, &stride) : nullptr;
+    bool done = false;
+    if (rec) {
+      if (sp->ndf == 6) done = launch_apply_rec<T, 6>(ctx->stream, c, rec, M, x, y, nl, n3, 6, dotv, pp, stop);
+      else if (sp->ndf == 8) done = launch_apply_rec<T, 8>(ctx->stream, c, rec, M, x, y, nl, n3, 8, dotv, pp, stop);
+    }
+    if (done) {
+      LFR_CHECK_LAUNCH(ctx);
+      off += (int)nbp;
+      continue;
+    }
 switch (sp->ndf) {
       case 6: launch_apply_shape<T, 6>(ctx->stream, c, M, x, y, sp->map0_d, nl, n3, sp->ndf, dotv, pp, stop); break;
       case 8: launch_apply_shape<T, 8>(ctx->stream, c, M, x, y, sp->map0_d, nl, n3, sp->ndf, dotv, pp, stop); break;
@@ -317,6 +369,14 @@ int check_op(lfr_ctx* ctx, const lfr_lma_op* A, const char* who) {
   LFR_REQUIRE(ctx, A->prec == 4 || A->prec == 8, "%s: prec %d", who, A->prec);
   LFR_REQUIRE(ctx, A->plan->space == A->space && A->plan->kind == 0,
               "%s: operator plan must be a matrix_vector plan of its space", who);
+  // build the column records now: never inside a captured graph
+  if (lfr_records_enabled()) {
+    int stride = 0;
+    ctx->err.clear();
+    for (size_t ci = 0; ci < A->plan->classes.size(); ++ci)
+      (void)lfr_plan_records(ctx, A->plan, ci, A->space, &stride);
+    if (!ctx->err.empty()) return LFR_ERR_NOMEM;
+  }
   return LFR_OK;
 }
 

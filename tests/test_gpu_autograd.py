@@ -32,12 +32,13 @@ def _ops(ctx, cfg, w2, w3, dt):
     dev = ctx.device
     M = torch.from_numpy(cfg.mass.astype(dt)).to(dev)
     D = torch.from_numpy(cfg.div.astype(dt)).to(dev)
-    return (api.LMAOperator(M, w2, w2, cfg.colouring), api.LMAOperator(D, w2, w3, cfg.colouring))
+    # div maps W2 -> W3: local_stencil(ncell_3d, 1, 6), to = W3, from = W2
+    return (api.LMAOperator(M, w2, w2, cfg.colouring), api.LMAOperator(D, w3, w2, cfg.colouring))
 
 
 def _loss(M, D, p, w, mop, dop, iters=ITERS):
     from lfric_core_amd import autograd as ag
-    b = ag.lma_apply_transpose(D, p, dop)           # W2 <- W3 (D^T p)
+    b = ag.lma_apply_transpose(D, p, dop)           # D^T p: W3 -> W2
     u = ag.cg_solve(M, b, mop, iters)
     return torch.dot(w, u)
 
@@ -49,6 +50,7 @@ def test_dot_product_tests(ctx, setup):
     st = Stream(40)
     dev = ctx.device
     for op, nin, nout in ((mop, cfg.w2.undf, cfg.w2.undf), (dop, cfg.w2.undf, cfg.w3.undf)):
+        # <A x, y> on the to-space, <x, A^T y> on the from-space
         x = torch.from_numpy(st.uniform(nin)).to(dev)
         y = torch.from_numpy(st.uniform(nout)).to(dev)
         Ax = ag._apply(op.matrix, x, op)
