@@ -268,6 +268,40 @@ int lfr_columnwise_op_appinv(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_
                              const lfr_space* space, const int32_t* indirection_d,
                              int fp64_pivots);
 
+/* ---------------------------------------------------------------------- */
+/* Multigrid with vertical line relaxation (W3, lowest order)              */
+/* ---------------------------------------------------------------------- */
+/* V-cycle for A = T + sigma_l * L_h on a hierarchy of nested cubed spheres:
+ * T is a column tridiagonal (the CMA operator of columnwise_op_appinv), L_h
+ * the 4-neighbour horizontal graph Laplacian.  Smoother: damped vertical line
+ * Jacobi (one Thomas solve per column on the residual), restriction by cell
+ * average (components/science/source/kernel/inter_mesh/
+ * sci_restrict_scalar_unweighted_kernel_mod.F90), prolongation by injection
+ * (.../sci_prolong_scalar_unweighted_kernel_mod.F90).  The whole V-cycle over
+ * nrhs right-hand sides is captured once into a hipGraph and replayed; the
+ * smoother's workgroup size is autotuned per level at creation.  No single
+ * counterpart in the reference (LFRic's multigrid preconditioner lives in
+ * lfric_apps); the components are the reference kernels named above.
+ *   ncell[l]       cells of level l (0 = finest)
+ *   cell_next_h[l] (ncell[l], 4) 0-based W,S,E,N neighbours
+ *   children_h[l]  (ncell[l], 4) 0-based cells of level l-1 under each cell,
+ *                  y outer, x inner (ignored for l = 0)
+ *   bands_h[l]     [3][nlayers][ncell[l]] sub, diag, super of T
+ *   sigma[l]       horizontal coupling of level l
+ * nu1/nu2/ncoarse must be even. */
+typedef struct lfr_mg lfr_mg;
+int lfr_mg_create(lfr_ctx* ctx, int prec, int nlev, int nlayers, int nrhs, const int* ncell,
+                  const int32_t* const* cell_next_h, const int32_t* const* children_h,
+                  const double* const* bands_h, const double* sigma, double omega, int nu1,
+                  int nu2, int ncoarse, lfr_mg** mg);
+int lfr_mg_destroy(lfr_mg* mg);
+int lfr_mg_block_sizes(const lfr_mg* mg, int* smooth_blocks);
+/* x_d, b_d: nrhs fine-level W3 fields (LFRic layout, dof = cell*nl + k),
+ * stored back to back.  Runs ncycles V-cycles starting from x; if res_h is
+ * not NULL it receives ||b - A x|| per right-hand side before the first cycle
+ * (res_h[0..nrhs)) and after every cycle (res_h[(c+1)*nrhs + r]). */
+int lfr_mg_solve(lfr_mg* mg, void* x_d, const void* b_d, int ncycles, double* res_h);
+
 #ifdef __cplusplus
 }
 #endif

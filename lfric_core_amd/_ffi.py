@@ -65,6 +65,10 @@ SIGNATURES = {
     # solver entry points take structure pointers; lfric_core_amd.solvers
     # installs the typed signatures before calling them
     "lfr_columnwise_op_appinv": [_P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I],
+    "lfr_mg_create": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _D, _I, _I, _I, C.POINTER(_P)],
+    "lfr_mg_destroy": [_P],
+    "lfr_mg_block_sizes": [_P, _P],
+    "lfr_mg_solve": [_P, _P, _P, _I, _P],
     "lfr_cg": None,
     "lfr_refined_cg": None,
     "lfr_gmres": None,

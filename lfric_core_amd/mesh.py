@@ -196,3 +196,47 @@ def extrusion_eta(nlayers: int, kind: str = "uniform") -> np.ndarray:
     if kind == "quadratic":
         return k ** 2
     raise ValueError(kind)
+
+
+def intergrid_map(coarse_n: int, fine_n: int) -> np.ndarray:
+    """Fine cells under each coarse cell of two nested cubed spheres.
+
+    Returns map[coarse_cell-1, y, x] = fine cell id (1-based), the layout of
+    the reference's mesh-mesh connectivity (cell_map(x, y, cell) in Fortran
+    order), following calc_global_cell_map
+    (mesh_tools/source/support/calc_global_cell_map_mod.F90:134-258): on each
+    panel the fine cells of a coarse cell are the ef x ef block below it in
+    panel row-major numbering, turned for rotated panels (PANEL_ROTATIONS) so
+    that (x, y) follow the coarse cell's own W->E, N->S directions.
+    """
+    if fine_n % coarse_n:
+        raise ValueError("fine mesh must refine the coarse mesh by an integer factor")
+    ef = fine_n // coarse_n
+    ccpp, fcpp = coarse_n * coarse_n, fine_n * fine_n
+    out = np.empty((6 * ccpp, ef, ef), dtype=np.int32)
+    ii, jj = np.meshgrid(np.arange(ef), np.arange(ef), indexing="xy")   # [y, x] grids
+    for p in range(6):
+        rot = PANEL_ROTATIONS[p]
+        for j in range(coarse_n):          # coarse row (y)
+            for i in range(coarse_n):      # coarse column (x)
+                # tmp(a, b) = fine cell at column i*ef + a, row j*ef + b (0-based)
+                def tmp(a, b):
+                    return p * fcpp + (j * ef + b) * fine_n + (i * ef + a) + 1
+                c = p * ccpp + j * coarse_n + i
+                if rot == 0:
+                    out[c] = tmp(ii, jj)
+                elif rot == 1:
+                    # map(ifine, jfine) = tmp(jfine, ef+1-ifine)
+                    out[c] = tmp(jj, ef - 1 - ii)
+                else:
+                    # map(ifine, jfine) = tmp(ef+1-jfine, ifine)
+                    out[c] = tmp(ef - 1 - jj, ii)
+    return out
+
+
+def coarse_of_fine(coarse_n: int, fine_n: int) -> np.ndarray:
+    """parent[fine_cell-1] = coarse cell id (1-based)."""
+    m = intergrid_map(coarse_n, fine_n)
+    parent = np.empty(6 * fine_n * fine_n, dtype=np.int32)
+    parent[m.reshape(m.shape[0], -1) - 1] = np.arange(1, m.shape[0] + 1)[:, None]
+    return parent

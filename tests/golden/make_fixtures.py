@@ -12,7 +12,8 @@ Writes, next to this script:
   3x3 biperiodic unit-test mesh, 3 layers).
 * ``meshes.npz`` - face-face and face-node tables of the reference's
   cubed-sphere mesh files (UGRID, NetCDF-3), read with scipy's NetCDF reader,
-  plus node and face coordinates of C12.
+  node and face coordinates of C12, and the intergrid cell maps of the
+  multigrid mesh file (C32/C16/C8/C4, C12/C4).
 
 Only data is written; no reference source is copied.
 """
@@ -91,11 +92,24 @@ def read_meshes(root: Path) -> dict:
     return out
 
 
+MG_FILE = "infrastructure/unit-test/data/mesh_C32_MG.nc"
+
+
+def read_mg_maps(root: Path) -> dict:
+    """Intergrid cell maps of the reference's multigrid mesh file, stored as
+    cell_map(x, y, source cell) (Fortran order) -> numpy [cell, y, x]."""
+    from scipy.io import netcdf_file
+    f = netcdf_file(str(root / MG_FILE), "r", mmap=False)
+    out = {k: f.variables[k].data.astype(np.int32).copy() for k in f.variables if k.endswith("_map")}
+    f.close()
+    return out
+
+
 def main(root: str = "/root/reference") -> None:
     r = Path(root)
     fs = parse_fs_test_data(r / "infrastructure/unit-test/support/fs_test_data_mod.F90")
     np.savez_compressed(HERE / "fs_test_data.npz", **fs)
-    np.savez_compressed(HERE / "meshes.npz", **read_meshes(r))
+    np.savez_compressed(HERE / "meshes.npz", **read_meshes(r), **read_mg_maps(r))
     print("wrote", sorted(fs)[:5], "...", len(fs), "arrays")
 
 

@@ -8,7 +8,7 @@ first-encounter renumbering (infrastructure/source/mesh/local_mesh_mod.f90:348-3
 import numpy as np
 import pytest
 
-from lfric_core_amd.mesh import (canonical_vertex_ids, cubed_sphere, extrusion_eta,
+from lfric_core_amd.mesh import (E, S, canonical_vertex_ids, cubed_sphere, extrusion_eta,
                                  unit_test_mesh)
 
 from conftest import GOLDEN
@@ -55,3 +55,23 @@ def test_unit_test_mesh_is_biperiodic_3x3():
 def test_extrusion():
     np.testing.assert_allclose(extrusion_eta(4), [0, .25, .5, .75, 1])
     np.testing.assert_allclose(extrusion_eta(4, "quadratic"), [0, .0625, .25, .5625, 1])
+
+
+@pytest.mark.parametrize("coarse,fine", [(16, 32), (8, 16), (4, 8), (4, 12)])
+def test_intergrid_map_matches_reference_multigrid_file(coarse, fine):
+    from lfric_core_amd.mesh import coarse_of_fine, intergrid_map
+    g = np.load(GOLDEN / "meshes.npz")
+    np.testing.assert_array_equal(intergrid_map(coarse, fine), g[f"C{coarse}_C{fine}_map"])
+    np.testing.assert_array_equal(coarse_of_fine(coarse, fine), g[f"C{fine}_C{coarse}_map"].ravel())
+
+
+def test_intergrid_map_children_are_nested():
+    """Children of a coarse cell form a connected 2x2 block: x-neighbours are east of each other."""
+    from lfric_core_amd.mesh import intergrid_map
+    fine = cubed_sphere(16)
+    m = intergrid_map(8, 16)
+    for c in range(m.shape[0]):
+        for y in range(2):
+            assert fine.cell_next[m[c, y, 0] - 1, E] == m[c, y, 1]
+        for x in range(2):
+            assert fine.cell_next[m[c, 0, x] - 1, S] == m[c, 1, x]

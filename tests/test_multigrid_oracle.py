@@ -1,0 +1,82 @@
+"""The multigrid hierarchy and the numpy V-cycle restatement, on the CPU."""
+import numpy as np
+import pytest
+
+from lfric_core_amd.multigrid import build_hierarchy
+from lfric_core_amd.rng import Stream
+from oracle import multigrid as omg
+
+
+def _dense(lv, nl):
+    n = lv.ncell * nl
+    A = np.zeros((n, n))
+    idx = lambda c, k: k * lv.ncell + c          # layer-major single rhs
+    a, d, u = lv.bands
+    for c in range(lv.ncell):
+        for k in range(nl):
+            i = idx(c, k)
+            A[i, i] += d[k, c] + 4 * lv.sigma
+            if k > 0:
+                A[i, idx(c, k - 1)] += a[k, c]
+            if k < nl - 1:
+                A[i, idx(c, k + 1)] += u[k, c]
+            for s in range(4):
+                A[i, idx(lv.cell_next[c, s], k)] -= lv.sigma
+    return A
+
+
+def test_galerkin_coarse_operator():
+    """R A_f P = A_c for cell-average R and injection P."""
+    H = build_hierarchy((8, 4), nlayers=5)
+    f, c = H.levels
+    nl = H.nlayers
+    Af, Ac = _dense(f, nl), _dense(c, nl)
+    nf, nc = f.ncell * nl, c.ncell * nl
+    P = np.zeros((nf, nc))
+    R = np.zeros((nc, nf))
+    for C in range(c.ncell):
+        for q in range(4):
+            ff = c.children[C, q]
+            for k in range(nl):
+                P[k * f.ncell + ff, k * c.ncell + C] = 1.0
+                R[k * c.ncell + C, k * f.ncell + ff] = 0.25
+    np.testing.assert_allclose(R @ Af @ P, Ac, atol=1e-13)
+
+
+def test_apply_and_smoother_against_dense():
+    H = build_hierarchy((4,), nlayers=6)
+    lv = H.levels[0]
+    nl = H.nlayers
+    A = _dense(lv, nl)
+    st = Stream(7)
+    x = st.uniform(lv.ncell * nl)
+    b = st.uniform(lv.ncell * nl)
+    X = x.reshape(1, nl, lv.ncell)
+    B = b.reshape(1, nl, lv.ncell)
+    np.testing.assert_allclose(omg.apply_A(X, lv.bands, lv.cell_next, lv.sigma).ravel(), A @ x,
+                               atol=1e-13)
+    # smoother with omega = 1: x + D^{-1}(b - A x), D the column-tridiagonal part
+    D = A.copy()
+    for i in range(D.shape[0]):
+        for j in range(D.shape[1]):
+            if (i % lv.ncell) != (j % lv.ncell):
+                D[i, j] = 0.0
+    want = x + np.linalg.solve(D, b - A @ x)
+    got = omg.smooth(X, B, lv.bands, lv.cell_next, lv.sigma, 1.0).ravel()
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_vcycle_converges_to_dense_solution():
+    H = build_hierarchy((8, 4, 2), nlayers=6)
+    lv = H.levels[0]
+    nl = H.nlayers
+    st = Stream(8)
+    b = st.uniform(2 * lv.ncell * nl).reshape(2, -1)
+    x, res = omg.solve(H, np.zeros_like(b), b, 12)
+    factors = res[1:, 0] / res[:-1, 0]
+    assert factors.max() < 0.5
+    A = _dense(lv, nl)
+    xl = omg.to_layer_major(x, lv.ncell, nl)
+    bl = omg.to_layer_major(b, lv.ncell, nl)
+    want = np.linalg.solve(A, bl[0].ravel())
+    assert np.linalg.norm(xl[0].ravel() - want) <= 1e-6 * np.linalg.norm(want)
