@@ -82,3 +82,41 @@ def precision_config(edge_cells: int = 96, nlayers: int = 70, seed: int = 1,
     b = st.uniform(w2.undf)
     return PrecisionConfig(edge_cells, nlayers, mesh, w2, w3, col,
                            80.0e3 * extrusion_eta(nlayers, "quadratic"), mass, div, grad, b)
+
+
+@dataclass
+class DiffusionConfig:
+    """Assembled W0 operator on a cubed-sphere shell (the C12L10 plain-method case)."""
+
+    edge_cells: int
+    nlayers: int
+    mesh: object
+    w0: fsm.FunctionSpace
+    colouring: fsm.Colouring
+    heights: np.ndarray
+    matrix: np.ndarray    # (8, 8, n3) float64: mass + dt_kappa * stiffness
+    mass: np.ndarray
+    stiffness: np.ndarray
+    x_true: np.ndarray    # (undf_w0,) float64
+
+
+def diffusion_config(node_lon_deg, node_lat_deg, face_nodes, nlayers: int = 10,
+                     depth: float = 10.0e3, radius: float = 6.371e6, dt_kappa: float = 1.0e-3,
+                     seed: int = 0, n_q: int = 3) -> DiffusionConfig:
+    """W0 lowest order (8 dofs/cell) on the shell given by a cubed-sphere mesh file's
+    node longitudes/latitudes (degrees) and face-node table (1-based), ``nlayers``
+    uniform layers over ``depth`` metres; local matrices mass + dt_kappa * stiffness
+    by ``n_q``-point Gaussian quadrature; x_true ~ U[-1, 1] from seed ``seed``."""
+    from .assembly import shell_corners, to_local_stencil, w0_mass_stiffness
+    ncell = face_nodes.shape[0]
+    edge = int(round(np.sqrt(ncell / 6)))
+    mesh = cubed_sphere(edge)
+    w0 = fsm.build_dofmap(mesh, "W0", nlayers)
+    col = fsm.colour_cells(mesh)
+    heights = depth * extrusion_eta(nlayers, "uniform")
+    chi = shell_corners(node_lon_deg, node_lat_deg, face_nodes, radius, heights)
+    mass, stiff = w0_mass_stiffness(chi, n_q)
+    A = mass + dt_kappa * stiff
+    x_true = Stream(seed).uniform(w0.undf)
+    return DiffusionConfig(edge, nlayers, mesh, w0, col, heights, to_local_stencil(A),
+                           to_local_stencil(mass), to_local_stencil(stiff), x_true)
