@@ -302,6 +302,34 @@ int lfr_mg_block_sizes(const lfr_mg* mg, int* smooth_blocks);
  * (res_h[0..nrhs)) and after every cycle (res_h[(c+1)*nrhs + r]). */
 int lfr_mg_solve(lfr_mg* mg, void* x_d, const void* b_d, int ncycles, double* res_h);
 
+/* ---- multi-right-hand-side LMA apply on MFMA -----------------------------
+ * 16 fields applied together: y_r (+)= A x_r.  Fields are multidata fields in
+ * LFRic's "ndata first" layout with ndata = 16 (element (dof d, rhs r) at
+ * d*16 + r, d the single-field dof index), FP32, 16-byte aligned.  The
+ * operator is the LMA local_stencil(ncell_3d, ndf_to, ndf_from) of
+ * infrastructure/source/operator/operator_mod.t90 (FP32 or FP64, prec_in 4/8),
+ * repacked at creation into the handle's own layout: mprec 4 keeps FP32 and
+ * runs FP32 MFMA; mprec 2 stores BF16 and rounds the inputs to BF16, with
+ * FP32 accumulation.  The colouring (same arrays as lfr_plan_create) orders
+ * the writes of a continuous output space; it may be NULL for a
+ * discontinuous one.  Semantics per field are those of matrix_vector_code
+ * (components/science/source/kernel/algebra/matrix_vector_kernel_mod.F90:68-140,
+ * overwrite = 0) or setval_c(y, 0) followed by it (overwrite = 1); the
+ * summation order is the MFMA's, so results are not bitwise those of the
+ * single-field kernels.  No counterpart in the reference (which has no
+ * multi-field LMA apply).  Shapes with kernels: W2 k=1 (36x36), W0 k=1
+ * (27x27), W3 k=1 (8x8), W3<-W2 k=1 (8x36), W0 k=0 (8x8), W2 k=0 (6x6). */
+typedef struct lfr_multi lfr_multi;
+int lfr_multi_create(lfr_ctx* ctx, int mprec, int prec_in, const void* matrix_d,
+                     const lfr_space* to_space, const lfr_space* from_space,
+                     const int32_t* cells_in_colour, const int32_t* ncells_per_colour,
+                     int ncolour, int maxcells, lfr_multi** out);
+int lfr_multi_destroy(lfr_multi* h);
+int lfr_multi_info(const lfr_multi* h, int64_t* packed_bytes, int* slots, int* vertical_pairs,
+                   int* launches);
+int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void* x_d, int nrhs,
+                            int overwrite);
+
 #ifdef __cplusplus
 }
 #endif

@@ -308,6 +308,64 @@ def dot(ctx, x, y) -> float:
 
 # --- column-wise operators --------------------------------------------------
 
+class MultiOperator:
+    """An LMA operator prepared for 16-field applies on MFMA (lfr_multi_create).
+
+    mprec 4: FP32 matrix and FP32 MFMA; mprec 2: BF16 matrix and inputs, FP32
+    accumulation.  Fields are (undf, 16) FP32 tensors (the 16 values of a dof
+    contiguous)."""
+
+    NRHS = 16
+
+    def __init__(self, op: LMAOperator, mprec: int = 4):
+        self.op, self.mprec = op, mprec
+        ctx = op.ctx
+        col = op.colouring
+        if col is not None:
+            cic = np.asfortranarray(col.cells_in_colour.astype(np.int32))
+            npc = np.ascontiguousarray(col.ncells_per_colour, dtype=np.int32)
+            args = (cic.ctypes.data_as(_P), npc.ctypes.data_as(_P), col.ncolours, cic.shape[1])
+            self._keep = (cic, npc)
+        else:
+            args = (None, None, 0, 0)
+        h = _P()
+        check(lib().lfr_multi_create(ctx.h, mprec, _prec(op.matrix), _P(op.matrix.data_ptr()),
+                                     op.to_space.h, op.from_space.h, *args, C.byref(h)), ctx.h)
+        self.h = h
+
+    def info(self):
+        nb, s, p, nlch = C.c_int64(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().lfr_multi_info(self.h, C.byref(nb), C.byref(s), C.byref(p), C.byref(nlch)))
+        return {"packed_bytes": nb.value, "slots": s.value, "vertical_pairs": p.value,
+                "launches": nlch.value}
+
+    def _check(self, t: torch.Tensor, space: Space, name: str):
+        shape = (space.undf, self.NRHS)
+        if tuple(t.shape) != shape or t.dtype != torch.float32 or not t.is_contiguous() \
+                or t.device != space.ctx.device:
+            raise LfrError(1, f"{name} must be a contiguous float32 tensor of shape {shape}")
+
+    def apply(self, lhs: torch.Tensor, x: torch.Tensor, overwrite: bool = True):
+        """lhs (+)= A x for the 16 fields."""
+        self._check(lhs, self.op.to_space, "lhs")
+        self._check(x, self.op.from_space, "x")
+        ctx = self.op.ctx
+        ctx.sync_stream()
+        check(lib().lfr_matrix_vector_multi(ctx.h, self.h, _P(lhs.data_ptr()), _P(x.data_ptr()),
+                                            self.NRHS, int(overwrite)), ctx.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lfr_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class CMAOperator:
     """columnwise_operator_type on one space: columnwise_matrix(bw, nrow, ncell_2d)
     as a tensor of shape (ncell_2d, nrow, bw) plus indirection_dofmap(nrow)."""

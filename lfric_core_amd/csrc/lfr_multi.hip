@@ -1,0 +1,455 @@
+// Multi-right-hand-side LMA apply on the matrix cores (MFMA).
+//
+//   y_r = A x_r,  r = 0..15, for 16 fields held as one multidata field with
+//   the 16 values of a dof contiguous (LFRic's "ndata first" layout:
+//   element (dof d, rhs r) at d*16 + r, d the single-field dof index).
+//
+// Per cell-layer e this is the small GEMM  Y_e (ndf_to x 16) = M_e X_e with
+// M_e (ndf_to x ndf_from) the local matrix of the LMA operator
+// (local_stencil(ncell_3d, ndf_to, ndf_from),
+// infrastructure/source/operator/operator_mod.t90:31-122) and X_e the
+// gathered input dofs.  It runs as C^T = X^T M^T on
+// v_mfma_f32_4x4x1_16b_f32 (FP32) or v_mfma_f32_4x4x4_16b_bf16 (BF16 inputs,
+// FP32 accumulation): the 16 blocks of one instruction are 16 consecutive
+// layers of one column, each block a 4 (rhs) x 4 (rows) tile, so a wave
+// walks one column 16 layers at a time with no padding of the GEMM shape
+// beyond ndf_to rounded up to a multiple of 4.
+//
+// Matrix: repacked once (at handle creation) into
+//   Mp[cell][j/4][slot][k][j%4]   (slot = permuted output row, k = layer)
+// so that the 16 layer-lanes of one row read 16 contiguous 16-byte (FP32)
+// or 8-byte (BF16) pieces: 256 B / 128 B per row per instruction.
+//
+// Continuous output (GH_INC): cells are processed colour by colour, one
+// launch per colour.  Inside a column the top-face/top-vertex rows of layer
+// k and the bottom rows of layer k+1 hit the same dof; the permuted slot
+// order puts a bottom row and its top partner at slots p and p + 4*PIG, so
+// they sit in the same lane position of two row tiles and are combined in
+// registers (lane shift by one layer).  Across columns a dof is written by
+// its first toucher in colour order (overwrite form) and incremented by the
+// later ones, so results are deterministic.
+#include <cstring>
+#include <vector>
+
+#include "lfr_internal.h"
+
+typedef float lfr_v4f __attribute__((ext_vector_type(4)));
+typedef short lfr_v4s __attribute__((ext_vector_type(4)));
+
+struct lfr_multi {
+  lfr_ctx* ctx = nullptr;
+  int mprec = 4;                  // 4: FP32 matrix/MFMA, 2: BF16 matrix and inputs
+  int nt = 0, nf = 0, ntp = 0, jq = 0, pig = 0, npair = 0;
+  int nl = 0, ncell = 0;
+  const lfr_space* to = nullptr;
+  const lfr_space* from = nullptr;
+  void* mp_d = nullptr;           // packed matrix
+  int32_t* slot_d = nullptr;      // (ntp) row df | kind << 16  (kind 0 pad, 1 plain, 2 bottom, 3 top)
+  uint64_t* first_d = nullptr;    // (ncell) bit p: this cell first touches slot p's dof column
+  int32_t* cells_d = nullptr;     // cells in launch order, 0-based
+  std::vector<int> launch_off;    // launches: [launch_off[i], launch_off[i+1])
+  int64_t mp_bytes = 0;
+};
+
+namespace {
+
+constexpr int kRhs = 16;
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
+  return (unsigned short)(u >> 16);
+}
+
+// Mp[((c*JQ + q)*NTP + p)*nl + k][jr] = M[c*nl + k, row(p), 4q + jr]
+template <typename TI, typename TO>
+__global__ void k_multi_pack(TO* __restrict__ mp, const TI* __restrict__ m, const int32_t* __restrict__ slot,
+                             int ncell, int nl, int nt, int nf, int ntp, int jqn) {
+  const int64_t n3 = (int64_t)ncell * nl;
+  const int64_t total = (int64_t)ncell * jqn * ntp * nl * 4;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t t = idx;
+    const int jr = (int)(t & 3);
+    t >>= 2;
+    const int k = (int)(t % nl);
+    t /= nl;
+    const int p = (int)(t % ntp);
+    t /= ntp;
+    const int q = (int)(t % jqn);
+    const int64_t c = t / jqn;
+    const int j = 4 * q + jr;
+    const int row = slot[p] & 0xffff;
+    const int kind = slot[p] >> 16;
+    float v = 0.f;
+    if (kind != 0 && j < nf) v = (float)m[(c * nl + k) + n3 * ((int64_t)row + (int64_t)nt * j)];
+    if constexpr (sizeof(TO) == 2)
+      mp[idx] = f2bf(v);
+    else
+      mp[idx] = v;
+  }
+}
+
+template <bool BF>
+__device__ __forceinline__ lfr_v4f mfma_step(const lfr_v4f (&xa)[4], int v, const void* mv, lfr_v4f acc);
+
+template <>
+__device__ __forceinline__ lfr_v4f mfma_step<false>(const lfr_v4f (&xa)[4], int v, const void* mv, lfr_v4f acc) {
+  const lfr_v4f m = *(const lfr_v4f*)mv;
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[0][v], m[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[1][v], m[1], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[2][v], m[2], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[3][v], m[3], acc, 0, 0, 0);
+  return acc;
+}
+
+template <>
+__device__ __forceinline__ lfr_v4f mfma_step<true>(const lfr_v4f (&xa)[4], int v, const void* mv, lfr_v4f acc) {
+  lfr_v4s a;
+  a[0] = (short)f2bf(xa[0][v]);
+  a[1] = (short)f2bf(xa[1][v]);
+  a[2] = (short)f2bf(xa[2][v]);
+  a[3] = (short)f2bf(xa[3][v]);
+  const lfr_v4s m = *(const lfr_v4s*)mv;
+  return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a, m, acc, 0, 0, 0);
+}
+
+// One workgroup per cell column, WS waves; wave w owns row tiles
+// [w*IGW, w*IGW + IGW).  Lane l: layer b = l >> 2 of the current 16-layer
+// block, q = l & 3 (rhs sub-index for the A operand, row-in-tile for B/D).
+// Vertically paired tiles (0..2*PIG) all belong to wave 0.
+template <bool BF, int NTP, int NF, int PIG, int IGW>
+__global__ void __launch_bounds__(64 * ((NTP / 4 + IGW - 1) / IGW))
+k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __restrict__ mp_,
+          const int32_t* __restrict__ cells, const int32_t* __restrict__ map_to,
+          const int32_t* __restrict__ map_from, const int32_t* __restrict__ slot,
+          const uint64_t* __restrict__ first, int nl, int nt, int overwrite) {
+  constexpr int IG = NTP / 4;
+  constexpr int JQ = (NF + 3) / 4;
+  constexpr int ME = BF ? 2 : 4;   // bytes per matrix element
+  static_assert(PIG == 0 || 2 * PIG <= IGW, "paired tiles must sit in wave 0");
+  const int c = cells[blockIdx.x];
+  const int l = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int g0 = w * IGW;
+  const int b = l >> 2, q = l & 3;
+  const char* mp = (const char*)mp_;
+  const int32_t* mt = map_to + (int64_t)c * nt;
+  const int32_t* mf = map_from + (int64_t)c * NF;
+  const uint64_t fmask = first[c];
+  int sinfo[IGW];
+#pragma unroll
+  for (int gi = 0; gi < IGW; ++gi) sinfo[gi] = g0 + gi < IG ? slot[4 * (g0 + gi) + q] : 0;
+
+  lfr_v4f carry[PIG > 0 ? PIG : 1][4];     // previous block's top-row tiles (wave 0)
+#pragma unroll
+  for (int g = 0; g < (PIG > 0 ? PIG : 1); ++g)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) carry[g][v] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < nl; k0 += 16) {
+    const int k = min(k0 + b, nl - 1);     // clamped: lanes past the top only load
+    lfr_v4f acc[IGW][4];
+#pragma unroll
+    for (int gi = 0; gi < IGW; ++gi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[gi][v] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
+
+    for (int jq = 0; jq < JQ; ++jq) {
+      lfr_v4f xa[4];
+#pragma unroll
+      for (int jr = 0; jr < 4; ++jr) {
+        const int j = 4 * jq + jr;
+        if (j < NF) {
+          const int64_t d = (int64_t)mf[j] + k;
+          xa[jr] = *(const lfr_v4f*)(x + d * kRhs + 4 * q);
+        } else {
+          xa[jr] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
+        }
+      }
+      const int64_t mrow = ((int64_t)c * JQ + jq) * NTP;
+#pragma unroll
+      for (int gi = 0; gi < IGW; ++gi) {
+        if (IG % IGW != 0 && g0 + gi >= IG) break;
+        const char* mv = mp + ((((mrow + 4 * (g0 + gi) + q) * nl) + k) * 4) * ME;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[gi][v] = mfma_step<BF>(xa, v, mv, acc[gi][v]);
+      }
+    }
+
+    // epilogue: lane (b, q) of tile g holds slot 4g+q of layer k0+b,
+    // acc[g][v][m] = rhs 4m + v.
+    const int kk = k0 + b;
+#pragma unroll
+    for (int gi = 0; gi < IGW; ++gi) {
+      if (IG % IGW != 0 && g0 + gi >= IG) break;
+      const int kind = sinfo[gi] >> 16;
+      const int df = sinfo[gi] & 0xffff;
+      lfr_v4f val[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) val[v] = acc[gi][v];
+      if constexpr (PIG > 0) {
+        if (gi < PIG && w == 0) {
+          // bottom rows: add the partner top row of the layer below
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            lfr_v4f t;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+              const float cur = __shfl(acc[gi + PIG][v][m], (l - 4) & 63);
+              const float prv = __shfl(carry[gi][v][m], (l + 60) & 63);
+              t[m] = b == 0 ? prv : cur;
+            }
+            if (kind == 2 && kk > 0) val[v] = val[v] + t;
+          }
+        }
+      }
+      const bool skip = kind == 0 || kk >= nl || (kind == 3 && kk < nl - 1);
+      if (!skip) {
+        const int64_t d = (int64_t)mt[df] + kk;
+        float* yp = y + d * kRhs;
+        const bool ow = overwrite && ((fmask >> (4 * (g0 + gi) + q)) & 1ull);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          lfr_v4f o = {val[0][m], val[1][m], val[2][m], val[3][m]};   // rhs 4m .. 4m+3
+          if (!ow) o = o + *(const lfr_v4f*)(yp + 4 * m);
+          *(lfr_v4f*)(yp + 4 * m) = o;
+        }
+      }
+    }
+    if constexpr (PIG > 0) {
+      if (w == 0) {
+#pragma unroll
+        for (int g = 0; g < PIG; ++g)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) carry[g][v] = acc[g + PIG][v];
+      }
+    }
+  }
+}
+
+template <bool BF, int NTP, int NF, int PIG, int IGW>
+void launch_multi(hipStream_t s, lfr_multi* h, float* y, const float* x, int overwrite) {
+  constexpr int WS = (NTP / 4 + IGW - 1) / IGW;
+  for (size_t i = 0; i + 1 < h->launch_off.size(); ++i) {
+    const int n = h->launch_off[i + 1] - h->launch_off[i];
+    if (n <= 0) continue;
+    hipLaunchKernelGGL((k_multi16<BF, NTP, NF, PIG, IGW>), dim3(n), dim3(64 * WS), 0, s, y, x,
+                       (const void*)h->mp_d, h->cells_d + h->launch_off[i], h->to->map0_d, h->from->map0_d,
+                       h->slot_d, h->first_d, h->nl, h->nt, overwrite);
+  }
+}
+
+void multi_free(lfr_multi* h) {
+  if (!h) return;
+  for (void* p : {h->mp_d, (void*)h->slot_d, (void*)h->first_d, (void*)h->cells_d})
+    if (p) (void)hipFree(p);
+  delete h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lfr_multi_create(lfr_ctx* ctx, int mprec, int prec_in, const void* matrix_d, const lfr_space* to,
+                     const lfr_space* from, const int32_t* cells_in_colour, const int32_t* ncells_per_colour,
+                     int ncolour, int maxcells, lfr_multi** out) {
+  if (!ctx || !out) return LFR_ERR_ARG;
+  *out = nullptr;
+  LFR_REQUIRE(ctx, to && from && matrix_d && (mprec == 4 || mprec == 2) && (prec_in == 4 || prec_in == 8),
+              "lfr_multi_create: bad arguments");
+  LFR_REQUIRE(ctx, to->ncell == from->ncell && to->nlayers == from->nlayers,
+              "lfr_multi_create: spaces on different meshes");
+  const int nt = to->ndf, nf = from->ndf, nl = to->nlayers, ncell = to->ncell;
+  // supported shapes (ndf_to, ndf_from): the slot layout must match a kernel instance
+  // bottom/top pairs of the output space: map(top) == map(bottom) + 1 in every cell
+  std::vector<int> bottoms, tops;
+  for (int a = 0; a < nt; ++a)
+    for (int t = 0; t < nt; ++t)
+      if (to->map_h[t] == to->map_h[a] + 1) {
+        bool all = true;
+        for (int c = 0; c < ncell && all; ++c)
+          all = to->map_h[(size_t)c * nt + t] == to->map_h[(size_t)c * nt + a] + 1;
+        if (all) {
+          bottoms.push_back(a);
+          tops.push_back(t);
+        }
+      }
+  const int P = (int)bottoms.size();
+  const int P4 = (P + 3) / 4 * 4;
+  int ntp = (nt + 3) / 4 * 4;
+  if (P > 0 && P4 + P > ntp) ntp = (P4 + P + 3) / 4 * 4;
+  std::vector<int32_t> slot(ntp, 0);
+  std::vector<char> used(nt, 0);
+  for (int i = 0; i < P; ++i) {
+    slot[i] = bottoms[i] | (2 << 16);
+    slot[P4 + i] = tops[i] | (3 << 16);
+    used[bottoms[i]] = used[tops[i]] = 1;
+  }
+  {
+    int p = 0;
+    for (int df = 0; df < nt; ++df) {
+      if (used[df]) continue;
+      while (p < ntp && (slot[p] >> 16) != 0) ++p;
+      if (p >= ntp) return lfr_set_error(ctx, LFR_ERR_ARG, "lfr_multi_create: slot layout overflow");
+      slot[p++] = df | (1 << 16);
+    }
+  }
+  const int pig = P4 / 4;
+  // kernel instances
+  const bool ok_shape = (ntp == 36 && nf == 36 && pig == 1) || (ntp == 28 && nf == 27 && pig == 3) ||
+                        (ntp == 8 && nf == 8 && (pig == 0 || pig == 1)) || (ntp == 8 && nf == 6 && pig == 1) ||
+                        (ntp == 8 && nf == 36 && pig == 0);
+  LFR_REQUIRE(ctx, ok_shape,
+              "lfr_multi_create: no kernel for ndf_to %d (padded %d, %d vertical pairs), ndf_from %d", nt, ntp,
+              P, nf);
+  // launch order and first touchers
+  std::vector<int32_t> order;
+  std::vector<int> off{0};
+  bool discontinuous = true;
+  {
+    std::vector<int> count(to->undf + 2, 0);
+    for (size_t i = 0; i < to->map_h.size(); ++i) count[to->map_h[i]]++;
+    for (int c = 0; c < ncell && discontinuous; ++c)
+      for (int df = 0; df < nt; ++df)
+        if (count[to->map_h[(size_t)c * nt + df]] > 1) discontinuous = false;
+  }
+  if (discontinuous) {
+    for (int c = 0; c < ncell; ++c) order.push_back(c);
+    off.push_back(ncell);
+  } else {
+    LFR_REQUIRE(ctx, cells_in_colour && ncells_per_colour && ncolour > 0,
+                "lfr_multi_create: a continuous output space needs the colouring");
+    for (int col = 0; col < ncolour; ++col)
+      LFR_REQUIRE(ctx, ncells_per_colour[col] >= 0 && ncells_per_colour[col] <= maxcells,
+                  "lfr_multi_create: bad ncells_per_colour");
+    for (int col = 0; col < ncolour; ++col) {
+      for (int i = 0; i < ncells_per_colour[col]; ++i) {
+        const int c = cells_in_colour[col + (size_t)ncolour * i] - 1;   // Fortran (ncolour, maxcells)
+        if (c < 0 || c >= ncell) return lfr_set_error(ctx, LFR_ERR_ARG, "lfr_multi_create: bad colour map");
+        order.push_back(c);
+      }
+      off.push_back((int)order.size());
+    }
+    if ((int)order.size() != ncell)
+      return lfr_set_error(ctx, LFR_ERR_ARG, "lfr_multi_create: colouring covers %zu of %d cells", order.size(),
+                           ncell);
+  }
+  std::vector<uint64_t> firstm(ncell, 0);
+  {
+    std::vector<char> seen(to->undf + 2, 0);
+    for (int c : order) {
+      uint64_t m = 0;
+      std::vector<int> cols;
+      for (int p = 0; p < ntp; ++p) {
+        const int kind = slot[p] >> 16;
+        if (!kind) continue;
+        const int df = slot[p] & 0xffff;
+        const int col = to->map_h[(size_t)c * nt + df] - (kind == 3 ? 1 : 0);
+        if (!seen[col]) m |= 1ull << p;
+        cols.push_back(col);
+      }
+      for (int col : cols) seen[col] = 1;
+      firstm[c] = m;
+    }
+  }
+  lfr_multi* h = new lfr_multi();
+  h->ctx = ctx;
+  h->mprec = mprec;
+  h->nt = nt;
+  h->nf = nf;
+  h->ntp = ntp;
+  h->jq = (nf + 3) / 4;
+  h->pig = pig;
+  h->npair = P;
+  h->nl = nl;
+  h->ncell = ncell;
+  h->to = to;
+  h->from = from;
+  h->launch_off = off;
+  const int64_t nel = (int64_t)ncell * h->jq * ntp * nl * 4;
+  h->mp_bytes = nel * (mprec == 4 ? 4 : 2);
+  if (hipMalloc(&h->mp_d, h->mp_bytes) != hipSuccess || hipMalloc(&h->slot_d, ntp * 4) != hipSuccess ||
+      hipMalloc(&h->first_d, (size_t)ncell * 8) != hipSuccess ||
+      hipMalloc(&h->cells_d, (size_t)ncell * 4) != hipSuccess) {
+    multi_free(h);
+    return lfr_set_error(ctx, LFR_ERR_NOMEM, "lfr_multi_create: allocation of %lld bytes failed",
+                         (long long)h->mp_bytes);
+  }
+  hipStream_t s = ctx->stream;
+  LFR_HIP(ctx, hipMemcpyAsync(h->slot_d, slot.data(), ntp * 4, hipMemcpyHostToDevice, s));
+  LFR_HIP(ctx, hipMemcpyAsync(h->first_d, firstm.data(), (size_t)ncell * 8, hipMemcpyHostToDevice, s));
+  LFR_HIP(ctx, hipMemcpyAsync(h->cells_d, order.data(), (size_t)ncell * 4, hipMemcpyHostToDevice, s));
+  const unsigned g = lfr_blocks(nel, 256, 8192);
+  if (prec_in == 4 && mprec == 4)
+    hipLaunchKernelGGL((k_multi_pack<float, float>), dim3(g), dim3(256), 0, s, (float*)h->mp_d,
+                       (const float*)matrix_d, h->slot_d, ncell, nl, nt, nf, ntp, h->jq);
+  else if (prec_in == 8 && mprec == 4)
+    hipLaunchKernelGGL((k_multi_pack<double, float>), dim3(g), dim3(256), 0, s, (float*)h->mp_d,
+                       (const double*)matrix_d, h->slot_d, ncell, nl, nt, nf, ntp, h->jq);
+  else if (prec_in == 4)
+    hipLaunchKernelGGL((k_multi_pack<float, unsigned short>), dim3(g), dim3(256), 0, s,
+                       (unsigned short*)h->mp_d, (const float*)matrix_d, h->slot_d, ncell, nl, nt, nf, ntp, h->jq);
+  else
+    hipLaunchKernelGGL((k_multi_pack<double, unsigned short>), dim3(g), dim3(256), 0, s,
+                       (unsigned short*)h->mp_d, (const double*)matrix_d, h->slot_d, ncell, nl, nt, nf, ntp,
+                       h->jq);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    multi_free(h);
+    return lfr_set_error(ctx, LFR_ERR_HIP, "lfr_multi_create: %s", hipGetErrorString(e));
+  }
+  *out = h;
+  return LFR_OK;
+}
+
+int lfr_multi_destroy(lfr_multi* h) {
+  multi_free(h);
+  return LFR_OK;
+}
+
+int lfr_multi_info(const lfr_multi* h, int64_t* packed_bytes, int* slots, int* vertical_pairs, int* launches) {
+  if (!h) return LFR_ERR_ARG;
+  if (packed_bytes) *packed_bytes = h->mp_bytes;
+  if (slots) *slots = h->ntp;
+  if (vertical_pairs) *vertical_pairs = h->npair;
+  if (launches) *launches = (int)h->launch_off.size() - 1;
+  return LFR_OK;
+}
+
+int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void* x_d, int nrhs, int overwrite) {
+  if (!ctx || !h) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, nrhs == kRhs, "lfr_matrix_vector_multi: nrhs must be %d (got %d)", kRhs, nrhs);
+  LFR_REQUIRE(ctx, lhs_d && x_d, "lfr_matrix_vector_multi: null field");
+  LFR_REQUIRE(ctx, ((uintptr_t)lhs_d & 15) == 0 && ((uintptr_t)x_d & 15) == 0,
+              "lfr_matrix_vector_multi: fields must be 16-byte aligned");
+  LFR_REQUIRE(ctx, lhs_d != x_d, "lfr_matrix_vector_multi: lhs and x must not alias");
+  hipStream_t s = ctx->stream;
+  float* y = (float*)lhs_d;
+  const float* x = (const float*)x_d;
+  const bool bf = h->mprec == 2;
+  const int key = h->ntp * 1000 + h->nf * 10 + h->pig;
+#define LFR_MULTI_CASE(NTP, NF, PIG, IGW)                               \
+  case NTP * 1000 + NF * 10 + PIG:                                      \
+    if (bf)                                                             \
+      launch_multi<true, NTP, NF, PIG, IGW>(s, h, y, x, overwrite);     \
+    else                                                                \
+      launch_multi<false, NTP, NF, PIG, IGW>(s, h, y, x, overwrite);    \
+    break;
+  switch (key) {
+    LFR_MULTI_CASE(36, 36, 1, 3)
+    LFR_MULTI_CASE(28, 27, 3, 7)
+    LFR_MULTI_CASE(8, 8, 0, 2)
+    LFR_MULTI_CASE(8, 8, 1, 2)
+    LFR_MULTI_CASE(8, 6, 1, 2)
+    LFR_MULTI_CASE(8, 36, 0, 2)
+    default:
+      return lfr_set_error(ctx, LFR_ERR_ARG, "lfr_matrix_vector_multi: no kernel instance %d", key);
+  }
+#undef LFR_MULTI_CASE
+  LFR_CHECK_LAUNCH(ctx);
+  return LFR_OK;
+}
+
+}  // extern "C"

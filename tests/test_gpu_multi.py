@@ -1,0 +1,101 @@
+"""16-field MFMA apply (lfr_matrix_vector_multi) against the oracle's FP64
+single-field loops, one field at a time.
+
+Tolerances (norm-wise relative error per field, against FP64 on FP64 data):
+FP32 MFMA 2e-6; BF16-input/FP32-accumulate 2e-2 (BF16 keeps 8 mantissa bits).
+The MFMA summation order differs from the single-field kernels, so these
+are tolerances, not bit-exactness; repeated applies are bitwise identical."""
+import numpy as np
+import pytest
+
+import oracle
+from lfric_core_amd import function_space as fsm
+from lfric_core_amd.configs import normal_lma, spd_lma
+from lfric_core_amd.mesh import cubed_sphere
+from lfric_core_amd.rng import Stream
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+TOL = {4: 2e-6, 2: 2e-2}
+CASES = [  # (to space, k, from space, k, matrix kind)
+    ("W2", 1, "W2", 1, "spd"),
+    ("W0", 1, "W0", 1, "spd"),
+    ("W3", 1, "W3", 1, "spd"),
+    ("W3", 1, "W2", 1, "normal"),
+    ("W0", 0, "W0", 0, "spd"),
+    ("W2", 0, "W2", 0, "spd"),
+]
+
+
+def _problem(to, kt, fr, kf, kind, nl, edge=4, seed=3):
+    mesh = cubed_sphere(edge)
+    fs_to = fsm.build_dofmap(mesh, to, nl, k=kt)
+    fs_fr = fs_to if (fr, kf) == (to, kt) else fsm.build_dofmap(mesh, fr, nl, k=kf)
+    col = fsm.colour_cells(mesh)
+    st = Stream(seed)
+    n3 = mesh.ncells * nl
+    if kind == "spd":
+        M = spd_lma(st, n3, fs_to.ndf, float(fs_to.ndf), std=1.0 / np.sqrt(fs_to.ndf))
+    else:
+        M = normal_lma(st, n3, fs_to.ndf, fs_fr.ndf)
+    X = st.uniform(fs_fr.undf * 16).reshape(fs_fr.undf, 16)
+    return mesh, fs_to, fs_fr, col, M, X
+
+
+def _oracle(fs_to, fs_fr, col, nl, M, X):
+    Y = np.zeros((fs_to.undf, 16))
+    for r in range(16):
+        x = np.ascontiguousarray(X[:, r])
+        if fs_to.is_discontinuous:
+            y = oracle.invoke_dg(0, nl, np.zeros(fs_to.undf), x, M, fs_to.dofmap, fs_fr.dofmap)
+        else:
+            y = oracle.invoke_coloured(0, col, nl, np.zeros(fs_to.undf), x, M, fs_to.dofmap,
+                                       fs_fr.dofmap)
+        Y[:, r] = y
+    return Y
+
+
+@pytest.mark.parametrize("mprec", [4, 2])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}k{c[1]}-{c[2]}k{c[3]}")
+def test_multi_apply_matches_fp64(ctx, case, mprec):
+    from lfric_core_amd import api
+    nl = 20                                     # one full 16-layer block and a partial one
+    to, kt, fr, kf, kind = case
+    print(f"multi {to}k{kt}<-{fr}k{kf} mprec={mprec}", flush=True)
+    mesh, fs_to, fs_fr, col, M, X = _problem(to, kt, fr, kf, kind, nl)
+    Y64 = _oracle(fs_to, fs_fr, col, nl, M, X)
+    s_to = api.Space(ctx, fs_to)
+    s_fr = s_to if fs_fr is fs_to else api.Space(ctx, fs_fr)
+    op = api.LMAOperator(torch.from_numpy(M.astype(np.float32)).to(ctx.device), s_to, s_fr, col)
+    mop = api.MultiOperator(op, mprec)
+    xd = torch.from_numpy(X.astype(np.float32)).to(ctx.device)
+    y = torch.full((fs_to.undf, 16), np.nan, dtype=torch.float32, device=ctx.device)
+    mop.apply(y, xd, overwrite=True)
+    Y = y.cpu().numpy().astype(np.float64)
+    err = np.linalg.norm(Y - Y64, axis=0) / np.linalg.norm(Y64, axis=0)
+    assert np.all(np.isfinite(Y)), "unwritten output dofs"
+    assert err.max() < TOL[mprec], err
+    # increment form adds onto lhs; repeated overwrite applies are bitwise identical
+    y2 = y.clone()
+    mop.apply(y2, xd, overwrite=False)
+    np.testing.assert_allclose(y2.cpu().numpy(), 2 * Y, rtol=1e-6, atol=1e-6 * np.abs(Y).max())
+    y3 = torch.zeros_like(y)
+    mop.apply(y3, xd, overwrite=True)
+    assert torch.equal(y3, y)
+    mop.close()
+
+
+def test_multi_info_and_errors(ctx):
+    from lfric_core_amd import api
+    from lfric_core_amd._ffi import LfrError
+    mesh, fs_to, fs_fr, col, M, X = _problem("W2", 1, "W2", 1, "spd", 16, edge=2)
+    s = api.Space(ctx, fs_to)
+    op = api.LMAOperator(torch.from_numpy(M.astype(np.float32)).to(ctx.device), s, s, col)
+    mop = api.MultiOperator(op, 4)
+    info = mop.info()
+    assert info["slots"] == 36 and info["vertical_pairs"] == 4 and info["launches"] == col.ncolours
+    with pytest.raises(LfrError):
+        mop.apply(torch.zeros(fs_to.undf, 8, device=ctx.device),
+                  torch.zeros(fs_to.undf, 16, device=ctx.device))
+    mop.close()
