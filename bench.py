@@ -148,6 +148,98 @@ def bench_solvers(ctx, cfg, w2, torch, steps_cg: int = 50):
     return out
 
 
+def bench_multi(ctx, torch, edge_cells: int = 48, nl: int = 70, steps: int = 20):
+    """16-field W2 (element order 1, 36x36) apply on MFMA, FP32 and BF16-input,
+    each against 16 single-field FP64 applies of the same operator."""
+    from lfric_core_amd import api
+    from lfric_core_amd.configs import multi_config
+    t0 = time.time()
+    cfg = multi_config(ctx.device, edge_cells, nl)
+    torch.cuda.synchronize()
+    n3 = cfg.mesh.ncells * nl
+    undf = cfg.w2.undf
+    log(f"multi config C{edge_cells}L{nl} W2 k=1 undf={undf} built in {time.time()-t0:.1f}s")
+    w2 = api.Space(ctx, cfg.w2)
+    op64 = api.LMAOperator(cfg.matrix, w2, w2, cfg.colouring)
+    Y64 = torch.empty_like(cfg.x)
+    y1 = torch.empty(undf, dtype=torch.float64, device=ctx.device)
+    for r in range(16):
+        api.matrix_vector(y1, cfg.x[:, r].contiguous(), op64, overwrite=True)
+        Y64[:, r] = y1
+    M32 = cfg.matrix.float()
+    op32 = api.LMAOperator(M32, w2, w2, cfg.colouring)
+    x32 = cfg.x.float().contiguous()
+    ref_n = torch.linalg.vector_norm(Y64, dim=0)
+    flops = 2.0 * 36 * 36 * 16 * n3
+    out = {"workload": f"y_r = A x_r, r=0..15, W2 element order 1 (36x36 local matrices), "
+                       f"C{edge_cells}L{nl}", "undf": undf, "ncell_3d": n3}
+    for name, mprec, sm in (("fp32", 4, 4), ("bf16_in_fp32_acc", 2, 2)):
+        mop = api.MultiOperator(op32, mprec)
+        y = torch.empty(undf, 16, dtype=torch.float32, device=ctx.device)
+        for _ in range(3):
+            mop.apply(y, x32)
+        ms = time_events(lambda: mop.apply(y, x32), steps, torch)
+        err = torch.linalg.vector_norm(y.double() - Y64, dim=0) / ref_n
+        nbytes = n3 * 1296 * sm + 2 * 16 * undf * 4 + cfg.mesh.ncells * 36 * 4
+        out[name] = {"ms": round(ms, 4), "gdofs": round(16 * undf / (ms * 1e-3) / 1e9, 2),
+                     "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "roofline_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                     "algorithmic_bytes": nbytes,
+                     "max_rel_err_vs_fp64": float(err.max()), "mean_rel_err_vs_fp64": float(err.mean())}
+        log(f"multi {name}: {ms*1e3:.0f} us, {16*undf/(ms*1e-3)/1e9:.1f} GDOF/s, "
+            f"{nbytes/(ms*1e-3)/1e9:.0f} GB/s, {flops/(ms*1e-3)/1e12:.1f} TFLOP/s, "
+            f"max rel err vs FP64 {float(err.max()):.2e}")
+        mop.close()
+        del y
+    # 16 simultaneous refined solves (FP32 MFMA CG inside FP64 residuals) to 1e-10,
+    # against 16 plain FP64 CG solves (single-field gather) to the same tolerance
+    from lfric_core_amd.solvers import ConjugateGradient, RefinedCGMulti
+    mop = api.MultiOperator(op32, 4)
+    xr = torch.zeros_like(cfg.x)
+    solver = RefinedCGMulti(mop, op64, r_tol=1e-10, max_outer=20, inner_r_tol=1e-5,
+                            inner_max_iter=1000)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    st = solver.apply(xr, cfg.x)
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t
+    mop.close()
+    x1 = torch.zeros(undf, dtype=torch.float64, device=ctx.device)
+    cg = ConjugateGradient(op64, r_tol=1e-10, max_iter=5000)
+    cg.apply(x1, cfg.x[:, 0].contiguous())        # warm
+    its, errs = [], []
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for r in range(16):
+        x1.zero_()
+        s64 = cg.apply(x1, cfg.x[:, r].contiguous())
+        its.append(s64.iterations)
+        errs.append(x1 - xr[:, r])
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t
+    xerr = max(float(torch.linalg.vector_norm(e) / torch.linalg.vector_norm(xr[:, i]))
+               for i, e in enumerate(errs))
+    out["refined_cg_16"] = {"seconds": round(t_ref, 4), "outer": st.outer,
+                            "inner_iterations": st.inner_total, "converged": bool(st.converged),
+                            "max_rel_residual": st.r_nrm,
+                            "fp64_cg_16_seconds": round(t64, 4), "fp64_cg_iterations": its,
+                            "max_rel_diff_vs_fp64_cg": xerr,
+                            "speedup_vs_fp64_cg": round(t64 / t_ref, 3)}
+    log(f"refined CG x16: {t_ref*1e3:.0f} ms ({st.outer} outer, {st.inner_total} inner), max rel "
+        f"residual {st.r_nrm:.2e}; FP64 CG x16 {t64*1e3:.0f} ms; x diff {xerr:.2e}")
+    del xr, x1, errs
+    # single-field FP32 gather applies of the same operator, for comparison
+    ys = torch.empty(undf, dtype=torch.float32, device=ctx.device)
+    xs = x32[:, 0].contiguous()
+    api.matrix_vector(ys, xs, op32, overwrite=True)
+    ms1 = time_events(lambda: api.matrix_vector(ys, xs, op32, overwrite=True), steps, torch)
+    out["single_field_fp32_x16_ms"] = round(16 * ms1, 4)
+    del op64, op32, M32, Y64, x32, cfg
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_multigrid(ctx, torch, edge_cells=(192, 96, 48, 24), nl: int = 70, nrhs: int = 8,
                     cycles: int = 10):
     """Batched V-cycles (2 pre / 2 post line-Jacobi sweeps, 8 coarse sweeps) on the
@@ -198,6 +290,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solvers", action="store_true")
     ap.add_argument("--no-mg", action="store_true", help="skip the batched multigrid V-cycles")
+    ap.add_argument("--no-multi", action="store_true", help="skip the 16-field MFMA apply")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a plain copy of 1 GiB (FP32) 5 times, for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes")
@@ -248,6 +341,8 @@ def main():
 
     if not args.no_solvers:
         extra["solvers"] = bench_solvers(ctx, cfg, w2, torch)
+    if not args.no_multi:
+        extra["multi_rhs"] = bench_multi(ctx, torch, 24 if args.quick else 48)
     if not args.no_mg:
         extra["multigrid"] = bench_multigrid(ctx, torch, (48, 24, 12, 6) if args.quick
                                              else (192, 96, 48, 24))

@@ -288,7 +288,8 @@ int lfr_columnwise_op_appinv(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_
  *                  y outer, x inner (ignored for l = 0)
  *   bands_h[l]     [3][nlayers][ncell[l]] sub, diag, super of T
  *   sigma[l]       horizontal coupling of level l
- * nu1/nu2/ncoarse must be even. */
+ * nu1/nu2/ncoarse must be even; nrhs is a power of 2 up to 32; nlayers <= 128;
+ * each level has 4x the cells of the next coarser one. */
 typedef struct lfr_mg lfr_mg;
 int lfr_mg_create(lfr_ctx* ctx, int prec, int nlev, int nlayers, int nrhs, const int* ncell,
                   const int32_t* const* cell_next_h, const int32_t* const* children_h,
@@ -329,6 +330,22 @@ int lfr_multi_info(const lfr_multi* h, int64_t* packed_bytes, int* slots, int* v
                    int* launches);
 int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void* x_d, int nrhs,
                             int overwrite);
+
+/* Sixteen simultaneous solves A x_r = b_r by FP32 CG on the multi-field
+ * operator A32 (lfr_matrix_vector_multi) inside an FP64 refinement loop
+ * whose residuals use the single-field FP64 operator A64 (same matrix).
+ * x64_d, b64_d: FP64 multidata fields (undf, 16); x holds the initial guess.
+ * Each inner solve starts from zero and freezes a field once its residual
+ * falls by inner_r_tol; the loop ends when ||b_r - A x_r|| <= r_tol ||b_r||
+ * for every field or after max_outer corrections.  stats->r_nrm is the
+ * largest final relative residual, stats->b_nrm the largest ||b_r||;
+ * rel_res_h (optional) receives the 16 final relative residuals.  The inner
+ * iteration is conjugate_gradient_type%cg_solve
+ * (components/science/source/solver/sci_iterative_solver_mod.f90:495-592) per
+ * field; the batching has no counterpart in the reference. */
+int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_op* A64, double r_tol,
+                         int max_outer, double inner_r_tol, int inner_max_iter, void* x64_d,
+                         const void* b64_d, int nrhs, lfr_refine_stats* stats, double* rel_res_h);
 
 #ifdef __cplusplus
 }

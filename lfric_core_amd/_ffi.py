@@ -75,6 +75,7 @@ SIGNATURES = {
     "lfr_matrix_vector_multi": [_P, _P, _P, _P, _I, _I],
     "lfr_cg": None,
     "lfr_refined_cg": None,
+    "lfr_refined_cg_multi": None,
     "lfr_gmres": None,
 }
 _RESTYPE = {"lfr_last_error": C.c_char_p, "lfr_get_stream": _P}

@@ -120,3 +120,44 @@ def diffusion_config(node_lon_deg, node_lat_deg, face_nodes, nlayers: int = 10,
     x_true = Stream(seed).uniform(w0.undf)
     return DiffusionConfig(edge, nlayers, mesh, w0, col, heights, to_local_stencil(A),
                            to_local_stencil(mass), to_local_stencil(stiff), x_true)
+
+
+def spd_lma_device(stream, n3: int, ndf: int, shift: float, std: float, chunk: int = 1 << 16):
+    """spd_lma drawn from a rng.TorchStream on its device (float64 tensor
+    (ndf, ndf, n3)); same draw order and construction as ``spd_lma``."""
+    import torch
+    out = torch.empty((ndf, ndf, n3), dtype=torch.float64, device=stream.device)
+    eye = shift * torch.eye(ndf, dtype=torch.float64, device=stream.device)
+    for s in range(0, n3, chunk):
+        m = min(chunk, n3 - s)
+        B = stream.normal(m * ndf * ndf).reshape(m, ndf, ndf) * std
+        A = torch.bmm(B, B.transpose(1, 2).contiguous()) + eye
+        out[:, :, s:s + m] = A.permute(2, 1, 0)
+    return out
+
+
+@dataclass
+class MultiConfig:
+    """Element-order-1 W2 mass operator and 16 fields (the C48L70 multi-field case)."""
+
+    edge_cells: int
+    nlayers: int
+    mesh: object
+    w2: fsm.FunctionSpace
+    colouring: fsm.Colouring
+    matrix: object        # float64 (36, 36, n3) on the device
+    x: object             # float64 (undf, 16) on the device
+
+
+def multi_config(device, edge_cells: int = 48, nlayers: int = 70, seed: int = 3) -> MultiConfig:
+    """W2 k=1 (36 dofs/cell) local matrices B B^T + 36 I, B ~ N(0, 1/36); then
+    16 fields ~ U[-1, 1] drawn dof-major ((undf, 16) row-major), one stream."""
+    from .rng import TorchStream
+    mesh = cubed_sphere(edge_cells)
+    w2 = fsm.build_dofmap(mesh, "W2", nlayers, k=1)
+    col = fsm.colour_cells(mesh)
+    st = TorchStream(seed, device)
+    n3 = mesh.ncells * nlayers
+    M = spd_lma_device(st, n3, w2.ndf, float(w2.ndf), 1.0 / np.sqrt(w2.ndf))
+    x = st.uniform(w2.undf * 16).reshape(w2.undf, 16)
+    return MultiConfig(edge_cells, nlayers, mesh, w2, col, M, x)

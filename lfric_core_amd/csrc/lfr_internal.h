@@ -69,6 +69,9 @@ bool lfr_gather_nt();
 
 int lfr_set_error(lfr_ctx* ctx, int code, const char* fmt, ...);
 
+// Output and input field sizes (undf) of a multi-field operator handle.
+extern "C" int lfr_multi_space_undf(const lfr_multi* h, int* undf_to, int* undf_from);
+
 #define LFR_HIP(ctx, call)                                                        \
   do {                                                                            \
     hipError_t _e = (call);                                                       \

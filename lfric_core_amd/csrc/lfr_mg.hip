@@ -13,12 +13,21 @@
 // and injection prolongation (sci_prolong_scalar_unweighted_kernel_mod.F90:108-116,
 // fine += coarse).
 //
-// Layout: level fields are stored layer-major, f[(rhs*nl + k)*ncell + c], so
-// that one thread per (column, right-hand side) walking its column reads
-// consecutive cells across a wavefront for every layer (coalesced), and the
-// four neighbour reads of a layer hit one contiguous plane.  Band arrays are
-// [k][ncell] per band.  The fine-level right-hand side and solution are
-// converted from/to LFRic's k-fastest W3 layout once per solve.
+// Layout: level fields are stored f[(k*ncell + c)*R + r] (layer, cell, then
+// the R right-hand sides fastest), so that the lanes of a wavefront cover R
+// right-hand sides of consecutive cells: every load of x, b and of a
+// neighbour value is a contiguous run, and the band values of a column are
+// shared by its R lanes.  Band arrays are [k][ncell].  The fine-level
+// right-hand side and solution are converted from/to LFRic's W3 layout
+// (field r at r*undf, dof = c*nl + k) once per solve with LDS-tiled
+// transposes.
+//
+// The line smoother never divides: the Thomas factors of D = T + 4 sigma I
+// (cp_k and 1/(d_k + 4 sigma - a_k cp_{k-1})) depend only on the operator and
+// are computed once per level at set-up with the same operations the
+// recurrence would use, so results are bitwise those of the textbook sweep.
+// The forward sweep stores d'_k in the output field and the backward sweep
+// finishes it in place, so a thread holds no per-layer arrays.
 //
 // One V-cycle over all right-hand sides is captured into a hipGraph on first
 // use and replayed; each level's kernels use the workgroup size chosen by a
@@ -39,7 +48,9 @@ struct lfr_mg_level {
   void* a_d = nullptr;              // bands [k][ncell]
   void* d_d = nullptr;
   void* u_d = nullptr;
-  void* x_d = nullptr;              // [rhs][k][ncell]
+  void* cp_d = nullptr;             // Thomas factors of D, [k][ncell]
+  void* iv_d = nullptr;
+  void* x_d = nullptr;              // [k][ncell][rhs]
   void* b_d = nullptr;
   void* t_d = nullptr;              // smoother ping-pong buffer
   double sigma = 0;
@@ -62,82 +73,87 @@ struct lfr_mg {
 namespace {
 
 constexpr int kMaxNl = 128;
+constexpr int kMaxRhs = 32;
 
 template <typename T>
-__device__ __forceinline__ T horiz(const T* __restrict__ xk, const int32_t* __restrict__ nb, int c,
+__device__ __forceinline__ T horiz(const T* __restrict__ xk, const int32_t* __restrict__ nb, int c, int R, int r,
                                    T xc, T sigma) {
-  const T s4 = xk[nb[4 * c + 0]] + xk[nb[4 * c + 1]];
-  const T s5 = s4 + xk[nb[4 * c + 2]];
-  const T sn = s5 + xk[nb[4 * c + 3]];
+  const T s4 = xk[(int64_t)nb[4 * c + 0] * R + r] + xk[(int64_t)nb[4 * c + 1] * R + r];
+  const T s5 = s4 + xk[(int64_t)nb[4 * c + 2] * R + r];
+  const T sn = s5 + xk[(int64_t)nb[4 * c + 3] * R + r];
   const T four = (T)4 * xc;
   return sigma * (four - sn);
 }
 
-// One damped line-Jacobi sweep: xo = xi + omega * D^{-1} (b - A xi)
-template <typename T, int NL>
-__global__ void k_mg_smooth(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b,
-                            const T* __restrict__ ba, const T* __restrict__ bd,
-                            const T* __restrict__ bu, const int32_t* __restrict__ nb, int ncell,
-                            int nl_rt, T sigma, T omega) {
-  const int nl = NL > 0 ? NL : nl_rt;
+// Thomas factors of D = tridiag(a, d + 4 sigma, u), per column
+template <typename T>
+__global__ void k_mg_factor(T* __restrict__ cp, T* __restrict__ iv, const T* __restrict__ ba,
+                            const T* __restrict__ bd, const T* __restrict__ bu, int ncell, int nl, T sigma) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int rhs = blockIdx.y;
   if (c >= ncell) return;
-  const int64_t off = (int64_t)rhs * nl * ncell;
-  const T* x = xi + off;
-  const T* bb = b + off;
-  T cp[NL > 0 ? NL : kMaxNl], dp[NL > 0 ? NL : kMaxNl];
-  T xm = 0, x0 = x[c];
-#pragma unroll
-  for (int k = 0; k < (NL > 0 ? NL : kMaxNl); ++k) {
-    if (NL == 0 && k >= nl) break;
+  T cprev = 0;
+  for (int k = 0; k < nl; ++k) {
     const int64_t ik = (int64_t)k * ncell + c;
-    const T xp = (k + 1 < nl) ? x[ik + ncell] : T(0);
+    const T dd = bd[ik] + (T)4 * sigma;
+    const T inv = k == 0 ? T(1) / dd : T(1) / (dd - ba[ik] * cprev);
+    iv[ik] = inv;
+    cprev = inv * bu[ik];
+    cp[ik] = cprev;
+  }
+}
+
+// One damped line-Jacobi sweep: xo = xi + omega * D^{-1} (b - A xi); thread = (cell, rhs)
+template <typename T>
+__global__ void __launch_bounds__(256) k_mg_smooth(T* __restrict__ xo, const T* __restrict__ xi, const T* __restrict__ b,
+                                                   const T* __restrict__ ba, const T* __restrict__ bd,
+                                                   const T* __restrict__ bu, const T* __restrict__ cp,
+                                                   const T* __restrict__ iv, const int32_t* __restrict__ nb,
+                                                   int ncell, int nl, int lgR, T sigma, T omega) {
+  const int R = 1 << lgR;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ncell * R) return;
+  const int c = (int)(t >> lgR), r = (int)(t & (R - 1));
+  const int64_t plane = (int64_t)ncell * R;
+  T xm = 0, x0 = xi[t], dprev = 0;
+  for (int k = 0; k < nl; ++k) {
+    const int64_t i = (int64_t)k * plane + t;
+    const int64_t ik = (int64_t)k * ncell + c;
+    const T xp = (k + 1 < nl) ? xi[i + plane] : T(0);
     const T ak = ba[ik], dk = bd[ik], uk = bu[ik];
-    // residual r = b - A x at (c, k)
     T ax = ak * xm;
     ax = ax + dk * x0;
     ax = ax + uk * xp;
-    ax = ax + horiz(x + (int64_t)k * ncell, nb, c, x0, sigma);
-    const T r = bb[ik] - ax;
-    // Thomas forward sweep on D = tridiag(a, d + 4 sigma, u)
-    const T dd = dk + (T)4 * sigma;
-    if (k == 0) {
-      const T inv = T(1) / dd;
-      cp[k] = inv * uk;
-      dp[k] = inv * r;
-    } else {
-      const T inv = T(1) / (dd - ak * cp[k - 1]);
-      cp[k] = inv * uk;
-      dp[k] = inv * (r - ak * dp[k - 1]);
-    }
+    ax = ax + horiz(xi + (int64_t)k * plane, nb, c, R, r, x0, sigma);
+    const T res = b[i] - ax;
+    const T dp = k == 0 ? iv[ik] * res : iv[ik] * (res - ak * dprev);
+    xo[i] = dp;
+    dprev = dp;
     xm = x0;
     x0 = xp;
   }
-  T* xout = xo + off;
-#pragma unroll
-  for (int k = (NL > 0 ? NL : kMaxNl) - 1; k >= 0; --k) {
-    if (NL == 0 && k >= nl) continue;
-    if (k < nl - 1) dp[k] = dp[k] - cp[k] * dp[k + 1];
-    const int64_t ik = (int64_t)k * ncell + c;
-    xout[ik] = x[ik] + omega * dp[k];
+  T dnext = 0;
+  for (int k = nl - 1; k >= 0; --k) {
+    const int64_t i = (int64_t)k * plane + t;
+    T dp = xo[i];
+    if (k < nl - 1) dp = dp - cp[(int64_t)k * ncell + c] * dnext;
+    xo[i] = xi[i] + omega * dp;
+    dnext = dp;
   }
 }
 
 // bc(C,k) = sum over the 4 children f (y outer, x inner) of r(f,k) * 1/4,
 // r = b - A x on the fine level.  Also zeroes the coarse solution.
 template <typename T>
-__global__ void k_mg_restrict(T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ xf,
-                              const T* __restrict__ bf, const T* __restrict__ ba,
-                              const T* __restrict__ bd, const T* __restrict__ bu,
-                              const int32_t* __restrict__ nb, const int32_t* __restrict__ children,
-                              int ncell_c, int ncell_f, int nl, T sigma) {
-  const int C = blockIdx.x * blockDim.x + threadIdx.x;
-  const int rhs = blockIdx.y;
-  if (C >= ncell_c) return;
-  const int64_t offf = (int64_t)rhs * nl * ncell_f, offc = (int64_t)rhs * nl * ncell_c;
-  const T* x = xf + offf;
-  const T* b = bf + offf;
+__global__ void __launch_bounds__(256) k_mg_restrict(T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ xf,
+                                                     const T* __restrict__ bf, const T* __restrict__ ba,
+                                                     const T* __restrict__ bd, const T* __restrict__ bu,
+                                                     const int32_t* __restrict__ nb, const int32_t* __restrict__ children,
+                                                     int ncell_c, int ncell_f, int nl, int lgR, T sigma) {
+  const int R = 1 << lgR;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ncell_c * R) return;
+  const int C = (int)(t >> lgR), r = (int)(t & (R - 1));
+  const int64_t pf = (int64_t)ncell_f * R, pc = (int64_t)ncell_c * R;
   const T denom = T(1) / T(4);
   int ch[4];
 #pragma unroll
@@ -147,102 +163,149 @@ __global__ void k_mg_restrict(T* __restrict__ bc, T* __restrict__ xc, const T* _
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int f = ch[q];
+      const int64_t i = (int64_t)k * pf + (int64_t)f * R + r;
       const int64_t ik = (int64_t)k * ncell_f + f;
-      const T x0 = x[ik];
-      const T xm = k > 0 ? x[ik - ncell_f] : T(0);
-      const T xp = k + 1 < nl ? x[ik + ncell_f] : T(0);
+      const T x0 = xf[i];
+      const T xm = k > 0 ? xf[i - pf] : T(0);
+      const T xp = k + 1 < nl ? xf[i + pf] : T(0);
       T ax = ba[ik] * xm;
       ax = ax + bd[ik] * x0;
       ax = ax + bu[ik] * xp;
-      ax = ax + horiz(x + (int64_t)k * ncell_f, nb, f, x0, sigma);
-      const T r = b[ik] - ax;
-      acc = acc + r * denom;
+      ax = ax + horiz(xf + (int64_t)k * pf, nb, f, R, r, x0, sigma);
+      const T res = bf[i] - ax;
+      acc = acc + res * denom;
     }
-    bc[offc + (int64_t)k * ncell_c + C] = acc;
-    xc[offc + (int64_t)k * ncell_c + C] = T(0);
+    bc[(int64_t)k * pc + t] = acc;
+    xc[(int64_t)k * pc + t] = T(0);
   }
 }
 
 // xf(f,k) += xc(parent(f), k)
 template <typename T>
-__global__ void k_mg_prolong(T* __restrict__ xf, const T* __restrict__ xc,
-                             const int32_t* __restrict__ parent, int ncell_f, int ncell_c, int nl) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  const int rhs = blockIdx.y;
-  if (f >= ncell_f) return;
-  const int P = parent[f];
-  T* x = xf + (int64_t)rhs * nl * ncell_f;
-  const T* y = xc + (int64_t)rhs * nl * ncell_c;
-  for (int k = 0; k < nl; ++k) x[(int64_t)k * ncell_f + f] += y[(int64_t)k * ncell_c + P];
+__global__ void __launch_bounds__(256) k_mg_prolong(T* __restrict__ xf, const T* __restrict__ xc,
+                                                    const int32_t* __restrict__ parent, int ncell_f, int ncell_c,
+                                                    int nl, int lgR) {
+  const int R = 1 << lgR;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ncell_f * R) return;
+  const int f = (int)(t >> lgR), r = (int)(t & (R - 1));
+  const int64_t src = (int64_t)parent[f] * R + r;
+  const int64_t pf = (int64_t)ncell_f * R, pc = (int64_t)ncell_c * R;
+  for (int k = 0; k < nl; ++k) xf[(int64_t)k * pf + t] += xc[(int64_t)k * pc + src];
 }
 
-// per-rhs partial sums of r^2, r = b - A x
+// per-rhs partial sums of r^2, r = b - A x; part[blk*R + r]
 template <typename T>
 __global__ void __launch_bounds__(lfr::kRedThreads)
 k_mg_resnorm(const T* __restrict__ xf, const T* __restrict__ bf, const T* __restrict__ ba,
              const T* __restrict__ bd, const T* __restrict__ bu, const int32_t* __restrict__ nb,
-             int ncell, int nl, T sigma, double* __restrict__ part, int nblk) {
-  __shared__ double lds[lfr::kRedThreads / 64];
-  const int rhs = blockIdx.y;
-  const int64_t off = (int64_t)rhs * nl * ncell;
-  const T* x = xf + off;
-  const T* b = bf + off;
-  const int64_t n = (int64_t)ncell * nl;
+             int ncell, int nl, int lgR, T sigma, double* __restrict__ part) {
+  __shared__ double lds[lfr::kRedThreads / 64][kMaxRhs];
+  const int R = 1 << lgR;
+  const int64_t plane = (int64_t)ncell * R;
+  const int64_t n = plane * nl;
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * lfr::kRedThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * lfr::kRedThreads) {
-    const int k = (int)(i / ncell);
-    const int c = (int)(i - (int64_t)k * ncell);
-    const T x0 = x[i];
-    const T xm = k > 0 ? x[i - ncell] : T(0);
-    const T xp = k + 1 < nl ? x[i + ncell] : T(0);
-    T ax = ba[i] * xm;
-    ax = ax + bd[i] * x0;
-    ax = ax + bu[i] * xp;
-    ax = ax + horiz(x + (int64_t)k * ncell, nb, c, x0, sigma);
-    const double r = (double)(b[i] - ax);
-    s += r * r;
+    const int k = (int)(i / plane);
+    const int64_t t = i - (int64_t)k * plane;
+    const int c = (int)(t >> lgR), r = (int)(t & (R - 1));
+    const int64_t ik = (int64_t)k * ncell + c;
+    const T x0 = xf[i];
+    const T xm = k > 0 ? xf[i - plane] : T(0);
+    const T xp = k + 1 < nl ? xf[i + plane] : T(0);
+    T ax = ba[ik] * xm;
+    ax = ax + bd[ik] * x0;
+    ax = ax + bu[ik] * xp;
+    ax = ax + horiz(xf + (int64_t)k * plane, nb, c, R, r, x0, sigma);
+    const double res = (double)(bf[i] - ax);
+    s += res * res;
   }
-  s = lfr::block_sum(s, lds);
-  if (threadIdx.x == 0) part[rhs * nblk + blockIdx.x] = s;
+  // lanes l and l + R, l + 2R, ... hold the same right-hand side
+  for (int off = 32; off >= R; off >>= 1) s += __shfl_xor(s, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane < R) lds[w][lane] = s;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    double v = 0.0;
+    for (int q = 0; q < lfr::kRedThreads / 64; ++q) v += lds[q][threadIdx.x];
+    part[(int64_t)blockIdx.x * R + threadIdx.x] = v;
+  }
 }
 
-// LFRic W3 layout (k fastest: dof = c*nl + k) <-> layer-major
-template <typename T>
-__global__ void k_to_layer_major(T* __restrict__ out, const T* __restrict__ in, int ncell, int nl, int nrhs) {
-  const int64_t n = (int64_t)ncell * nl * nrhs;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t per = (int64_t)ncell * nl;
-    const int64_t rhs = i / per, rem = i - rhs * per;
-    const int k = (int)(rem / ncell), c = (int)(rem - (int64_t)k * ncell);
-    out[i] = in[rhs * per + (int64_t)c * nl + k];
+// out[r] = sum over blocks (fixed order) of part[blk*R + r]; one block of 256
+__global__ void __launch_bounds__(256) k_mg_resnorm_final(const double* __restrict__ part, int nb, int R,
+                                                          double* __restrict__ out) {
+  __shared__ double lds[256];
+  const int r = threadIdx.x % R, j = threadIdx.x / R, nj = 256 / R;
+  double s = 0.0;
+  for (int bb = j; bb < nb; bb += nj) s += part[(int64_t)bb * R + r];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    double t = 0.0;
+    for (int q = 0; q < nj; ++q) t += lds[q * R + threadIdx.x];
+    out[threadIdx.x] = t;
   }
 }
+
+// LFRic W3 fields (field r at r*undf, dof = c*nl + k) -> [k][c][r], a tile of
+// TC cells per block staged through LDS (contiguous reads and writes).
 template <typename T>
-__global__ void k_from_layer_major(T* __restrict__ out, const T* __restrict__ in, int ncell, int nl, int nrhs) {
-  const int64_t n = (int64_t)ncell * nl * nrhs;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t per = (int64_t)ncell * nl;
-    const int64_t rhs = i / per, rem = i - rhs * per;
-    const int c = (int)(rem / nl), k = (int)(rem - (int64_t)c * nl);
-    out[i] = in[rhs * per + (int64_t)k * ncell + c];
+__global__ void __launch_bounds__(256) k_to_mg(T* __restrict__ out, const T* __restrict__ in, int ncell, int nl,
+                                               int R, int TC) {
+  extern __shared__ char smem_raw[];
+  T* tile = (T*)smem_raw;          // [R][TC*nl]
+  const int c0 = blockIdx.x * TC;
+  const int tc = min(TC, ncell - c0);
+  const int64_t undf = (int64_t)ncell * nl;
+  const int span = tc * nl;
+  for (int idx = threadIdx.x; idx < R * span; idx += blockDim.x) {
+    const int r = idx / span, j = idx - r * span;
+    tile[r * TC * nl + j] = in[(int64_t)r * undf + (int64_t)c0 * nl + j];
   }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < R * span; idx += blockDim.x) {
+    const int r = idx % R, rest = idx / R;
+    const int cc = rest % tc, k = rest / tc;
+    out[((int64_t)k * ncell + c0 + cc) * R + r] = tile[r * TC * nl + cc * nl + k];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_from_mg(T* __restrict__ out, const T* __restrict__ in, int ncell, int nl,
+                                                 int R, int TC) {
+  extern __shared__ char smem_raw[];
+  T* tile = (T*)smem_raw;
+  const int c0 = blockIdx.x * TC;
+  const int tc = min(TC, ncell - c0);
+  const int64_t undf = (int64_t)ncell * nl;
+  const int span = tc * nl;
+  for (int idx = threadIdx.x; idx < R * span; idx += blockDim.x) {
+    const int r = idx % R, rest = idx / R;
+    const int cc = rest % tc, k = rest / tc;
+    tile[r * TC * nl + cc * nl + k] = in[((int64_t)k * ncell + c0 + cc) * R + r];
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < R * span; idx += blockDim.x) {
+    const int r = idx / span, j = idx - r * span;
+    out[(int64_t)r * undf + (int64_t)c0 * nl + j] = tile[r * TC * nl + j];
+  }
+}
+
+int lg2(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
 }
 
 template <typename T>
 void launch_smooth(hipStream_t s, lfr_mg* mg, lfr_mg_level& L, T* xo, const T* xi) {
   const int bs = L.block_smooth;
-  dim3 grid((L.ncell + bs - 1) / bs, mg->nrhs);
-  const T sig = (T)L.sigma, om = (T)mg->omega;
-  if (mg->nl == 70)
-    hipLaunchKernelGGL((k_mg_smooth<T, 70>), grid, dim3(bs), 0, s, xo, xi, (const T*)L.b_d,
-                       (const T*)L.a_d, (const T*)L.d_d, (const T*)L.u_d, L.cell_next_d, L.ncell, 70, sig, om);
-  else if (mg->nl == 10)
-    hipLaunchKernelGGL((k_mg_smooth<T, 10>), grid, dim3(bs), 0, s, xo, xi, (const T*)L.b_d,
-                       (const T*)L.a_d, (const T*)L.d_d, (const T*)L.u_d, L.cell_next_d, L.ncell, 10, sig, om);
-  else
-    hipLaunchKernelGGL((k_mg_smooth<T, 0>), grid, dim3(bs), 0, s, xo, xi, (const T*)L.b_d,
-                       (const T*)L.a_d, (const T*)L.d_d, (const T*)L.u_d, L.cell_next_d, L.ncell, mg->nl, sig, om);
+  const int64_t n = (int64_t)L.ncell * mg->nrhs;
+  hipLaunchKernelGGL(k_mg_smooth<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, xo, xi, (const T*)L.b_d,
+                     (const T*)L.a_d, (const T*)L.d_d, (const T*)L.u_d, (const T*)L.cp_d, (const T*)L.iv_d,
+                     L.cell_next_d, L.ncell, mg->nl, lg2(mg->nrhs), (T)L.sigma, (T)mg->omega);
 }
 
 // two sweeps: x -> t -> x
@@ -261,19 +324,20 @@ void vcycle(hipStream_t s, lfr_mg* mg, int l) {
   }
   for (int i = 0; i < mg->nu1 / 2; ++i) smooth2<T>(s, mg, L);
   lfr_mg_level& C = mg->lev[l + 1];
+  const int lgR = lg2(mg->nrhs);
   {
     const int bs = L.block_other;
-    dim3 grid((C.ncell + bs - 1) / bs, mg->nrhs);
-    hipLaunchKernelGGL(k_mg_restrict<T>, grid, dim3(bs), 0, s, (T*)C.b_d, (T*)C.x_d, (const T*)L.x_d,
-                       (const T*)L.b_d, (const T*)L.a_d, (const T*)L.d_d, (const T*)L.u_d,
-                       L.cell_next_d, C.children_d, C.ncell, L.ncell, mg->nl, (T)L.sigma);
+    const int64_t n = (int64_t)C.ncell * mg->nrhs;
+    hipLaunchKernelGGL(k_mg_restrict<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, (T*)C.b_d,
+                       (T*)C.x_d, (const T*)L.x_d, (const T*)L.b_d, (const T*)L.a_d, (const T*)L.d_d,
+                       (const T*)L.u_d, L.cell_next_d, C.children_d, C.ncell, L.ncell, mg->nl, lgR, (T)L.sigma);
   }
   vcycle<T>(s, mg, l + 1);
   {
     const int bs = L.block_other;
-    dim3 grid((L.ncell + bs - 1) / bs, mg->nrhs);
-    hipLaunchKernelGGL(k_mg_prolong<T>, grid, dim3(bs), 0, s, (T*)L.x_d, (const T*)C.x_d, L.parent_d,
-                       L.ncell, C.ncell, mg->nl);
+    const int64_t n = (int64_t)L.ncell * mg->nrhs;
+    hipLaunchKernelGGL(k_mg_prolong<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, (T*)L.x_d,
+                       (const T*)C.x_d, L.parent_d, L.ncell, C.ncell, mg->nl, lgR);
   }
   for (int i = 0; i < mg->nu2 / 2; ++i) smooth2<T>(s, mg, L);
 }
@@ -285,7 +349,7 @@ void mg_free(lfr_mg* mg) {
   if (mg->graph) (void)hipGraphExecDestroy(mg->graph);
   for (auto& L : mg->lev)
     for (void* p : {(void*)L.cell_next_d, (void*)L.children_d, (void*)L.parent_d, L.a_d, L.d_d, L.u_d,
-                    L.x_d, L.b_d, L.t_d})
+                    L.cp_d, L.iv_d, L.x_d, L.b_d, L.t_d})
       if (p) (void)hipFree(p);
   if (mg->part_d) (void)hipFree(mg->part_d);
   delete mg;
@@ -299,7 +363,7 @@ int autotune(lfr_mg* mg) {
   hipEvent_t e0, e1;
   LFR_HIP(ctx, hipEventCreate(&e0));
   LFR_HIP(ctx, hipEventCreate(&e1));
-  const int cand[3] = {64, 128, 256};
+  const int cand[3] = {64, 128, 256};   // launch bound of the smoother is 256
   for (auto& L : mg->lev) {
     float best = 1e30f;
     int bestb = 256;
@@ -336,8 +400,9 @@ int lfr_mg_create(lfr_ctx* ctx, int prec, int nlev, int nlayers, int nrhs, const
   if (!ctx || !out) return LFR_ERR_ARG;
   *out = nullptr;
   LFR_REQUIRE(ctx, (prec == 4 || prec == 8) && nlev >= 1 && nlayers >= 2 && nlayers <= kMaxNl &&
-                       nrhs >= 1 && nrhs <= 64,
-              "lfr_mg_create: bad sizes (prec %d nlev %d nlayers %d nrhs %d)", prec, nlev, nlayers, nrhs);
+                       nrhs >= 1 && nrhs <= kMaxRhs && (nrhs & (nrhs - 1)) == 0,
+              "lfr_mg_create: bad sizes (prec %d nlev %d nlayers %d nrhs %d; nrhs a power of 2 <= %d)", prec,
+              nlev, nlayers, nrhs, kMaxRhs);
   LFR_REQUIRE(ctx, nu1 % 2 == 0 && nu2 % 2 == 0 && ncoarse % 2 == 0 && ncoarse > 0,
               "lfr_mg_create: smoothing counts must be even (ping-pong sweeps)");
   lfr_mg* mg = new lfr_mg();
@@ -366,7 +431,8 @@ int lfr_mg_create(lfr_ctx* ctx, int prec, int nlev, int nlayers, int nrhs, const
     const size_t nb = (size_t)nc * nlayers;
     bool ok = hipMalloc(&L.cell_next_d, (size_t)nc * 4 * 4) == hipSuccess &&
               hipMalloc(&L.a_d, nb * s) == hipSuccess && hipMalloc(&L.d_d, nb * s) == hipSuccess &&
-              hipMalloc(&L.u_d, nb * s) == hipSuccess &&
+              hipMalloc(&L.u_d, nb * s) == hipSuccess && hipMalloc(&L.cp_d, nb * s) == hipSuccess &&
+              hipMalloc(&L.iv_d, nb * s) == hipSuccess &&
               hipMalloc(&L.x_d, nb * nrhs * s) == hipSuccess &&
               hipMalloc(&L.b_d, nb * nrhs * s) == hipSuccess &&
               hipMalloc(&L.t_d, nb * nrhs * s) == hipSuccess;
@@ -385,6 +451,21 @@ int lfr_mg_create(lfr_ctx* ctx, int prec, int nlev, int nlayers, int nrhs, const
         memcpy(buf.data(), src, nb * s);
       void* dst = band == 0 ? L.a_d : (band == 1 ? L.d_d : L.u_d);
       (void)hipMemcpy(dst, buf.data(), nb * s, hipMemcpyHostToDevice);
+    }
+    {
+      const unsigned g = (unsigned)((nc + 255) / 256);
+      if (prec == 4)
+        hipLaunchKernelGGL(k_mg_factor<float>, dim3(g), dim3(256), 0, ctx->stream, (float*)L.cp_d, (float*)L.iv_d,
+                           (const float*)L.a_d, (const float*)L.d_d, (const float*)L.u_d, nc, nlayers,
+                           (float)L.sigma);
+      else
+        hipLaunchKernelGGL(k_mg_factor<double>, dim3(g), dim3(256), 0, ctx->stream, (double*)L.cp_d,
+                           (double*)L.iv_d, (const double*)L.a_d, (const double*)L.d_d, (const double*)L.u_d, nc,
+                           nlayers, L.sigma);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        mg_free(mg);
+        return lfr_set_error(ctx, LFR_ERR_HIP, "lfr_mg_create: factor kernel failed");
+      }
     }
     (void)hipMemset(L.x_d, 0, nb * nrhs * s);
     (void)hipMemset(L.b_d, 0, nb * nrhs * s);
@@ -446,16 +527,21 @@ int lfr_mg_solve(lfr_mg* mg, void* x_d, const void* b_d, int ncycles, double* re
   lfr_ctx* ctx = mg->ctx;
   LFR_REQUIRE(ctx, x_d && b_d && ncycles >= 0, "lfr_mg_solve: bad arguments");
   lfr_mg_level& F = mg->lev[0];
-  const int64_t n = (int64_t)F.ncell * mg->nl * mg->nrhs;
-  const unsigned g = lfr_blocks(n, 256, 4096);
   hipStream_t s = ctx->stream;
   const bool f32 = mg->prec == 4;
+  const int R = mg->nrhs, lgR = lg2(R);
+  const size_t es = f32 ? 4 : 8;
+  int TC = (int)(32768 / ((size_t)mg->nl * R * es));
+  if (TC > 16) TC = 16;
+  if (TC < 1) TC = 1;
+  const size_t smem = (size_t)TC * mg->nl * R * es;
+  const unsigned gt = (unsigned)((F.ncell + TC - 1) / TC);
   if (f32) {
-    hipLaunchKernelGGL(k_to_layer_major<float>, dim3(g), dim3(256), 0, s, (float*)F.b_d, (const float*)b_d, F.ncell, mg->nl, mg->nrhs);
-    hipLaunchKernelGGL(k_to_layer_major<float>, dim3(g), dim3(256), 0, s, (float*)F.x_d, (const float*)x_d, F.ncell, mg->nl, mg->nrhs);
+    hipLaunchKernelGGL(k_to_mg<float>, dim3(gt), dim3(256), smem, s, (float*)F.b_d, (const float*)b_d, F.ncell, mg->nl, R, TC);
+    hipLaunchKernelGGL(k_to_mg<float>, dim3(gt), dim3(256), smem, s, (float*)F.x_d, (const float*)x_d, F.ncell, mg->nl, R, TC);
   } else {
-    hipLaunchKernelGGL(k_to_layer_major<double>, dim3(g), dim3(256), 0, s, (double*)F.b_d, (const double*)b_d, F.ncell, mg->nl, mg->nrhs);
-    hipLaunchKernelGGL(k_to_layer_major<double>, dim3(g), dim3(256), 0, s, (double*)F.x_d, (const double*)x_d, F.ncell, mg->nl, mg->nrhs);
+    hipLaunchKernelGGL(k_to_mg<double>, dim3(gt), dim3(256), smem, s, (double*)F.b_d, (const double*)b_d, F.ncell, mg->nl, R, TC);
+    hipLaunchKernelGGL(k_to_mg<double>, dim3(gt), dim3(256), smem, s, (double*)F.x_d, (const double*)x_d, F.ncell, mg->nl, R, TC);
   }
   LFR_CHECK_LAUNCH(ctx);
   if (!mg->graph) {
@@ -472,24 +558,22 @@ int lfr_mg_solve(lfr_mg* mg, void* x_d, const void* b_d, int ncycles, double* re
     }
     (void)hipGetLastError();
   }
-  const int nblk = lfr::red_blocks((int64_t)F.ncell * mg->nl);
+  const int nblk = lfr::red_blocks((int64_t)F.ncell * mg->nl * R);
   auto resnorm = [&](double* dst) -> int {
-    dim3 grid(nblk, mg->nrhs);
     if (f32)
-      hipLaunchKernelGGL(k_mg_resnorm<float>, grid, dim3(lfr::kRedThreads), 0, s, (const float*)F.x_d,
+      hipLaunchKernelGGL(k_mg_resnorm<float>, dim3(nblk), dim3(lfr::kRedThreads), 0, s, (const float*)F.x_d,
                          (const float*)F.b_d, (const float*)F.a_d, (const float*)F.d_d, (const float*)F.u_d,
-                         F.cell_next_d, F.ncell, mg->nl, (float)F.sigma, mg->part_d, nblk);
+                         F.cell_next_d, F.ncell, mg->nl, lgR, (float)F.sigma, mg->part_d);
     else
-      hipLaunchKernelGGL(k_mg_resnorm<double>, grid, dim3(lfr::kRedThreads), 0, s, (const double*)F.x_d,
+      hipLaunchKernelGGL(k_mg_resnorm<double>, dim3(nblk), dim3(lfr::kRedThreads), 0, s, (const double*)F.x_d,
                          (const double*)F.b_d, (const double*)F.a_d, (const double*)F.d_d, (const double*)F.u_d,
-                         F.cell_next_d, F.ncell, mg->nl, F.sigma, mg->part_d, nblk);
-    double* sums = mg->part_d + (size_t)mg->nrhs * lfr::kRedMaxBlocks;
-    hipLaunchKernelGGL(lfr::k_reduce_final, dim3(mg->nrhs), dim3(lfr::kRedThreads), 0, s, mg->part_d,
-                       nblk, nblk, sums);
+                         F.cell_next_d, F.ncell, mg->nl, lgR, F.sigma, mg->part_d);
+    double* sums = mg->part_d + (size_t)R * lfr::kRedMaxBlocks;
+    hipLaunchKernelGGL(k_mg_resnorm_final, dim3(1), dim3(256), 0, s, mg->part_d, nblk, R, sums);
     LFR_CHECK_LAUNCH(ctx);
-    LFR_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, sums, mg->nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
+    LFR_HIP(ctx, hipMemcpyAsync(ctx->host_scalars, sums, R * sizeof(double), hipMemcpyDeviceToHost, s));
     LFR_HIP(ctx, hipStreamSynchronize(s));
-    for (int r = 0; r < mg->nrhs; ++r) dst[r] = sqrt(ctx->host_scalars[r]);
+    for (int r = 0; r < R; ++r) dst[r] = sqrt(ctx->host_scalars[r]);
     return LFR_OK;
   };
   if (res_h) {
@@ -512,9 +596,9 @@ int lfr_mg_solve(lfr_mg* mg, void* x_d, const void* b_d, int ncycles, double* re
     }
   }
   if (f32)
-    hipLaunchKernelGGL(k_from_layer_major<float>, dim3(g), dim3(256), 0, s, (float*)x_d, (const float*)F.x_d, F.ncell, mg->nl, mg->nrhs);
+    hipLaunchKernelGGL(k_from_mg<float>, dim3(gt), dim3(256), smem, s, (float*)x_d, (const float*)F.x_d, F.ncell, mg->nl, R, TC);
   else
-    hipLaunchKernelGGL(k_from_layer_major<double>, dim3(g), dim3(256), 0, s, (double*)x_d, (const double*)F.x_d, F.ncell, mg->nl, mg->nrhs);
+    hipLaunchKernelGGL(k_from_mg<double>, dim3(gt), dim3(256), smem, s, (double*)x_d, (const double*)F.x_d, F.ncell, mg->nl, R, TC);
   LFR_CHECK_LAUNCH(ctx);
   return LFR_OK;
 }

@@ -418,6 +418,13 @@ int lfr_multi_info(const lfr_multi* h, int64_t* packed_bytes, int* slots, int* v
   return LFR_OK;
 }
 
+int lfr_multi_space_undf(const lfr_multi* h, int* undf_to, int* undf_from) {
+  if (!h) return LFR_ERR_ARG;
+  *undf_to = h->to->undf;
+  *undf_from = h->from->undf;
+  return LFR_OK;
+}
+
 int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void* x_d, int nrhs, int overwrite) {
   if (!ctx || !h) return LFR_ERR_ARG;
   LFR_REQUIRE(ctx, nrhs == kRhs, "lfr_matrix_vector_multi: nrhs must be %d (got %d)", kRhs, nrhs);

@@ -62,3 +62,57 @@ class Stream:
         z[:, 0] = rad * np.cos(ang)
         z[:, 1] = rad * np.sin(ang)
         return mean + std * z.reshape(-1)[:n]
+
+
+_I64 = lambda u: u - (1 << 64) if u >= (1 << 63) else u
+
+
+class TorchStream:
+    """The same SplitMix64 stream and transforms as ``Stream``, evaluated on a
+    torch device (int64 arithmetic wraps like uint64; logical shifts are
+    masked arithmetic shifts).  Raw draws are bitwise those of ``Stream``;
+    uniforms are exact; normals may differ from numpy's in the last bit of
+    log/cos/sin."""
+
+    def __init__(self, seed: int, device, chunk: int = 1 << 26):
+        self.seed = int(seed)
+        self.pos = 0
+        self.device = device
+        self.chunk = chunk
+
+    @staticmethod
+    def _srl(z, s: int):
+        import torch
+        return torch.bitwise_and(z >> s, (1 << (64 - s)) - 1)
+
+    def raw(self, n: int):
+        import torch
+        out = torch.empty(n, dtype=torch.int64, device=self.device)
+        g, m1, m2 = _I64(0x9E3779B97F4A7C15), _I64(0xBF58476D1CE4E5B9), _I64(0x94D049BB133111EB)
+        seed = _I64(self.seed & 0xFFFFFFFFFFFFFFFF)
+        for s in range(0, n, self.chunk):
+            m = min(self.chunk, n - s)
+            z = torch.arange(self.pos + s + 1, self.pos + s + m + 1, dtype=torch.int64,
+                             device=self.device)
+            z = z * g + seed
+            z = (z ^ self._srl(z, 30)) * m1
+            z = (z ^ self._srl(z, 27)) * m2
+            out[s:s + m] = z ^ self._srl(z, 31)
+        self.pos += n
+        return out
+
+    def uniform01(self, n: int):
+        import torch
+        return self._srl(self.raw(n), 11).to(torch.float64) * (1.0 / 9007199254740992.0)
+
+    def uniform(self, n: int, lo: float = -1.0, hi: float = 1.0):
+        return lo + (hi - lo) * self.uniform01(n)
+
+    def normal(self, n: int, mean: float = 0.0, std: float = 1.0):
+        import torch
+        m = (n + 1) // 2
+        u = self.uniform01(2 * m).reshape(m, 2)
+        rad = torch.sqrt(-2.0 * torch.log1p(-u[:, 0]))
+        ang = 2.0 * np.pi * u[:, 1]
+        z = torch.stack([rad * torch.cos(ang), rad * torch.sin(ang)], dim=1)
+        return mean + std * z.reshape(-1)[:n]

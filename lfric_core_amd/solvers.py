@@ -13,6 +13,8 @@ from __future__ import annotations
 import ctypes as C
 from dataclasses import dataclass
 
+import numpy as np
+
 import torch
 
 from . import api
@@ -52,6 +54,9 @@ def _sig():
     L.lfr_refined_cg.restype = C.c_int
     L.lfr_gmres.argtypes = L.lfr_cg.argtypes
     L.lfr_gmres.restype = C.c_int
+    L.lfr_refined_cg_multi.argtypes = [_P, _P, C.POINTER(LmaOp), C.c_double, C.c_int, C.c_double,
+                                       C.c_int, _P, _P, C.c_int, C.POINTER(RefineStats), _P]
+    L.lfr_refined_cg_multi.restype = C.c_int
     return L
 
 
@@ -134,6 +139,38 @@ class RefinedCG:
                                C.byref(self.inner), _P(x64.data_ptr()), _P(b64.data_ptr()),
                                C.byref(st)), ctx.h)
         self.stats = st
+        return st
+
+
+class RefinedCGMulti:
+    """Sixteen simultaneous refined solves: FP32 CG on the MFMA multi-field
+    operator inside an FP64 residual loop (lfr_refined_cg_multi)."""
+
+    def __init__(self, mop: "api.MultiOperator", op64: "api.LMAOperator", r_tol: float = 1e-10,
+                 max_outer: int = 20, inner_r_tol: float = 1e-5, inner_max_iter: int = 500):
+        self.mop, self.op64 = mop, op64
+        self.r_tol, self.max_outer = r_tol, max_outer
+        self.inner_r_tol, self.inner_max_iter = inner_r_tol, inner_max_iter
+        self.stats = None
+        self.rel_res = None
+
+    def apply(self, x64: torch.Tensor, b64: torch.Tensor) -> RefineStats:
+        shape = (self.op64.to_space.undf, 16)
+        for t, nm in ((x64, "x"), (b64, "b")):
+            if tuple(t.shape) != shape or t.dtype != torch.float64 or not t.is_contiguous() \
+                    or t.device != self.op64.ctx.device:
+                raise LfrError(1, f"{nm} must be a contiguous float64 tensor of shape {shape}")
+        L = _sig()
+        ctx = self.op64.ctx
+        ctx.sync_stream()
+        A64 = lma_op(self.op64)
+        st = RefineStats()
+        rel = np.zeros(16)
+        check(L.lfr_refined_cg_multi(ctx.h, self.mop.h, C.byref(A64), self.r_tol, self.max_outer,
+                                     self.inner_r_tol, self.inner_max_iter, _P(x64.data_ptr()),
+                                     _P(b64.data_ptr()), 16, C.byref(st),
+                                     rel.ctypes.data_as(_P)), ctx.h)
+        self.stats, self.rel_res = st, rel
         return st
 
 

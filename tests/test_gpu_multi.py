@@ -99,3 +99,28 @@ def test_multi_info_and_errors(ctx):
         mop.apply(torch.zeros(fs_to.undf, 8, device=ctx.device),
                   torch.zeros(fs_to.undf, 16, device=ctx.device))
     mop.close()
+
+
+def test_refined_cg_multi_16_fields(ctx):
+    """16 simultaneous refined solves to 1e-10; residuals re-measured with the
+    oracle's FP64 coloured apply on the CPU."""
+    from lfric_core_amd import api
+    from lfric_core_amd.solvers import RefinedCGMulti
+    nl = 20
+    mesh, fs, _, col, M, X = _problem("W2", 1, "W2", 1, "spd", nl)
+    s = api.Space(ctx, fs)
+    M64 = torch.from_numpy(M).to(ctx.device)
+    op64 = api.LMAOperator(M64, s, s, col)
+    op32 = api.LMAOperator(M64.float().contiguous(), s, s, col)
+    mop = api.MultiOperator(op32, 4)
+    b = torch.from_numpy(X).to(ctx.device)
+    x = torch.zeros_like(b)
+    st = RefinedCGMulti(mop, op64, r_tol=1e-10).apply(x, b)
+    assert st.converged and st.r_nrm <= 1e-10 and st.outer >= 1
+    xs = x.cpu().numpy()
+    for r in range(16):
+        Ax = oracle.invoke_coloured(0, col, nl, np.zeros(fs.undf), np.ascontiguousarray(xs[:, r]), M,
+                                    fs.dofmap, fs.dofmap)
+        rel = np.linalg.norm(X[:, r] - Ax) / np.linalg.norm(X[:, r])
+        assert rel <= 1.5e-10, (r, rel)
+    mop.close()

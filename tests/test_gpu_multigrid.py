@@ -38,7 +38,7 @@ def _run(ctx, H, prec, nrhs, ncycles, seed=11, **kw):
 def test_vcycle_bit_exact(ctx, prec, nl):
     H = build_hierarchy((16, 8, 4), nlayers=nl)
     print(f"mg prec={prec} nl={nl}", flush=True)
-    x, res, xo, reso = _run(ctx, H, prec, nrhs=3, ncycles=3)
+    x, res, xo, reso = _run(ctx, H, prec, nrhs=4, ncycles=3)
     np.testing.assert_array_equal(x, xo)
     np.testing.assert_allclose(res, reso, rtol=1e-12 if prec == 8 else 1e-5)
 
@@ -82,6 +82,8 @@ def test_bad_arguments(ctx):
         mg.solve(torch.zeros(2, H.fine_undf, dtype=torch.float64, device=ctx.device),
                  torch.zeros(2, H.fine_undf, dtype=torch.float64, device=ctx.device))
     mg.close()
+    with pytest.raises(LfrError):
+        Multigrid(ctx, H, prec=8, nrhs=3)              # right-hand sides: a power of 2
     H.levels[1].children[0, 0] = H.levels[1].children[0, 1]
     with pytest.raises(LfrError):
         Multigrid(ctx, H, prec=8, nrhs=1)

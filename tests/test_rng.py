@@ -27,3 +27,13 @@ def test_uniform_and_normal_moments():
     z = Stream(4).normal(200_001)
     assert z.size == 200_001
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1.0) < 0.01
+
+
+def test_torch_stream_matches_numpy_stream():
+    import torch
+    from lfric_core_amd.rng import TorchStream
+    a, b = Stream(11), TorchStream(11, "cpu", chunk=333)
+    assert np.array_equal(a.raw(1000), b.raw(1000).numpy().view(np.uint64))
+    assert np.array_equal(a.uniform(501, -2.0, 3.0), b.uniform(501, -2.0, 3.0).numpy())
+    np.testing.assert_allclose(b.normal(999).numpy(), a.normal(999), rtol=0, atol=1e-14)
+    assert a.pos == b.pos
