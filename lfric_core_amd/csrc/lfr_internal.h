@@ -65,7 +65,7 @@ const int32_t* lfr_plan_records(lfr_ctx* ctx, const lfr_plan* plan, size_t ci, c
                                 int* stride);
 bool lfr_record_shape(int ndf_in, int ngroup, int gsize);
 bool lfr_records_enabled();
-bool lfr_gather_nt();
+bool lfr_gather_nt(int prec);
 
 int lfr_set_error(lfr_ctx* ctx, int code, const char* fmt, ...);
 

@@ -34,14 +34,17 @@ unsigned pad8(unsigned nb) { return (nb + 7u) & ~7u; }
 
 }  // namespace
 
-// Non-temporal (streaming) loads of the matrix in the record kernels;
-// LFR_GATHER_NT=0 uses ordinary loads.
-bool lfr_gather_nt() {
-  static const bool on = [] {
+// Non-temporal (streaming) loads of the matrix in the record kernels.
+// Default: on for FP64 matrices, off for FP32 (measured faster each way on
+// the W2 C96L70 apply); LFR_GATHER_NT=0 or =1 forces one setting for both.
+bool lfr_gather_nt(int prec) {
+  static const int mode = [] {
     const char* e = getenv("LFR_GATHER_NT");
-    return !(e && e[0] == '0');
+    if (e && e[0] == '0') return 0;
+    if (e && e[0] == '1') return 1;
+    return -1;
   }();
-  return on;
+  return mode < 0 ? prec == 8 : mode == 1;
 }
 
 // LFR_NO_RECORDS=1 switches the record kernels off (plan-entry kernels only).
@@ -112,7 +115,7 @@ void launch_rec_t(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, 
                   const int32_t* rec, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
                   int overwrite) {
   const unsigned nbp = pad8(nb);
-  const bool nt = lfr_gather_nt();
+  const bool nt = lfr_gather_nt((int)sizeof(T));
 #define LREC(KD, NTV)                                                                             \
   hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, KD, false, NTV>), dim3(nbp), dim3(kThreads), \
                      0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, overwrite, nbp,       \

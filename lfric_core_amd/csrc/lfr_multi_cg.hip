@@ -131,19 +131,41 @@ k_mcg_dir(float* __restrict__ p, const float* __restrict__ res, int64_t n16, con
     p[i] = res[i] + bt * p[i];
 }
 
-// field r of an interleaved FP64 array -> contiguous
-__global__ void k_extract(double* __restrict__ out, const double* __restrict__ in, int64_t n, int r) {
-  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < n; d += (int64_t)gridDim.x * blockDim.x)
-    out[d] = in[d * kR + r];
+// Interleaved (n, 16) -> field-major (16, n), 64 dofs per block through LDS.
+constexpr int kTile = 64;
+__global__ void __launch_bounds__(256) k_split16(double* __restrict__ out, const double* __restrict__ in, int64_t n) {
+  __shared__ double tile[kR][kTile + 1];
+  const int64_t d0 = (int64_t)blockIdx.x * kTile;
+  for (int i = threadIdx.x; i < kTile * kR; i += 256) {
+    const int dd = i >> 4, f = i & 15;
+    if (d0 + dd < n) tile[f][dd] = in[(d0 + dd) * kR + f];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kTile * kR; i += 256) {
+    const int f = i / kTile, dd = i % kTile;
+    if (d0 + dd < n) out[(int64_t)f * n + d0 + dd] = tile[f][dd];
+  }
 }
 
-// res64[d*16+r] = b[d*16+r] - y[d]; res32 = (float)res64
-__global__ void k_residual_field(double* __restrict__ res64, float* __restrict__ res32, const double* __restrict__ b,
-                                 const double* __restrict__ y, int64_t n, int r) {
-  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < n; d += (int64_t)gridDim.x * blockDim.x) {
-    const double v = b[d * kR + r] - y[d];
-    res64[d * kR + r] = v;
-    res32[d * kR + r] = (float)v;
+// res64[d*16+f] = b[d*16+f] - Y[f*n + d]; res32 = (float)res64
+__global__ void __launch_bounds__(256) k_residual16(double* __restrict__ res64, float* __restrict__ res32,
+                                                    const double* __restrict__ b, const double* __restrict__ Y,
+                                                    int64_t n) {
+  __shared__ double tile[kR][kTile + 1];
+  const int64_t d0 = (int64_t)blockIdx.x * kTile;
+  for (int i = threadIdx.x; i < kTile * kR; i += 256) {
+    const int f = i / kTile, dd = i % kTile;
+    if (d0 + dd < n) tile[f][dd] = Y[(int64_t)f * n + d0 + dd];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kTile * kR; i += 256) {
+    const int dd = i >> 4, f = i & 15;
+    if (d0 + dd < n) {
+      const int64_t j = (d0 + dd) * kR + f;
+      const double v = b[j] - tile[f][dd];
+      res64[j] = v;
+      res32[j] = (float)v;
+    }
   }
 }
 
@@ -183,8 +205,8 @@ extern "C" int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_
   const int64_t n = ut, n16 = n * kR;
   const int nb = lfr::red_blocks(n16);
   // workspace: 4 FP32 multidata fields (d, res, p, z), FP64 residual field,
-  // one FP64 single field (y), one FP64 extracted field, partials, scalars
-  const size_t f32 = up(n16 * 4), f64 = up(n16 * 8), s64 = up(n * 8);
+  // x and A x of the 16 fields field-major (FP64), partials, scalars
+  const size_t f32 = up(n16 * 4), f64 = up(n16 * 8), s64 = up(n16 * 8);
   const size_t need = 4 * f32 + f64 + 2 * s64 + up((size_t)nb * kR * 8) + up(128 * 8) + up(2 * kR * 8);
   int rc = lfr_work_reserve(ctx, need);
   if (rc) return rc;
@@ -194,13 +216,14 @@ extern "C" int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_
   float* p = (float*)(w + 2 * f32);
   float* z = (float*)(w + 3 * f32);
   double* r64 = (double*)(w + 4 * f32);
-  double* y1 = (double*)(w + 4 * f32 + f64);
-  double* x1 = (double*)(w + 4 * f32 + f64 + s64);
+  double* Y16 = (double*)(w + 4 * f32 + f64);
+  double* X16 = (double*)(w + 4 * f32 + f64 + s64);
   double* part = (double*)(w + 4 * f32 + f64 + 2 * s64);
   double* S = (double*)(w + 4 * f32 + f64 + 2 * s64 + up((size_t)nb * kR * 8));
   double* tmp = S + 128;
   hipStream_t s = ctx->stream;
-  const unsigned g = lfr_blocks(n16, 256, 8192), g1 = lfr_blocks(n, 256, 8192);
+  const unsigned g = lfr_blocks(n16, 256, 8192);
+  const unsigned gt = (unsigned)((n + kTile - 1) / kTile);
   double* x64 = (double*)x64_d;
   const double* b64 = (const double*)b64_d;
   double* Sh = ctx->host_scalars;   // 64 doubles
@@ -216,14 +239,15 @@ extern "C" int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_
   int outer = 0, inner_total = 0, conv = 0;
   double rel[kR];
   for (;;) {
-    // FP64 residual per field
+    // FP64 residual: fields to field-major, one FP64 apply per field, back
+    hipLaunchKernelGGL(k_split16, dim3(gt), dim3(256), 0, s, X16, (const double*)x64, n);
+    LFR_CHECK_LAUNCH(ctx);
     for (int r = 0; r < kR; ++r) {
-      hipLaunchKernelGGL(k_extract, dim3(g1), dim3(256), 0, s, x1, (const double*)x64, n, r);
-      LFR_CHECK_LAUNCH(ctx);
-      rc = lfr_matrix_vector(ctx, 8, y1, x1, A64->matrix_d, A64->space, A64->space, A64->plan, 1);
+      rc = lfr_matrix_vector(ctx, 8, Y16 + (size_t)r * n, X16 + (size_t)r * n, A64->matrix_d, A64->space,
+                             A64->space, A64->plan, 1);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_residual_field, dim3(g1), dim3(256), 0, s, r64, res, b64, (const double*)y1, n, r);
     }
+    hipLaunchKernelGGL(k_residual16, dim3(gt), dim3(256), 0, s, r64, res, b64, (const double*)Y16, n);
     hipLaunchKernelGGL((k_mdot<double, double>), dim3(nb), dim3(256), 0, s, (const double*)r64,
                        (const double*)r64, n16, part);
     hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(256), 0, s, part, nb, tmp);

@@ -275,7 +275,7 @@ void launch_apply_rec_t(hipStream_t s, const lfr_plan_class& c, const int32_t* r
                         double* part, const double* stop) {
   const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
   const unsigned nbp = (nb + 7u) & ~7u;
-  const bool nt = lfr_gather_nt();
+  const bool nt = lfr_gather_nt((int)sizeof(T));
   if (dotv) {
     if (nt)
       hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, 0, true, true>), dim3(nbp), dim3(kThreads),
