@@ -303,6 +303,35 @@ int lfr_mg_block_sizes(const lfr_mg* mg, int* smooth_blocks);
  * (res_h[0..nrhs)) and after every cycle (res_h[(c+1)*nrhs + r]). */
 int lfr_mg_solve(lfr_mg* mg, void* x_d, const void* b_d, int ncycles, double* res_h);
 
+/* Banded column-operator geometry (columnwise_operator_type's alpha, beta,
+ * gamma_m, gamma_p, bandwidth, nrow, ncol and its column_banded_dofmap /
+ * indirection_dofmap arrays, infrastructure/source/operator/
+ * columnwise_operator_mod.f90:168-191,350-396), uploaded once.  banded_*_h are
+ * column_banded_dofmap(ndf, nlayers) in Fortran order, ind_*_h
+ * indirection_dofmap(nrow|ncol); all 1-based.  Every (row, column) entry the
+ * maps produce must lie inside the band. */
+typedef struct lfr_cma lfr_cma;
+int lfr_cma_create(lfr_ctx* ctx, int nlayers, int nrow, int ncol, int bandwidth, int alpha,
+                   int beta, int gamma_m, int gamma_p, int ndf_to, const int32_t* banded_to_h,
+                   const int32_t* ind_to_h, int ndf_from, const int32_t* banded_from_h,
+                   const int32_t* ind_from_h, lfr_cma** cma);
+int lfr_cma_destroy(lfr_cma* cma);
+
+/* invoke(columnwise_op_asm_kernel_type(cma_op, lma_op)) over all columns:
+ * columnwise_matrix(bandwidth, nrow, ncell_2d) assembled from
+ * local_stencil(ncell_3d, ndf_to, ndf_from) (components/science/source/kernel/
+ * columnwise_operators/sci_columnwise_op_asm_kernel_mod.F90:72-128), bitwise in
+ * the reference's summation order. */
+int lfr_columnwise_op_asm(lfr_ctx* ctx, int prec, const lfr_cma* cma, const void* local_stencil_d,
+                          void* cm_d, int ncell_2d);
+
+/* invoke(columnwise_op_app_kernel_type(lhs, x, cma_op)) over all columns:
+ * lhs += A x with A banded (components/science/source/kernel/
+ * columnwise_operators/sci_columnwise_op_app_kernel_mod.F90:73-124); the band
+ * limits are formed in the operator's real kind as there. */
+int lfr_columnwise_op_app(lfr_ctx* ctx, int prec, const lfr_cma* cma, void* lhs_d, const void* x_d,
+                          const void* cm_d, const lfr_space* to_space, const lfr_space* from_space);
+
 /* ---- multi-right-hand-side LMA apply on MFMA -----------------------------
  * 16 fields applied together: y_r (+)= A x_r.  Fields are multidata fields in
  * LFRic's "ndata first" layout with ndata = 16 (element (dof d, rhs r) at

@@ -69,6 +69,10 @@ SIGNATURES = {
     "lfr_mg_destroy": [_P],
     "lfr_mg_block_sizes": [_P, _P],
     "lfr_mg_solve": [_P, _P, _P, _I, _P],
+    "lfr_cma_create": [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, C.POINTER(_P)],
+    "lfr_cma_destroy": [_P],
+    "lfr_columnwise_op_asm": [_P, _I, _P, _P, _P, _I],
+    "lfr_columnwise_op_app": [_P, _I, _P, _P, _P, _P, _P, _P],
     "lfr_multi_create": [_P, _I, _I, _P, _P, _P, _P, _P, _I, _I, C.POINTER(_P)],
     "lfr_multi_destroy": [_P],
     "lfr_multi_info": [_P, _P, _P, _P, _P],

@@ -399,3 +399,65 @@ def columnwise_op_appinv(lhs: torch.Tensor, x: torch.Tensor, op: CMAOperator,
                                          op.space.fs.ncells, op.space.h,
                                          _P(op.indirection.data_ptr()), int(fp64_pivots)), ctx.h)
     return lhs
+
+
+class ColumnwiseGeometry:
+    """Banded column-operator geometry between two horizontally discontinuous
+    spaces (lfr_cma_create), from lfric_core_amd.cma.cma_geometry."""
+
+    def __init__(self, to_space: Space, from_space: Space, geom=None):
+        from .cma import cma_geometry
+        g = geom if geom is not None else cma_geometry(to_space.fs, from_space.fs)
+        self.geom, self.to_space, self.from_space = g, to_space, from_space
+        bt = np.ascontiguousarray(g.banded_to, dtype=np.int32)       # (nl, ndf) = Fortran (ndf, nl)
+        bf = np.ascontiguousarray(g.banded_from, dtype=np.int32)
+        it = np.ascontiguousarray(g.indirection_to, dtype=np.int32)
+        if_ = np.ascontiguousarray(g.indirection_from, dtype=np.int32)
+        h = _P()
+        ctx = to_space.ctx
+        check(lib().lfr_cma_create(ctx.h, to_space.fs.nlayers, g.nrow, g.ncol, g.bandwidth, g.alpha,
+                                   g.beta, g.gamma_m, g.gamma_p, to_space.ndf, bt.ctypes.data_as(_P),
+                                   it.ctypes.data_as(_P), from_space.ndf, bf.ctypes.data_as(_P),
+                                   if_.ctypes.data_as(_P), C.byref(h)), ctx.h)
+        self.h = h
+        self.indirection_to = torch.from_numpy(it).to(ctx.device)
+
+    def empty_matrix(self, dtype=torch.float32) -> torch.Tensor:
+        g = self.geom
+        return torch.empty((self.to_space.fs.ncells, g.nrow, g.bandwidth), dtype=dtype,
+                           device=self.to_space.ctx.device)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lfr_cma_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def columnwise_op_asm(cm: torch.Tensor, lma: LMAOperator, geom: ColumnwiseGeometry) -> torch.Tensor:
+    """columnwise_matrix(bw, nrow, ncell_2d) (tensor (ncell_2d, nrow, bw)) assembled from an LMA operator."""
+    g = geom.geom
+    shape = (geom.to_space.fs.ncells, g.nrow, g.bandwidth)
+    if tuple(cm.shape) != shape or cm.dtype != lma.matrix.dtype or not cm.is_contiguous():
+        raise LfrError(1, f"cm must be a contiguous {lma.matrix.dtype} tensor of shape {shape}")
+    ctx = geom.to_space.ctx
+    ctx.sync_stream()
+    check(lib().lfr_columnwise_op_asm(ctx.h, _prec(cm), geom.h, _P(lma.matrix.data_ptr()),
+                                      _P(cm.data_ptr()), geom.to_space.fs.ncells), ctx.h)
+    return cm
+
+
+def columnwise_op_app(lhs: torch.Tensor, x: torch.Tensor, cm: torch.Tensor, geom: ColumnwiseGeometry):
+    """lhs += A x for a banded column operator."""
+    _field(lhs, geom.to_space, "lhs", cm.dtype)
+    _field(x, geom.from_space, "x", cm.dtype)
+    ctx = geom.to_space.ctx
+    ctx.sync_stream()
+    check(lib().lfr_columnwise_op_app(ctx.h, _prec(cm), geom.h, _P(lhs.data_ptr()), _P(x.data_ptr()),
+                                      _P(cm.data_ptr()), geom.to_space.h, geom.from_space.h), ctx.h)
+    return lhs

@@ -1,5 +1,8 @@
 // Column-wise (CMA) operators: the tridiagonal column solve
-// columnwise_op_appinv and the banded apply columnwise_op_app.
+// columnwise_op_appinv, the banded apply columnwise_op_app and the assembly
+// columnwise_op_asm (components/science/source/kernel/columnwise_operators/
+// sci_columnwise_op_{app,asm}_kernel_mod.F90), one thread per (column, row)
+// for the latter two, each row's sums formed in the reference order.
 //
 // appinv (components/science/source/kernel/columnwise_operators/
 // sci_columnwise_op_appinv_kernel_mod.F90:77-145): one Thomas solve per
@@ -14,6 +17,9 @@
 // keeps c' and d' in registers (NROW is a compile-time constant and every
 // loop over rows is unrolled, so the arrays never spill to scratch).  The
 // solution goes back through LDS for a coalesced store.
+#include <cmath>
+#include <vector>
+
 #include "lfr_internal.h"
 
 namespace {
@@ -192,3 +198,226 @@ extern "C" int lfr_columnwise_op_appinv(lfr_ctx* ctx, int prec, void* lhs, const
   return launch_appinv<double, double>(ctx, (double*)lhs, (const double*)x, (const double*)cm,
                                        space, indirection_d, nrow, ncell_2d);
 }
+
+// ---------------------------------------------------------------------------
+// Banded (CMA) geometry handle, assembly and apply
+// ---------------------------------------------------------------------------
+struct lfr_cma {
+  lfr_ctx* ctx = nullptr;
+  int nrow = 0, ncol = 0, bw = 0, alpha = 0, beta = 0, gamma_m = 0, gamma_p = 0;
+  int ndf_to = 0, ndf_from = 0, nlayers = 0;
+  int32_t* ind_to_d = nullptr;      // indirection_dofmap_to(nrow), 1-based
+  int32_t* ind_from_d = nullptr;    // indirection_dofmap_from(ncol), 1-based
+  int32_t* banded_from_d = nullptr; // column_banded_dofmap_from(ndf_from, nlayers)
+  int32_t* row_ptr_d = nullptr;     // per row i: its (k, df1) entries, in (k, df1) order
+  int32_t* row_ent_d = nullptr;     // k * ndf_to + df1
+};
+
+namespace {
+
+// band limit of row i (1-based), formed in the operator's real kind as the
+// reference does: ceiling((alpha*i - gamma_p) / (1.0_r_solver*beta))
+template <typename T>
+__device__ __forceinline__ int band_lo(int i, int alpha, int beta, int gamma_p) {
+  return (int)ceil((T)(alpha * i - gamma_p) / ((T)1 * beta));
+}
+template <typename T>
+__device__ __forceinline__ int band_hi(int i, int alpha, int beta, int gamma_m) {
+  return (int)floor((T)(alpha * i + gamma_m) / ((T)1 * beta));
+}
+
+// columnwise_op_asm_kernel_code, thread = (column, row i): cm(:, i, cell)
+// accumulates its (k, df1, df2) contributions in the reference loop order.
+template <typename T, int BW>
+__global__ void __launch_bounds__(256)
+k_cma_asm(T* __restrict__ cm, const T* __restrict__ ls, const int32_t* __restrict__ row_ptr,
+          const int32_t* __restrict__ row_ent, const int32_t* __restrict__ banded_from, int ncell, int nl,
+          int nrow, int ndf_to, int ndf_from, int alpha, int beta, int gamma_p) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ncell * nrow) return;
+  const int cell = (int)(t / nrow), i0 = (int)(t - (int64_t)cell * nrow);
+  const int64_t n3 = (int64_t)ncell * nl;
+  const int jm = band_lo<T>(i0 + 1, alpha, beta, gamma_p);
+  T acc[BW];
+#pragma unroll
+  for (int b = 0; b < BW; ++b) acc[b] = T(0);
+  for (int e = row_ptr[i0]; e < row_ptr[i0 + 1]; ++e) {
+    const int k = row_ent[e] / ndf_to, df1 = row_ent[e] - k * ndf_to;
+    const int64_t ik = (int64_t)cell * nl + k;
+    for (int df2 = 0; df2 < ndf_from; ++df2) {
+      const int b = banded_from[df2 + ndf_from * k] - jm;
+      const T v = ls[ik + n3 * (df1 + (int64_t)ndf_to * df2)];
+#pragma unroll
+      for (int q = 0; q < BW; ++q)
+        if (q == b) acc[q] = acc[q] + v;
+    }
+  }
+  T* out = cm + ((int64_t)cell * nrow + i0) * BW;
+#pragma unroll
+  for (int b = 0; b < BW; ++b) out[b] = acc[b];
+}
+
+// columnwise_op_app_kernel_code, thread = (column, row i): lhs(mu_i) += sum_j
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_cma_app(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
+          const int32_t* __restrict__ map_to, const int32_t* __restrict__ map_from,
+          const int32_t* __restrict__ ind_to, const int32_t* __restrict__ ind_from, int ncell, int nrow,
+          int ncol, int bw, int ndf_to, int ndf_from, int alpha, int beta, int gamma_m, int gamma_p) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ncell * nrow) return;
+  const int cell = (int)(t / nrow), i0 = (int)(t - (int64_t)cell * nrow);
+  const int i = i0 + 1;
+  const int64_t mu_i = (int64_t)map_to[(int64_t)cell * ndf_to] + ind_to[i0] - 1;
+  const int64_t base_from = map_from[(int64_t)cell * ndf_from];
+  const int jm = band_lo<T>(i, alpha, beta, gamma_p);
+  const int jp = band_hi<T>(i, alpha, beta, gamma_m);
+  const int lo = jm > 1 ? jm : 1, hi = jp < ncol ? jp : ncol;
+  const T* c = cm + ((int64_t)cell * nrow + i0) * bw;
+  T acc = lhs[mu_i];
+  for (int j = lo; j <= hi; ++j) {
+    const T prod = c[j - jm] * x[base_from + ind_from[j - 1] - 1];
+    acc = acc + prod;
+  }
+  lhs[mu_i] = acc;
+}
+
+void cma_free(lfr_cma* h) {
+  if (!h) return;
+  for (void* p : {(void*)h->ind_to_d, (void*)h->ind_from_d, (void*)h->banded_from_d, (void*)h->row_ptr_d,
+                  (void*)h->row_ent_d})
+    if (p) (void)hipFree(p);
+  delete h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lfr_cma_create(lfr_ctx* ctx, int nlayers, int nrow, int ncol, int bandwidth, int alpha, int beta,
+                   int gamma_m, int gamma_p, int ndf_to, const int32_t* banded_to_h, const int32_t* ind_to_h,
+                   int ndf_from, const int32_t* banded_from_h, const int32_t* ind_from_h, lfr_cma** out) {
+  if (!ctx || !out) return LFR_ERR_ARG;
+  *out = nullptr;
+  LFR_REQUIRE(ctx, nlayers > 0 && nrow > 0 && ncol > 0 && bandwidth >= 1 && bandwidth <= 8 && beta > 0 &&
+                       ndf_to > 0 && ndf_from > 0 && banded_to_h && ind_to_h && banded_from_h && ind_from_h,
+              "lfr_cma_create: bad arguments");
+  // validate the maps against nrow/ncol and the band
+  for (int q = 0; q < nrow; ++q)
+    LFR_REQUIRE(ctx, ind_to_h[q] >= 1 && ind_to_h[q] <= nrow, "lfr_cma_create: indirection_dofmap_to out of range");
+  for (int q = 0; q < ncol; ++q)
+    LFR_REQUIRE(ctx, ind_from_h[q] >= 1 && ind_from_h[q] <= ncol,
+                "lfr_cma_create: indirection_dofmap_from out of range");
+  std::vector<std::vector<int32_t>> rows(nrow);
+  for (int k = 0; k < nlayers; ++k)
+    for (int df1 = 0; df1 < ndf_to; ++df1) {
+      const int i = banded_to_h[df1 + ndf_to * k];
+      LFR_REQUIRE(ctx, i >= 1 && i <= nrow, "lfr_cma_create: column_banded_dofmap_to out of range");
+      const int jm = (int)std::ceil((double)(alpha * i - gamma_p) / (double)beta);
+      for (int df2 = 0; df2 < ndf_from; ++df2) {
+        const int j = banded_from_h[df2 + ndf_from * k];
+        LFR_REQUIRE(ctx, j >= 1 && j <= ncol && j - jm >= 0 && j - jm < bandwidth,
+                    "lfr_cma_create: entry (row %d, column %d) outside the band", i, j);
+      }
+      rows[i - 1].push_back(k * ndf_to + df1);   // appended in (k, df1) order
+    }
+  std::vector<int32_t> ptr(nrow + 1, 0), ent;
+  for (int i = 0; i < nrow; ++i) {
+    ptr[i + 1] = ptr[i] + (int)rows[i].size();
+    ent.insert(ent.end(), rows[i].begin(), rows[i].end());
+  }
+  lfr_cma* h = new lfr_cma();
+  h->ctx = ctx;
+  h->nrow = nrow;
+  h->ncol = ncol;
+  h->bw = bandwidth;
+  h->alpha = alpha;
+  h->beta = beta;
+  h->gamma_m = gamma_m;
+  h->gamma_p = gamma_p;
+  h->ndf_to = ndf_to;
+  h->ndf_from = ndf_from;
+  h->nlayers = nlayers;
+  const size_t nbf = (size_t)ndf_from * nlayers;
+  const size_t nent = ent.empty() ? 1 : ent.size();
+  if (hipMalloc(&h->ind_to_d, nrow * 4) != hipSuccess || hipMalloc(&h->ind_from_d, ncol * 4) != hipSuccess ||
+      hipMalloc(&h->banded_from_d, nbf * 4) != hipSuccess || hipMalloc(&h->row_ptr_d, (nrow + 1) * 4) != hipSuccess ||
+      hipMalloc(&h->row_ent_d, nent * 4) != hipSuccess) {
+    cma_free(h);
+    return lfr_set_error(ctx, LFR_ERR_NOMEM, "lfr_cma_create: allocation failed");
+  }
+  hipError_t e = hipMemcpy(h->ind_to_d, ind_to_h, nrow * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h->ind_from_d, ind_from_h, ncol * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h->banded_from_d, banded_from_h, nbf * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(h->row_ptr_d, ptr.data(), (nrow + 1) * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !ent.empty()) e = hipMemcpy(h->row_ent_d, ent.data(), ent.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    cma_free(h);
+    return lfr_set_error(ctx, LFR_ERR_HIP, "lfr_cma_create: %s", hipGetErrorString(e));
+  }
+  *out = h;
+  return LFR_OK;
+}
+
+int lfr_cma_destroy(lfr_cma* h) {
+  cma_free(h);
+  return LFR_OK;
+}
+
+int lfr_columnwise_op_asm(lfr_ctx* ctx, int prec, const lfr_cma* h, const void* local_stencil_d, void* cm_d,
+                          int ncell_2d) {
+  if (!ctx || !h) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, (prec == 4 || prec == 8) && local_stencil_d && cm_d && ncell_2d > 0,
+              "lfr_columnwise_op_asm: bad arguments");
+  const int64_t n = (int64_t)ncell_2d * h->nrow;
+  const unsigned g = (unsigned)((n + 255) / 256);
+  hipStream_t s = ctx->stream;
+#define LFR_ASM(T, BWV)                                                                                 \
+  hipLaunchKernelGGL((k_cma_asm<T, BWV>), dim3(g), dim3(256), 0, s, (T*)cm_d, (const T*)local_stencil_d, \
+                     h->row_ptr_d, h->row_ent_d, h->banded_from_d, ncell_2d, h->nlayers, h->nrow, h->ndf_to, \
+                     h->ndf_from, h->alpha, h->beta, h->gamma_p)
+#define LFR_ASM_BW(T)                      \
+  switch (h->bw) {                         \
+    case 1: LFR_ASM(T, 1); break;          \
+    case 2: LFR_ASM(T, 2); break;          \
+    case 3: LFR_ASM(T, 3); break;          \
+    case 4: LFR_ASM(T, 4); break;          \
+    case 5: LFR_ASM(T, 5); break;          \
+    default: LFR_ASM(T, 8); break;         \
+  }
+  if (h->bw > 5 && h->bw != 8) return lfr_set_error(ctx, LFR_ERR_ARG, "lfr_columnwise_op_asm: bandwidth %d", h->bw);
+  if (prec == 4) {
+    LFR_ASM_BW(float)
+  } else {
+    LFR_ASM_BW(double)
+  }
+#undef LFR_ASM_BW
+#undef LFR_ASM
+  LFR_CHECK_LAUNCH(ctx);
+  return LFR_OK;
+}
+
+int lfr_columnwise_op_app(lfr_ctx* ctx, int prec, const lfr_cma* h, void* lhs_d, const void* x_d,
+                          const void* cm_d, const lfr_space* to_space, const lfr_space* from_space) {
+  if (!ctx || !h) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, (prec == 4 || prec == 8) && lhs_d && x_d && cm_d && to_space && from_space,
+              "lfr_columnwise_op_app: bad arguments");
+  LFR_REQUIRE(ctx, to_space->ndf == h->ndf_to && from_space->ndf == h->ndf_from &&
+                       to_space->ncell == from_space->ncell && to_space->nlayers == h->nlayers,
+              "lfr_columnwise_op_app: spaces do not match the operator geometry");
+  const int ncell = to_space->ncell;
+  const int64_t n = (int64_t)ncell * h->nrow;
+  const unsigned g = (unsigned)((n + 255) / 256);
+  if (prec == 4)
+    hipLaunchKernelGGL(k_cma_app<float>, dim3(g), dim3(256), 0, ctx->stream, (float*)lhs_d, (const float*)x_d,
+                       (const float*)cm_d, to_space->map0_d, from_space->map0_d, h->ind_to_d, h->ind_from_d, ncell,
+                       h->nrow, h->ncol, h->bw, h->ndf_to, h->ndf_from, h->alpha, h->beta, h->gamma_m, h->gamma_p);
+  else
+    hipLaunchKernelGGL(k_cma_app<double>, dim3(g), dim3(256), 0, ctx->stream, (double*)lhs_d, (const double*)x_d,
+                       (const double*)cm_d, to_space->map0_d, from_space->map0_d, h->ind_to_d, h->ind_from_d, ncell,
+                       h->nrow, h->ncol, h->bw, h->ndf_to, h->ndf_from, h->alpha, h->beta, h->gamma_m, h->gamma_p);
+  LFR_CHECK_LAUNCH(ctx);
+  return LFR_OK;
+}
+
+}  // extern "C"

@@ -66,6 +66,11 @@ const int32_t* lfr_plan_records(lfr_ctx* ctx, const lfr_plan* plan, size_t ci, c
 bool lfr_record_shape(int ndf_in, int ngroup, int gsize);
 bool lfr_records_enabled();
 bool lfr_gather_nt(int prec);
+// Levels per thread of the record gather for plan class `c` (1: one level per
+// thread; 2 or 4: the vectorised kernel, used for classes of exactly nl
+// levels with a record shape); blocks of 256 threads that class launches.
+int lfr_class_lpt(const lfr_plan_class& c, int nl, int prec, int ndf_in);
+unsigned lfr_class_blocks(const lfr_plan_class& c, int lpt);
 
 int lfr_set_error(lfr_ctx* ctx, int code, const char* fmt, ...);
 

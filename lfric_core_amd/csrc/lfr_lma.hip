@@ -47,6 +47,28 @@ bool lfr_gather_nt(int prec) {
   return mode < 0 ? prec == 8 : mode == 1;
 }
 
+// Levels per thread of the vectorised record gather: 4 (FP32) or 2 (FP64) by
+// default; LFR_GATHER_LPT=1, 2 or 4 forces one value.
+int lfr_gather_lpt(int prec) {
+  static const int forced = [] {
+    const char* e = getenv("LFR_GATHER_LPT");
+    if (e && (e[0] == '1' || e[0] == '2' || e[0] == '4')) return e[0] - '0';
+    return 0;
+  }();
+  return forced ? forced : (prec == 4 ? 4 : 2);
+}
+
+int lfr_class_lpt(const lfr_plan_class& c, int nl, int prec, int ndf_in) {
+  if (!lfr_records_enabled() || !lfr_record_shape(ndf_in, c.ngroup, c.gsize) || c.height != nl) return 1;
+  const int lpt = lfr_gather_lpt(prec);
+  return nl >= lpt ? lpt : 1;
+}
+
+unsigned lfr_class_blocks(const lfr_plan_class& c, int lpt) {
+  const int64_t threads = (int64_t)c.ncol * ((c.height + lpt - 1) / lpt);
+  return lfr_blocks(threads, lfr::kThreads);
+}
+
 // LFR_NO_RECORDS=1 switches the record kernels off (plan-entry kernels only).
 bool lfr_records_enabled() {
   static const bool on = [] {
@@ -128,6 +150,41 @@ void launch_rec_t(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, 
 #undef LREC
 }
 
+template <typename T, int NDF_IN, int NG, int GS, int LPT>
+void launch_vec_t(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y, const int32_t* rec,
+                  const lfr_plan_class& c, int nl, int64_t n3, int ndf_out, int overwrite) {
+  const unsigned nbp = pad8(nb);
+  if (kind == 0)
+    hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, 0, false, LPT>), dim3(nbp), dim3(kThreads), 0, s, M,
+                       x, y, rec, c.ncol, nl, n3, ndf_out, overwrite, nbp, (const T*)nullptr, (double*)nullptr,
+                       (const double*)nullptr);
+  else
+    hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, 1, false, LPT>), dim3(nbp), dim3(kThreads), 0, s, M,
+                       x, y, rec, c.ncol, nl, n3, ndf_out, overwrite, nbp, (const T*)nullptr, (double*)nullptr,
+                       (const double*)nullptr);
+}
+
+template <typename T, int NDF_IN, int LPT>
+bool launch_vec_shape(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y, const int32_t* rec,
+                      const lfr_plan_class& c, int nl, int64_t n3, int ndf_out, int overwrite) {
+#define SHAPE(NGV, GSV)                                                                               \
+  if (c.ngroup == NGV && c.gsize == GSV) {                                                            \
+    launch_vec_t<T, NDF_IN, NGV, GSV, LPT>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite); \
+    return true;                                                                                      \
+  }
+  SHAPE(2, 1) SHAPE(1, 1) SHAPE(4, 1) SHAPE(3, 1) SHAPE(1, 2) SHAPE(2, 2) SHAPE(3, 2) SHAPE(4, 2)
+#undef SHAPE
+  return false;
+}
+
+template <typename T, int NDF_IN>
+bool launch_vec_any(hipStream_t s, int lpt, int kind, unsigned nb, const T* M, const T* x, T* y,
+                    const int32_t* rec, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out, int overwrite) {
+  if (lpt == 2) return launch_vec_shape<T, NDF_IN, 2>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite);
+  if (lpt == 4) return launch_vec_shape<T, NDF_IN, 4>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite);
+  return false;
+}
+
 template <typename T, int NDF_IN>
 void launch_rec_shape(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
                       const int32_t* rec, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
@@ -149,6 +206,19 @@ int run_gather(lfr_ctx* ctx, const lfr_plan* plan, int kind, const T* M, const T
     const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
     int stride = 0;
     const int32_t* rec = lfr_records_enabled() ? lfr_plan_records(ctx, plan, ci, in, &stride) : nullptr;
+    const int lpt = rec ? lfr_class_lpt(c, nl, (int)sizeof(T), in->ndf) : 1;
+    if (rec && lpt > 1) {
+      const unsigned nbv = lfr_class_blocks(c, lpt);
+      bool ok = false;
+      switch (in->ndf) {
+        case 1: ok = launch_vec_any<T, 1>(ctx->stream, lpt, kind, nbv, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
+        case 6: ok = launch_vec_any<T, 6>(ctx->stream, lpt, kind, nbv, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
+        case 8: ok = launch_vec_any<T, 8>(ctx->stream, lpt, kind, nbv, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
+      }
+      if (!ok) return lfr_set_error(ctx, LFR_ERR_ARG, "no vectorised gather for this class shape");
+      LFR_CHECK_LAUNCH(ctx);
+      continue;
+    }
     if (rec) {
       switch (in->ndf) {
         case 1: launch_rec_shape<T, 1>(ctx->stream, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;

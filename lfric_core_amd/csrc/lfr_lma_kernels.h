@@ -254,6 +254,145 @@ k_gather_rec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y
 }
 
 // ---------------------------------------------------------------------------
+// Record gather, LPT consecutive levels per thread, for classes whose columns
+// have exactly nl levels (no bottom/top shift: every entry reads layer k = L).
+// Each matrix plane segment, operand column and output column is then read
+// or written LPT values per instruction (multi-dword loads; the hardware
+// needs only 4-byte alignment), which divides the vector-memory instruction
+// count by LPT.  Threads per column = ceil(nl / LPT); the last one starts at
+// nl - LPT so that its window stays inside the column, recomputes the levels
+// it shares with its neighbour (identical operations, identical bits) and
+// writes only the levels that neighbour does not own.  The arithmetic per
+// output is exactly that of k_gather_rec.
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+struct __attribute__((aligned(4))) LVec {
+  T v[N];
+};
+
+template <typename T, int N, bool NT>
+__device__ __forceinline__ LVec<T, N> ldvec(const T* p) {
+  if constexpr (N == 1) {
+    LVec<T, 1> r;
+    r.v[0] = NT ? __builtin_nontemporal_load(p) : *p;
+    return r;
+  } else {
+    return *reinterpret_cast<const LVec<T, N>*>(p);
+  }
+}
+
+template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT, int LPT>
+__global__ void __launch_bounds__(kThreads)
+k_gather_vec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
+             const int32_t* __restrict__ rec, int ncol, int nl, int64_t n3, int ndf_out,
+             int overwrite, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
+             double* __restrict__ part = nullptr, const double* __restrict__ stop = nullptr) {
+  using S = RecShape<NDF_IN, NG, GS>;
+  if (DOT && stop && stop[0] != 0.0) return;
+  const int tpc = (nl + LPT - 1) / LPT;
+  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
+  const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
+  const bool valid = t < (int64_t)ncol * tpc;
+  const int64_t tt = valid ? t : 0;
+  const int col = (int)(tt / tpc);
+  const int p = (int)(tt - (int64_t)col * tpc);
+  const int L0 = min(p * LPT, nl - LPT);      // window [L0, L0 + LPT) inside the column
+  const int own = p * LPT - L0;               // levels below L0 + own belong to thread p - 1
+  int r[S::kStride];
+  const int4* r4 = reinterpret_cast<const int4*>(rec + (int64_t)col * S::kStride);
+#pragma unroll
+  for (int q = 0; q < S::kStride / 4; ++q) {
+    const int4 v = r4[q];
+    r[4 * q + 0] = v.x;
+    r[4 * q + 1] = v.y;
+    r[4 * q + 2] = v.z;
+    r[4 * q + 3] = v.w;
+  }
+  const int32_t g0 = r[0];
+  T acc[LPT];
+  if (overwrite || !valid) {
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) acc[u] = T(0);
+  } else {
+    const LVec<T, LPT> yv = ldvec<T, LPT, false>(y + g0 + L0);
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) acc[u] = yv.v[u];
+  }
+  LVec<T, LPT> mv[NG][GS][NDF_IN], xv[NG][GS][NDF_IN];
+  bool ok[NG][GS];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const int* rg = r + 1 + g * S::kGroup;
+    const int64_t mbase = (int64_t)rg[0];
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+      const int code = rg[1 + q];
+      ok[g][q] = code >= 0;
+      const int ii = code >= 0 ? (code >> 1) : 0;
+#pragma unroll
+      for (int j = 0; j < NDF_IN; ++j) {
+        const int64_t mo = KIND == 0 ? (int64_t)ii + (int64_t)ndf_out * j
+                                     : (int64_t)j + (int64_t)NDF_IN * ii;
+        mv[g][q][j] = ldvec<T, LPT, false>(M + mbase + L0 + n3 * mo);
+        xv[g][q][j] = ldvec<T, LPT, false>(x + rg[1 + GS + j] + L0);
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    if (KIND == 0) {
+#pragma unroll
+      for (int j = 0; j < NDF_IN; ++j)
+#pragma unroll
+        for (int q = 0; q < GS; ++q)
+#pragma unroll
+          for (int u = 0; u < LPT; ++u) {
+            const T pr = mv[g][q][j].v[u] * xv[g][q][j].v[u];
+            const T s = acc[u] + pr;
+            acc[u] = ok[g][q] ? s : acc[u];
+          }
+    } else {
+#pragma unroll
+      for (int q = 0; q < GS; ++q)
+#pragma unroll
+        for (int u = 0; u < LPT; ++u) {
+          T dot = T(0);
+#pragma unroll
+          for (int j = 0; j < NDF_IN; ++j) {
+            const T pr = mv[g][q][j].v[u] * xv[g][q][j].v[u];
+            dot = dot + pr;
+          }
+          const T s = acc[u] + dot;
+          acc[u] = ok[g][q] ? s : acc[u];
+        }
+    }
+  }
+  if (valid) {
+    if (own == 0) {
+      LVec<T, LPT> o;
+#pragma unroll
+      for (int u = 0; u < LPT; ++u) o.v[u] = acc[u];
+      *reinterpret_cast<LVec<T, LPT>*>(y + g0 + L0) = o;
+    } else {
+#pragma unroll
+      for (int u = 0; u < LPT; ++u)
+        if (u >= own) y[g0 + L0 + u] = acc[u];
+    }
+  }
+  if (DOT) {
+    __shared__ double lds[kThreads / 64];
+    double v = 0.0;
+    if (valid) {
+#pragma unroll
+      for (int u = 0; u < LPT; ++u)
+        if (u >= own) v += (double)acc[u] * (double)dotv[g0 + L0 + u];
+    }
+    v = block_sum(v, lds);
+    if (threadIdx.x == 0) part[lb] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Cell-centric apply for a discontinuous output space: thread <-> (cell, k).
 // Each output dof has exactly one (cell, df) contributor, so per output the
 // sum over df2 in ascending order is the reference order

@@ -28,6 +28,7 @@
 // registers (lane shift by one layer).  Across columns a dof is written by
 // its first toucher in colour order (overwrite form) and incremented by the
 // later ones, so results are deterministic.
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -444,6 +445,20 @@ int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void*
     else                                                                \
       launch_multi<false, NTP, NF, PIG, IGW>(s, h, y, x, overwrite);    \
     break;
+  // W2 k=1: row tiles per wave (2: 5 waves per column, ~100 registers;
+  // 3: 3 waves, ~200 registers); LFR_MULTI_IGW=3 selects the latter
+  static const int igw36 = [] {
+    const char* e = getenv("LFR_MULTI_IGW");
+    return (e && e[0] == '3') ? 3 : 2;
+  }();
+  if (key == 36 * 1000 + 36 * 10 + 1 && igw36 == 2) {
+    if (bf)
+      launch_multi<true, 36, 36, 1, 2>(s, h, y, x, overwrite);
+    else
+      launch_multi<false, 36, 36, 1, 2>(s, h, y, x, overwrite);
+    LFR_CHECK_LAUNCH(ctx);
+    return LFR_OK;
+  }
   switch (key) {
     LFR_MULTI_CASE(36, 36, 1, 3)
     LFR_MULTI_CASE(28, 27, 3, 7)

@@ -218,10 +218,13 @@ struct Ws {
 
 size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
 
-int apply_partials(const lfr_plan* plan) {
+// Partial sums written by one fused apply: padded block count of every class
+// under the kernel apply_op launches for it.
+int apply_partials(const lfr_plan* plan, int prec) {
   int np = 0;
+  const lfr_space* sp = plan->space;
   for (const auto& c : plan->classes) {
-    const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
+    const unsigned nb = lfr_class_blocks(c, lfr_class_lpt(c, sp->nlayers, prec, sp->ndf));
     np += (int)((nb + 7u) & ~7u);
   }
   return np;
@@ -230,7 +233,7 @@ int apply_partials(const lfr_plan* plan) {
 // Carve the solver workspace out of one grow-only device buffer.
 int ws_get(lfr_ctx* ctx, int64_t n, int prec, const lfr_plan* plan, int naux, Ws& w) {
   const size_t vb = align_up((size_t)n * prec);
-  const int napply = apply_partials(plan);
+  const int napply = apply_partials(plan, prec);
   const size_t need = 3 * vb + (size_t)naux * vb + align_up(napply * sizeof(double)) +
                       2 * align_up(lfr::kRedMaxBlocks * sizeof(double)) +
                       align_up(S_NSLOT * sizeof(double)) + align_up(8 * sizeof(double));
@@ -295,6 +298,32 @@ void launch_apply_rec_t(hipStream_t s, const lfr_plan_class& c, const int32_t* r
   }
 }
 
+template <typename T, int NDF_IN, int NG, int GS, int LPT>
+void launch_apply_vec_t(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M, const T* x, T* y,
+                        int nl, int64_t n3, int ndf_out, const T* dotv, double* part, const double* stop) {
+  const unsigned nbp = (lfr_class_blocks(c, LPT) + 7u) & ~7u;
+  if (dotv)
+    hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, 0, true, LPT>), dim3(nbp), dim3(kThreads), 0, s, M, x,
+                       y, rec, c.ncol, nl, n3, ndf_out, 1, nbp, dotv, part, stop);
+  else
+    hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, 0, false, LPT>), dim3(nbp), dim3(kThreads), 0, s, M, x,
+                       y, rec, c.ncol, nl, n3, ndf_out, 1, nbp, (const T*)nullptr, (double*)nullptr,
+                       (const double*)nullptr);
+}
+
+template <typename T, int NDF_IN, int LPT>
+bool launch_apply_vec(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M, const T* x, T* y,
+                      int nl, int64_t n3, int ndf, const T* dotv, double* part, const double* stop) {
+#define SHAPE(NGV, GSV)                                                                                   \
+  if (c.ngroup == NGV && c.gsize == GSV) {                                                                \
+    launch_apply_vec_t<T, NDF_IN, NGV, GSV, LPT>(s, c, rec, M, x, y, nl, n3, ndf, dotv, part, stop);      \
+    return true;                                                                                          \
+  }
+  SHAPE(2, 1) SHAPE(1, 1) SHAPE(4, 1) SHAPE(3, 1) SHAPE(1, 2) SHAPE(2, 2) SHAPE(3, 2) SHAPE(4, 2)
+#undef SHAPE
+  return false;
+}
+
 template <typename T, int NDF_IN>
 bool launch_apply_rec(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M,
                       const T* x, T* y, int nl, int64_t n3, int ndf, const T* dotv, double* part,
@@ -333,14 +362,28 @@ int apply_op(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv,
   int off = 0;
   for (size_t ci = 0; ci < A->plan->classes.size(); ++ci) {
     const auto& c = A->plan->classes[ci];
-    const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
+    const int lpt = lfr_class_lpt(c, nl, (int)sizeof(T), sp->ndf);
+    const unsigned nb = lfr_class_blocks(c, lpt);
     const unsigned nbp = (nb + 7u) & ~7u;
     double* pp = part ? part + off : nullptr;
     const T* M = (const T*)A->matrix_d;
     int stride = 0;
     const int32_t* rec = lfr_records_enabled() ? lfr_plan_records(ctx, A->plan, ci, sp, &stride) : nullptr;
     bool done = false;
-    if (rec) {
+    if (lpt > 1) {
+      if (!rec) return lfr_set_error(ctx, LFR_ERR_NOMEM, "apply: column records unavailable");
+      if (sp->ndf == 6) {
+        done = lpt == 2 ? launch_apply_vec<T, 6, 2>(ctx->stream, c, rec, M, x, y, nl, n3, 6, dotv, pp, stop)
+                        : launch_apply_vec<T, 6, 4>(ctx->stream, c, rec, M, x, y, nl, n3, 6, dotv, pp, stop);
+      } else if (sp->ndf == 8) {
+        done = lpt == 2 ? launch_apply_vec<T, 8, 2>(ctx->stream, c, rec, M, x, y, nl, n3, 8, dotv, pp, stop)
+                        : launch_apply_vec<T, 8, 4>(ctx->stream, c, rec, M, x, y, nl, n3, 8, dotv, pp, stop);
+      } else if (sp->ndf == 1) {
+        done = lpt == 2 ? launch_apply_vec<T, 1, 2>(ctx->stream, c, rec, M, x, y, nl, n3, 1, dotv, pp, stop)
+                        : launch_apply_vec<T, 1, 4>(ctx->stream, c, rec, M, x, y, nl, n3, 1, dotv, pp, stop);
+      }
+      if (!done) return lfr_set_error(ctx, LFR_ERR_ARG, "apply: no vectorised gather for this class shape");
+    } else if (rec) {
       if (sp->ndf == 6) done = launch_apply_rec<T, 6>(ctx->stream, c, rec, M, x, y, nl, n3, 6, dotv, pp, stop);
       else if (sp->ndf == 8) done = launch_apply_rec<T, 8>(ctx->stream, c, rec, M, x, y, nl, n3, 8, dotv, pp, stop);
     }

@@ -169,6 +169,62 @@ def appinv(lhs, x, cm, dofmap, indirection, nthreads=None, use_ref=False):
     return lhs
 
 
+def cma_asm(local_stencil, geom, ncell, nlayers, use_ref=False):
+    """columnwise_op_asm_kernel_code over all columns: returns columnwise_matrix
+    (bw, nrow, ncell_2d) as a C array (ncell_2d, nrow, bw).  local_stencil in
+    numpy layout (ndf_from, ndf_to, ncell_3d); geom from lfric_core_amd.cma."""
+    dt = local_stencil.dtype
+    ndf_from, ndf_to, n3 = local_stencil.shape
+    cm = np.zeros((ncell, geom.nrow, geom.bandwidth), dtype=dt)
+    bt = np.ascontiguousarray(geom.banded_to, dtype=np.int32)      # (nl, ndf_to) = (ndf_to, nl) Fortran
+    bf = np.ascontiguousarray(geom.banded_from, dtype=np.int32)
+    ls = np.ascontiguousarray(local_stencil)
+    for c in range(1, ncell + 1):
+        if use_ref:
+            assert dt == np.float32
+            ref().ref_asm_r32(_I(c), _I(nlayers), _I(ncell), _I(n3), ls.ctypes.data_as(C.c_void_p),
+                              cm.ctypes.data_as(C.c_void_p), _I(geom.nrow), _I(geom.ncol),
+                              _I(geom.bandwidth), _I(geom.alpha), _I(geom.beta), _I(geom.gamma_m),
+                              _I(geom.gamma_p), _I(ndf_to), bt.ctypes.data_as(C.c_void_p),
+                              _I(ndf_from), bf.ctypes.data_as(C.c_void_p))
+        else:
+            getattr(lib(), f"orc_columnwise_op_asm_code_{_suf(dt)}")(
+                _I(c), _I(nlayers), _I(n3), ls.ctypes.data_as(C.c_void_p), cm.ctypes.data_as(C.c_void_p),
+                _I(geom.nrow), _I(geom.bandwidth), _I(geom.alpha), _I(geom.beta), _I(geom.gamma_p),
+                _I(ndf_to), bt.ctypes.data_as(C.c_void_p), _I(ndf_from), bf.ctypes.data_as(C.c_void_p))
+    return cm
+
+
+def cma_app(lhs, x, cm, geom, dofmap_to, dofmap_from, use_ref=False):
+    """columnwise_op_app_kernel_code over all columns: lhs += A x (in place)."""
+    dt = lhs.dtype
+    ncell = cm.shape[0]
+    it = np.ascontiguousarray(geom.indirection_to, dtype=np.int32)
+    if_ = np.ascontiguousarray(geom.indirection_from, dtype=np.int32)
+    dmt = np.ascontiguousarray(dofmap_to, dtype=np.int32)
+    dmf = np.ascontiguousarray(dofmap_from, dtype=np.int32)
+    for c in range(1, ncell + 1):
+        m1 = dmt[c - 1]
+        m2 = dmf[c - 1]
+        if use_ref:
+            assert dt == np.float32
+            ref().ref_app_r32(_I(c), _I(ncell), lhs.ctypes.data_as(C.c_void_p),
+                              x.ctypes.data_as(C.c_void_p), cm.ctypes.data_as(C.c_void_p),
+                              _I(geom.nrow), _I(geom.ncol), _I(geom.bandwidth), _I(geom.alpha),
+                              _I(geom.beta), _I(geom.gamma_m), _I(geom.gamma_p), _I(m1.size),
+                              _I(lhs.size), m1.ctypes.data_as(C.c_void_p), it.ctypes.data_as(C.c_void_p),
+                              _I(m2.size), _I(x.size), m2.ctypes.data_as(C.c_void_p),
+                              if_.ctypes.data_as(C.c_void_p))
+        else:
+            getattr(lib(), f"orc_columnwise_op_app_code_{_suf(dt)}")(
+                _I(c), lhs.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                cm.ctypes.data_as(C.c_void_p), _I(geom.nrow), _I(geom.ncol), _I(geom.bandwidth),
+                _I(geom.alpha), _I(geom.beta), _I(geom.gamma_m), _I(geom.gamma_p),
+                m1.ctypes.data_as(C.c_void_p), it.ctypes.data_as(C.c_void_p),
+                m2.ctypes.data_as(C.c_void_p), if_.ctypes.data_as(C.c_void_p))
+    return lhs
+
+
 def dot(x, y, nthreads=None) -> float:
     nt = nthreads_default() if nthreads is None else nthreads
     fn = getattr(lib(), f"orc_dot_{_suf(x.dtype)}")
