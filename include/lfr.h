@@ -195,6 +195,9 @@ typedef struct lfr_lma_op {
   const void* matrix_d;     /* local_stencil(ncell_3d, ndf, ndf)            */
   const lfr_space* space;
   const lfr_plan* plan;
+  int apply_bc;             /* 1: enforce_bc after every apply (the space's
+                               boundary values must be set), as
+                               mass_matrix_operator_type%apply with apply_bc */
 } lfr_lma_op;
 
 /* Solver settings as in the reference constructors
@@ -249,6 +252,20 @@ typedef struct lfr_refine_stats {
 int lfr_refined_cg(lfr_ctx* ctx, const lfr_lma_op* A32, const lfr_lma_op* A64, double r_tol,
                    int max_outer, const lfr_solver_params* inner, void* x64_d, const void* b64_d,
                    lfr_refine_stats* stats);
+
+/* ---------------------------------------------------------------------- */
+/* Boundary conditions                                                     */
+/* ---------------------------------------------------------------------- */
+
+/* Attach boundary_value(ndf, 2) (function_space_type%get_boundary_dofs(),
+ * infrastructure/source/function_space/function_space_mod.F90:784-794;
+ * Fortran order, 0/1 flags) to a space. */
+int lfr_space_set_boundary(lfr_space* space, const int32_t* boundary_value_h);
+
+/* invoke(enforce_bc_kernel_type(field)) over all cells: field(map(df)) *=
+ * boundary_value(df,1), field(map(df)+nlayers-1) *= boundary_value(df,2)
+ * (components/science/source/kernel/algebra/sci_enforce_bc_kernel_mod.F90:63-91). */
+int lfr_enforce_bc(lfr_ctx* ctx, int prec, void* field_d, const lfr_space* space);
 
 /* ---------------------------------------------------------------------- */
 /* Column-wise (CMA) operators                                             */

@@ -69,6 +69,8 @@ SIGNATURES = {
     "lfr_mg_destroy": [_P],
     "lfr_mg_block_sizes": [_P, _P],
     "lfr_mg_solve": [_P, _P, _P, _I, _P],
+    "lfr_space_set_boundary": [_P, _P],
+    "lfr_enforce_bc": [_P, _I, _P, _P],
     "lfr_cma_create": [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, C.POINTER(_P)],
     "lfr_cma_destroy": [_P],
     "lfr_columnwise_op_asm": [_P, _I, _P, _P, _P, _I],

@@ -90,6 +90,15 @@ class Space:
     def ndf(self) -> int:
         return self.fs.ndf
 
+    def set_boundary(self, boundary=None) -> None:
+        """Attach boundary_value(ndf, 2) (default: function_space.boundary_dofs)."""
+        from .function_space import boundary_dofs
+        bv = boundary_dofs(self.fs) if boundary is None else np.asarray(boundary, dtype=np.int32)
+        if bv.shape != (self.ndf, 2):
+            raise LfrError(1, f"boundary values must have shape {(self.ndf, 2)}")
+        self._bv = np.asfortranarray(bv, dtype=np.int32)
+        check(lib().lfr_space_set_boundary(self.h, self._bv.ctypes.data_as(_P)), self.ctx.h)
+
     def plan(self, colouring: Colouring, kind: int) -> "Plan":
         key = (id(colouring), kind)
         if key not in self._plans:
@@ -140,6 +149,7 @@ class LMAOperator:
     to_space: Space
     from_space: Space
     colouring: Colouring | None = None
+    apply_bc: bool = False        # solvers: enforce_bc after every apply (mass_matrix_operator apply_bc)
     _cache: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -236,6 +246,15 @@ def transpose_matrix_vector(lhs: torch.Tensor, x: torch.Tensor, op: LMAOperator,
 
 
 # --- vector algebra ---------------------------------------------------------
+
+def enforce_bc(field_: torch.Tensor, space: Space) -> torch.Tensor:
+    """invoke(enforce_bc_kernel_type(field)): zero the bottom/top boundary dofs."""
+    _field(field_, space, "field")
+    ctx = space.ctx
+    ctx.sync_stream()
+    check(lib().lfr_enforce_bc(ctx.h, _prec(field_), _P(field_.data_ptr()), space.h), ctx.h)
+    return field_
+
 
 def _vec(ctx: Context, *ts: torch.Tensor):
     n = ts[0].numel()

@@ -30,7 +30,12 @@ struct lfr_space {
   int ndf = 0, ncell = 0, nlayers = 0, undf = 0;
   int32_t* map0_d = nullptr;        // map(ndf, ncell) converted to 0-based, device
   std::vector<int32_t> map_h;       // 1-based host copy (plan construction)
+  std::vector<int32_t> boundary_h;  // boundary_value(ndf, 2) for enforce_bc (empty: not set)
 };
+
+// Queue enforce_bc over all cells of `sp` on `field` (sp must have its
+// boundary values set).
+int lfr_enforce_bc_queue(lfr_ctx* ctx, int prec, void* field, const lfr_space* sp);
 
 // One class of output-dof columns with identical structure: every column has
 // `height` dofs and its contributions come from `ngroup` cells, each

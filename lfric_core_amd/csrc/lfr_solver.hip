@@ -352,10 +352,11 @@ void launch_apply_shape(hipStream_t s, const lfr_plan_class& c, const T* M, cons
     launch_apply_t<T, NDF_IN, 0, 2>(s, c, M, x, y, mi, nl, n3, ndf, ndf, dotv, part, stop);
 }
 
-// y = A x (overwrite); with dotv: partials of <y, dotv> into part[0..napply)
+// y = A x (overwrite) by the gather kernels; with dotv: partials of
+// <y, dotv> into part[0..napply)
 template <typename T>
-int apply_op(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv, double* part,
-             const double* stop) {
+int apply_gather(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv, double* part,
+                 const double* stop) {
   const lfr_space* sp = A->space;
   const int nl = sp->nlayers;
   const int64_t n3 = (int64_t)sp->ncell * nl;
@@ -407,11 +408,47 @@ switch (sp->ndf) {
   return LFR_OK;
 }
 
+// Partial sums of <a, b> over a grid of exactly nblk blocks (part[0..nblk)).
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_dot_partials(const T* __restrict__ a, const T* __restrict__ b, int64_t n, double* __restrict__ part,
+               const double* __restrict__ stop) {
+  if (stop && stop[0] != 0.0) return;
+  __shared__ double lds[kThreads / 64];
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    v += (double)a[i] * (double)b[i];
+  v = lfr::block_sum(v, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// y = A x as the operator object applies it: the gather apply, then
+// enforce_bc when the operator asks for it (the fused dot product is then
+// formed after the mask, over the same napply partial slots).
+template <typename T>
+int apply_op(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv, double* part,
+             const double* stop) {
+  if (!A->apply_bc) return apply_gather<T>(ctx, A, x, y, dotv, part, stop);
+  int rc = apply_gather<T>(ctx, A, x, y, (const T*)nullptr, nullptr, stop);
+  if (rc) return rc;
+  rc = lfr_enforce_bc_queue(ctx, (int)sizeof(T), y, A->space);
+  if (rc) return rc;
+  if (dotv) {
+    const int np = apply_partials(A->plan, (int)sizeof(T));
+    hipLaunchKernelGGL(k_dot_partials<T>, dim3(np), dim3(kThreads), 0, ctx->stream, (const T*)y, dotv,
+                       (int64_t)A->space->undf, part, stop);
+    LFR_CHECK_LAUNCH(ctx);
+  }
+  return LFR_OK;
+}
+
 int check_op(lfr_ctx* ctx, const lfr_lma_op* A, const char* who) {
   LFR_REQUIRE(ctx, A && A->matrix_d && A->space && A->plan, "%s: incomplete operator", who);
   LFR_REQUIRE(ctx, A->prec == 4 || A->prec == 8, "%s: prec %d", who, A->prec);
   LFR_REQUIRE(ctx, A->plan->space == A->space && A->plan->kind == 0,
               "%s: operator plan must be a matrix_vector plan of its space", who);
+  LFR_REQUIRE(ctx, !A->apply_bc || (int)A->space->boundary_h.size() == 2 * A->space->ndf,
+              "%s: apply_bc set but the space has no boundary values", who);
   // build the column records now: never inside a captured graph
   if (lfr_records_enabled()) {
     int stride = 0;

@@ -241,3 +241,61 @@ int lfr_vec_dot_dev(lfr_ctx* ctx, int prec, int64_t n, const void* x, const void
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// enforce_bc (components/science/source/kernel/algebra/sci_enforce_bc_kernel_mod.F90:63-91)
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kBcMaxNdf = 64;
+struct BcValues {
+  int v[2][kBcMaxNdf];
+};
+
+// thread = (cell, df, end): field(map(df) + (end ? nl-1 : 0)) *= value.  Cells
+// sharing a dof carry the same flag for it, so concurrent updates agree.
+template <typename T>
+__global__ void k_enforce_bc(T* __restrict__ field, const int32_t* __restrict__ map0, int ncell, int ndf, int nl,
+                             BcValues bv) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ncell * ndf * 2) return;
+  const int e = (int)(t & 1);
+  const int64_t cd = t >> 1;
+  const int df = (int)(cd % ndf);
+  const int v = bv.v[e][df];
+  if (v == 1) return;
+  const int64_t d = (int64_t)map0[cd] + (e ? nl - 1 : 0);
+  field[d] = field[d] * (T)v;
+}
+
+}  // namespace
+
+int lfr_enforce_bc_queue(lfr_ctx* ctx, int prec, void* field, const lfr_space* sp) {
+  LFR_REQUIRE(ctx, sp && field && (prec == 4 || prec == 8), "enforce_bc: bad arguments");
+  LFR_REQUIRE(ctx, (int)sp->boundary_h.size() == 2 * sp->ndf, "enforce_bc: the space has no boundary values");
+  LFR_REQUIRE(ctx, sp->ndf <= kBcMaxNdf, "enforce_bc: ndf %d > %d", sp->ndf, kBcMaxNdf);
+  BcValues bv;
+  for (int e = 0; e < 2; ++e)
+    for (int df = 0; df < sp->ndf; ++df) bv.v[e][df] = sp->boundary_h[df + sp->ndf * e];
+  const int64_t n = (int64_t)sp->ncell * sp->ndf * 2;
+  const unsigned g = (unsigned)((n + 255) / 256);
+  if (prec == 4)
+    hipLaunchKernelGGL(k_enforce_bc<float>, dim3(g), dim3(256), 0, ctx->stream, (float*)field, sp->map0_d, sp->ncell,
+                       sp->ndf, sp->nlayers, bv);
+  else
+    hipLaunchKernelGGL(k_enforce_bc<double>, dim3(g), dim3(256), 0, ctx->stream, (double*)field, sp->map0_d,
+                       sp->ncell, sp->ndf, sp->nlayers, bv);
+  LFR_CHECK_LAUNCH(ctx);
+  return LFR_OK;
+}
+
+extern "C" int lfr_space_set_boundary(lfr_space* sp, const int32_t* bv) {
+  if (!sp || !bv) return LFR_ERR_ARG;
+  sp->boundary_h.assign(bv, bv + 2 * sp->ndf);
+  return LFR_OK;
+}
+
+extern "C" int lfr_enforce_bc(lfr_ctx* ctx, int prec, void* field_d, const lfr_space* space) {
+  if (!ctx) return LFR_ERR_ARG;
+  return lfr_enforce_bc_queue(ctx, prec, field_d, space);
+}

@@ -152,6 +152,19 @@ class FunctionSpace:
         return np.ascontiguousarray(self.dofmap)  # row c holds map(:, c)
 
 
+def boundary_dofs(fs: "FunctionSpace") -> np.ndarray:
+    """boundary_value(ndf, 2) of get_boundary_dofs: 0 for a dof on the bottom
+    (column 1) / top (column 2) entity of the cell, 1 otherwise
+    (function_space_constructor_helper_functions_mod.F90:678-680, 884-885,
+    930-932: dofs of the bottom/top face, edges and vertices).  Those are
+    exactly the dofs of nl+1-level columns: shift 0 bottom, shift 1 top."""
+    bv = np.ones((fs.ndf, 2), dtype=np.int32)
+    stack = fs.dof_height == fs.nlayers + 1
+    bv[stack & (fs.dof_shift == 0), 0] = 0
+    bv[stack & (fs.dof_shift == 1), 1] = 0
+    return bv
+
+
 def build_dofmap(mesh: Mesh2D, space: str, nlayers: int, k: int = 0,
                  k_v: int | None = None) -> FunctionSpace:
     """Serial dofmap_setup (all cells owned, ndata = 1)."""

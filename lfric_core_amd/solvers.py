@@ -24,7 +24,7 @@ _P = C.c_void_p
 
 
 class LmaOp(C.Structure):
-    _fields_ = [("prec", C.c_int), ("matrix_d", _P), ("space", _P), ("plan", _P)]
+    _fields_ = [("prec", C.c_int), ("matrix_d", _P), ("space", _P), ("plan", _P), ("apply_bc", C.c_int)]
 
 
 class SolverParams(C.Structure):
@@ -64,7 +64,8 @@ def lma_op(op: "api.LMAOperator") -> LmaOp:
     if op.to_space is not op.from_space:
         raise LfrError(1, "solvers need a square operator (to_space is from_space)")
     plan = op.to_space.plan(op.colouring, 0)
-    return LmaOp(api._prec(op.matrix), _P(op.matrix.data_ptr()), op.to_space.h, plan.h)
+    return LmaOp(api._prec(op.matrix), _P(op.matrix.data_ptr()), op.to_space.h, plan.h,
+                 int(bool(op.apply_bc)))
 
 
 @dataclass

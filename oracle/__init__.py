@@ -225,6 +225,22 @@ def cma_app(lhs, x, cm, geom, dofmap_to, dofmap_from, use_ref=False):
     return lhs
 
 
+def enforce_bc(field, dofmap, boundary, nlayers, use_ref=False):
+    """enforce_bc over all cells, in place.  boundary: (ndf, 2) flags."""
+    bv = np.asfortranarray(boundary, dtype=np.int32)
+    dm = np.ascontiguousarray(dofmap, dtype=np.int32)
+    ncell, ndf = dm.shape
+    if use_ref:
+        getattr(ref(), f"ref_invoke_enforce_bc_{_suf(field.dtype)}")(
+            _I(ncell), _I(nlayers), field.ctypes.data_as(C.c_void_p), _I(ndf), _I(field.size),
+            dm.ctypes.data_as(C.c_void_p), bv.ctypes.data_as(C.c_void_p))
+    else:
+        getattr(lib(), f"orc_invoke_enforce_bc_{_suf(field.dtype)}")(
+            _I(ncell), _I(nlayers), field.ctypes.data_as(C.c_void_p), _I(ndf),
+            dm.ctypes.data_as(C.c_void_p), bv.ctypes.data_as(C.c_void_p))
+    return field
+
+
 def dot(x, y, nthreads=None) -> float:
     nt = nthreads_default() if nthreads is None else nthreads
     fn = getattr(lib(), f"orc_dot_{_suf(x.dtype)}")

@@ -199,6 +199,27 @@ DEF_CMAAPP(double, r64)
 DEF_CMAASM(float, r32)
 DEF_CMAASM(double, r64)
 
+/* enforce_bc_code (components/science/source/kernel/algebra/
+ * sci_enforce_bc_kernel_mod.F90:63-91) over all cells in cell order:
+ * field(map(df)) *= bv(df,1); field(map(df)+nl-1) *= bv(df,2); bv(ndf,2). */
+#define DEF_ENFORCE_BC(T, SUF)                                                          \
+  void orc_invoke_enforce_bc_##SUF(int ncell, int nlayers, T* field, int ndf,          \
+                                   const int* dofmap, const int* bv) {                 \
+    for (int c = 0; c < ncell; ++c) {                                                   \
+      const int* map = dofmap + (int64_t)c * ndf;                                       \
+      for (int df = 0; df < ndf; ++df) {                                                \
+        int64_t d = (int64_t)map[df] - 1;                                               \
+        field[d] = field[d] * (T)bv[df];                                                \
+      }                                                                                 \
+      for (int df = 0; df < ndf; ++df) {                                                \
+        int64_t d = (int64_t)map[df] - 1 + nlayers - 1;                                 \
+        field[d] = field[d] * (T)bv[df + ndf];                                          \
+      }                                                                                 \
+    }                                                                                   \
+  }
+DEF_ENFORCE_BC(float, r32)
+DEF_ENFORCE_BC(double, r64)
+
 /* ------------------------------------------------------------------------ */
 /* Invoke-level loops (what the generated PSy layer does around a kernel)   */
 /* ------------------------------------------------------------------------ */

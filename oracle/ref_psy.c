@@ -146,3 +146,24 @@ void ref_asm_r32(int cell, int nlayers, int ncell_2d, int ncell_3d, const float*
       &cell, &nlayers, &ncell_2d, &ncell_3d, ls, cm, &nrow, &ncol, &bw, &alpha, &beta,
       &gamma_m, &gamma_p, &ndf_to, cbd_to, &ndf_from, cbd_from);
 }
+
+extern void FK(sci_enforce_bc_kernel_mod, enforce_bc_code_r_single)(const int* nlayers, float* field,
+                                                                    const int* ndf, const int* undf,
+                                                                    const int* map, const int* bv);
+extern void FK(sci_enforce_bc_kernel_mod, enforce_bc_code_r_double)(const int* nlayers, double* field,
+                                                                    const int* ndf, const int* undf,
+                                                                    const int* map, const int* bv);
+
+/* enforce_bc over all cells in cell order (the kernel is called per column) */
+void ref_invoke_enforce_bc_r32(int ncell, int nlayers, float* field, int ndf, int undf, const int* dofmap,
+                               const int* bv) {
+  for (int c = 0; c < ncell; ++c)
+    FK(sci_enforce_bc_kernel_mod, enforce_bc_code_r_single)(&nlayers, field, &ndf, &undf,
+                                                            dofmap + (int64_t)c * ndf, bv);
+}
+void ref_invoke_enforce_bc_r64(int ncell, int nlayers, double* field, int ndf, int undf, const int* dofmap,
+                               const int* bv) {
+  for (int c = 0; c < ncell; ++c)
+    FK(sci_enforce_bc_kernel_mod, enforce_bc_code_r_double)(&nlayers, field, &ndf, &undf,
+                                                            dofmap + (int64_t)c * ndf, bv);
+}

@@ -138,3 +138,49 @@ def test_gmres_fp64_matches_oracle(ctx, prob):
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-10 * np.linalg.norm(xo)
     res = cfg.b - A(x.cpu().numpy())
     assert np.linalg.norm(res) < 1e-9 * np.linalg.norm(cfg.b)
+
+
+def test_enforce_bc_bitexact(ctx, prob):
+    from lfric_core_amd import api
+    from lfric_core_amd.function_space import boundary_dofs
+    cfg, w2 = prob
+    w2.set_boundary()
+    for dt in (np.float32, np.float64):
+        f = cfg.b.astype(dt)
+        ref = oracle.enforce_bc(f.copy(), cfg.w2.dofmap, boundary_dofs(cfg.w2), cfg.nlayers)
+        g = torch.from_numpy(f.copy()).to(ctx.device)
+        api.enforce_bc(g, w2)
+        np.testing.assert_array_equal(g.cpu().numpy(), ref)
+
+
+def test_cg_with_boundary_mask_matches_oracle(ctx, prob):
+    """mass_matrix_operator with apply_bc: y = enforce_bc(M x) inside CG (FP64)."""
+    from lfric_core_amd import api
+    from lfric_core_amd.function_space import boundary_dofs
+    from lfric_core_amd.solvers import ConjugateGradient
+    cfg, w2 = prob
+    w2.set_boundary()
+    bv = boundary_dofs(cfg.w2)
+    M = cfg.mass
+    b = oracle.enforce_bc(cfg.b.copy(), cfg.w2.dofmap, bv, cfg.nlayers)
+
+    def apply(v):
+        y = oracle.invoke_coloured(0, cfg.colouring, cfg.nlayers, np.zeros_like(v), v, M,
+                                   cfg.w2.dofmap, cfg.w2.dofmap)
+        return oracle.enforce_bc(y, cfg.w2.dofmap, bv, cfg.nlayers)
+
+    xo = np.zeros(cfg.w2.undf)
+    it_o, conv_o, _, _ = cg_solve(apply, xo, b, r_tol=1e-10, max_iter=500)
+    op = api.LMAOperator(torch.from_numpy(M).to(ctx.device), w2, w2, cfg.colouring, apply_bc=True)
+    x = torch.zeros(cfg.w2.undf, dtype=torch.float64, device=ctx.device)
+    st = ConjugateGradient(op, r_tol=1e-10, max_iter=500).apply(x, torch.from_numpy(b).to(ctx.device))
+    assert conv_o and st.converged and st.iterations == it_o
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo)
+    bnd = np.zeros(cfg.w2.undf, bool)
+    for df in range(cfg.w2.ndf):
+        if bv[df, 0] == 0:
+            bnd[cfg.w2.dofmap[:, df] - 1] = True
+        if bv[df, 1] == 0:
+            bnd[cfg.w2.dofmap[:, df] - 1 + cfg.nlayers - 1] = True
+    assert np.all(xg[bnd] == 0.0)
