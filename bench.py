@@ -44,6 +44,19 @@ def apply_bytes(n3: int, ndf_to: int, ndf_from: int, undf_to: int, undf_from: in
     return b
 
 
+def profiled_traffic(key: str = "traffic_bytes_fp32_apply"):
+    """HBM bytes per apply measured by the rocprofv3 PMC runs recorded in
+    profiles/k_gather_w2_c96l70.md (FETCH_SIZE x 2 + WRITE_SIZE), or None."""
+    p = ROOT / "profiles" / "k_gather_w2_c96l70.md"
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith(key + ":"):
+                return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def time_events(fn, steps: int, torch) -> float:
     """Average ms per call of fn over `steps` calls, HIP events on the current stream."""
     st = torch.cuda.current_stream()
@@ -390,7 +403,7 @@ def main():
                    "ncell_3d": n3},
         "roofline": {"bound": "hbm", "achieved": round(r32["gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r32["gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": profiled_traffic() if nc == 96 else None},
         "fp64": {"gdofs": round(res["fp64"]["gdofs"], 3), "ms": round(res["fp64"]["ms"], 6),
                  "gbs": round(res["fp64"]["gbs"], 1),
                  "roofline_frac": round(res["fp64"]["gbs"] / HBM_PEAK_GBS, 4)},

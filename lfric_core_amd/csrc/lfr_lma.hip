@@ -34,17 +34,16 @@ unsigned pad8(unsigned nb) { return (nb + 7u) & ~7u; }
 
 }  // namespace
 
-// Non-temporal (streaming) loads of the matrix in the record kernels.
-// Default: on for FP64 matrices, off for FP32 (measured faster each way on
-// the W2 C96L70 apply); LFR_GATHER_NT=0 or =1 forces one setting for both.
+// Non-temporal (streaming) loads of the matrix in the one-level-per-thread
+// record kernels (measured faster in both precisions on the W2 C96L70 face
+// stacks); LFR_GATHER_NT=0 turns them off.
 bool lfr_gather_nt(int prec) {
-  static const int mode = [] {
+  (void)prec;
+  static const bool on = [] {
     const char* e = getenv("LFR_GATHER_NT");
-    if (e && e[0] == '0') return 0;
-    if (e && e[0] == '1') return 1;
-    return -1;
+    return !(e && e[0] == '0');
   }();
-  return mode < 0 ? prec == 8 : mode == 1;
+  return on;
 }
 
 // Levels per thread of the vectorised record gather: 4 (FP32) or 2 (FP64) by

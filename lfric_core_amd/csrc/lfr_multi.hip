@@ -445,11 +445,11 @@ int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void*
     else                                                                \
       launch_multi<false, NTP, NF, PIG, IGW>(s, h, y, x, overwrite);    \
     break;
-  // W2 k=1: row tiles per wave (2: 5 waves per column, ~100 registers;
-  // 3: 3 waves, ~200 registers); LFR_MULTI_IGW=3 selects the latter
+  // W2 k=1: row tiles per wave (3: 3 waves per column, ~200 registers,
+  // measured fastest; 2: 5 waves, ~120 registers, LFR_MULTI_IGW=2)
   static const int igw36 = [] {
     const char* e = getenv("LFR_MULTI_IGW");
-    return (e && e[0] == '3') ? 3 : 2;
+    return (e && e[0] == '2') ? 2 : 3;
   }();
   if (key == 36 * 1000 + 36 * 10 + 1 && igw36 == 2) {
     if (bf)
