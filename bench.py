@@ -367,11 +367,18 @@ def main():
         dst.copy_(src)
     ms_copy = time_events(lambda: dst.copy_(src), 20, torch)
     ms_sum = time_events(lambda: src.sum(), 20, torch)
-    extra["hbm_reference"] = {"torch_copy_gbs": round(2 * src.numel() * 4 / (ms_copy * 1e-3) / 1e9, 1),
-                              "torch_sum_read_gbs": round(src.numel() * 4 / (ms_sum * 1e-3) / 1e9, 1),
-                              "bytes": src.numel() * 4}
+    ms_lcopy = time_events(lambda: api.copy(ctx, dst, src), 20, torch)
+    ms_ldot = time_events(lambda: api.dot(ctx, src, dst), 20, torch)     # reads both (2 GiB)
+    nb = src.numel() * 4
+    extra["hbm_reference"] = {"torch_copy_gbs": round(2 * nb / (ms_copy * 1e-3) / 1e9, 1),
+                              "torch_sum_read_gbs": round(nb / (ms_sum * 1e-3) / 1e9, 1),
+                              "lfr_copy_gbs": round(2 * nb / (ms_lcopy * 1e-3) / 1e9, 1),
+                              "lfr_dot_read_gbs": round(2 * nb / (ms_ldot * 1e-3) / 1e9, 1),
+                              "bytes": nb}
     log(f"reference: torch copy {extra['hbm_reference']['torch_copy_gbs']} GB/s, "
-        f"sum (read) {extra['hbm_reference']['torch_sum_read_gbs']} GB/s")
+        f"sum (read) {extra['hbm_reference']['torch_sum_read_gbs']} GB/s, lfr copy "
+        f"{extra['hbm_reference']['lfr_copy_gbs']} GB/s, lfr dot (read, incl. host sync) "
+        f"{extra['hbm_reference']['lfr_dot_read_gbs']} GB/s")
     del src, dst
     torch.cuda.empty_cache()
 

@@ -171,15 +171,14 @@ struct RecShape {
 template <typename T>
 __device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
 
-template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT = false, bool NT = true>
-__global__ void __launch_bounds__(kThreads)
-k_gather_rec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
-             const int32_t* __restrict__ rec, int ncol, int H, int nl, int64_t n3, int ndf_out,
-             int overwrite, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
-             double* __restrict__ part = nullptr, const double* __restrict__ stop = nullptr) {
+// One thread's work of the record gather for logical block lb; returns
+// acc * dotv (FP64) for the fused inner product (0 when no dotv or invalid).
+template <typename T, int NDF_IN, int NG, int GS, int KIND, bool NT>
+__device__ __forceinline__ double gather_rec_body(const T* __restrict__ M, const T* __restrict__ x,
+                                                  T* __restrict__ y, const int32_t* __restrict__ rec, int ncol,
+                                                  int H, int nl, int64_t n3, int ndf_out, int overwrite,
+                                                  unsigned lb, const T* __restrict__ dotv) {
   using S = RecShape<NDF_IN, NG, GS>;
-  if (DOT && stop && stop[0] != 0.0) return;
-  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
   const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
   const bool valid = t < (int64_t)ncol * H;
   const int64_t tt = valid ? t : 0;
@@ -245,9 +244,21 @@ k_gather_rec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y
     }
   }
   if (valid) y[g0 + L] = acc;
+  return (dotv && valid) ? (double)acc * (double)dotv[g0 + L] : 0.0;
+}
+
+template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT = false, bool NT = true>
+__global__ void __launch_bounds__(kThreads)
+k_gather_rec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
+             const int32_t* __restrict__ rec, int ncol, int H, int nl, int64_t n3, int ndf_out,
+             int overwrite, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
+             double* __restrict__ part = nullptr, const double* __restrict__ stop = nullptr) {
+  if (DOT && stop && stop[0] != 0.0) return;
+  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
+  double v = gather_rec_body<T, NDF_IN, NG, GS, KIND, NT>(M, x, y, rec, ncol, H, nl, n3, ndf_out, overwrite, lb,
+                                                          DOT ? dotv : nullptr);
   if (DOT) {
     __shared__ double lds[kThreads / 64];
-    double v = valid ? (double)acc * (double)dotv[g0 + L] : 0.0;
     v = block_sum(v, lds);
     if (threadIdx.x == 0) part[lb] = v;
   }
@@ -281,16 +292,13 @@ __device__ __forceinline__ LVec<T, N> ldvec(const T* p) {
   }
 }
 
-template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT, int LPT>
-__global__ void __launch_bounds__(kThreads)
-k_gather_vec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
-             const int32_t* __restrict__ rec, int ncol, int nl, int64_t n3, int ndf_out,
-             int overwrite, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
-             double* __restrict__ part = nullptr, const double* __restrict__ stop = nullptr) {
+template <typename T, int NDF_IN, int NG, int GS, int KIND, int LPT>
+__device__ __forceinline__ double gather_vec_body(const T* __restrict__ M, const T* __restrict__ x,
+                                                  T* __restrict__ y, const int32_t* __restrict__ rec, int ncol,
+                                                  int nl, int64_t n3, int ndf_out, int overwrite, unsigned lb,
+                                                  const T* __restrict__ dotv) {
   using S = RecShape<NDF_IN, NG, GS>;
-  if (DOT && stop && stop[0] != 0.0) return;
   const int tpc = (nl + LPT - 1) / LPT;
-  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
   const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
   const bool valid = t < (int64_t)ncol * tpc;
   const int64_t tt = valid ? t : 0;
@@ -379,14 +387,27 @@ k_gather_vec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y
         if (u >= own) y[g0 + L0 + u] = acc[u];
     }
   }
+  double v = 0.0;
+  if (dotv && valid) {
+#pragma unroll
+    for (int u = 0; u < LPT; ++u)
+      if (u >= own) v += (double)acc[u] * (double)dotv[g0 + L0 + u];
+  }
+  return v;
+}
+
+template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT, int LPT>
+__global__ void __launch_bounds__(kThreads)
+k_gather_vec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
+             const int32_t* __restrict__ rec, int ncol, int nl, int64_t n3, int ndf_out,
+             int overwrite, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
+             double* __restrict__ part = nullptr, const double* __restrict__ stop = nullptr) {
+  if (DOT && stop && stop[0] != 0.0) return;
+  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
+  double v = gather_vec_body<T, NDF_IN, NG, GS, KIND, LPT>(M, x, y, rec, ncol, nl, n3, ndf_out, overwrite, lb,
+                                                           DOT ? dotv : nullptr);
   if (DOT) {
     __shared__ double lds[kThreads / 64];
-    double v = 0.0;
-    if (valid) {
-#pragma unroll
-      for (int u = 0; u < LPT; ++u)
-        if (u >= own) v += (double)acc[u] * (double)dotv[g0 + L0 + u];
-    }
     v = block_sum(v, lds);
     if (threadIdx.x == 0) part[lb] = v;
   }

@@ -119,7 +119,7 @@ __device__ __forceinline__ lfr_v4f mfma_step<true>(const lfr_v4f (&xa)[4], int v
 // [w*IGW, w*IGW + IGW).  Lane l: layer b = l >> 2 of the current 16-layer
 // block, q = l & 3 (rhs sub-index for the A operand, row-in-tile for B/D).
 // Vertically paired tiles (0..2*PIG) all belong to wave 0.
-template <bool BF, int NTP, int NF, int PIG, int IGW>
+template <bool BF, int NTP, int NF, int PIG, int IGW, int UNR = 1>
 __global__ void __launch_bounds__(64 * ((NTP / 4 + IGW - 1) / IGW))
 k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __restrict__ mp_,
           const int32_t* __restrict__ cells, const int32_t* __restrict__ map_to,
@@ -156,6 +156,7 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[gi][v] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
 
+#pragma unroll UNR
     for (int jq = 0; jq < JQ; ++jq) {
       lfr_v4f xa[4];
 #pragma unroll
@@ -229,13 +230,13 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
   }
 }
 
-template <bool BF, int NTP, int NF, int PIG, int IGW>
+template <bool BF, int NTP, int NF, int PIG, int IGW, int UNR = 1>
 void launch_multi(hipStream_t s, lfr_multi* h, float* y, const float* x, int overwrite) {
   constexpr int WS = (NTP / 4 + IGW - 1) / IGW;
   for (size_t i = 0; i + 1 < h->launch_off.size(); ++i) {
     const int n = h->launch_off[i + 1] - h->launch_off[i];
     if (n <= 0) continue;
-    hipLaunchKernelGGL((k_multi16<BF, NTP, NF, PIG, IGW>), dim3(n), dim3(64 * WS), 0, s, y, x,
+    hipLaunchKernelGGL((k_multi16<BF, NTP, NF, PIG, IGW, UNR>), dim3(n), dim3(64 * WS), 0, s, y, x,
                        (const void*)h->mp_d, h->cells_d + h->launch_off[i], h->to->map0_d, h->from->map0_d,
                        h->slot_d, h->first_d, h->nl, h->nt, overwrite);
   }
@@ -445,17 +446,21 @@ int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void*
     else                                                                \
       launch_multi<false, NTP, NF, PIG, IGW>(s, h, y, x, overwrite);    \
     break;
-  // W2 k=1: row tiles per wave (3: 3 waves per column, ~200 registers,
-  // measured fastest; 2: 5 waves, ~120 registers, LFR_MULTI_IGW=2)
-  static const int igw36 = [] {
-    const char* e = getenv("LFR_MULTI_IGW");
-    return (e && e[0] == '2') ? 2 : 3;
+  // W2 k=1 variants (row tiles per wave, j-quad loop unroll), selected by
+  // LFR_MULTI_VARIANT: 0 = 3 tiles (3 waves/column), 1 = 2 tiles (5 waves),
+  // 2 = 2 tiles with the j loop unrolled by 3, 3 = 3 tiles unrolled by 2
+  static const int var36 = [] {
+    const char* e = getenv("LFR_MULTI_VARIANT");
+    return (e && e[0] >= '0' && e[0] <= '3') ? e[0] - '0' : 0;
   }();
-  if (key == 36 * 1000 + 36 * 10 + 1 && igw36 == 2) {
-    if (bf)
-      launch_multi<true, 36, 36, 1, 2>(s, h, y, x, overwrite);
-    else
-      launch_multi<false, 36, 36, 1, 2>(s, h, y, x, overwrite);
+  if (key == 36 * 1000 + 36 * 10 + 1 && var36 != 0) {
+#define LFR_V(IG, UN)                                                   \
+  if (bf) launch_multi<true, 36, 36, 1, IG, UN>(s, h, y, x, overwrite); \
+  else launch_multi<false, 36, 36, 1, IG, UN>(s, h, y, x, overwrite);
+    if (var36 == 1) { LFR_V(2, 1) }
+    else if (var36 == 2) { LFR_V(2, 3) }
+    else { LFR_V(3, 2) }
+#undef LFR_V
     LFR_CHECK_LAUNCH(ctx);
     return LFR_OK;
   }
