@@ -198,6 +198,7 @@ __device__ __forceinline__ double gather_rec_body(const T* __restrict__ M, const
   T acc = (overwrite || !valid) ? T(0) : y[g0 + L];
   T mv[NG][GS][NDF_IN], xv[NG][GS][NDF_IN];
   bool ok[NG][GS];
+  int iis[NG][GS];
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
     const int* rg = r + 1 + g * S::kGroup;
@@ -208,6 +209,7 @@ __device__ __forceinline__ double gather_rec_body(const T* __restrict__ M, const
       const int k = L - (code & 1);
       ok[g][q] = code >= 0 && k >= 0 && k < nl;
       const int ii = code >= 0 ? (code >> 1) : 0;
+      iis[g][q] = ii;
       const int kk = k < 0 ? 0 : (k >= nl ? nl - 1 : k);
 #pragma unroll
       for (int j = 0; j < NDF_IN; ++j) {
@@ -244,7 +246,24 @@ __device__ __forceinline__ double gather_rec_body(const T* __restrict__ M, const
     }
   }
   if (valid) y[g0 + L] = acc;
-  return (dotv && valid) ? (double)acc * (double)dotv[g0 + L] : 0.0;
+  if (!dotv || !valid) return 0.0;
+  // Fused <y, x> of the solvers (dotv == x, square operator on one space):
+  // x at the output dof is the operand of any valid entry's own local dof
+  // (input dof ii of the cell at layer k is output dof g0 + L), so it is
+  // taken from the registers instead of being loaded again.
+  T self = T(0);
+  bool found = false;
+#pragma unroll
+  for (int g = 0; g < NG; ++g)
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+      T v = xv[g][q][0];
+#pragma unroll
+      for (int j = 1; j < NDF_IN; ++j) v = iis[g][q] == j ? xv[g][q][j] : v;
+      self = (ok[g][q] && !found) ? v : self;
+      found = found || ok[g][q];
+    }
+  return (double)acc * (double)self;
 }
 
 template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT = false, bool NT = true>
@@ -328,6 +347,7 @@ __device__ __forceinline__ double gather_vec_body(const T* __restrict__ M, const
   }
   LVec<T, LPT> mv[NG][GS][NDF_IN], xv[NG][GS][NDF_IN];
   bool ok[NG][GS];
+  int iis[NG][GS];
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
     const int* rg = r + 1 + g * S::kGroup;
@@ -337,6 +357,7 @@ __device__ __forceinline__ double gather_vec_body(const T* __restrict__ M, const
       const int code = rg[1 + q];
       ok[g][q] = code >= 0;
       const int ii = code >= 0 ? (code >> 1) : 0;
+      iis[g][q] = ii;
 #pragma unroll
       for (int j = 0; j < NDF_IN; ++j) {
         const int64_t mo = KIND == 0 ? (int64_t)ii + (int64_t)ndf_out * j
@@ -389,9 +410,28 @@ __device__ __forceinline__ double gather_vec_body(const T* __restrict__ M, const
   }
   double v = 0.0;
   if (dotv && valid) {
+    // fused <y, x> with dotv == x: x at the output dofs is the own-dof
+    // operand of the first valid entry (see gather_rec_body)
+    T self[LPT];
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) self[u] = T(0);
+    bool found = false;
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+      for (int q = 0; q < GS; ++q) {
+#pragma unroll
+        for (int u = 0; u < LPT; ++u) {
+          T w = xv[g][q][0].v[u];
+#pragma unroll
+          for (int j = 1; j < NDF_IN; ++j) w = iis[g][q] == j ? xv[g][q][j].v[u] : w;
+          self[u] = (ok[g][q] && !found) ? w : self[u];
+        }
+        found = found || ok[g][q];
+      }
 #pragma unroll
     for (int u = 0; u < LPT; ++u)
-      if (u >= own) v += (double)acc[u] * (double)dotv[g0 + L0 + u];
+      if (u >= own) v += (double)acc[u] * (double)self[u];
   }
   return v;
 }

@@ -41,14 +41,30 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return s;
 }
 
+// sum of p[threadIdx.x + k*kRedThreads] over k in increasing order (the
+// thread's share of a block-wide sum of n values); eight loads are issued
+// before they are added, the addition order is unchanged.
+__device__ __forceinline__ double strided_sum(const double* __restrict__ p, int n) {
+  double s = 0.0;
+  int b = threadIdx.x;
+  for (; b + 7 * kRedThreads < n; b += 8 * kRedThreads) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = p[b + q * kRedThreads];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += v[q];
+  }
+  for (; b < n; b += kRedThreads) s += p[b];
+  return s;
+}
+
 // Final stage: out[q] = sum over b of part[q*stride + b], b < nb, for
 // q < nq (one block per q).
 static __global__ void __launch_bounds__(kRedThreads)
 k_reduce_final(const double* __restrict__ part, int nb, int stride, double* __restrict__ out) {
   __shared__ double lds[kRedThreads / 64];
   const double* p = part + (int64_t)blockIdx.x * stride;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kRedThreads) s += p[b];
+  double s = strided_sum(p, nb);
   s = block_sum(s, lds);
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }

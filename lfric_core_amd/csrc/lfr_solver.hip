@@ -48,9 +48,7 @@ k_cg_alpha(const double* __restrict__ part, int np, double* __restrict__ S, int 
     return;
   }
   __shared__ double lds[kThreads / 64];
-  double s = 0.0;
-  for (int b = threadIdx.x; b < np; b += kThreads) s += part[b];
-  s = lfr::block_sum(s, lds);
+  const double s = lfr::block_sum(lfr::strided_sum(part, np), lds);
   if (threadIdx.x == 0) {
     S[S_ALPHA] = S[S_RZ] / s;
     S[S_ITER] = S[S_ITER] + 1.0;
@@ -159,7 +157,7 @@ __global__ void __launch_bounds__(kThreads)
 k_store_sum(const double* __restrict__ part, int np, double* __restrict__ S, int slot, int take_sqrt) {
   __shared__ double lds[kThreads / 64];
   double s = 0.0;
-  for (int b = threadIdx.x; b < np; b += kThreads) s += part[b];
+  s = lfr::strided_sum(part, np);
   s = lfr::block_sum(s, lds);
   if (threadIdx.x == 0) S[slot] = take_sqrt ? sqrt(s) : s;
 }
@@ -357,6 +355,9 @@ void launch_apply_shape(hipStream_t s, const lfr_plan_class& c, const T* M, cons
 template <typename T>
 int apply_gather(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv, double* part,
                  const double* stop) {
+  // the record kernels form the fused inner product from the operand
+  // registers, which holds for <A x, x> only
+  LFR_REQUIRE(ctx, !dotv || dotv == x, "apply: fused inner product must be <A x, x>");
   const lfr_space* sp = A->space;
   const int nl = sp->nlayers;
   const int64_t n3 = (int64_t)sp->ncell * nl;
