@@ -349,7 +349,32 @@ def main():
                      "adj_ms": ms_adj, "adj_ratio": ms_adj / ms}
         log(f"{name}: apply {ms*1e3:.1f} us  {res[name]['gdofs']:.1f} GDOF/s  "
             f"{res[name]['gbs']:.0f} GB/s; adjoint {ms_adj*1e3:.1f} us (ratio {ms_adj/ms:.2f})")
-        del M, op, x, y, xb
+        # the same operator packed once into gather order (lfr_lma_pack)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        pk = api.PackedOperator(op, kind=0)
+        t_pack = time.perf_counter() - t
+        pkt = api.PackedOperator(op, kind=1)
+        yp = torch.empty_like(x)
+        fwdp = lambda: pk.apply(yp, x, overwrite=True)
+        for _ in range(args.warmup):
+            fwdp()
+        ms_p = time_events(fwdp, args.steps, torch)
+        same = bool(torch.equal(yp, y))
+        adjp = lambda: pkt.apply(xb, y, overwrite=True)
+        for _ in range(args.warmup):
+            adjp()
+        ms_padj = time_events(adjp, args.steps, torch)
+        res[name].update({"packed_ms": ms_p, "packed_gdofs": cfg.w2.undf / (ms_p * 1e-3) / 1e9,
+                          "packed_gbs": nbytes / (ms_p * 1e-3) / 1e9, "packed_adj_ms": ms_padj,
+                          "packed_adj_ratio": ms_padj / ms_p, "pack_ms": t_pack * 1e3,
+                          "packed_bitwise_equal": same, "packed_nbytes": pk.nbytes})
+        log(f"{name} packed: apply {ms_p*1e3:.1f} us  {cfg.w2.undf/(ms_p*1e-3)/1e9:.1f} GDOF/s  "
+            f"{nbytes/(ms_p*1e-3)/1e9:.0f} GB/s; adjoint {ms_padj*1e3:.1f} us (ratio {ms_padj/ms_p:.2f}); "
+            f"pack {t_pack*1e3:.1f} ms; bitwise equal {same}")
+        pk.close()
+        pkt.close()
+        del M, op, x, y, xb, yp
         torch.cuda.empty_cache()
 
     if not args.no_solvers:

@@ -51,6 +51,7 @@ enum {
 typedef struct lfr_ctx lfr_ctx;
 typedef struct lfr_space lfr_space;
 typedef struct lfr_plan lfr_plan;
+typedef struct lfr_lma_packed lfr_lma_packed;
 
 /* ---------------------------------------------------------------------- */
 /* Context, memory, streams                                                */
@@ -142,6 +143,29 @@ int lfr_transpose_matrix_vector(lfr_ctx* ctx, int prec, void* lhs_d, const void*
                                 const void* matrix_d, const lfr_space* lhs_space,
                                 const lfr_space* x_space, const lfr_plan* plan, int overwrite);
 
+/* Packed operator for repeated applies of one LMA operator (the
+ * operator_type whose local_stencil, INF/operator/operator_mod.t90:31-122,
+ * is assembled once and applied many times, as the mass matrices in the
+ * solvers are).  Copies, in one device pass, the matrix rows that each
+ * output column of `plan` reads into one contiguous block per column, in
+ * the plan's summation order, so an apply streams the matrix front to back.
+ * kind-0 plan: the operator of lfr_matrix_vector (x_space = from_space);
+ * kind-1 plan: the operator of lfr_transpose_matrix_vector (x_space = the
+ * space of x).  The handle does not reference matrix_d afterwards; it holds
+ * about the matrix's bytes (plus zero padding where a column entry has no
+ * layer, 1.4 % for W2 lowest order).  Every class of the plan must have a
+ * record kernel (ndf_in 1, 6 or 8); otherwise LFR_ERR_ARG. */
+int lfr_lma_pack(lfr_ctx* ctx, int prec, const void* matrix_d, const lfr_space* x_space,
+                 const lfr_plan* plan, lfr_lma_packed** out);
+int lfr_lma_packed_destroy(lfr_lma_packed* pk);
+/* Device bytes held by a packed operator. */
+int lfr_lma_packed_info(const lfr_lma_packed* pk, long long* nbytes);
+/* lfr_matrix_vector (kind-0 pack) or lfr_transpose_matrix_vector (kind-1
+ * pack) with a packed operator: bit-identical results; lhs on the plan's
+ * space, x on x_space, overwrite != 0: lhs = A x. */
+int lfr_matrix_vector_packed(lfr_ctx* ctx, const lfr_lma_packed* pk, void* lhs_d, const void* x_d,
+                             int overwrite);
+
 
 /* Gradient of an LMA apply with respect to the local matrices:
  * Mbar(ik,i,j) (+)= alpha * ybar(map_to(i)+k) * x(map_from(j)+k), for the
@@ -198,6 +222,8 @@ typedef struct lfr_lma_op {
   int apply_bc;             /* 1: enforce_bc after every apply (the space's
                                boundary values must be set), as
                                mass_matrix_operator_type%apply with apply_bc */
+  const lfr_lma_packed* packed; /* NULL, or a kind-0 pack of matrix_d with
+                               `plan`: applies then read the packed copy     */
 } lfr_lma_op;
 
 /* Solver settings as in the reference constructors

@@ -50,6 +50,10 @@ SIGNATURES = {
     "lfr_dg_matrix_vector": [_P, _I, _P, _P, _P, _P, _P],
     "lfr_dg_inc_matrix_vector": [_P, _I, _P, _P, _P, _P, _P],
     "lfr_transpose_matrix_vector": [_P, _I, _P, _P, _P, _P, _P, _P, _I],
+    "lfr_lma_pack": [_P, _I, _P, _P, _P, C.POINTER(_P)],
+    "lfr_lma_packed_destroy": [_P],
+    "lfr_lma_packed_info": [_P, C.POINTER(C.c_longlong)],
+    "lfr_matrix_vector_packed": [_P, _P, _P, _P, _I],
     "lfr_lma_outer": [_P, _I, _P, _P, _P, _P, _P, _D, _I],
     "lfr_vec_setval": [_P, _I, _I64, _P, _D],
     "lfr_vec_copy": [_P, _I, _I64, _P, _P],

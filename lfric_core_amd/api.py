@@ -150,6 +150,7 @@ class LMAOperator:
     from_space: Space
     colouring: Colouring | None = None
     apply_bc: bool = False        # solvers: enforce_bc after every apply (mass_matrix_operator apply_bc)
+    packed: "PackedOperator | None" = None  # solvers: apply through this kind-0 pack
     _cache: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -243,6 +244,61 @@ def transpose_matrix_vector(lhs: torch.Tensor, x: torch.Tensor, op: LMAOperator,
                                             _P(x.data_ptr()), _P(op.matrix.data_ptr()),
                                             lsp.h, xsp.h, plan.h, int(overwrite)), ctx.h)
     return lhs
+
+
+class PackedOperator:
+    """Gather-order copy of an LMAOperator for repeated applies (lfr_lma_pack).
+
+    kind 0 packs the operator of ``matrix_vector`` (output on ``to_space``),
+    kind 1 that of ``transpose_matrix_vector`` (output on ``from_space``).
+    ``apply`` gives the same bits as the unpacked call; the pack holds its own
+    copy of the matrix (``nbytes``), so ``op.matrix`` may change or go away
+    afterwards without affecting it.
+    """
+
+    def __init__(self, op: LMAOperator, kind: int = 0, colouring: Colouring | None = None):
+        col = colouring or op.colouring
+        if col is None:
+            raise LfrError(1, "packing needs a colouring of the output space")
+        if kind not in (0, 1):
+            raise LfrError(1, f"kind must be 0 or 1, got {kind}")
+        self.kind = kind
+        self.out_space, self.x_space = ((op.to_space, op.from_space) if kind == 0
+                                        else (op.from_space, op.to_space))
+        self.plan = self.out_space.plan(col, kind)
+        self.dtype = op.matrix.dtype
+        self.ctx = op.ctx
+        self.ctx.sync_stream()
+        h = _P()
+        check(lib().lfr_lma_pack(self.ctx.h, _prec(op.matrix), _P(op.matrix.data_ptr()),
+                                 self.x_space.h, self.plan.h, C.byref(h)), self.ctx.h)
+        self.h = h
+
+    @property
+    def nbytes(self) -> int:
+        n = C.c_longlong()
+        check(lib().lfr_lma_packed_info(self.h, C.byref(n)))
+        return n.value
+
+    def apply(self, lhs: torch.Tensor, x: torch.Tensor, overwrite: bool = False) -> torch.Tensor:
+        """lhs (+)= A x (kind 0) or A^T x (kind 1), bit-identical to the unpacked apply."""
+        _field(lhs, self.out_space, "lhs", self.dtype)
+        _field(x, self.x_space, "x", self.dtype)
+        self.ctx.sync_stream()
+        check(lib().lfr_matrix_vector_packed(self.ctx.h, self.h, _P(lhs.data_ptr()),
+                                             _P(x.data_ptr()), int(overwrite)), self.ctx.h)
+        return lhs
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lfr_lma_packed_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # --- vector algebra ---------------------------------------------------------

@@ -62,6 +62,8 @@ struct lfr_plan {
   // nullptr where the class shape has no record kernel)
   // (keyed by the input space's uid, which is never reused)
   mutable std::vector<std::pair<uint64_t, std::vector<int32_t*>>> records;
+  // records of the unified memory-order gather, per input space
+  mutable std::vector<std::pair<uint64_t, int32_t*>> urecords;
 };
 
 // Column records of `plan` class `ci` for input space `in`, built on first
@@ -70,12 +72,44 @@ const int32_t* lfr_plan_records(lfr_ctx* ctx, const lfr_plan* plan, size_t ci, c
                                 int* stride);
 bool lfr_record_shape(int ndf_in, int ngroup, int gsize);
 bool lfr_records_enabled();
+bool lfr_gather_vec_nt(bool packed);
+// The plan qualifies for the unified memory-order gather (every column has
+// two entries and nl or nl+1 levels; ndf_in 1 or 6; nl >= lpt).
+bool lfr_plan_unified(const lfr_plan* plan, int ndf_in, int lpt);
+int lfr_plan_ncolumns_total(const lfr_plan* plan);
+int lfr_unified_lpt(int prec);
+// Unified records for input space `in` (built on first use), in column
+// memory order; nullptr on allocation failure (error set on ctx).
+const int32_t* lfr_plan_urecords(lfr_ctx* ctx, const lfr_plan* plan, const lfr_space* in);
+// (cell, code) of entries 0 and 1 of every column in memory order (host).
+void lfr_plan_uentries(const lfr_plan* plan, std::vector<int32_t>& ent);
+// Blocks of the unified launch for levels-per-thread lpt.
+unsigned lfr_unified_blocks(const lfr_plan* plan, int lpt);
 bool lfr_gather_nt(int prec);
 // Levels per thread of the record gather for plan class `c` (1: one level per
 // thread; 2 or 4: the vectorised kernel, used for classes of exactly nl
 // levels with a record shape); blocks of 256 threads that class launches.
-int lfr_class_lpt(const lfr_plan_class& c, int nl, int prec, int ndf_in);
+int lfr_class_lpt(const lfr_plan_class& c, int nl, int prec, int ndf_in, bool packed = false);
 unsigned lfr_class_blocks(const lfr_plan_class& c, int lpt);
+
+// Packed operator: the matrix rows each output column of `plan` reads,
+// stored per column in the plan's order, entry-major, then input dof, then
+// level (class heights), zero where an entry has no cell layer.
+struct lfr_lma_packed {
+  lfr_ctx* ctx = nullptr;
+  int prec = 0;
+  int kind = 0;
+  const lfr_plan* plan = nullptr;
+  const lfr_space* in = nullptr;
+  int ndf_out = 0;
+  void* data_d = nullptr;
+  std::vector<int64_t> off;  // element offset of each class's block
+  std::vector<int64_t> pcs;  // elements per column of each class
+  int64_t nbytes = 0;
+  // unified layout: every column in memory order, 2 entries x ndf_in rows
+  // of nl+1 levels (the unified gather's packed form); off/pcs have one slot
+  bool unified = false;
+};
 
 int lfr_set_error(lfr_ctx* ctx, int code, const char* fmt, ...);
 

@@ -57,8 +57,20 @@ int lfr_gather_lpt(int prec) {
   return forced ? forced : (prec == 4 ? 4 : 2);
 }
 
-int lfr_class_lpt(const lfr_plan_class& c, int nl, int prec, int ndf_in) {
-  if (!lfr_records_enabled() || !lfr_record_shape(ndf_in, c.ngroup, c.gsize) || c.height != nl) return 1;
+// Non-temporal matrix loads in the vectorised and unified gathers: off by
+// default (measured slower, packed or not); LFR_GATHER_VEC_NT=1 forces.
+bool lfr_gather_vec_nt(bool packed) {
+  static const int forced = [] {
+    const char* e = getenv("LFR_GATHER_VEC_NT");
+    return (e && (e[0] == '0' || e[0] == '1')) ? e[0] - '0' : -1;
+  }();
+  (void)packed;
+  return forced == 1;
+}
+
+int lfr_class_lpt(const lfr_plan_class& c, int nl, int prec, int ndf_in, bool packed) {
+  if (!(packed || lfr_records_enabled()) || !lfr_record_shape(ndf_in, c.ngroup, c.gsize) || c.height != nl)
+    return 1;
   const int lpt = lfr_gather_lpt(prec);
   return nl >= lpt ? lpt : 1;
 }
@@ -134,12 +146,12 @@ void launch_gather_any(hipStream_t s, int kind, unsigned nb, const T* M, const T
 template <typename T, int NDF_IN, int NG, int GS>
 void launch_rec_t(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
                   const int32_t* rec, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
-                  int overwrite) {
+                  int overwrite, int64_t pcs) {
   const unsigned nbp = pad8(nb);
   const bool nt = lfr_gather_nt((int)sizeof(T));
 #define LREC(KD, NTV)                                                                             \
   hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, KD, false, NTV>), dim3(nbp), dim3(kThreads), \
-                     0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, overwrite, nbp,       \
+                     0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, overwrite, pcs, nbp,  \
                      (const T*)nullptr, (double*)nullptr, (const double*)nullptr)
   if (kind == 0) {
     if (nt) LREC(0, true); else LREC(0, false);
@@ -151,24 +163,27 @@ void launch_rec_t(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, 
 
 template <typename T, int NDF_IN, int NG, int GS, int LPT>
 void launch_vec_t(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y, const int32_t* rec,
-                  const lfr_plan_class& c, int nl, int64_t n3, int ndf_out, int overwrite) {
+                  const lfr_plan_class& c, int nl, int64_t n3, int ndf_out, int overwrite, int64_t pcs) {
   const unsigned nbp = pad8(nb);
-  if (kind == 0)
-    hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, 0, false, LPT>), dim3(nbp), dim3(kThreads), 0, s, M,
-                       x, y, rec, c.ncol, nl, n3, ndf_out, overwrite, nbp, (const T*)nullptr, (double*)nullptr,
-                       (const double*)nullptr);
-  else
-    hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, 1, false, LPT>), dim3(nbp), dim3(kThreads), 0, s, M,
-                       x, y, rec, c.ncol, nl, n3, ndf_out, overwrite, nbp, (const T*)nullptr, (double*)nullptr,
-                       (const double*)nullptr);
+  const bool nt = lfr_gather_vec_nt(pcs > 0);
+#define LVEC(KD, NTV)                                                                                   \
+  hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, KD, false, LPT, NTV>), dim3(nbp), dim3(kThreads), 0, \
+                     s, M, x, y, rec, c.ncol, nl, n3, ndf_out, overwrite, pcs, nbp, (const T*)nullptr,  \
+                     (double*)nullptr, (const double*)nullptr)
+  if (kind == 0) {
+    if (nt) LVEC(0, true); else LVEC(0, false);
+  } else {
+    if (nt) LVEC(1, true); else LVEC(1, false);
+  }
+#undef LVEC
 }
 
 template <typename T, int NDF_IN, int LPT>
 bool launch_vec_shape(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y, const int32_t* rec,
-                      const lfr_plan_class& c, int nl, int64_t n3, int ndf_out, int overwrite) {
+                      const lfr_plan_class& c, int nl, int64_t n3, int ndf_out, int overwrite, int64_t pcs) {
 #define SHAPE(NGV, GSV)                                                                               \
   if (c.ngroup == NGV && c.gsize == GSV) {                                                            \
-    launch_vec_t<T, NDF_IN, NGV, GSV, LPT>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite); \
+    launch_vec_t<T, NDF_IN, NGV, GSV, LPT>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite, pcs); \
     return true;                                                                                      \
   }
   SHAPE(2, 1) SHAPE(1, 1) SHAPE(4, 1) SHAPE(3, 1) SHAPE(1, 2) SHAPE(2, 2) SHAPE(3, 2) SHAPE(4, 2)
@@ -178,41 +193,88 @@ bool launch_vec_shape(hipStream_t s, int kind, unsigned nb, const T* M, const T*
 
 template <typename T, int NDF_IN>
 bool launch_vec_any(hipStream_t s, int lpt, int kind, unsigned nb, const T* M, const T* x, T* y,
-                    const int32_t* rec, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out, int overwrite) {
-  if (lpt == 2) return launch_vec_shape<T, NDF_IN, 2>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite);
-  if (lpt == 4) return launch_vec_shape<T, NDF_IN, 4>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite);
+                    const int32_t* rec, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out, int overwrite,
+                    int64_t pcs) {
+  if (lpt == 2) return launch_vec_shape<T, NDF_IN, 2>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite, pcs);
+  if (lpt == 4) return launch_vec_shape<T, NDF_IN, 4>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite, pcs);
   return false;
 }
 
 template <typename T, int NDF_IN>
 void launch_rec_shape(hipStream_t s, int kind, unsigned nb, const T* M, const T* x, T* y,
                       const int32_t* rec, const lfr_plan_class& c, int nl, int64_t n3, int ndf_out,
-                      int overwrite) {
+                      int overwrite, int64_t pcs) {
 #define SHAPE(NGV, GSV) \
   if (c.ngroup == NGV && c.gsize == GSV) \
-    return launch_rec_t<T, NDF_IN, NGV, GSV>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite);
+    return launch_rec_t<T, NDF_IN, NGV, GSV>(s, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite, pcs);
   SHAPE(2, 1) SHAPE(1, 2) SHAPE(1, 1) SHAPE(4, 2) SHAPE(3, 2) SHAPE(2, 2) SHAPE(4, 1) SHAPE(3, 1)
 #undef SHAPE
 }
 
+template <typename T, int NDF_IN, int KIND, int LPT>
+void launch_uni_t(hipStream_t s, bool nt, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur,
+                  int ncol, int nl, int64_t n3, int ndf_out, int overwrite, int64_t pcs) {
+  if (nt)
+    hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, KIND, false, LPT, true>), dim3(nbp), dim3(kThreads), 0, s, M, x,
+                       y, ur, ncol, nl, n3, ndf_out, overwrite, pcs, nbp, (const T*)nullptr, (double*)nullptr,
+                       (const double*)nullptr);
+  else
+    hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, KIND, false, LPT, false>), dim3(nbp), dim3(kThreads), 0, s, M,
+                       x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs, nbp, (const T*)nullptr, (double*)nullptr,
+                       (const double*)nullptr);
+}
+
+template <typename T, int NDF_IN>
+void launch_uni_k(hipStream_t s, int kind, int lpt, bool nt, unsigned nbp, const T* M, const T* x, T* y,
+                  const int32_t* ur, int ncol, int nl, int64_t n3, int ndf_out, int overwrite, int64_t pcs) {
+  if (kind == 0) {
+    if (lpt == 4) launch_uni_t<T, NDF_IN, 0, 4>(s, nt, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+    else launch_uni_t<T, NDF_IN, 0, 2>(s, nt, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+  } else {
+    if (lpt == 4) launch_uni_t<T, NDF_IN, 1, 4>(s, nt, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+    else launch_uni_t<T, NDF_IN, 1, 2>(s, nt, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+  }
+}
+
 template <typename T>
 int run_gather(lfr_ctx* ctx, const lfr_plan* plan, int kind, const T* M, const T* x, T* y,
-               const lfr_space* in, int ndf_out, int64_t n3, int overwrite) {
+               const lfr_space* in, int ndf_out, int64_t n3, int overwrite,
+               const lfr_lma_packed* pk = nullptr) {
   const int nl = plan->space->nlayers;
+  const int lptu = lfr_unified_lpt((int)sizeof(T));
+  if (pk ? pk->unified : (lfr_records_enabled() && lfr_plan_unified(plan, in->ndf, lptu))) {
+    const int32_t* ur = lfr_plan_urecords(ctx, plan, in);
+    if (!ur) return LFR_ERR_NOMEM;
+    const unsigned nbp = pad8(lfr_unified_blocks(plan, lptu));
+    const T* Mu = pk ? (const T*)pk->data_d : M;
+    const int64_t pcs = pk ? pk->pcs[0] : 0;
+    const bool nt = lfr_gather_vec_nt(pk != nullptr);
+    const int ncol = lfr_plan_ncolumns_total(plan);
+    if (in->ndf == 6)
+      launch_uni_k<T, 6>(ctx->stream, kind, lptu, nt, nbp, Mu, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+    else
+      launch_uni_k<T, 1>(ctx->stream, kind, lptu, nt, nbp, Mu, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+    LFR_CHECK_LAUNCH(ctx);
+    return LFR_OK;
+  }
   for (size_t ci = 0; ci < plan->classes.size(); ++ci) {
     const auto& c = plan->classes[ci];
     if (c.ncol == 0) continue;
     const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
     int stride = 0;
-    const int32_t* rec = lfr_records_enabled() ? lfr_plan_records(ctx, plan, ci, in, &stride) : nullptr;
-    const int lpt = rec ? lfr_class_lpt(c, nl, (int)sizeof(T), in->ndf) : 1;
+    const int32_t* rec = (pk || lfr_records_enabled()) ? lfr_plan_records(ctx, plan, ci, in, &stride) : nullptr;
+    // packed operator: this class's block, columns pcs elements apart
+    const T* Mc = pk ? (const T*)pk->data_d + pk->off[ci] : M;
+    const int64_t pcs = pk ? pk->pcs[ci] : 0;
+    if (pk && !rec) return lfr_set_error(ctx, LFR_ERR_NOMEM, "packed apply: column records unavailable");
+    const int lpt = rec ? lfr_class_lpt(c, nl, (int)sizeof(T), in->ndf, pk != nullptr) : 1;
     if (rec && lpt > 1) {
       const unsigned nbv = lfr_class_blocks(c, lpt);
       bool ok = false;
       switch (in->ndf) {
-        case 1: ok = launch_vec_any<T, 1>(ctx->stream, lpt, kind, nbv, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
-        case 6: ok = launch_vec_any<T, 6>(ctx->stream, lpt, kind, nbv, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
-        case 8: ok = launch_vec_any<T, 8>(ctx->stream, lpt, kind, nbv, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
+        case 1: ok = launch_vec_any<T, 1>(ctx->stream, lpt, kind, nbv, Mc, x, y, rec, c, nl, n3, ndf_out, overwrite, pcs); break;
+        case 6: ok = launch_vec_any<T, 6>(ctx->stream, lpt, kind, nbv, Mc, x, y, rec, c, nl, n3, ndf_out, overwrite, pcs); break;
+        case 8: ok = launch_vec_any<T, 8>(ctx->stream, lpt, kind, nbv, Mc, x, y, rec, c, nl, n3, ndf_out, overwrite, pcs); break;
       }
       if (!ok) return lfr_set_error(ctx, LFR_ERR_ARG, "no vectorised gather for this class shape");
       LFR_CHECK_LAUNCH(ctx);
@@ -220,9 +282,9 @@ int run_gather(lfr_ctx* ctx, const lfr_plan* plan, int kind, const T* M, const T
     }
     if (rec) {
       switch (in->ndf) {
-        case 1: launch_rec_shape<T, 1>(ctx->stream, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
-        case 6: launch_rec_shape<T, 6>(ctx->stream, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
-        case 8: launch_rec_shape<T, 8>(ctx->stream, kind, nb, M, x, y, rec, c, nl, n3, ndf_out, overwrite); break;
+        case 1: launch_rec_shape<T, 1>(ctx->stream, kind, nb, Mc, x, y, rec, c, nl, n3, ndf_out, overwrite, pcs); break;
+        case 6: launch_rec_shape<T, 6>(ctx->stream, kind, nb, Mc, x, y, rec, c, nl, n3, ndf_out, overwrite, pcs); break;
+        case 8: launch_rec_shape<T, 8>(ctx->stream, kind, nb, Mc, x, y, rec, c, nl, n3, ndf_out, overwrite, pcs); break;
       }
       LFR_CHECK_LAUNCH(ctx);
       continue;
@@ -475,11 +537,110 @@ const int32_t* lfr_plan_records(lfr_ctx* ctx, const lfr_plan* plan, size_t ci, c
   return recs[ci];
 }
 
+// ---------------------------------------------------------------------------
+// Unified memory-order gather (plans whose columns all have two entries)
+// ---------------------------------------------------------------------------
+// LFR_GATHER_UNIFIED=0 switches it off (per-class launches instead).
+bool lfr_plan_unified(const lfr_plan* plan, int ndf_in, int lpt) {
+  static const bool on = [] {
+    const char* e = getenv("LFR_GATHER_UNIFIED");
+    return !(e && e[0] == '0');
+  }();
+  if (!on || (ndf_in != 1 && ndf_in != 6)) return false;
+  const int nl = plan->space->nlayers;
+  if (nl < lpt) return false;
+  bool any = false;
+  for (const auto& c : plan->classes) {
+    if (c.ncol == 0) continue;
+    if (!((c.ngroup == 2 && c.gsize == 1) || (c.ngroup == 1 && c.gsize == 2))) return false;
+    if (c.height != nl && c.height != nl + 1) return false;
+    any = true;
+  }
+  return any;
+}
+
+// Levels per thread of the unified gather (2 or 4; 1 is not offered).
+int lfr_unified_lpt(int prec) {
+  const int l = lfr_gather_lpt(prec);
+  return l >= 2 ? l : 2;
+}
+
+int lfr_plan_ncolumns_total(const lfr_plan* plan) {
+  int n = 0;
+  for (const auto& c : plan->classes) n += c.ncol;
+  return n;
+}
+
+unsigned lfr_unified_blocks(const lfr_plan* plan, int lpt) {
+  const int tpc = (plan->space->nlayers + lpt) / lpt;
+  return lfr_blocks((int64_t)lfr_plan_ncolumns_total(plan) * tpc, lfr::kThreads);
+}
+
+namespace {
+// (start, class, column-in-class) of every column, in memory order
+std::vector<std::tuple<int32_t, int, int>> memory_order(const lfr_plan* plan) {
+  std::vector<std::tuple<int32_t, int, int>> v;
+  for (size_t k = 0; k < plan->classes.size(); ++k)
+    for (int q = 0; q < plan->classes[k].ncol; ++q) v.emplace_back(plan->classes[k].starts_h[q], (int)k, q);
+  std::sort(v.begin(), v.end());
+  return v;
+}
+}  // namespace
+
+void lfr_plan_uentries(const lfr_plan* plan, std::vector<int32_t>& ent) {
+  const auto order = memory_order(plan);
+  ent.assign(order.size() * 4, 0);
+  for (size_t i = 0; i < order.size(); ++i) {
+    const auto& pc = plan->classes[std::get<1>(order[i])];
+    const int32_t* e = &pc.ent_h[(size_t)std::get<2>(order[i]) * 4];  // two (cell, code) pairs
+    for (int q = 0; q < 4; ++q) ent[i * 4 + q] = e[q];
+  }
+}
+
+const int32_t* lfr_plan_urecords(lfr_ctx* ctx, const lfr_plan* plan, const lfr_space* in) {
+  for (auto& kv : plan->urecords)
+    if (kv.first == in->uid) return kv.second;
+  const int ndf_in = in->ndf;
+  const int kEntry = 2 + ndf_in;
+  const int stride = (3 + 2 * kEntry + 3) & ~3;
+  const int nl = plan->space->nlayers;
+  const auto order = memory_order(plan);
+  std::vector<int32_t> rec(order.size() * stride, 0);
+  for (size_t i = 0; i < order.size(); ++i) {
+    const auto& pc = plan->classes[std::get<1>(order[i])];
+    const int q = std::get<2>(order[i]);
+    int32_t* r = &rec[i * stride];
+    r[0] = pc.starts_h[q];
+    r[1] = pc.height;
+    r[2] = pc.gsize == 2 ? 1 : 0;  // one cell, two local dofs: input-dof-outer order (kind 0)
+    const int32_t* e = &pc.ent_h[(size_t)q * 4];
+    for (int k = 0; k < 2; ++k) {
+      const int cell = e[2 * k];  // padded slots carry a valid cell id
+      int32_t* re = r + 3 + k * kEntry;
+      re[0] = cell * nl;
+      re[1] = e[2 * k + 1];
+      for (int j = 0; j < ndf_in; ++j) re[2 + j] = in->map_h[(size_t)cell * ndf_in + j] - 1;
+    }
+  }
+  int32_t* d = nullptr;
+  if (hipMalloc(&d, std::max<size_t>(rec.size(), 4) * sizeof(int32_t)) != hipSuccess ||
+      hipMemcpy(d, rec.data(), rec.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    if (d) (void)hipFree(d);
+    (void)hipGetLastError();
+    lfr_set_error(ctx, LFR_ERR_NOMEM, "unified column record allocation failed");
+    return nullptr;
+  }
+  plan->urecords.emplace_back(in->uid, d);
+  return d;
+}
+
 extern "C" int lfr_plan_destroy(lfr_plan* plan) {
   if (!plan) return LFR_OK;
   for (auto& kv : plan->records)
     for (int32_t* p : kv.second)
       if (p) (void)hipFree(p);
+  for (auto& kv : plan->urecords)
+    if (kv.second) (void)hipFree(kv.second);
   for (auto& c : plan->classes) {
     if (c.col_start_d) (void)hipFree(c.col_start_d);
     if (c.ent_d) (void)hipFree(c.ent_d);
@@ -532,6 +693,178 @@ extern "C" int lfr_transpose_matrix_vector(lfr_ctx* ctx, int prec, void* lhs, co
                              lsp->ndf, n3, overwrite);
   return run_gather<double>(ctx, plan, 1, (const double*)M, (const double*)x, (double*)lhs, xsp,
                             lsp->ndf, n3, overwrite);
+}
+
+// ---------------------------------------------------------------------------
+// Packed operator
+// ---------------------------------------------------------------------------
+namespace {
+
+// P[col][e][j][L] = the matrix value entry e of column col multiplies with
+// input dof j at output level L (the same index arithmetic as the gather
+// kernels on the reference layout), 0 where the entry has no layer there.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_lma_pack(const T* __restrict__ M, T* __restrict__ P, const int32_t* __restrict__ ent, int ne, int ndf_in,
+           int H, int nl, int64_t n3, int ndf_out, int kind, int64_t total) {
+  for (int64_t idx = (int64_t)blockIdx.x * kThreads + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * kThreads) {
+    const int L = (int)(idx % H);
+    int64_t r = idx / H;
+    const int j = (int)(r % ndf_in);
+    r /= ndf_in;  // col * ne + e
+    const int cell = ent[2 * r];
+    const int code = ent[2 * r + 1];
+    const int k = L - (code & 1);
+    const bool ok = code >= 0 && k >= 0 && k < nl;
+    const int ii = code >= 0 ? (code >> 1) : 0;
+    const int64_t mo = kind == 0 ? (int64_t)ii + (int64_t)ndf_out * j : (int64_t)j + (int64_t)ndf_in * ii;
+    P[idx] = ok ? M[(int64_t)cell * nl + k + n3 * mo] : T(0);
+  }
+}
+
+}  // namespace
+
+extern "C" int lfr_lma_pack(lfr_ctx* ctx, int prec, const void* M, const lfr_space* in, const lfr_plan* plan,
+                            lfr_lma_packed** out) {
+  if (!ctx || !out) return LFR_ERR_ARG;
+  *out = nullptr;
+  LFR_REQUIRE(ctx, prec == 4 || prec == 8, "lfr_lma_pack: prec must be 4 or 8, got %d", prec);
+  LFR_REQUIRE(ctx, M && in && plan, "lfr_lma_pack: null argument");
+  const lfr_space* sp = plan->space;
+  LFR_REQUIRE(ctx, sp->ncell == in->ncell && sp->nlayers == in->nlayers,
+              "lfr_lma_pack: spaces on different meshes");
+  LFR_REQUIRE(ctx, in->ndf == 1 || in->ndf == 6 || in->ndf == 8,
+              "lfr_lma_pack: no packed kernel for %d input dofs per cell", in->ndf);
+  for (const auto& c : plan->classes)
+    LFR_REQUIRE(ctx, c.ncol == 0 || lfr_record_shape(in->ndf, c.ngroup, c.gsize),
+                "lfr_lma_pack: plan class (%d cells x %d dofs) has no packed kernel", c.ngroup, c.gsize);
+  auto* pk = new lfr_lma_packed();
+  pk->ctx = ctx;
+  pk->prec = prec;
+  pk->kind = plan->kind;
+  pk->plan = plan;
+  pk->in = in;
+  pk->ndf_out = sp->ndf;
+  pk->unified = lfr_plan_unified(plan, in->ndf, lfr_unified_lpt(prec));
+  int64_t tot = 0;
+  if (pk->unified) {
+    pk->off.push_back(0);
+    pk->pcs.push_back((int64_t)2 * in->ndf * (sp->nlayers + 1));
+    tot = ((int64_t)lfr_plan_ncolumns_total(plan) * pk->pcs[0] + 63) & ~(int64_t)63;
+  } else {
+    for (const auto& c : plan->classes) {
+      const int64_t pcs = (int64_t)c.ngroup * c.gsize * in->ndf * c.height;
+      pk->off.push_back(tot);
+      pk->pcs.push_back(pcs);
+      tot += ((int64_t)c.ncol * pcs + 63) & ~(int64_t)63;  // 256-byte aligned blocks
+    }
+  }
+  pk->nbytes = tot * prec;
+  if (hipMalloc(&pk->data_d, (size_t)std::max<int64_t>(pk->nbytes, 256)) != hipSuccess) {
+    (void)hipGetLastError();
+    delete pk;
+    return lfr_set_error(ctx, LFR_ERR_NOMEM, "lfr_lma_pack: cannot allocate %lld bytes", (long long)tot * prec);
+  }
+  const int nl = sp->nlayers;
+  const int64_t n3 = (int64_t)sp->ncell * nl;
+  if (pk->unified) {
+    // entries of every column in memory order, rows of nl+1 levels
+    std::vector<int32_t> ent;
+    lfr_plan_uentries(plan, ent);
+    int32_t* ent_d = nullptr;
+    const int64_t total = (int64_t)lfr_plan_ncolumns_total(plan) * pk->pcs[0];
+    hipError_t e = hipMalloc(&ent_d, ent.size() * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMemcpy(ent_d, ent.data(), ent.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      const unsigned nb = lfr_blocks(total, kThreads, 65536);
+      if (prec == 4)
+        hipLaunchKernelGGL(k_lma_pack<float>, dim3(nb), dim3(kThreads), 0, ctx->stream, (const float*)M,
+                           (float*)pk->data_d, ent_d, 2, in->ndf, nl + 1, nl, n3, sp->ndf, plan->kind, total);
+      else
+        hipLaunchKernelGGL(k_lma_pack<double>, dim3(nb), dim3(kThreads), 0, ctx->stream, (const double*)M,
+                           (double*)pk->data_d, ent_d, 2, in->ndf, nl + 1, nl, n3, sp->ndf, plan->kind, total);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    }
+    if (ent_d) (void)hipFree(ent_d);
+    if (e != hipSuccess) {
+      (void)hipFree(pk->data_d);
+      delete pk;
+      return lfr_set_error(ctx, LFR_ERR_HIP, "lfr_lma_pack: %s", hipGetErrorString(e));
+    }
+  }
+  for (size_t ci = 0; !pk->unified && ci < plan->classes.size(); ++ci) {
+    const auto& c = plan->classes[ci];
+    const int64_t total = (int64_t)c.ncol * pk->pcs[ci];
+    if (total == 0) continue;
+    const unsigned nb = lfr_blocks(total, kThreads, 65536);
+    if (prec == 4)
+      hipLaunchKernelGGL(k_lma_pack<float>, dim3(nb), dim3(kThreads), 0, ctx->stream, (const float*)M,
+                         (float*)pk->data_d + pk->off[ci], c.ent_d, c.ngroup * c.gsize, in->ndf, c.height, nl,
+                         n3, sp->ndf, plan->kind, total);
+    else
+      hipLaunchKernelGGL(k_lma_pack<double>, dim3(nb), dim3(kThreads), 0, ctx->stream, (const double*)M,
+                         (double*)pk->data_d + pk->off[ci], c.ent_d, c.ngroup * c.gsize, in->ndf, c.height, nl,
+                         n3, sp->ndf, plan->kind, total);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      (void)hipFree(pk->data_d);
+      delete pk;
+      return lfr_set_error(ctx, LFR_ERR_HIP, "lfr_lma_pack: launch failed: %s", hipGetErrorString(e));
+    }
+  }
+  // build the column records now so the first apply does no host work
+  if (pk->unified && !lfr_plan_urecords(ctx, plan, in)) {
+    (void)hipFree(pk->data_d);
+    delete pk;
+    return LFR_ERR_NOMEM;
+  }
+  for (size_t ci = 0; !pk->unified && ci < plan->classes.size(); ++ci) {
+    int stride = 0;
+    if (plan->classes[ci].ncol && !lfr_plan_records(ctx, plan, ci, in, &stride)) {
+      (void)hipFree(pk->data_d);
+      delete pk;
+      return LFR_ERR_NOMEM;  // message set by lfr_plan_records
+    }
+  }
+  // matrix_d may be released as soon as this returns
+  const hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(pk->data_d);
+    delete pk;
+    return lfr_set_error(ctx, LFR_ERR_HIP, "lfr_lma_pack: %s", hipGetErrorString(e));
+  }
+  *out = pk;
+  return LFR_OK;
+}
+
+extern "C" int lfr_lma_packed_destroy(lfr_lma_packed* pk) {
+  if (!pk) return LFR_OK;
+  if (pk->data_d) (void)hipFree(pk->data_d);
+  delete pk;
+  return LFR_OK;
+}
+
+extern "C" int lfr_lma_packed_info(const lfr_lma_packed* pk, long long* nbytes) {
+  if (!pk || !nbytes) return LFR_ERR_ARG;
+  *nbytes = (long long)pk->nbytes;
+  return LFR_OK;
+}
+
+extern "C" int lfr_matrix_vector_packed(lfr_ctx* ctx, const lfr_lma_packed* pk, void* lhs, const void* x,
+                                        int overwrite) {
+  if (!ctx) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, pk && lhs && x, "lfr_matrix_vector_packed: null argument");
+  LFR_REQUIRE(ctx, pk->ctx == ctx, "lfr_matrix_vector_packed: operator packed on another context");
+  LFR_REQUIRE(ctx, lhs != x, "lfr_matrix_vector_packed: lhs must not alias x");
+  const lfr_space* sp = pk->plan->space;
+  const int64_t n3 = (int64_t)sp->ncell * sp->nlayers;
+  if (pk->prec == 4)
+    return run_gather<float>(ctx, pk->plan, pk->kind, nullptr, (const float*)x, (float*)lhs, pk->in, sp->ndf, n3,
+                             overwrite, pk);
+  return run_gather<double>(ctx, pk->plan, pk->kind, nullptr, (const double*)x, (double*)lhs, pk->in, sp->ndf, n3,
+                            overwrite, pk);
 }
 
 static int dg_common(lfr_ctx* ctx, int prec, void* lhs, const void* x, const void* M,

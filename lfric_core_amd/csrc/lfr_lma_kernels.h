@@ -177,7 +177,7 @@ template <typename T, int NDF_IN, int NG, int GS, int KIND, bool NT>
 __device__ __forceinline__ double gather_rec_body(const T* __restrict__ M, const T* __restrict__ x,
                                                   T* __restrict__ y, const int32_t* __restrict__ rec, int ncol,
                                                   int H, int nl, int64_t n3, int ndf_out, int overwrite,
-                                                  unsigned lb, const T* __restrict__ dotv) {
+                                                  int64_t pcs, unsigned lb, const T* __restrict__ dotv) {
   using S = RecShape<NDF_IN, NG, GS>;
   const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
   const bool valid = t < (int64_t)ncol * H;
@@ -211,11 +211,14 @@ __device__ __forceinline__ double gather_rec_body(const T* __restrict__ M, const
       const int ii = code >= 0 ? (code >> 1) : 0;
       iis[g][q] = ii;
       const int kk = k < 0 ? 0 : (k >= nl ? nl - 1 : k);
+      // reference layout: M(cell*nl + k, row, column) planes n3 apart;
+      // packed (pcs > 0): row (entry, j) of this column at level L
+      const int64_t eb = pcs > 0 ? (int64_t)col * pcs + (int64_t)(g * GS + q) * NDF_IN * H + L
+                                 : mbase + kk + n3 * (KIND == 0 ? (int64_t)ii : (int64_t)NDF_IN * ii);
+      const int64_t js = pcs > 0 ? (int64_t)H : (KIND == 0 ? n3 * ndf_out : n3);
 #pragma unroll
       for (int j = 0; j < NDF_IN; ++j) {
-        const int64_t mo = KIND == 0 ? (int64_t)ii + (int64_t)ndf_out * j
-                                     : (int64_t)j + (int64_t)NDF_IN * ii;
-        mv[g][q][j] = NT ? ldnt(M + mbase + kk + n3 * mo) : M[mbase + kk + n3 * mo];
+        mv[g][q][j] = NT ? ldnt(M + eb + js * j) : M[eb + js * j];
         xv[g][q][j] = x[rg[1 + GS + j] + kk];
       }
     }
@@ -270,12 +273,12 @@ template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT = false, bo
 __global__ void __launch_bounds__(kThreads)
 k_gather_rec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
              const int32_t* __restrict__ rec, int ncol, int H, int nl, int64_t n3, int ndf_out,
-             int overwrite, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
+             int overwrite, int64_t pcs, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
              double* __restrict__ part = nullptr, const double* __restrict__ stop = nullptr) {
   if (DOT && stop && stop[0] != 0.0) return;
   const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
-  double v = gather_rec_body<T, NDF_IN, NG, GS, KIND, NT>(M, x, y, rec, ncol, H, nl, n3, ndf_out, overwrite, lb,
-                                                          DOT ? dotv : nullptr);
+  double v = gather_rec_body<T, NDF_IN, NG, GS, KIND, NT>(M, x, y, rec, ncol, H, nl, n3, ndf_out, overwrite, pcs,
+                                                          lb, DOT ? dotv : nullptr);
   if (DOT) {
     __shared__ double lds[kThreads / 64];
     v = block_sum(v, lds);
@@ -302,23 +305,26 @@ struct __attribute__((aligned(4))) LVec {
 
 template <typename T, int N, bool NT>
 __device__ __forceinline__ LVec<T, N> ldvec(const T* p) {
-  if constexpr (N == 1) {
-    LVec<T, 1> r;
-    r.v[0] = NT ? __builtin_nontemporal_load(p) : *p;
+  if constexpr (NT) {
+    // element-wise non-temporal loads; the compiler merges them into one
+    // multi-dword load with the streaming cache policy
+    LVec<T, N> r;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.v[i] = __builtin_nontemporal_load(p + i);
     return r;
   } else {
     return *reinterpret_cast<const LVec<T, N>*>(p);
   }
 }
 
-template <typename T, int NDF_IN, int NG, int GS, int KIND, int LPT>
+template <typename T, int NDF_IN, int NG, int GS, int KIND, int LPT, bool NT = false, int BS = kThreads>
 __device__ __forceinline__ double gather_vec_body(const T* __restrict__ M, const T* __restrict__ x,
                                                   T* __restrict__ y, const int32_t* __restrict__ rec, int ncol,
-                                                  int nl, int64_t n3, int ndf_out, int overwrite, unsigned lb,
-                                                  const T* __restrict__ dotv) {
+                                                  int nl, int64_t n3, int ndf_out, int overwrite, int64_t pcs,
+                                                  unsigned lb, const T* __restrict__ dotv) {
   using S = RecShape<NDF_IN, NG, GS>;
   const int tpc = (nl + LPT - 1) / LPT;
-  const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
+  const int64_t t = (int64_t)lb * BS + threadIdx.x;
   const bool valid = t < (int64_t)ncol * tpc;
   const int64_t tt = valid ? t : 0;
   const int col = (int)(tt / tpc);
@@ -358,11 +364,12 @@ __device__ __forceinline__ double gather_vec_body(const T* __restrict__ M, const
       ok[g][q] = code >= 0;
       const int ii = code >= 0 ? (code >> 1) : 0;
       iis[g][q] = ii;
+      const int64_t eb = pcs > 0 ? (int64_t)col * pcs + (int64_t)(g * GS + q) * NDF_IN * nl + L0
+                                 : mbase + L0 + n3 * (KIND == 0 ? (int64_t)ii : (int64_t)NDF_IN * ii);
+      const int64_t js = pcs > 0 ? (int64_t)nl : (KIND == 0 ? n3 * ndf_out : n3);
 #pragma unroll
       for (int j = 0; j < NDF_IN; ++j) {
-        const int64_t mo = KIND == 0 ? (int64_t)ii + (int64_t)ndf_out * j
-                                     : (int64_t)j + (int64_t)NDF_IN * ii;
-        mv[g][q][j] = ldvec<T, LPT, false>(M + mbase + L0 + n3 * mo);
+        mv[g][q][j] = ldvec<T, LPT, NT>(M + eb + js * j);
         xv[g][q][j] = ldvec<T, LPT, false>(x + rg[1 + GS + j] + L0);
       }
     }
@@ -436,17 +443,198 @@ __device__ __forceinline__ double gather_vec_body(const T* __restrict__ M, const
   return v;
 }
 
-template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT, int LPT>
-__global__ void __launch_bounds__(kThreads)
+template <typename T, int NDF_IN, int NG, int GS, int KIND, bool DOT, int LPT, bool NT = false, int BS = kThreads>
+__global__ void __launch_bounds__(BS)
 k_gather_vec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
              const int32_t* __restrict__ rec, int ncol, int nl, int64_t n3, int ndf_out,
-             int overwrite, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
+             int overwrite, int64_t pcs, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
              double* __restrict__ part = nullptr, const double* __restrict__ stop = nullptr) {
   if (DOT && stop && stop[0] != 0.0) return;
   const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
-  double v = gather_vec_body<T, NDF_IN, NG, GS, KIND, LPT>(M, x, y, rec, ncol, nl, n3, ndf_out, overwrite, lb,
-                                                           DOT ? dotv : nullptr);
+  double v = gather_vec_body<T, NDF_IN, NG, GS, KIND, LPT, NT, BS>(M, x, y, rec, ncol, nl, n3, ndf_out, overwrite,
+                                                                   pcs, lb, DOT ? dotv : nullptr);
   if (DOT) {
+    __shared__ double lds[BS / 64];
+    v = block_sum<BS>(v, lds);
+    if (threadIdx.x == 0) part[lb] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Unified gather over every output column of a plan in memory order, for
+// plans whose columns all have two entries and nl or nl+1 levels (W2 lowest
+// order: side-face columns shared by two cells, and bottom/top face stacks
+// touched by one cell through two local dofs whose layers are one apart).
+// One launch writes y front to back, so the lines a column shares with its
+// neighbours are completed while they are still in L2, and the operand is
+// fetched once.  Record per column (ints):
+//   [ g0 | H | jouter | per entry e = 0, 1: cell*nl, code, map_in(cell, 0..NDF_IN-1) ]
+// padded to x4.  A thread takes LPT consecutive output levels [L0, L0+LPT) of
+// one column (ceil((nl+1)/LPT) threads per column; windows clamped inside the
+// column, overlapping levels written by their owner only).  Entry e reads
+// cell layers k = L - shift: its loads start at kst = L0 - shift clamped to
+// [0, nl-LPT], and lane u takes element u + (kst - start) of what was loaded
+// (a rotation by -1, 0 or +1); levels whose k falls outside [0, nl) are
+// masked.  Summation order per output: kind 0 adds the twelve products entry
+// by entry (two cells, `jouter` = 0) or input dof by input dof (one cell with
+// two local dofs, `jouter` = 1), as k_gather_rec does for each class; kind 1
+// adds one dot product per entry.
+// ---------------------------------------------------------------------------
+template <int NDF_IN>
+struct UniRec {
+  static constexpr int kEntry = 2 + NDF_IN;
+  static constexpr int kInts = 3 + 2 * kEntry;
+  static constexpr int kStride = (kInts + 3) & ~3;
+};
+
+template <typename T, int LPT>
+__device__ __forceinline__ T rot_pick(const LVec<T, LPT>& v, int u, int d) {
+  const T a = v.v[u];
+  const T b = v.v[u + 1 < LPT ? u + 1 : LPT - 1];
+  const T c = v.v[u > 0 ? u - 1 : 0];
+  return d == 0 ? a : (d > 0 ? b : c);
+}
+
+template <typename T, int NDF_IN, int KIND, bool DOT, int LPT, bool NT>
+__global__ void __launch_bounds__(kThreads)
+k_gather_uni(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
+             const int32_t* __restrict__ urec, int ncol, int nl, int64_t n3, int ndf_out, int overwrite,
+             int64_t pcs, unsigned nb_pad, const T* __restrict__ dotv = nullptr,
+             double* __restrict__ part = nullptr, const double* __restrict__ stop = nullptr) {
+  if (DOT && stop && stop[0] != 0.0) return;
+  using S = UniRec<NDF_IN>;
+  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
+  const int tpc = (nl + LPT) / LPT;  // ceil((nl + 1) / LPT)
+  const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
+  const bool valid = t < (int64_t)ncol * tpc;
+  const int64_t tt = valid ? t : 0;
+  const int col = (int)(tt / tpc);
+  const int p = (int)(tt - (int64_t)col * tpc);
+  int r[S::kStride];
+  const int4* r4 = reinterpret_cast<const int4*>(urec + (int64_t)col * S::kStride);
+#pragma unroll
+  for (int q = 0; q < S::kStride / 4; ++q) {
+    const int4 v = r4[q];
+    r[4 * q + 0] = v.x;
+    r[4 * q + 1] = v.y;
+    r[4 * q + 2] = v.z;
+    r[4 * q + 3] = v.w;
+  }
+  const int32_t g0 = r[0];
+  const int H = r[1];
+  const bool jouter = r[2] != 0;
+  const int L0 = min(p * LPT, H - LPT);
+  const int own = p * LPT - L0;  // levels below L0 + own belong to the previous thread
+  T acc[LPT];
+  if (overwrite || !valid) {
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) acc[u] = T(0);
+  } else {
+    const LVec<T, LPT> yv = ldvec<T, LPT, false>(y + g0 + L0);
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) acc[u] = yv.v[u];
+  }
+  T mv[2][NDF_IN][LPT], xv[2][NDF_IN][LPT];
+  bool ok[2][LPT];
+  int iis[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int* re = r + 3 + e * S::kEntry;
+    const int64_t cb = (int64_t)re[0];
+    const int code = re[1];
+    const int ii = code >= 0 ? (code >> 1) : 0;
+    iis[e] = ii;
+    const int kst = L0 - (code & 1);
+    const int xs = min(max(kst, 0), nl - LPT);
+    const int d = kst - xs;
+#pragma unroll
+    for (int u = 0; u < LPT; ++u) ok[e][u] = code >= 0 && kst + u >= 0 && kst + u < nl;
+    const int64_t js = pcs > 0 ? (int64_t)(nl + 1) : (KIND == 0 ? n3 * ndf_out : n3);
+    const int64_t eb = pcs > 0 ? (int64_t)col * pcs + (int64_t)e * NDF_IN * (nl + 1) + L0
+                               : cb + xs + n3 * (KIND == 0 ? (int64_t)ii : (int64_t)NDF_IN * ii);
+    LVec<T, LPT> lm[NDF_IN], lx[NDF_IN];
+#pragma unroll
+    for (int j = 0; j < NDF_IN; ++j) {
+      lm[j] = ldvec<T, LPT, NT>(M + eb + js * j);
+      lx[j] = ldvec<T, LPT, false>(x + re[2 + j] + xs);
+    }
+#pragma unroll
+    for (int j = 0; j < NDF_IN; ++j)
+#pragma unroll
+      for (int u = 0; u < LPT; ++u) {
+        // the packed copy is stored by output level: no rotation
+        mv[e][j][u] = pcs > 0 ? lm[j].v[u] : rot_pick<T, LPT>(lm[j], u, d);
+        xv[e][j][u] = rot_pick<T, LPT>(lx[j], u, d);
+      }
+  }
+#pragma unroll
+  for (int u = 0; u < LPT; ++u) {
+    if (KIND == 0) {
+      T a = acc[u];
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int j = 0; j < NDF_IN; ++j) {
+          const T pr = mv[e][j][u] * xv[e][j][u];
+          const T s = a + pr;
+          a = ok[e][u] ? s : a;
+        }
+      T b = acc[u];
+#pragma unroll
+      for (int j = 0; j < NDF_IN; ++j)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const T pr = mv[e][j][u] * xv[e][j][u];
+          const T s = b + pr;
+          b = ok[e][u] ? s : b;
+        }
+      acc[u] = jouter ? b : a;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        T dot = T(0);
+#pragma unroll
+        for (int j = 0; j < NDF_IN; ++j) {
+          const T pr = mv[e][j][u] * xv[e][j][u];
+          dot = dot + pr;
+        }
+        const T s = acc[u] + dot;
+        acc[u] = ok[e][u] ? s : acc[u];
+      }
+    }
+  }
+  if (valid) {
+    if (own == 0) {
+      LVec<T, LPT> o;
+#pragma unroll
+      for (int u = 0; u < LPT; ++u) o.v[u] = acc[u];
+      *reinterpret_cast<LVec<T, LPT>*>(y + g0 + L0) = o;
+    } else {
+#pragma unroll
+      for (int u = 0; u < LPT; ++u)
+        if (u >= own) y[g0 + L0 + u] = acc[u];
+    }
+  }
+  if (DOT) {
+    // fused <y, x> with dotv == x: x at the output dof is the operand of the
+    // own local dof of the first entry valid at that level
+    double v = 0.0;
+    if (valid) {
+#pragma unroll
+      for (int u = 0; u < LPT; ++u) {
+        T self = T(0);
+        bool found = false;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          T w = xv[e][0][u];
+#pragma unroll
+          for (int j = 1; j < NDF_IN; ++j) w = iis[e] == j ? xv[e][j][u] : w;
+          self = (ok[e][u] && !found) ? w : self;
+          found = found || ok[e][u];
+        }
+        if (u >= own) v += (double)acc[u] * (double)self;
+      }
+    }
     __shared__ double lds[kThreads / 64];
     v = block_sum(v, lds);
     if (threadIdx.x == 0) part[lb] = v;

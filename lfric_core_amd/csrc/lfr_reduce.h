@@ -25,8 +25,9 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sum of one value per thread over a block of kRedThreads threads; the
-// result is valid in thread 0.  `lds` must hold kRedThreads/64 doubles.
+// Sum of one value per thread over a block of BS threads; the result is
+// valid in thread 0.  `lds` must hold BS/64 doubles.
+template <int BS = kRedThreads>
 __device__ __forceinline__ double block_sum(double v, double* lds) {
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -35,7 +36,7 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   double s = 0.0;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int w = 0; w < kRedThreads / 64; ++w) s += lds[w];
+    for (int w = 0; w < BS / 64; ++w) s += lds[w];
   }
   __syncthreads();
   return s;

@@ -217,21 +217,38 @@ struct Ws {
 size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Partial sums written by one fused apply: padded block count of every class
-// under the kernel apply_op launches for it.
-int apply_partials(const lfr_plan* plan, int prec) {
+// under the kernel apply_op launches for it (depends on the precision, which
+// sets the levels per thread, and on whether the operator is packed).
+bool apply_unified(const lfr_plan* plan, int prec, const lfr_lma_packed* pk) {
+  if (pk) return pk->unified;
+  return lfr_records_enabled() && lfr_plan_unified(plan, plan->space->ndf, lfr_unified_lpt(prec));
+}
+
+int apply_partials(const lfr_plan* plan, int prec, bool packed) {
   int np = 0;
   const lfr_space* sp = plan->space;
+  if (packed ? lfr_plan_unified(plan, sp->ndf, lfr_unified_lpt(prec)) : apply_unified(plan, prec, nullptr))
+    return (int)((lfr_unified_blocks(plan, lfr_unified_lpt(prec)) + 7u) & ~7u);
   for (const auto& c : plan->classes) {
-    const unsigned nb = lfr_class_blocks(c, lfr_class_lpt(c, sp->nlayers, prec, sp->ndf));
+    const unsigned nb = lfr_class_blocks(c, lfr_class_lpt(c, sp->nlayers, prec, sp->ndf, packed));
     np += (int)((nb + 7u) & ~7u);
   }
+  return np;
+}
+
+int apply_partials_max(const lfr_plan* plan) {
+  int np = 0;
+  for (int prec : {4, 8})
+    for (bool packed : {false, true}) np = std::max(np, apply_partials(plan, prec, packed));
   return np;
 }
 
 // Carve the solver workspace out of one grow-only device buffer.
 int ws_get(lfr_ctx* ctx, int64_t n, int prec, const lfr_plan* plan, int naux, Ws& w) {
   const size_t vb = align_up((size_t)n * prec);
-  const int napply = apply_partials(plan, prec);
+  // room for the partials of an apply in either precision (the refined
+  // solver runs FP32 applies in an FP64-sized workspace)
+  const int napply = apply_partials_max(plan);
   const size_t need = 3 * vb + (size_t)naux * vb + align_up(napply * sizeof(double)) +
                       2 * align_up(lfr::kRedMaxBlocks * sizeof(double)) +
                       align_up(S_NSLOT * sizeof(double)) + align_up(8 * sizeof(double));
@@ -272,7 +289,7 @@ void launch_apply_t(hipStream_t s, const lfr_plan_class& c, const T* M, const T*
 
 template <typename T, int NDF_IN, int NG, int GS>
 void launch_apply_rec_t(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M,
-                        const T* x, T* y, int nl, int64_t n3, int ndf_out, const T* dotv,
+                        const T* x, T* y, int nl, int64_t n3, int ndf_out, int64_t pcs, const T* dotv,
                         double* part, const double* stop) {
   const unsigned nb = lfr_blocks((int64_t)c.ncol * c.height, kThreads);
   const unsigned nbp = (nb + 7u) & ~7u;
@@ -280,41 +297,52 @@ void launch_apply_rec_t(hipStream_t s, const lfr_plan_class& c, const int32_t* r
   if (dotv) {
     if (nt)
       hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, 0, true, true>), dim3(nbp), dim3(kThreads),
-                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, nbp, dotv, part, stop);
+                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, pcs, nbp, dotv, part, stop);
     else
       hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, 0, true, false>), dim3(nbp), dim3(kThreads),
-                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, nbp, dotv, part, stop);
+                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, pcs, nbp, dotv, part, stop);
   } else {
     if (nt)
       hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, 0, false, true>), dim3(nbp), dim3(kThreads),
-                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, nbp,
+                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, pcs, nbp,
                          (const T*)nullptr, (double*)nullptr, (const double*)nullptr);
     else
       hipLaunchKernelGGL((lfr::k_gather_rec<T, NDF_IN, NG, GS, 0, false, false>), dim3(nbp), dim3(kThreads),
-                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, nbp,
+                         0, s, M, x, y, rec, c.ncol, c.height, nl, n3, ndf_out, 1, pcs, nbp,
                          (const T*)nullptr, (double*)nullptr, (const double*)nullptr);
   }
 }
 
-template <typename T, int NDF_IN, int NG, int GS, int LPT>
-void launch_apply_vec_t(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M, const T* x, T* y,
-                        int nl, int64_t n3, int ndf_out, const T* dotv, double* part, const double* stop) {
+template <typename T, int NDF_IN, int NG, int GS, int LPT, bool NT>
+void launch_apply_vec_nt(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M, const T* x, T* y,
+                         int nl, int64_t n3, int ndf_out, int64_t pcs, const T* dotv, double* part,
+                         const double* stop) {
   const unsigned nbp = (lfr_class_blocks(c, LPT) + 7u) & ~7u;
   if (dotv)
-    hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, 0, true, LPT>), dim3(nbp), dim3(kThreads), 0, s, M, x,
-                       y, rec, c.ncol, nl, n3, ndf_out, 1, nbp, dotv, part, stop);
+    hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, 0, true, LPT, NT>), dim3(nbp), dim3(kThreads), 0, s, M,
+                       x, y, rec, c.ncol, nl, n3, ndf_out, 1, pcs, nbp, dotv, part, stop);
   else
-    hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, 0, false, LPT>), dim3(nbp), dim3(kThreads), 0, s, M, x,
-                       y, rec, c.ncol, nl, n3, ndf_out, 1, nbp, (const T*)nullptr, (double*)nullptr,
+    hipLaunchKernelGGL((lfr::k_gather_vec<T, NDF_IN, NG, GS, 0, false, LPT, NT>), dim3(nbp), dim3(kThreads), 0, s, M,
+                       x, y, rec, c.ncol, nl, n3, ndf_out, 1, pcs, nbp, (const T*)nullptr, (double*)nullptr,
                        (const double*)nullptr);
+}
+
+template <typename T, int NDF_IN, int NG, int GS, int LPT>
+void launch_apply_vec_t(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M, const T* x, T* y,
+                        int nl, int64_t n3, int ndf_out, int64_t pcs, const T* dotv, double* part,
+                        const double* stop) {
+  if (lfr_gather_vec_nt(pcs > 0))
+    launch_apply_vec_nt<T, NDF_IN, NG, GS, LPT, true>(s, c, rec, M, x, y, nl, n3, ndf_out, pcs, dotv, part, stop);
+  else
+    launch_apply_vec_nt<T, NDF_IN, NG, GS, LPT, false>(s, c, rec, M, x, y, nl, n3, ndf_out, pcs, dotv, part, stop);
 }
 
 template <typename T, int NDF_IN, int LPT>
 bool launch_apply_vec(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M, const T* x, T* y,
-                      int nl, int64_t n3, int ndf, const T* dotv, double* part, const double* stop) {
+                      int nl, int64_t n3, int ndf, int64_t pcs, const T* dotv, double* part, const double* stop) {
 #define SHAPE(NGV, GSV)                                                                                   \
   if (c.ngroup == NGV && c.gsize == GSV) {                                                                \
-    launch_apply_vec_t<T, NDF_IN, NGV, GSV, LPT>(s, c, rec, M, x, y, nl, n3, ndf, dotv, part, stop);      \
+    launch_apply_vec_t<T, NDF_IN, NGV, GSV, LPT>(s, c, rec, M, x, y, nl, n3, ndf, pcs, dotv, part, stop); \
     return true;                                                                                          \
   }
   SHAPE(2, 1) SHAPE(1, 1) SHAPE(4, 1) SHAPE(3, 1) SHAPE(1, 2) SHAPE(2, 2) SHAPE(3, 2) SHAPE(4, 2)
@@ -324,11 +352,11 @@ bool launch_apply_vec(hipStream_t s, const lfr_plan_class& c, const int32_t* rec
 
 template <typename T, int NDF_IN>
 bool launch_apply_rec(hipStream_t s, const lfr_plan_class& c, const int32_t* rec, const T* M,
-                      const T* x, T* y, int nl, int64_t n3, int ndf, const T* dotv, double* part,
+                      const T* x, T* y, int nl, int64_t n3, int ndf, int64_t pcs, const T* dotv, double* part,
                       const double* stop) {
 #define SHAPE(NGV, GSV) \
   if (c.ngroup == NGV && c.gsize == GSV) { \
-    launch_apply_rec_t<T, NDF_IN, NGV, GSV>(s, c, rec, M, x, y, nl, n3, ndf, dotv, part, stop); \
+    launch_apply_rec_t<T, NDF_IN, NGV, GSV>(s, c, rec, M, x, y, nl, n3, ndf, pcs, dotv, part, stop); \
     return true; }
   SHAPE(2, 1) SHAPE(1, 2) SHAPE(1, 1) SHAPE(4, 2) SHAPE(3, 2) SHAPE(2, 2) SHAPE(4, 1) SHAPE(3, 1)
 #undef SHAPE
@@ -350,6 +378,25 @@ void launch_apply_shape(hipStream_t s, const lfr_plan_class& c, const T* M, cons
     launch_apply_t<T, NDF_IN, 0, 2>(s, c, M, x, y, mi, nl, n3, ndf, ndf, dotv, part, stop);
 }
 
+template <typename T, int NDF_IN, int LPT, bool NT>
+void launch_apply_uni_nt(hipStream_t s, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur, int ncol,
+                         int nl, int64_t n3, int64_t pcs, const T* dotv, double* part, const double* stop) {
+  if (dotv)
+    hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, 0, true, LPT, NT>), dim3(nbp), dim3(kThreads), 0, s, M, x, y,
+                       ur, ncol, nl, n3, NDF_IN, 1, pcs, nbp, dotv, part, stop);
+  else
+    hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, 0, false, LPT, NT>), dim3(nbp), dim3(kThreads), 0, s, M, x, y,
+                       ur, ncol, nl, n3, NDF_IN, 1, pcs, nbp, (const T*)nullptr, (double*)nullptr,
+                       (const double*)nullptr);
+}
+
+template <typename T, int NDF_IN, int LPT>
+void launch_apply_uni(hipStream_t s, bool nt, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur,
+                      int ncol, int nl, int64_t n3, int64_t pcs, const T* dotv, double* part, const double* stop) {
+  if (nt) launch_apply_uni_nt<T, NDF_IN, LPT, true>(s, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+  else launch_apply_uni_nt<T, NDF_IN, LPT, false>(s, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+}
+
 // y = A x (overwrite) by the gather kernels; with dotv: partials of
 // <y, dotv> into part[0..napply)
 template <typename T>
@@ -361,34 +408,59 @@ int apply_gather(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* d
   const lfr_space* sp = A->space;
   const int nl = sp->nlayers;
   const int64_t n3 = (int64_t)sp->ncell * nl;
+  if (apply_unified(A->plan, (int)sizeof(T), A->packed)) {
+    const int32_t* ur = lfr_plan_urecords(ctx, A->plan, sp);
+    if (!ur) return LFR_ERR_NOMEM;
+    const int lpt = lfr_unified_lpt((int)sizeof(T));
+    const unsigned nbp = (lfr_unified_blocks(A->plan, lpt) + 7u) & ~7u;
+    const T* M = A->packed ? (const T*)A->packed->data_d : (const T*)A->matrix_d;
+    const int64_t pcs = A->packed ? A->packed->pcs[0] : 0;
+    const int ncol = lfr_plan_ncolumns_total(A->plan);
+    const bool nt = lfr_gather_vec_nt(A->packed != nullptr);
+    if (sp->ndf == 6) {
+      if (lpt == 4) launch_apply_uni<T, 6, 4>(ctx->stream, nt, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+      else launch_apply_uni<T, 6, 2>(ctx->stream, nt, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+    } else {
+      if (lpt == 4) launch_apply_uni<T, 1, 4>(ctx->stream, nt, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+      else launch_apply_uni<T, 1, 2>(ctx->stream, nt, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+    }
+    LFR_CHECK_LAUNCH(ctx);
+    return LFR_OK;
+  }
   int off = 0;
   for (size_t ci = 0; ci < A->plan->classes.size(); ++ci) {
     const auto& c = A->plan->classes[ci];
-    const int lpt = lfr_class_lpt(c, nl, (int)sizeof(T), sp->ndf);
+    const lfr_lma_packed* pk = A->packed;
+    const int lpt = lfr_class_lpt(c, nl, (int)sizeof(T), sp->ndf, pk != nullptr);
     const unsigned nb = lfr_class_blocks(c, lpt);
     const unsigned nbp = (nb + 7u) & ~7u;
     double* pp = part ? part + off : nullptr;
-    const T* M = (const T*)A->matrix_d;
+    // packed operator: this class's block, columns pcs elements apart
+    const T* M = pk ? (const T*)pk->data_d + pk->off[ci] : (const T*)A->matrix_d;
+    const int64_t pcs = pk ? pk->pcs[ci] : 0;
     int stride = 0;
-    const int32_t* rec = lfr_records_enabled() ? lfr_plan_records(ctx, A->plan, ci, sp, &stride) : nullptr;
+    const int32_t* rec = (pk || lfr_records_enabled()) ? lfr_plan_records(ctx, A->plan, ci, sp, &stride) : nullptr;
+    if (pk && !rec) return lfr_set_error(ctx, LFR_ERR_NOMEM, "apply: column records unavailable");
     bool done = false;
     if (lpt > 1) {
       if (!rec) return lfr_set_error(ctx, LFR_ERR_NOMEM, "apply: column records unavailable");
       if (sp->ndf == 6) {
-        done = lpt == 2 ? launch_apply_vec<T, 6, 2>(ctx->stream, c, rec, M, x, y, nl, n3, 6, dotv, pp, stop)
-                        : launch_apply_vec<T, 6, 4>(ctx->stream, c, rec, M, x, y, nl, n3, 6, dotv, pp, stop);
+        done = lpt == 2 ? launch_apply_vec<T, 6, 2>(ctx->stream, c, rec, M, x, y, nl, n3, 6, pcs, dotv, pp, stop)
+                        : launch_apply_vec<T, 6, 4>(ctx->stream, c, rec, M, x, y, nl, n3, 6, pcs, dotv, pp, stop);
       } else if (sp->ndf == 8) {
-        done = lpt == 2 ? launch_apply_vec<T, 8, 2>(ctx->stream, c, rec, M, x, y, nl, n3, 8, dotv, pp, stop)
-                        : launch_apply_vec<T, 8, 4>(ctx->stream, c, rec, M, x, y, nl, n3, 8, dotv, pp, stop);
+        done = lpt == 2 ? launch_apply_vec<T, 8, 2>(ctx->stream, c, rec, M, x, y, nl, n3, 8, pcs, dotv, pp, stop)
+                        : launch_apply_vec<T, 8, 4>(ctx->stream, c, rec, M, x, y, nl, n3, 8, pcs, dotv, pp, stop);
       } else if (sp->ndf == 1) {
-        done = lpt == 2 ? launch_apply_vec<T, 1, 2>(ctx->stream, c, rec, M, x, y, nl, n3, 1, dotv, pp, stop)
-                        : launch_apply_vec<T, 1, 4>(ctx->stream, c, rec, M, x, y, nl, n3, 1, dotv, pp, stop);
+        done = lpt == 2 ? launch_apply_vec<T, 1, 2>(ctx->stream, c, rec, M, x, y, nl, n3, 1, pcs, dotv, pp, stop)
+                        : launch_apply_vec<T, 1, 4>(ctx->stream, c, rec, M, x, y, nl, n3, 1, pcs, dotv, pp, stop);
       }
       if (!done) return lfr_set_error(ctx, LFR_ERR_ARG, "apply: no vectorised gather for this class shape");
     } else if (rec) {
-      if (sp->ndf == 6) done = launch_apply_rec<T, 6>(ctx->stream, c, rec, M, x, y, nl, n3, 6, dotv, pp, stop);
-      else if (sp->ndf == 8) done = launch_apply_rec<T, 8>(ctx->stream, c, rec, M, x, y, nl, n3, 8, dotv, pp, stop);
+      if (sp->ndf == 6) done = launch_apply_rec<T, 6>(ctx->stream, c, rec, M, x, y, nl, n3, 6, pcs, dotv, pp, stop);
+      else if (sp->ndf == 8) done = launch_apply_rec<T, 8>(ctx->stream, c, rec, M, x, y, nl, n3, 8, pcs, dotv, pp, stop);
+      else if (sp->ndf == 1) done = launch_apply_rec<T, 1>(ctx->stream, c, rec, M, x, y, nl, n3, 1, pcs, dotv, pp, stop);
     }
+    if (pk && !done) return lfr_set_error(ctx, LFR_ERR_ARG, "apply: no packed kernel for this class shape");
     if (done) {
       LFR_CHECK_LAUNCH(ctx);
       off += (int)nbp;
@@ -435,7 +507,7 @@ int apply_op(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv,
   rc = lfr_enforce_bc_queue(ctx, (int)sizeof(T), y, A->space);
   if (rc) return rc;
   if (dotv) {
-    const int np = apply_partials(A->plan, (int)sizeof(T));
+    const int np = apply_partials(A->plan, (int)sizeof(T), A->packed != nullptr);
     hipLaunchKernelGGL(k_dot_partials<T>, dim3(np), dim3(kThreads), 0, ctx->stream, (const T*)y, dotv,
                        (int64_t)A->space->undf, part, stop);
     LFR_CHECK_LAUNCH(ctx);
@@ -450,8 +522,17 @@ int check_op(lfr_ctx* ctx, const lfr_lma_op* A, const char* who) {
               "%s: operator plan must be a matrix_vector plan of its space", who);
   LFR_REQUIRE(ctx, !A->apply_bc || (int)A->space->boundary_h.size() == 2 * A->space->ndf,
               "%s: apply_bc set but the space has no boundary values", who);
+  if (A->packed) {
+    const lfr_lma_packed* pk = A->packed;
+    LFR_REQUIRE(ctx, pk->ctx == ctx && pk->plan == A->plan && pk->in == A->space && pk->prec == A->prec &&
+                         pk->kind == 0,
+                "%s: packed operator was not packed from this operator's plan, space and precision", who);
+  }
   // build the column records now: never inside a captured graph
-  if (lfr_records_enabled()) {
+  if (apply_unified(A->plan, A->prec, A->packed)) {
+    ctx->err.clear();
+    if (!lfr_plan_urecords(ctx, A->plan, A->space)) return LFR_ERR_NOMEM;
+  } else if (A->packed || lfr_records_enabled()) {
     int stride = 0;
     ctx->err.clear();
     for (size_t ci = 0; ci < A->plan->classes.size(); ++ci)
@@ -475,11 +556,12 @@ int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P,
   T* z = (T*)w.z;
   const T* dg = (const T*)P->diag_d;
   const int G = lfr::red_blocks(n);
+  const int np_apply = apply_partials(A->plan, (int)sizeof(T), A->packed != nullptr);
   auto one_iteration = [&]() -> int {
     int rc = apply_op<T>(ctx, A, p, z, p, w.part_apply, w.S);
     if (rc) return rc;
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_apply,
-                       w.napply, w.S, P->max_iter);
+                       np_apply, w.S, P->max_iter);
     if (diag)
       hipLaunchKernelGGL((k_cg_update<T, true>), dim3(G), dim3(kThreads), 0, ctx->stream, x, r, p,
                          z, dg, n, w.S, w.part_rr, w.part_rz);

@@ -24,7 +24,8 @@ _P = C.c_void_p
 
 
 class LmaOp(C.Structure):
-    _fields_ = [("prec", C.c_int), ("matrix_d", _P), ("space", _P), ("plan", _P), ("apply_bc", C.c_int)]
+    _fields_ = [("prec", C.c_int), ("matrix_d", _P), ("space", _P), ("plan", _P), ("apply_bc", C.c_int),
+                ("packed", _P)]
 
 
 class SolverParams(C.Structure):
@@ -64,8 +65,11 @@ def lma_op(op: "api.LMAOperator") -> LmaOp:
     if op.to_space is not op.from_space:
         raise LfrError(1, "solvers need a square operator (to_space is from_space)")
     plan = op.to_space.plan(op.colouring, 0)
+    pk = op.packed
+    if pk is not None and (pk.kind != 0 or pk.plan is not plan):
+        raise LfrError(1, "op.packed must be a kind-0 pack of this operator with its colouring")
     return LmaOp(api._prec(op.matrix), _P(op.matrix.data_ptr()), op.to_space.h, plan.h,
-                 int(bool(op.apply_bc)))
+                 int(bool(op.apply_bc)), pk.h if pk is not None else None)
 
 
 @dataclass
