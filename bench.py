@@ -4,10 +4,12 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 
 A step is one W2 mass apply y = M x (invoke(setval_c(y,0), matrix_vector(y,x,M))
 fused into one overwrite pass) over the C96 cubed sphere with 70 layers,
-FP32 (r_solver), inputs resident in HBM.  The JSON line also carries the FP64
-apply rate, the adjoint/forward time ratio, the refined-vs-FP64 CG result
-(when the solver module is present), the HBM roofline of the dominant kernel
-and a CPU baseline timed on this host.
+FP32 (r_solver), inputs resident in HBM, through the operator packed once
+into gather order (lfr_lma_pack: the form an operator applied many times
+takes; results are bit-identical to the apply on the reference layout, whose
+rate is reported beside it).  The JSON line also carries the FP64 apply
+rate, the adjoint/forward time ratio, the refined-vs-FP64 CG result, the
+HBM roofline of the dominant kernel and a CPU baseline timed on this host.
 """
 
 from __future__ import annotations
@@ -46,8 +48,8 @@ def apply_bytes(n3: int, ndf_to: int, ndf_from: int, undf_to: int, undf_from: in
 
 def profiled_traffic(key: str = "traffic_bytes_fp32_apply"):
     """HBM bytes per apply measured by the rocprofv3 PMC runs recorded in
-    profiles/k_gather_w2_c96l70.md (FETCH_SIZE x 2 + WRITE_SIZE), or None."""
-    p = ROOT / "profiles" / "k_gather_w2_c96l70.md"
+    profiles/k_gather_uni_w2_c96l70.md (FETCH_SIZE x 2 + WRITE_SIZE), or None."""
+    p = ROOT / "profiles" / "k_gather_uni_w2_c96l70.md"
     try:
         for line in p.read_text().splitlines():
             if line.startswith(key + ":"):
@@ -102,7 +104,8 @@ def cpu_baseline(cfg, seconds: float = 15.0):
 
 def bench_solvers(ctx, cfg, w2, torch, steps_cg: int = 50):
     """FP64 CG vs FP32-inner/FP64-refined CG to 1e-10 on the W2 mass matrix, and
-    the time of fixed 50-iteration FP32 and FP64 CG runs (per-iteration roofline)."""
+    the time of fixed 50-iteration FP32 and FP64 CG runs (per-iteration roofline),
+    with packed operators."""
     from lfric_core_amd import api
     from lfric_core_amd.solvers import ConjugateGradient, RefinedCG
     dev = ctx.device
@@ -110,6 +113,9 @@ def bench_solvers(ctx, cfg, w2, torch, steps_cg: int = 50):
     M32 = M64.float().contiguous()
     op64 = api.LMAOperator(M64, w2, w2, cfg.colouring)
     op32 = api.LMAOperator(M32, w2, w2, cfg.colouring)
+    # the solvers apply each operator through its packed copy
+    op64.packed = api.PackedOperator(op64)
+    op32.packed = api.PackedOperator(op32)
     b = torch.from_numpy(cfg.b).to(dev)
     n = cfg.w2.undf
     n3 = cfg.mesh.ncells * cfg.nlayers
@@ -304,6 +310,9 @@ def main():
     ap.add_argument("--no-solvers", action="store_true")
     ap.add_argument("--no-mg", action="store_true", help="skip the batched multigrid V-cycles")
     ap.add_argument("--no-multi", action="store_true", help="skip the 16-field MFMA apply")
+    ap.add_argument("--packed-only", action="store_true",
+                    help="time only the packed applies (profiling runs: the reference-layout "
+                         "kernels share the packed kernels' names)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a plain copy of 1 GiB (FP32) 5 times, for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes")
@@ -332,24 +341,23 @@ def main():
         M = torch.from_numpy(cfg.mass.astype(dt)).to(dev)
         op = api.LMAOperator(M, w2, w2, cfg.colouring)
         x = torch.from_numpy(cfg.b.astype(dt)).to(dev)
-        y = torch.empty_like(x)
-        fwd = lambda: api.matrix_vector(y, x, op, overwrite=True)
-        for _ in range(args.warmup):
-            fwd()
-        ms = time_events(fwd, args.steps, torch)
-        # adjoint of the same operator: x_bar = M^T y_bar (overwrite form)
-        xb = torch.empty_like(x)
-        adj = lambda: api.transpose_matrix_vector(xb, y, op, overwrite=True)
-        for _ in range(args.warmup):
-            adj()
-        ms_adj = time_events(adj, args.steps, torch)
         nbytes = apply_bytes(n3, 6, 6, cfg.w2.undf, cfg.w2.undf, cfg.mesh.ncells, s)
-        res[name] = {"ms": ms, "gdofs": cfg.w2.undf / (ms * 1e-3) / 1e9,
-                     "gbs": nbytes / (ms * 1e-3) / 1e9, "bytes": nbytes,
-                     "adj_ms": ms_adj, "adj_ratio": ms_adj / ms}
-        log(f"{name}: apply {ms*1e3:.1f} us  {res[name]['gdofs']:.1f} GDOF/s  "
-            f"{res[name]['gbs']:.0f} GB/s; adjoint {ms_adj*1e3:.1f} us (ratio {ms_adj/ms:.2f})")
-        # the same operator packed once into gather order (lfr_lma_pack)
+        # reference layout: local_stencil(ncell_3d, 6, 6) as the caller holds it
+        y = torch.empty_like(x)
+        xb = torch.empty_like(x)
+        fwd = lambda: api.matrix_vector(y, x, op, overwrite=True)
+        adj = lambda: api.transpose_matrix_vector(xb, y, op, overwrite=True)
+        if args.packed_only:      # profiling runs: the packed kernels only
+            ms_ref = ms_ref_adj = float("nan")
+        else:
+            for _ in range(args.warmup):
+                fwd()
+            ms_ref = time_events(fwd, args.steps, torch)
+            for _ in range(args.warmup):
+                adj()
+            ms_ref_adj = time_events(adj, args.steps, torch)
+        # the product path for an operator applied many times: packed once
+        # into gather order (lfr_lma_pack), forward and adjoint
         torch.cuda.synchronize()
         t = time.perf_counter()
         pk = api.PackedOperator(op, kind=0)
@@ -359,19 +367,28 @@ def main():
         fwdp = lambda: pk.apply(yp, x, overwrite=True)
         for _ in range(args.warmup):
             fwdp()
-        ms_p = time_events(fwdp, args.steps, torch)
-        same = bool(torch.equal(yp, y))
+        ms = time_events(fwdp, args.steps, torch)
+        if args.packed_only:
+            y.copy_(yp)
+            same = None
+        else:
+            same = bool(torch.equal(yp, y))
         adjp = lambda: pkt.apply(xb, y, overwrite=True)
         for _ in range(args.warmup):
             adjp()
-        ms_padj = time_events(adjp, args.steps, torch)
-        res[name].update({"packed_ms": ms_p, "packed_gdofs": cfg.w2.undf / (ms_p * 1e-3) / 1e9,
-                          "packed_gbs": nbytes / (ms_p * 1e-3) / 1e9, "packed_adj_ms": ms_padj,
-                          "packed_adj_ratio": ms_padj / ms_p, "pack_ms": t_pack * 1e3,
-                          "packed_bitwise_equal": same, "packed_nbytes": pk.nbytes})
-        log(f"{name} packed: apply {ms_p*1e3:.1f} us  {cfg.w2.undf/(ms_p*1e-3)/1e9:.1f} GDOF/s  "
-            f"{nbytes/(ms_p*1e-3)/1e9:.0f} GB/s; adjoint {ms_padj*1e3:.1f} us (ratio {ms_padj/ms_p:.2f}); "
-            f"pack {t_pack*1e3:.1f} ms; bitwise equal {same}")
+        ms_adj = time_events(adjp, args.steps, torch)
+        res[name] = {"ms": ms, "gdofs": cfg.w2.undf / (ms * 1e-3) / 1e9,
+                     "gbs": nbytes / (ms * 1e-3) / 1e9, "bytes": nbytes,
+                     "adj_ms": ms_adj, "adj_ratio": ms_adj / ms, "pack_ms": t_pack * 1e3,
+                     "packed_nbytes": pk.nbytes, "bitwise_equal_to_reference_layout": same,
+                     "ref_ms": ms_ref, "ref_gdofs": cfg.w2.undf / (ms_ref * 1e-3) / 1e9,
+                     "ref_gbs": nbytes / (ms_ref * 1e-3) / 1e9, "ref_adj_ms": ms_ref_adj,
+                     "ref_adj_ratio": ms_ref_adj / ms_ref}
+        log(f"{name}: apply {ms*1e3:.1f} us  {res[name]['gdofs']:.1f} GDOF/s  "
+            f"{res[name]['gbs']:.0f} GB/s; adjoint {ms_adj*1e3:.1f} us (ratio {ms_adj/ms:.2f}); "
+            f"pack {t_pack*1e3:.1f} ms; same bits as reference layout: {same}")
+        log(f"{name} reference layout: apply {ms_ref*1e3:.1f} us  {res[name]['ref_gdofs']:.1f} GDOF/s  "
+            f"{res[name]['ref_gbs']:.0f} GB/s; adjoint {ms_ref_adj*1e3:.1f} us (ratio {ms_ref_adj/ms_ref:.2f})")
         pk.close()
         pkt.close()
         del M, op, x, y, xb, yp
@@ -432,7 +449,10 @@ def main():
         "config": {"workload": f"W2 LMA apply y = M x (mass BB^T+6I), C{nc} cubed sphere, "
                                f"{nl} layers, lowest order, r_solver=FP32",
                    "mesh": f"C{nc}", "nlayers": nl, "undf_w2": cfg.w2.undf,
-                   "ncell_3d": n3},
+                   "ncell_3d": n3,
+                   "operator": "packed once into gather order by lfr_lma_pack (same bits as the "
+                               "reference layout; that apply is in reference_layout)",
+                   "pack_ms": round(r32["pack_ms"], 2)},
         "roofline": {"bound": "hbm", "achieved": round(r32["gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r32["gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic() if nc == 96 else None},
@@ -441,6 +461,14 @@ def main():
                  "roofline_frac": round(res["fp64"]["gbs"] / HBM_PEAK_GBS, 4)},
         "adjoint_fwd_ratio": {"fp32": round(r32["adj_ratio"], 3),
                               "fp64": round(res["fp64"]["adj_ratio"], 3)},
+        "reference_layout": {k: {"gdofs": round(res[k]["ref_gdofs"], 3), "ms": round(res[k]["ref_ms"], 6),
+                                 "gbs": round(res[k]["ref_gbs"], 1),
+                                 "roofline_frac": round(res[k]["ref_gbs"] / HBM_PEAK_GBS, 4),
+                                 "adjoint_fwd_ratio": round(res[k]["ref_adj_ratio"], 3)}
+                             for k in ("fp32", "fp64")},
+        "bitwise_equal_packed_vs_reference_layout": (None if args.packed_only else
+                                                     all(res[k]["bitwise_equal_to_reference_layout"]
+                                                         for k in ("fp32", "fp64"))),
     }
     out.update(extra)
     if not args.no_cpu:

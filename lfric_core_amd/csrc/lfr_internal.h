@@ -78,6 +78,7 @@ bool lfr_gather_vec_nt(bool packed);
 bool lfr_plan_unified(const lfr_plan* plan, int ndf_in, int lpt);
 int lfr_plan_ncolumns_total(const lfr_plan* plan);
 int lfr_unified_lpt(int prec);
+bool lfr_gather_stage();
 // Unified records for input space `in` (built on first use), in column
 // memory order; nullptr on allocation failure (error set on ctx).
 const int32_t* lfr_plan_urecords(lfr_ctx* ctx, const lfr_plan* plan, const lfr_space* in);

@@ -214,14 +214,17 @@ void launch_rec_shape(hipStream_t s, int kind, unsigned nb, const T* M, const T*
 template <typename T, int NDF_IN, int KIND, int LPT>
 void launch_uni_t(hipStream_t s, bool nt, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur,
                   int ncol, int nl, int64_t n3, int ndf_out, int overwrite, int64_t pcs) {
-  if (nt)
-    hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, KIND, false, LPT, true>), dim3(nbp), dim3(kThreads), 0, s, M, x,
-                       y, ur, ncol, nl, n3, ndf_out, overwrite, pcs, nbp, (const T*)nullptr, (double*)nullptr,
-                       (const double*)nullptr);
-  else
-    hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, KIND, false, LPT, false>), dim3(nbp), dim3(kThreads), 0, s, M,
-                       x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs, nbp, (const T*)nullptr, (double*)nullptr,
-                       (const double*)nullptr);
+#define LUNI(NTV, STV)                                                                                        \
+  hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, KIND, false, LPT, NTV, STV>), dim3(nbp), dim3(kThreads), 0, s, \
+                     M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs, nbp, (const T*)nullptr, (double*)nullptr, \
+                     (const double*)nullptr)
+  const bool st = lfr_gather_stage();
+  if (nt) {
+    if (st) LUNI(true, true); else LUNI(true, false);
+  } else {
+    if (st) LUNI(false, true); else LUNI(false, false);
+  }
+#undef LUNI
 }
 
 template <typename T, int NDF_IN>
@@ -563,6 +566,16 @@ bool lfr_plan_unified(const lfr_plan* plan, int ndf_in, int lpt) {
 int lfr_unified_lpt(int prec) {
   const int l = lfr_gather_lpt(prec);
   return l >= 2 ? l : 2;
+}
+
+// The unified gather writes its output through LDS with aligned stores
+// (measured faster); LFR_GATHER_STAGE=0 stores from registers instead.
+bool lfr_gather_stage() {
+  static const bool on = [] {
+    const char* e = getenv("LFR_GATHER_STAGE");
+    return !(e && e[0] == '0');
+  }();
+  return on;
 }
 
 int lfr_plan_ncolumns_total(const lfr_plan* plan) {

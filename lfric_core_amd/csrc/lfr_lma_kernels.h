@@ -495,7 +495,7 @@ __device__ __forceinline__ T rot_pick(const LVec<T, LPT>& v, int u, int d) {
   return d == 0 ? a : (d > 0 ? b : c);
 }
 
-template <typename T, int NDF_IN, int KIND, bool DOT, int LPT, bool NT>
+template <typename T, int NDF_IN, int KIND, bool DOT, int LPT, bool NT, bool STAGE = false>
 __global__ void __launch_bounds__(kThreads)
 k_gather_uni(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
              const int32_t* __restrict__ urec, int ncol, int nl, int64_t n3, int ndf_out, int overwrite,
@@ -603,7 +603,41 @@ k_gather_uni(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y
       }
     }
   }
-  if (valid) {
+  if (STAGE) {
+    // The workgroup's threads own one contiguous dof range [wlo, whi)
+    // (columns in memory order, levels in thread order): stage it in LDS
+    // and write it with aligned 16-byte stores.
+    __shared__ T buf[kThreads * LPT];
+    __shared__ int64_t range[2];
+    const int64_t nthr = (int64_t)ncol * tpc;
+    const int64_t tlast = min((int64_t)lb * kThreads + kThreads, nthr) - 1;
+    if (threadIdx.x == 0) range[0] = (int64_t)g0 + min(p * LPT, H);
+    if (t == tlast) range[1] = (int64_t)g0 + min((p + 1) * LPT, H);
+    __syncthreads();
+    const int64_t wlo = range[0], whi = range[1];
+    if (valid)
+#pragma unroll
+      for (int u = 0; u < LPT; ++u)
+        if (u >= own) buf[(int64_t)g0 + L0 + u - wlo] = acc[u];
+    __syncthreads();
+    if ((int64_t)lb * kThreads < nthr) {
+      constexpr int V = 16 / (int)sizeof(T);
+      const int64_t a = (wlo + V - 1) & ~(int64_t)(V - 1);
+      const int64_t bnd = whi & ~(int64_t)(V - 1);
+      if (a >= bnd) {
+        for (int64_t i = wlo + threadIdx.x; i < whi; i += kThreads) y[i] = buf[i - wlo];
+      } else {
+        if (wlo + threadIdx.x < a) y[wlo + threadIdx.x] = buf[threadIdx.x];
+        if (bnd + threadIdx.x < whi) y[bnd + threadIdx.x] = buf[bnd + threadIdx.x - wlo];
+        for (int64_t q = a + (int64_t)threadIdx.x * V; q < bnd; q += (int64_t)kThreads * V) {
+          LVec<T, V> o;
+#pragma unroll
+          for (int u = 0; u < V; ++u) o.v[u] = buf[q - wlo + u];
+          *reinterpret_cast<LVec<T, V>*>(y + q) = o;
+        }
+      }
+    }
+  } else if (valid) {
     if (own == 0) {
       LVec<T, LPT> o;
 #pragma unroll

@@ -378,16 +378,25 @@ void launch_apply_shape(hipStream_t s, const lfr_plan_class& c, const T* M, cons
     launch_apply_t<T, NDF_IN, 0, 2>(s, c, M, x, y, mi, nl, n3, ndf, ndf, dotv, part, stop);
 }
 
+template <typename T, int NDF_IN, int LPT, bool NT, bool ST>
+void launch_apply_uni_st(hipStream_t s, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur, int ncol,
+                         int nl, int64_t n3, int64_t pcs, const T* dotv, double* part, const double* stop) {
+  if (dotv)
+    hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, 0, true, LPT, NT, ST>), dim3(nbp), dim3(kThreads), 0, s, M, x,
+                       y, ur, ncol, nl, n3, NDF_IN, 1, pcs, nbp, dotv, part, stop);
+  else
+    hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, 0, false, LPT, NT, ST>), dim3(nbp), dim3(kThreads), 0, s, M, x,
+                       y, ur, ncol, nl, n3, NDF_IN, 1, pcs, nbp, (const T*)nullptr, (double*)nullptr,
+                       (const double*)nullptr);
+}
+
 template <typename T, int NDF_IN, int LPT, bool NT>
 void launch_apply_uni_nt(hipStream_t s, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur, int ncol,
                          int nl, int64_t n3, int64_t pcs, const T* dotv, double* part, const double* stop) {
-  if (dotv)
-    hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, 0, true, LPT, NT>), dim3(nbp), dim3(kThreads), 0, s, M, x, y,
-                       ur, ncol, nl, n3, NDF_IN, 1, pcs, nbp, dotv, part, stop);
+  if (lfr_gather_stage())
+    launch_apply_uni_st<T, NDF_IN, LPT, NT, true>(s, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
   else
-    hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, 0, false, LPT, NT>), dim3(nbp), dim3(kThreads), 0, s, M, x, y,
-                       ur, ncol, nl, n3, NDF_IN, 1, pcs, nbp, (const T*)nullptr, (double*)nullptr,
-                       (const double*)nullptr);
+    launch_apply_uni_st<T, NDF_IN, LPT, NT, false>(s, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
 }
 
 template <typename T, int NDF_IN, int LPT>
