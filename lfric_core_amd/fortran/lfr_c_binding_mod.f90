@@ -139,6 +139,27 @@ module lfr_c_binding_mod
       integer(c_int), value :: prec, overwrite
     end function lfr_transpose_matrix_vector
 
+    ! packed copy of an operator applied many times (e.g. a mass matrix)
+    integer(c_int) function lfr_lma_pack(ctx, prec, matrix, x_space, plan, packed) &
+        bind(C, name="lfr_lma_pack")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, matrix, x_space, plan
+      integer(c_int), value :: prec
+      type(c_ptr), intent(out) :: packed
+    end function lfr_lma_pack
+
+    integer(c_int) function lfr_lma_packed_destroy(packed) bind(C, name="lfr_lma_packed_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: packed
+    end function lfr_lma_packed_destroy
+
+    integer(c_int) function lfr_matrix_vector_packed(ctx, packed, lhs, x, overwrite) &
+        bind(C, name="lfr_matrix_vector_packed")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, packed, lhs, x
+      integer(c_int), value :: overwrite
+    end function lfr_matrix_vector_packed
+
     integer(c_int) function lfr_columnwise_op_appinv(ctx, prec, lhs, x, cm, bandwidth, nrow,  &
                                                      ncell_2d, space, indirection, fp64_pivots) &
         bind(C, name="lfr_columnwise_op_appinv")
