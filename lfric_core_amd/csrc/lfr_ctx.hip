@@ -60,8 +60,22 @@ int lfr_init(int device, lfr_ctx** out) {
     delete ctx;
     return lfr_set_error(nullptr, LFR_ERR_NOMEM, "pinned scalar buffer allocation failed");
   }
+  auto drop_events = [ctx] {
+    for (auto& e : ctx->ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  for (auto& e : ctx->ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      e = nullptr;
+      drop_events();
+      (void)hipHostFree(ctx->host_scalars);
+      (void)hipStreamDestroy(ctx->own_stream);
+      delete ctx;
+      return lfr_set_error(nullptr, LFR_ERR_HIP, "event creation failed");
+    }
   int rc = lfr_work_reserve(ctx, 1u << 20);
   if (rc != LFR_OK) {
+    drop_events();
     (void)hipHostFree(ctx->host_scalars);
     (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
@@ -74,6 +88,9 @@ int lfr_init(int device, lfr_ctx** out) {
 int lfr_finalise(lfr_ctx* ctx) {
   if (!ctx) return LFR_OK;
   (void)hipStreamSynchronize(ctx->stream);
+  for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->work) (void)hipFree(ctx->work);
   if (ctx->host_scalars) (void)hipHostFree(ctx->host_scalars);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);

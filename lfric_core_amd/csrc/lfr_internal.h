@@ -19,9 +19,14 @@ struct lfr_ctx {
   // grow-only device workspace for reductions and solver scalars
   void* work = nullptr;
   size_t work_bytes = 0;
-  // pinned host staging for scalar read-backs
+  // pinned host staging for scalar read-backs (64 doubles)
   double* host_scalars = nullptr;
   int cu_count = 256;
+  // events marking scalar read-backs (solver chunks in flight)
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  // instantiated solver graphs, most recently used last; the key holds
+  // every pointer and scalar the captured kernels were given
+  std::vector<std::pair<std::vector<uint64_t>, hipGraphExec_t>> graphs;
 };
 
 struct lfr_space {

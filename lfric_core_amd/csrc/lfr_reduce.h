@@ -42,20 +42,21 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return s;
 }
 
-// sum of p[threadIdx.x + k*kRedThreads] over k in increasing order (the
-// thread's share of a block-wide sum of n values); eight loads are issued
-// before they are added, the addition order is unchanged.
+// sum of p[threadIdx.x + k*BS] over k in increasing order (the thread's
+// share of a block-wide sum of n values over BS threads); eight loads are
+// issued before they are added, the addition order is unchanged.
+template <int BS = kRedThreads>
 __device__ __forceinline__ double strided_sum(const double* __restrict__ p, int n) {
   double s = 0.0;
   int b = threadIdx.x;
-  for (; b + 7 * kRedThreads < n; b += 8 * kRedThreads) {
+  for (; b + 7 * BS < n; b += 8 * BS) {
     double v[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = p[b + q * kRedThreads];
+    for (int q = 0; q < 8; ++q) v[q] = p[b + q * BS];
 #pragma unroll
     for (int q = 0; q < 8; ++q) s += v[q];
   }
-  for (; b < n; b += kRedThreads) s += p[b];
+  for (; b < n; b += BS) s += p[b];
   return s;
 }
 

@@ -22,6 +22,7 @@
 // (sci_null_preconditioner_alg_mod.x90:84); r is used in its place.
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "lfr_internal.h"
@@ -40,17 +41,30 @@ constexpr double kEps = 2.220446049250313e-16;  // epsilon(1.0_r_def), constants
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads)
-k_cg_alpha(const double* __restrict__ part, int np, double* __restrict__ S, int max_iter) {
+// alpha = rz / <p, A p> from the apply's per-workgroup partials (one per
+// gather workgroup, ~12,000 at C96L70): 1024 threads share the sum.
+constexpr int kAlphaThreads = 1024;
+
+// rz = <r, z> (z = r without a preconditioner) is summed here from the
+// partials of the pass that formed r (residual or update), stored in S_RZ
+// for this iteration's beta, and used for alpha.
+__global__ void __launch_bounds__(kAlphaThreads)
+k_cg_alpha(const double* __restrict__ part, int np, const double* __restrict__ part_rz, int nrz,
+           double* __restrict__ S, int max_iter) {
   if (S[S_STOP] != 0.0) return;
   if (S[S_ITER] >= (double)max_iter) {
     if (threadIdx.x == 0) S[S_STOP] = 2.0;
     return;
   }
-  __shared__ double lds[kThreads / 64];
-  const double s = lfr::block_sum(lfr::strided_sum(part, np), lds);
+  __shared__ double lds[kAlphaThreads / 64];
+  const double s = lfr::block_sum<kAlphaThreads>(lfr::strided_sum<kAlphaThreads>(part, np), lds);
+  // rz in the order k_cg_beta_direction sums the same partials (256-thread
+  // stride; the extra waves add zeros), so both see the same bits
+  const double rz = lfr::block_sum<kAlphaThreads>(
+      threadIdx.x < kThreads ? lfr::strided_sum<kThreads>(part_rz, nrz) : 0.0, lds);
   if (threadIdx.x == 0) {
-    S[S_ALPHA] = S[S_RZ] / s;
+    S[S_RZ] = rz;
+    S[S_ALPHA] = rz / s;
     S[S_ITER] = S[S_ITER] + 1.0;
   }
 }
@@ -88,42 +102,39 @@ k_cg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, T* __
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
-k_cg_beta(const double* __restrict__ part_rr, const double* __restrict__ part_rz, int np,
-          double* __restrict__ S, int monitor, double r_tol, double a_tol, int diag) {
-  if (S[S_STOP] != 0.0) return;
-  __shared__ double lds[kThreads / 64];
-  double rr = 0.0, rz = 0.0;
-  for (int b = threadIdx.x; b < np; b += kThreads) {
-    rr += part_rr[b];
-    if (diag) rz += part_rz[b];
-  }
-  rr = lfr::block_sum(rr, lds);
-  if (diag) rz = lfr::block_sum(rz, lds);
-  if (threadIdx.x == 0) {
-    if (!diag) rz = rr;
-    const double rn = sqrt(rr);
-    S[S_RNRM] = rn;
-    S[S_RR] = rr;
-    if (rn != rn) {
-      S[S_NAN] = 1.0;
-      S[S_STOP] = 3.0;
-      return;
-    }
-    if (monitor && ((rn / S[S_RNRM0] <= r_tol) || (rn <= a_tol))) {
-      S[S_STOP] = 1.0;
-      return;
-    }
-    S[S_BETA] = rz / S[S_RZ];
-    S[S_RZ] = rz;
-  }
-}
-
+// beta and the convergence test, then p = beta * p + z.  Every workgroup
+// sums the update's partials in the same order (identical bits); workgroup 0
+// records the results; S_RZ (read here) is only written by k_cg_alpha.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_cg_direction(T* __restrict__ p, const T* __restrict__ z, int64_t n, const double* __restrict__ S) {
+k_cg_beta_direction(T* __restrict__ p, const T* __restrict__ z, int64_t n, const double* __restrict__ part_rr,
+                    const double* __restrict__ part_rz, int np, double* __restrict__ S, int monitor, double r_tol,
+                    double a_tol, int diag) {
   if (S[S_STOP] != 0.0) return;
-  const T b = (T)S[S_BETA];
+  __shared__ double lds[kThreads / 64];
+  __shared__ double bc[2];
+  const double rr = lfr::block_sum(lfr::strided_sum(part_rr, np), lds);
+  const double rzs = diag ? lfr::block_sum(lfr::strided_sum(part_rz, np), lds) : 0.0;
+  if (threadIdx.x == 0) {
+    const double rz = diag ? rzs : rr;
+    const double rn = sqrt(rr);
+    int stop = 0;
+    if (rn != rn) stop = 3;
+    else if (monitor && ((rn / S[S_RNRM0] <= r_tol) || (rn <= a_tol))) stop = 1;
+    const double beta = rz / S[S_RZ];
+    if (blockIdx.x == 0) {
+      S[S_RNRM] = rn;
+      S[S_RR] = rr;
+      if (stop == 3) S[S_NAN] = 1.0;
+      if (stop) S[S_STOP] = (double)stop;
+      else S[S_BETA] = beta;
+    }
+    bc[0] = (double)stop;
+    bc[1] = beta;
+  }
+  __syncthreads();
+  if (bc[0] != 0.0) return;
+  const T b = (T)bc[1];
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kThreads) {
     const T bp = b * p[i];
@@ -150,6 +161,14 @@ k_residual(T* __restrict__ r, const T* __restrict__ b, int64_t n, double* __rest
   }
   rr = lfr::block_sum(rr, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// ||r0|| < epsilon: cg converged in 0 iterations
+// (sci_iterative_solver_mod.f90:523-529); every later kernel returns at entry.
+__global__ void k_cg_start(double* __restrict__ S, double eps) {
+  if (threadIdx.x != 0) return;
+  S[S_RNRM] = S[S_RNRM0];
+  if (S[S_RNRM0] < eps) S[S_STOP] = 1.0;
 }
 
 // S[slot] = (sqrt?) sum(part)
@@ -569,42 +588,74 @@ int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P,
   auto one_iteration = [&]() -> int {
     int rc = apply_op<T>(ctx, A, p, z, p, w.part_apply, w.S);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_apply,
-                       np_apply, w.S, P->max_iter);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kAlphaThreads), 0, ctx->stream, w.part_apply,
+                       np_apply, diag ? w.part_rz : w.part_rr, G, w.S, P->max_iter);
     if (diag)
       hipLaunchKernelGGL((k_cg_update<T, true>), dim3(G), dim3(kThreads), 0, ctx->stream, x, r, p,
                          z, dg, n, w.S, w.part_rr, w.part_rz);
     else
       hipLaunchKernelGGL((k_cg_update<T, false>), dim3(G), dim3(kThreads), 0, ctx->stream, x, r,
                          p, z, dg, n, w.S, w.part_rr, w.part_rz);
-    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rr, w.part_rz,
-                       G, w.S, P->monitor_convergence, P->r_tol, P->a_tol, (int)diag);
-    hipLaunchKernelGGL(k_cg_direction<T>, dim3(G), dim3(kThreads), 0, ctx->stream, p,
-                       diag ? (const T*)z : (const T*)r, n, w.S);
+    hipLaunchKernelGGL(k_cg_beta_direction<T>, dim3(G), dim3(kThreads), 0, ctx->stream, p,
+                       diag ? (const T*)z : (const T*)r, n, w.part_rr, w.part_rz, G, w.S,
+                       P->monitor_convergence, P->r_tol, P->a_tol, (int)diag);
     LFR_CHECK_LAUNCH(ctx);
     return LFR_OK;
   };
 
+  // The captured chunk depends only on the pointers and scalars below, so
+  // its instantiation is kept on the context and reused by later solves
+  // with the same operator, vectors and settings.
   hipGraphExec_t exec = nullptr;
   if (use_graphs()) {
-    hipGraph_t graph = nullptr;
-    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed) == hipSuccess) {
-      int rc = LFR_OK;
-      for (int i = 0; i < chunk && rc == LFR_OK; ++i) rc = one_iteration();
-      hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-      if (rc != LFR_OK) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
+    uint64_t rt, at;
+    memcpy(&rt, &P->r_tol, 8);
+    memcpy(&at, &P->a_tol, 8);
+    std::vector<uint64_t> key = {
+        0xC6ull, sizeof(T), (uint64_t)(uintptr_t)A->matrix_d, (uint64_t)(uintptr_t)A->packed,
+        (uint64_t)(uintptr_t)A->plan, (uint64_t)(uintptr_t)A->space, (uint64_t)A->apply_bc,
+        (uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)r, (uint64_t)(uintptr_t)p, (uint64_t)(uintptr_t)z,
+        (uint64_t)(uintptr_t)w.part_apply, (uint64_t)(uintptr_t)w.part_rr, (uint64_t)(uintptr_t)w.part_rz,
+        (uint64_t)(uintptr_t)w.S, (uint64_t)(uintptr_t)dg, (uint64_t)P->max_iter,
+        (uint64_t)P->monitor_convergence, rt, at, (uint64_t)diag, (uint64_t)chunk, (uint64_t)n};
+    // enforce_bc takes the boundary values by value: they are part of the graph
+    if (A->apply_bc)
+      for (int32_t v : A->space->boundary_h) key.push_back((uint64_t)(uint32_t)v);
+    for (size_t i = 0; i < ctx->graphs.size(); ++i)
+      if (ctx->graphs[i].first == key) {
+        auto e = ctx->graphs[i];
+        ctx->graphs.erase(ctx->graphs.begin() + i);
+        ctx->graphs.push_back(e);
+        exec = e.second;
+        break;
       }
-      if (e == hipSuccess && graph) {
-        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
-        (void)hipGraphDestroy(graph);
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed) == hipSuccess) {
+        int rc = LFR_OK;
+        for (int i = 0; i < chunk && rc == LFR_OK; ++i) rc = one_iteration();
+        hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        if (rc != LFR_OK) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return rc;
+        }
+        if (e == hipSuccess && graph) {
+          if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
+          (void)hipGraphDestroy(graph);
+        }
+      }
+      (void)hipGetLastError();
+      if (exec) {
+        constexpr size_t kKeep = 8;
+        if (ctx->graphs.size() >= kKeep) {
+          (void)hipGraphExecDestroy(ctx->graphs.front().second);
+          ctx->graphs.erase(ctx->graphs.begin());
+        }
+        ctx->graphs.emplace_back(key, exec);
       }
     }
-    (void)hipGetLastError();
   }
-  double* Sh = ctx->host_scalars;
-  for (;;) {
+  auto run_chunk = [&]() -> int {
     if (exec) {
       LFR_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
     } else {
@@ -613,11 +664,36 @@ int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P,
         if (rc) return rc;
       }
     }
-    LFR_HIP(ctx, hipMemcpyAsync(Sh, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (Sh[S_STOP] != 0.0) break;
+    return LFR_OK;
+  };
+  double* Sh = ctx->host_scalars;
+  if (!P->monitor_convergence) {
+    // a fixed number of iterations: all chunks, then one read-back
+    // (iterations past max_iter return at entry)
+    for (int done = 0; done < P->max_iter; done += chunk) {
+      int rc = run_chunk();
+      if (rc) return rc;
+    }
+  } else {
+    // one chunk stays queued while the host checks the previous one; a
+    // chunk that starts after convergence returns at entry
+    int rc = run_chunk();
+    if (rc) return rc;
+    LFR_HIP(ctx, hipMemcpyAsync(Sh + 16, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LFR_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    for (int k = 0;; ++k) {
+      rc = run_chunk();
+      if (rc) return rc;
+      const int nxt = (k + 1) & 1;
+      LFR_HIP(ctx, hipMemcpyAsync(Sh + 16 + 16 * nxt, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+      LFR_HIP(ctx, hipEventRecord(ctx->ev[nxt], ctx->stream));
+      LFR_HIP(ctx, hipEventSynchronize(ctx->ev[k & 1]));
+      if (Sh[16 + 16 * (k & 1) + S_STOP] != 0.0) break;
+    }
   }
-  if (exec) (void)hipGraphExecDestroy(exec);
+  LFR_HIP(ctx, hipMemcpyAsync(Sh, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LFR_OK;
 }
 
@@ -648,35 +724,28 @@ int cg_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x
   }
   hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rr, G, w.S,
                      (int)S_RNRM0, 1);
+  hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, ctx->stream, w.S, kEps);
   LFR_CHECK_LAUNCH(ctx);
   double* Sh = ctx->host_scalars;
-  LFR_HIP(ctx, hipMemcpyAsync(Sh, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  st->r_nrm0 = Sh[S_RNRM0];
-  st->r_nrm = Sh[S_RNRM0];
-  st->iterations = 0;
-  st->converged = 0;
-  if (Sh[S_RNRM0] < kEps) {  // cg converged in 0 iterations (:523-529)
-    st->converged = 1;
-    return LFR_OK;
-  }
   // z = P^{-1} r ; rz = <r, z> ; p = z
   if (diag) {
+    // (rz itself is summed by the first k_cg_alpha from these partials)
     hipLaunchKernelGGL(k_prec_diag<T>, dim3(G), dim3(kThreads), 0, ctx->stream, z, r,
                        (const T*)P->diag_d, n, w.part_rz);
-    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rz, G, w.S,
-                       (int)S_RZ, 0);
     hipLaunchKernelGGL(k_copy_v<T>, dim3(G), dim3(kThreads), 0, ctx->stream, p, (const T*)z, n);
   } else {
-    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rr, G, w.S,
-                       (int)S_RZ, 0);
     hipLaunchKernelGGL(k_copy_v<T>, dim3(G), dim3(kThreads), 0, ctx->stream, p, (const T*)r, n);
   }
   LFR_CHECK_LAUNCH(ctx);
-  if (P->max_iter <= 0) return LFR_OK;
-  const int chunk = P->max_iter < 10 ? P->max_iter : 10;
-  int rc = cg_iterations<T>(ctx, A, P, x, w, n, chunk);
-  if (rc) return rc;
+  if (P->max_iter > 0) {
+    const int chunk = P->max_iter < 10 ? P->max_iter : 10;
+    int rc = cg_iterations<T>(ctx, A, P, x, w, n, chunk);  // ends with S in Sh
+    if (rc) return rc;
+  } else {
+    LFR_HIP(ctx, hipMemcpyAsync(Sh, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  st->r_nrm0 = Sh[S_RNRM0];
   st->iterations = (int)Sh[S_ITER];
   st->r_nrm = Sh[S_RNRM];
   st->converged = Sh[S_STOP] == 1.0 ? 1 : 0;

@@ -184,3 +184,35 @@ def test_cg_with_boundary_mask_matches_oracle(ctx, prob):
         if bv[df, 1] == 0:
             bnd[cfg.w2.dofmap[:, df] - 1 + cfg.nlayers - 1] = True
     assert np.all(xg[bnd] == 0.0)
+
+
+def test_cg_repeated_solves_and_fixed_counts(ctx, prob):
+    """Repeated solves reuse the context's instantiated iteration graph: the
+    same bits every time, fixed iteration counts that are not multiples of
+    the 10-iteration chunk, and monitored solves that stop where they did."""
+    from lfric_core_amd.solvers import ConjugateGradient
+    cfg, w2 = prob
+    print("case cg repeated solves", flush=True)
+    b = torch.from_numpy(cfg.b.astype(np.float32)).to(ctx.device)
+    op = _op(ctx, cfg, w2, np.float32)
+    for its in (25, 7):
+        outs = []
+        for _ in range(3):
+            x = torch.zeros_like(b)
+            st = ConjugateGradient(op, max_iter=its, monitor_convergence=False).apply(x, b, x_is_zero=True)
+            assert st.iterations == its and not st.converged
+            outs.append(x.cpu().numpy())
+        np.testing.assert_array_equal(outs[0], outs[1])
+        np.testing.assert_array_equal(outs[0], outs[2])
+    res = []
+    for _ in range(2):
+        x = torch.zeros_like(b)
+        st = ConjugateGradient(op, r_tol=1e-4, max_iter=500).apply(x, b, x_is_zero=True)
+        assert st.converged
+        res.append((st.iterations, x.cpu().numpy()))
+    assert res[0][0] == res[1][0]
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+    # fixed-count run to the monitored solve's iteration count: same iterate
+    x = torch.zeros_like(b)
+    ConjugateGradient(op, max_iter=res[0][0], monitor_convergence=False).apply(x, b, x_is_zero=True)
+    np.testing.assert_array_equal(x.cpu().numpy(), res[0][1])
