@@ -182,16 +182,30 @@ def bench_multi(ctx, torch, edge_cells: int = 48, nl: int = 70, steps: int = 20)
     op64 = api.LMAOperator(cfg.matrix, w2, w2, cfg.colouring)
     Y64 = torch.empty_like(cfg.x)
     y1 = torch.empty(undf, dtype=torch.float64, device=ctx.device)
+    xs = [cfg.x[:, r].contiguous() for r in range(16)]
     for r in range(16):
-        api.matrix_vector(y1, cfg.x[:, r].contiguous(), op64, overwrite=True)
+        api.matrix_vector(y1, xs[r], op64, overwrite=True)
         Y64[:, r] = y1
+    # the FP64 operator on all 16 fields: 16 single-field applies against one
+    # multidata apply (the refined solve's residual), same bits
+    ms_1x16 = time_events(lambda: [api.matrix_vector(y1, xs[r], op64, overwrite=True) for r in range(16)],
+                          3, torch)
+    Ynd = torch.empty_like(cfg.x)
+    api.matrix_vector_ndata(Ynd, cfg.x, op64, overwrite=True)
+    ms_nd = time_events(lambda: api.matrix_vector_ndata(Ynd, cfg.x, op64, overwrite=True), 5, torch)
+    nd_same = bool(torch.equal(Ynd, Y64))
+    del xs, Ynd
+    log(f"multi fp64 operator on 16 fields: 16 single-field applies {ms_1x16:.2f} ms, one multidata "
+        f"apply {ms_nd:.2f} ms, same bits {nd_same}")
     M32 = cfg.matrix.float()
     op32 = api.LMAOperator(M32, w2, w2, cfg.colouring)
     x32 = cfg.x.float().contiguous()
     ref_n = torch.linalg.vector_norm(Y64, dim=0)
     flops = 2.0 * 36 * 36 * 16 * n3
     out = {"workload": f"y_r = A x_r, r=0..15, W2 element order 1 (36x36 local matrices), "
-                       f"C{edge_cells}L{nl}", "undf": undf, "ncell_3d": n3}
+                       f"C{edge_cells}L{nl}", "undf": undf, "ncell_3d": n3,
+           "fp64_16_single_field_applies_ms": round(ms_1x16, 3), "fp64_multidata_apply_ms": round(ms_nd, 3),
+           "fp64_multidata_same_bits": nd_same}
     for name, mprec, sm in (("fp32", 4, 4), ("bf16_in_fp32_acc", 2, 2)):
         mop = api.MultiOperator(op32, mprec)
         y = torch.empty(undf, 16, dtype=torch.float32, device=ctx.device)

@@ -143,6 +143,17 @@ int lfr_transpose_matrix_vector(lfr_ctx* ctx, int prec, void* lhs_d, const void*
                                 const void* matrix_d, const lfr_space* lhs_space,
                                 const lfr_space* x_space, const lfr_plan* plan, int overwrite);
 
+/* lfr_matrix_vector (kind-0 plan) or lfr_transpose_matrix_vector (kind-1
+ * plan of lhs' space) on a multidata field of `ndata` fields stored "ndata
+ * first" (element (dof d, field r) at d*ndata + r, the layout of LFRic's
+ * multidata function spaces,
+ * infrastructure/source/function_space/function_space_constructor_helper_functions_mod.F90:1838-1846):
+ * the same operator applied to every field, each field's result bit-identical
+ * to a single-field apply.  1 <= ndata <= 64. */
+int lfr_matrix_vector_ndata(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_d, const void* matrix_d,
+                            const lfr_space* lhs_space, const lfr_space* x_space, const lfr_plan* plan,
+                            int ndata, int overwrite);
+
 /* Packed operator for repeated applies of one LMA operator (the
  * operator_type whose local_stencil, INF/operator/operator_mod.t90:31-122,
  * is assembled once and applied many times, as the mass matrices in the

@@ -199,6 +199,39 @@ def matrix_vector(lhs: torch.Tensor, x: torch.Tensor, op: LMAOperator, overwrite
     return lhs
 
 
+def _ndata_field(t: torch.Tensor, space: Space, name: str, dtype) -> int:
+    if not isinstance(t, torch.Tensor) or t.device != space.ctx.device:
+        raise LfrError(1, f"{name} must be a tensor on {space.ctx.device}")
+    if t.dim() != 2 or t.shape[0] != space.undf or not t.is_contiguous():
+        raise LfrError(1, f"{name} must be a contiguous (undf={space.undf}, ndata) tensor, got "
+                          f"{tuple(t.shape)}")
+    if t.dtype != dtype:
+        raise LfrError(1, f"{name} dtype {t.dtype} differs from the operator's {dtype}")
+    return t.shape[1]
+
+
+def matrix_vector_ndata(lhs: torch.Tensor, x: torch.Tensor, op: LMAOperator, overwrite: bool = False,
+                        transpose: bool = False, colouring: Colouring | None = None):
+    """matrix_vector (or, with transpose=True, transpose_matrix_vector) on a
+    multidata field: lhs[:, r] (+)= A x[:, r] for every column r of the
+    (undf, ndata) tensors ("ndata first" storage); each field bit-identical to
+    the single-field call."""
+    col = colouring or op.colouring
+    if col is None:
+        raise LfrError(1, "matrix_vector_ndata needs a colouring of the output space")
+    lsp, xsp = (op.from_space, op.to_space) if transpose else (op.to_space, op.from_space)
+    nd = _ndata_field(lhs, lsp, "lhs", op.matrix.dtype)
+    if _ndata_field(x, xsp, "x", op.matrix.dtype) != nd:
+        raise LfrError(1, "lhs and x have different ndata")
+    plan = lsp.plan(col, 1 if transpose else 0)
+    ctx = op.ctx
+    ctx.sync_stream()
+    check(lib().lfr_matrix_vector_ndata(ctx.h, _prec(lhs), _P(lhs.data_ptr()), _P(x.data_ptr()),
+                                        _P(op.matrix.data_ptr()), lsp.h, xsp.h, plan.h, nd,
+                                        int(overwrite)), ctx.h)
+    return lhs
+
+
 def dg_matrix_vector(lhs: torch.Tensor, x: torch.Tensor, op: LMAOperator):
     """lhs = M x on a discontinuous output space."""
     _field(lhs, op.to_space, "lhs", op.matrix.dtype)

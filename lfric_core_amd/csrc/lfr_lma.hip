@@ -880,6 +880,88 @@ extern "C" int lfr_matrix_vector_packed(lfr_ctx* ctx, const lfr_lma_packed* pk, 
                             overwrite, pk);
 }
 
+// ---------------------------------------------------------------------------
+// Multidata (ndata fields, "ndata first") apply
+// ---------------------------------------------------------------------------
+namespace {
+// fields per thread of the multidata gather: the widest of 16/8/4/1 that
+// divides ndata, capped at 4 (measured fastest on the 16-field FP64 W2
+// order-1 apply: 13.8 ms against 16.5 with 8 and 23.2 with 16);
+// LFR_NDATA_FPT=1/4/8/16 sets another cap
+int ndata_fpt(int nd) {
+  static const int cap = [] {
+    const char* e = getenv("LFR_NDATA_FPT");
+    const int v = e ? atoi(e) : 4;
+    return (v == 1 || v == 4 || v == 8 || v == 16) ? v : 4;
+  }();
+  for (int f : {16, 8, 4})
+    if (f <= cap && nd % f == 0) return f;
+  return 1;
+}
+
+template <typename T>
+int run_gather_ndata(lfr_ctx* ctx, const lfr_plan* plan, const T* M, const T* x, T* y, const lfr_space* in,
+                     int ndf_out, int64_t n3, int nd, int overwrite) {
+  const int nl = plan->space->nlayers;
+  for (const auto& c : plan->classes) {
+    if (c.ncol == 0) continue;
+    LFR_REQUIRE(ctx, c.gsize == 1 || c.gsize == 2, "plan class with %d entries per cell unsupported", c.gsize);
+    const int fpt = ndata_fpt(nd);
+    const unsigned nbp = pad8(lfr_blocks((int64_t)c.ncol * c.height * (nd / fpt), kThreads));
+#define LND(GSV, KD, F)                                                                                       \
+  hipLaunchKernelGGL((lfr::k_gather_nf<T, GSV, KD, F>), dim3(nbp), dim3(kThreads), 0, ctx->stream, M, x, y,  \
+                     in->map0_d, c.col_start_d, c.ent_d, c.ncol, c.height, c.ngroup, nl, n3, ndf_out, in->ndf, nd, \
+                     overwrite, nbp)
+    if (fpt == 16) {
+      if (plan->kind == 0) {
+        if (c.gsize == 1) LND(1, 0, 16); else LND(2, 0, 16);
+      } else {
+        if (c.gsize == 1) LND(1, 1, 16); else LND(2, 1, 16);
+      }
+    } else if (fpt == 8) {
+      if (plan->kind == 0) {
+        if (c.gsize == 1) LND(1, 0, 8); else LND(2, 0, 8);
+      } else {
+        if (c.gsize == 1) LND(1, 1, 8); else LND(2, 1, 8);
+      }
+    } else if (fpt == 4) {
+      if (plan->kind == 0) {
+        if (c.gsize == 1) LND(1, 0, 4); else LND(2, 0, 4);
+      } else {
+        if (c.gsize == 1) LND(1, 1, 4); else LND(2, 1, 4);
+      }
+    } else {
+      if (plan->kind == 0) {
+        if (c.gsize == 1) LND(1, 0, 1); else LND(2, 0, 1);
+      } else {
+        if (c.gsize == 1) LND(1, 1, 1); else LND(2, 1, 1);
+      }
+    }
+#undef LND
+    LFR_CHECK_LAUNCH(ctx);
+  }
+  return LFR_OK;
+}
+}  // namespace
+
+extern "C" int lfr_matrix_vector_ndata(lfr_ctx* ctx, int prec, void* lhs, const void* x, const void* M,
+                                       const lfr_space* lhs_space, const lfr_space* x_space, const lfr_plan* plan,
+                                       int ndata, int overwrite) {
+  if (!ctx) return LFR_ERR_ARG;
+  int rc = check_pair(ctx, prec, lhs, x, M, lhs_space, x_space, "lfr_matrix_vector_ndata");
+  if (rc) return rc;
+  LFR_REQUIRE(ctx, lhs != x, "lfr_matrix_vector_ndata: lhs must not alias x");
+  LFR_REQUIRE(ctx, ndata >= 1 && ndata <= 64, "lfr_matrix_vector_ndata: ndata must be 1..64, got %d", ndata);
+  LFR_REQUIRE(ctx, plan && plan->space == lhs_space,
+              "lfr_matrix_vector_ndata: plan must be a plan of the output space");
+  const int64_t n3 = (int64_t)lhs_space->ncell * lhs_space->nlayers;
+  if (prec == 4)
+    return run_gather_ndata<float>(ctx, plan, (const float*)M, (const float*)x, (float*)lhs, x_space,
+                                   lhs_space->ndf, n3, ndata, overwrite);
+  return run_gather_ndata<double>(ctx, plan, (const double*)M, (const double*)x, (double*)lhs, x_space,
+                                  lhs_space->ndf, n3, ndata, overwrite);
+}
+
 static int dg_common(lfr_ctx* ctx, int prec, void* lhs, const void* x, const void* M,
                      const lfr_space* to, const lfr_space* from, int inc, const char* who) {
   if (!ctx) return LFR_ERR_ARG;

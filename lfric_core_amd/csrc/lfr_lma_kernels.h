@@ -461,6 +461,104 @@ k_gather_vec(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y
 }
 
 // ---------------------------------------------------------------------------
+// k_gather for nf fields at once (a multidata field, "ndata first": element
+// (dof d, field r) at d*nf + r).  Thread <-> (column, level, FPT fields); the
+// threads of one output dof sit side by side in a wavefront, so a matrix
+// element is one load shared by nf/FPT lanes and each operand dof is one
+// nf-wide contiguous read.  The terms of every field are added in exactly the
+// order of k_gather's runtime-ndf path (bit-identical to NF single-field
+// applies).
+// ---------------------------------------------------------------------------
+template <typename T, int GS, int KIND, int FPT>
+__global__ void __launch_bounds__(kThreads)
+k_gather_nf(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
+            const int32_t* __restrict__ map_in, const int32_t* __restrict__ col_start,
+            const int32_t* __restrict__ ent, int ncol, int H, int ngroup, int nl, int64_t n3, int ndf_out,
+            int ndf_in, int nf, int overwrite, unsigned nb_pad) {
+  // FPT fields per thread (nf % FPT == 0): thread <-> (column, level, field group)
+  const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
+  const int ng = nf / FPT;
+  const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
+  const bool valid = t < (int64_t)ncol * H * ng;
+  const int64_t tt = valid ? t : 0;
+  const int64_t slot = tt / ng;
+  const int r0 = (int)(tt - slot * ng) * FPT;
+  const int col = (int)(slot / H);
+  const int L = (int)(slot - (int64_t)col * H);
+  const int32_t g0 = col_start[col];
+  T acc[FPT];
+  if (overwrite || !valid) {
+#pragma unroll
+    for (int f = 0; f < FPT; ++f) acc[f] = T(0);
+  } else {
+    const LVec<T, FPT> yv = ldvec<T, FPT, false>(y + (int64_t)(g0 + L) * nf + r0);
+#pragma unroll
+    for (int f = 0; f < FPT; ++f) acc[f] = yv.v[f];
+  }
+  const int32_t* e = ent + (int64_t)col * ngroup * GS * 2;
+  for (int g = 0; g < ngroup; ++g) {
+    const int32_t* eg = e + g * GS * 2;
+    const int cc = eg[0];
+    int ii[GS], kk[GS];
+    bool ok[GS];
+#pragma unroll
+    for (int q = 0; q < GS; ++q) {
+      const int code = eg[2 * q + 1];
+      const int k = L - (code & 1);
+      ok[q] = code >= 0 && k >= 0 && k < nl;
+      ii[q] = code >= 0 ? (code >> 1) : 0;
+      kk[q] = k < 0 ? 0 : (k >= nl ? nl - 1 : k);
+    }
+    const int32_t* mi = map_in + (int64_t)cc * ndf_in;
+    if (KIND == 0) {
+      for (int j = 0; j < ndf_in; ++j) {
+        const int xbj = mi[j];
+#pragma unroll
+        for (int q = 0; q < GS; ++q) {
+          const int64_t ik = (int64_t)cc * nl + kk[q];
+          const T m = M[ik + n3 * ((int64_t)ii[q] + (int64_t)ndf_out * j)];
+          const LVec<T, FPT> xv = ldvec<T, FPT, false>(x + (int64_t)(xbj + kk[q]) * nf + r0);
+#pragma unroll
+          for (int f = 0; f < FPT; ++f) {
+            const T p = m * xv.v[f];
+            const T s = acc[f] + p;
+            acc[f] = ok[q] ? s : acc[f];
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < GS; ++q) {
+        const int64_t ik = (int64_t)cc * nl + kk[q];
+        T dot[FPT];
+#pragma unroll
+        for (int f = 0; f < FPT; ++f) dot[f] = T(0);
+        for (int j = 0; j < ndf_in; ++j) {
+          const T m = M[ik + n3 * ((int64_t)j + (int64_t)ndf_in * ii[q])];
+          const LVec<T, FPT> xv = ldvec<T, FPT, false>(x + (int64_t)(mi[j] + kk[q]) * nf + r0);
+#pragma unroll
+          for (int f = 0; f < FPT; ++f) {
+            const T p = m * xv.v[f];
+            dot[f] = dot[f] + p;
+          }
+        }
+#pragma unroll
+        for (int f = 0; f < FPT; ++f) {
+          const T s = acc[f] + dot[f];
+          acc[f] = ok[q] ? s : acc[f];
+        }
+      }
+    }
+  }
+  if (valid) {
+    LVec<T, FPT> o;
+#pragma unroll
+    for (int f = 0; f < FPT; ++f) o.v[f] = acc[f];
+    *reinterpret_cast<LVec<T, FPT>*>(y + (int64_t)(g0 + L) * nf + r0) = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Unified gather over every output column of a plan in memory order, for
 // plans whose columns all have two entries and nl or nl+1 levels (W2 lowest
 // order: side-face columns shared by two cells, and bottom/top face stacks

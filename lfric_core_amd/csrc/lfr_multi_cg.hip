@@ -1,7 +1,8 @@
 // Sixteen simultaneous refined CG solves A x_r = b_r, r = 0..15.
 //
-// Outer loop in FP64 on the device: residual r_r = b_r - A64 x_r by the
-// single-field FP64 gather apply (one call per field), convergence when
+// Outer loop in FP64 on the device: residual r_r = b_r - A64 x_r by one
+// multidata FP64 gather apply of all fields (lfr_matrix_vector_ndata: each
+// field as a single-field apply computes it), convergence when
 // ||r_r|| <= r_tol ||b_r|| for every field.  Inner solves: unpreconditioned
 // CG on the FP32 multidata field (16 values per dof contiguous) with the
 // MFMA apply (lfr_matrix_vector_multi), per-field FP64 scalars
@@ -131,41 +132,14 @@ k_mcg_dir(float* __restrict__ p, const float* __restrict__ res, int64_t n16, con
     p[i] = res[i] + bt * p[i];
 }
 
-// Interleaved (n, 16) -> field-major (16, n), 64 dofs per block through LDS.
-constexpr int kTile = 64;
-__global__ void __launch_bounds__(256) k_split16(double* __restrict__ out, const double* __restrict__ in, int64_t n) {
-  __shared__ double tile[kR][kTile + 1];
-  const int64_t d0 = (int64_t)blockIdx.x * kTile;
-  for (int i = threadIdx.x; i < kTile * kR; i += 256) {
-    const int dd = i >> 4, f = i & 15;
-    if (d0 + dd < n) tile[f][dd] = in[(d0 + dd) * kR + f];
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < kTile * kR; i += 256) {
-    const int f = i / kTile, dd = i % kTile;
-    if (d0 + dd < n) out[(int64_t)f * n + d0 + dd] = tile[f][dd];
-  }
-}
-
-// res64[d*16+f] = b[d*16+f] - Y[f*n + d]; res32 = (float)res64
+// res64 = b - Y (multidata fields); res32 = (float)res64
 __global__ void __launch_bounds__(256) k_residual16(double* __restrict__ res64, float* __restrict__ res32,
                                                     const double* __restrict__ b, const double* __restrict__ Y,
-                                                    int64_t n) {
-  __shared__ double tile[kR][kTile + 1];
-  const int64_t d0 = (int64_t)blockIdx.x * kTile;
-  for (int i = threadIdx.x; i < kTile * kR; i += 256) {
-    const int f = i / kTile, dd = i % kTile;
-    if (d0 + dd < n) tile[f][dd] = Y[(int64_t)f * n + d0 + dd];
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < kTile * kR; i += 256) {
-    const int dd = i >> 4, f = i & 15;
-    if (d0 + dd < n) {
-      const int64_t j = (d0 + dd) * kR + f;
-      const double v = b[j] - tile[f][dd];
-      res64[j] = v;
-      res32[j] = (float)v;
-    }
+                                                    int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256) {
+    const double v = b[i] - Y[i];
+    res64[i] = v;
+    res32[i] = (float)v;
   }
 }
 
@@ -205,9 +179,9 @@ extern "C" int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_
   const int64_t n = ut, n16 = n * kR;
   const int nb = lfr::red_blocks(n16);
   // workspace: 4 FP32 multidata fields (d, res, p, z), FP64 residual field,
-  // x and A x of the 16 fields field-major (FP64), partials, scalars
+  // A x of the 16 fields (FP64, multidata), partials, scalars
   const size_t f32 = up(n16 * 4), f64 = up(n16 * 8), s64 = up(n16 * 8);
-  const size_t need = 4 * f32 + f64 + 2 * s64 + up((size_t)nb * kR * 8) + up(128 * 8) + up(2 * kR * 8);
+  const size_t need = 4 * f32 + f64 + s64 + up((size_t)nb * kR * 8) + up(128 * 8) + up(2 * kR * 8);
   int rc = lfr_work_reserve(ctx, need);
   if (rc) return rc;
   char* w = (char*)ctx->work;
@@ -217,13 +191,11 @@ extern "C" int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_
   float* z = (float*)(w + 3 * f32);
   double* r64 = (double*)(w + 4 * f32);
   double* Y16 = (double*)(w + 4 * f32 + f64);
-  double* X16 = (double*)(w + 4 * f32 + f64 + s64);
-  double* part = (double*)(w + 4 * f32 + f64 + 2 * s64);
-  double* S = (double*)(w + 4 * f32 + f64 + 2 * s64 + up((size_t)nb * kR * 8));
+  double* part = (double*)(w + 4 * f32 + f64 + s64);
+  double* S = (double*)(w + 4 * f32 + f64 + s64 + up((size_t)nb * kR * 8));
   double* tmp = S + 128;
   hipStream_t s = ctx->stream;
   const unsigned g = lfr_blocks(n16, 256, 8192);
-  const unsigned gt = (unsigned)((n + kTile - 1) / kTile);
   double* x64 = (double*)x64_d;
   const double* b64 = (const double*)b64_d;
   double* Sh = ctx->host_scalars;   // 64 doubles
@@ -239,15 +211,11 @@ extern "C" int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_
   int outer = 0, inner_total = 0, conv = 0;
   double rel[kR];
   for (;;) {
-    // FP64 residual: fields to field-major, one FP64 apply per field, back
-    hipLaunchKernelGGL(k_split16, dim3(gt), dim3(256), 0, s, X16, (const double*)x64, n);
-    LFR_CHECK_LAUNCH(ctx);
-    for (int r = 0; r < kR; ++r) {
-      rc = lfr_matrix_vector(ctx, 8, Y16 + (size_t)r * n, X16 + (size_t)r * n, A64->matrix_d, A64->space,
-                             A64->space, A64->plan, 1);
-      if (rc) return rc;
-    }
-    hipLaunchKernelGGL(k_residual16, dim3(gt), dim3(256), 0, s, r64, res, b64, (const double*)Y16, n);
+    // FP64 residual: one multidata FP64 apply of all 16 fields (the matrix
+    // is read once; each field as a single-field apply would compute it)
+    rc = lfr_matrix_vector_ndata(ctx, 8, Y16, x64, A64->matrix_d, A64->space, A64->space, A64->plan, kR, 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_residual16, dim3(g), dim3(256), 0, s, r64, res, b64, (const double*)Y16, n16);
     hipLaunchKernelGGL((k_mdot<double, double>), dim3(nb), dim3(256), 0, s, (const double*)r64,
                        (const double*)r64, n16, part);
     hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(256), 0, s, part, nb, tmp);
