@@ -91,28 +91,42 @@ __global__ void k_multi_pack(TO* __restrict__ mp, const TI* __restrict__ m, cons
   }
 }
 
+// One row tile's matrix operand for one jq step: 4 FP32 (16 B) or 4 BF16 (8 B)
 template <bool BF>
-__device__ __forceinline__ lfr_v4f mfma_step(const lfr_v4f (&xa)[4], int v, const void* mv, lfr_v4f acc);
+struct MTile;
+template <>
+struct MTile<false> {
+  lfr_v4f m;
+  __device__ __forceinline__ void load(const void* p) { m = *(const lfr_v4f*)p; }
+};
+template <>
+struct MTile<true> {
+  lfr_v4s m;
+  __device__ __forceinline__ void load(const void* p) { m = *(const lfr_v4s*)p; }
+};
+
+template <bool BF>
+__device__ __forceinline__ lfr_v4f mfma_step(const lfr_v4f (&xa)[4], int v, const MTile<BF>& mt, lfr_v4f acc);
 
 template <>
-__device__ __forceinline__ lfr_v4f mfma_step<false>(const lfr_v4f (&xa)[4], int v, const void* mv, lfr_v4f acc) {
-  const lfr_v4f m = *(const lfr_v4f*)mv;
-  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[0][v], m[0], acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[1][v], m[1], acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[2][v], m[2], acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[3][v], m[3], acc, 0, 0, 0);
+__device__ __forceinline__ lfr_v4f mfma_step<false>(const lfr_v4f (&xa)[4], int v, const MTile<false>& mt,
+                                                    lfr_v4f acc) {
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[0][v], mt.m[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[1][v], mt.m[1], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[2][v], mt.m[2], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(xa[3][v], mt.m[3], acc, 0, 0, 0);
   return acc;
 }
 
 template <>
-__device__ __forceinline__ lfr_v4f mfma_step<true>(const lfr_v4f (&xa)[4], int v, const void* mv, lfr_v4f acc) {
+__device__ __forceinline__ lfr_v4f mfma_step<true>(const lfr_v4f (&xa)[4], int v, const MTile<true>& mt,
+                                                   lfr_v4f acc) {
   lfr_v4s a;
   a[0] = (short)f2bf(xa[0][v]);
   a[1] = (short)f2bf(xa[1][v]);
   a[2] = (short)f2bf(xa[2][v]);
   a[3] = (short)f2bf(xa[3][v]);
-  const lfr_v4s m = *(const lfr_v4s*)mv;
-  return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a, m, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a, mt.m, acc, 0, 0, 0);
 }
 
 // One workgroup per cell column, WS waves; wave w owns row tiles
@@ -156,9 +170,9 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[gi][v] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
 
-#pragma unroll UNR
-    for (int jq = 0; jq < JQ; ++jq) {
-      lfr_v4f xa[4];
+    // operands of step jq: 4 input dofs (4 rhs each) and one matrix tile per
+    // row tile; the loads of step jq+1 are issued before the MFMAs of step jq
+    auto load_step = [&](int jq, lfr_v4f (&xa)[4], MTile<BF> (&tl)[IGW]) {
 #pragma unroll
       for (int jr = 0; jr < 4; ++jr) {
         const int j = 4 * jq + jr;
@@ -173,10 +187,25 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
 #pragma unroll
       for (int gi = 0; gi < IGW; ++gi) {
         if (IG % IGW != 0 && g0 + gi >= IG) break;
-        const char* mv = mp + ((((mrow + 4 * (g0 + gi) + q) * nl) + k) * 4) * ME;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[gi][v] = mfma_step<BF>(xa, v, mv, acc[gi][v]);
+        tl[gi].load(mp + ((((mrow + 4 * (g0 + gi) + q) * nl) + k) * 4) * ME);
       }
+    };
+    lfr_v4f xa[4], xn[4];
+    MTile<BF> ta[IGW], tn[IGW];
+    load_step(0, xa, ta);
+#pragma unroll UNR
+    for (int jq = 0; jq < JQ; ++jq) {
+      if (jq + 1 < JQ) load_step(jq + 1, xn, tn);
+#pragma unroll
+      for (int gi = 0; gi < IGW; ++gi) {
+        if (IG % IGW != 0 && g0 + gi >= IG) break;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[gi][v] = mfma_step<BF>(xa, v, ta[gi], acc[gi][v]);
+      }
+#pragma unroll
+      for (int jr = 0; jr < 4; ++jr) xa[jr] = xn[jr];
+#pragma unroll
+      for (int gi = 0; gi < IGW; ++gi) ta[gi] = tn[gi];
     }
 
     // epilogue: lane (b, q) of tile g holds slot 4g+q of layer k0+b,
