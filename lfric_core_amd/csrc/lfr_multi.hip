@@ -162,17 +162,11 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
 #pragma unroll
     for (int v = 0; v < 4; ++v) carry[g][v] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < nl; k0 += 16) {
-    const int k = min(k0 + b, nl - 1);     // clamped: lanes past the top only load
-    lfr_v4f acc[IGW][4];
-#pragma unroll
-    for (int gi = 0; gi < IGW; ++gi)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[gi][v] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
-
-    // operands of step jq: 4 input dofs (4 rhs each) and one matrix tile per
-    // row tile; the loads of step jq+1 are issued before the MFMAs of step jq
-    auto load_step = [&](int jq, lfr_v4f (&xa)[4], MTile<BF> (&tl)[IGW]) {
+  // operands of step (k, jq): 4 input dofs (4 rhs each) at layer k and one
+  // matrix tile per row tile; the loads of the next step (the next jq, or
+  // the first of the next 16-layer block) are issued before the MFMAs of
+  // the current one
+  auto load_step = [&](int k, int jq, lfr_v4f (&xa)[4], MTile<BF> (&tl)[IGW]) {
 #pragma unroll
       for (int jr = 0; jr < 4; ++jr) {
         const int j = 4 * jq + jr;
@@ -189,13 +183,24 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
         if (IG % IGW != 0 && g0 + gi >= IG) break;
         tl[gi].load(mp + ((((mrow + 4 * (g0 + gi) + q) * nl) + k) * 4) * ME);
       }
-    };
-    lfr_v4f xa[4], xn[4];
-    MTile<BF> ta[IGW], tn[IGW];
-    load_step(0, xa, ta);
+  };
+  lfr_v4f xa[4], xn[4];
+  MTile<BF> ta[IGW], tn[IGW];
+  load_step(min(b, nl - 1), 0, xa, ta);
+
+  for (int k0 = 0; k0 < nl; k0 += 16) {
+    const int k = min(k0 + b, nl - 1);     // clamped: lanes past the top only load
+    const int kn = min(k0 + 16 + b, nl - 1);
+    lfr_v4f acc[IGW][4];
+#pragma unroll
+    for (int gi = 0; gi < IGW; ++gi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[gi][v] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
+
 #pragma unroll UNR
     for (int jq = 0; jq < JQ; ++jq) {
-      if (jq + 1 < JQ) load_step(jq + 1, xn, tn);
+      if (jq + 1 < JQ) load_step(k, jq + 1, xn, tn);
+      else if (k0 + 16 < nl) load_step(kn, 0, xn, tn);
 #pragma unroll
       for (int gi = 0; gi < IGW; ++gi) {
         if (IG % IGW != 0 && g0 + gi >= IG) break;
