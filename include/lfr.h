@@ -322,6 +322,25 @@ int lfr_columnwise_op_appinv(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_
                              const lfr_space* space, const int32_t* indirection_d,
                              int fp64_pivots);
 
+/* invoke(tri_solve_kernel_type(y, x, tri_0, tri_plus, tri_minus)) over every
+ * column (components/science/source/kernel/solver/sci_tri_solve_kernel_mod.f90:95-118):
+ * Thomas solve of tri_minus(k) y(k-1) + tri_0(k) y(k) + tri_plus(k) y(k+1) = x(k)
+ * on the nlayers dofs of each column of a lowest-order W3 space (ndf 1); y is
+ * overwritten and must not alias an input.  Uses undf values of context
+ * workspace. */
+int lfr_tri_solve(lfr_ctx* ctx, int prec, void* y_d, const void* x_d, const void* tri_0_d,
+                  const void* tri_plus_d, const void* tri_minus_d, const lfr_space* space);
+
+/* invoke(operator_tri_solve_kernel_type(y, x, matrix)) over every column
+ * (components/science/source/kernel/solver/sci_operator_tri_solve_kernel_mod.f90:65-120):
+ * the tridiagonal of the nlayers+1 dofs of each column of a lowest-order
+ * Wtheta or W2v space (ndf 2, top dof = bottom dof + 1) assembled from the
+ * 2x2 local matrices local_stencil(ncell_3d, 2, 2), then the Thomas solve;
+ * y is overwritten.  The reference kernel is r_def; prec 4 runs the same
+ * steps in FP32. */
+int lfr_operator_tri_solve(lfr_ctx* ctx, int prec, void* y_d, const void* x_d, const void* matrix_d,
+                           const lfr_space* space);
+
 /* ---------------------------------------------------------------------- */
 /* Multigrid with vertical line relaxation (W3, lowest order)              */
 /* ---------------------------------------------------------------------- */

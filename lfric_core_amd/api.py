@@ -509,6 +509,38 @@ def columnwise_op_appinv(lhs: torch.Tensor, x: torch.Tensor, op: CMAOperator,
     return lhs
 
 
+def tri_solve(y: torch.Tensor, x: torch.Tensor, tri_0: torch.Tensor, tri_plus: torch.Tensor,
+              tri_minus: torch.Tensor, space: Space) -> torch.Tensor:
+    """invoke(tri_solve_kernel_type(y, x, tri_0, tri_plus, tri_minus)): the
+    vertical tridiagonal solve on every column of a lowest-order W3 space;
+    y is overwritten."""
+    dt = x.dtype
+    for name, t in (("y", y), ("x", x), ("tri_0", tri_0), ("tri_plus", tri_plus), ("tri_minus", tri_minus)):
+        _field(t, space, name, dt)
+    ctx = space.ctx
+    ctx.sync_stream()
+    check(lib().lfr_tri_solve(ctx.h, _prec(y), _P(y.data_ptr()), _P(x.data_ptr()), _P(tri_0.data_ptr()),
+                              _P(tri_plus.data_ptr()), _P(tri_minus.data_ptr()), space.h), ctx.h)
+    return y
+
+
+def operator_tri_solve(y: torch.Tensor, x: torch.Tensor, matrix: torch.Tensor, space: Space) -> torch.Tensor:
+    """invoke(operator_tri_solve_kernel_type(y, x, matrix)): the column
+    tridiagonal of a lowest-order Wtheta/W2v space taken from its 2x2 LMA
+    operator (matrix (2, 2, ncell_3d), the local_stencil memory), then the
+    Thomas solve; y is overwritten."""
+    _field(y, space, "y", x.dtype)
+    _field(x, space, "x", x.dtype)
+    n3 = space.fs.mesh.ncells * space.fs.nlayers
+    if tuple(matrix.shape) != (2, 2, n3) or not matrix.is_contiguous() or matrix.dtype != x.dtype:
+        raise LfrError(1, f"matrix must be a contiguous (2, 2, {n3}) {x.dtype} tensor")
+    ctx = space.ctx
+    ctx.sync_stream()
+    check(lib().lfr_operator_tri_solve(ctx.h, _prec(y), _P(y.data_ptr()), _P(x.data_ptr()),
+                                       _P(matrix.data_ptr()), space.h), ctx.h)
+    return y
+
+
 class ColumnwiseGeometry:
     """Banded column-operator geometry between two horizontally discontinuous
     spaces (lfr_cma_create), from lfric_core_amd.cma.cma_geometry."""

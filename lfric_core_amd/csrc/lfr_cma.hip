@@ -421,3 +421,142 @@ int lfr_columnwise_op_app(lfr_ctx* ctx, int prec, const lfr_cma* h, void* lhs_d,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Vertical tridiagonal solves on fields:
+//   tri_solve_code (components/science/source/kernel/solver/sci_tri_solve_kernel_mod.f90:95-118)
+//   operator_tri_solve_code (.../solver/sci_operator_tri_solve_kernel_mod.f90:65-120)
+// One lane per column; the forward sweep keeps the eliminated right-hand side
+// in y and the eliminated super-diagonal in a scratch field, the backward
+// sweep finishes y in place.  Operation order as the reference: one
+// reciprocal of the pivot per row, multiplied in.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_tri_solve(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ t0, const T* __restrict__ tp,
+            const T* __restrict__ tm, T* __restrict__ tpn, const int32_t* __restrict__ map0, int ncell, int ndf,
+            int nl) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncell) return;
+  const int64_t ij = map0[(int64_t)c * ndf];
+  T denom = T(1) / t0[ij];
+  T tpk = tp[ij] * denom;
+  T xk = x[ij] * denom;
+  tpn[ij] = tpk;
+  y[ij] = xk;
+  for (int k = 1; k < nl; ++k) {
+    const T pm = tm[ij + k] * tpk;
+    denom = T(1) / (t0[ij + k] - pm);
+    const T xm = tm[ij + k] * xk;
+    tpk = tp[ij + k] * denom;
+    xk = (x[ij + k] - xm) * denom;
+    tpn[ij + k] = tpk;
+    y[ij + k] = xk;
+  }
+  T yk = xk;   // y(nl-1)
+  for (int k = nl - 2; k >= 0; --k) {
+    const T py = tpn[ij + k] * yk;
+    yk = y[ij + k] - py;
+    y[ij + k] = yk;
+  }
+}
+
+// The column's tridiagonal from the 2x2 local matrices m(ik, i, j) at
+// ik + n3 * (i + 2 j) (0-based): diag(k) = m(k,0,0) + m(k-1,1,1),
+// sub(k) = m(k-1,1,0), super(k) = m(k,0,1), for the nl+1 dofs of the column.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_operator_tri_solve(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ m, T* __restrict__ tpn,
+                     const int32_t* __restrict__ map0, int ncell, int ndf, int nl) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncell) return;
+  const int64_t n3 = (int64_t)ncell * nl;
+  const int64_t ij = map0[(int64_t)c * ndf];
+  const int64_t ik0 = (int64_t)c * nl;
+  // k = 0
+  T d = m[ik0] + T(0);
+  T denom = T(1) / d;
+  T tpk = m[ik0 + 2 * n3] * denom;
+  T xk = x[ij] * denom;
+  tpn[ij] = tpk;
+  y[ij] = xk;
+  for (int k = 1; k <= nl; ++k) {
+    const int64_t ikm = ik0 + k - 1;
+    const T sub = m[ikm + n3];                          // (k-1; 2,1)
+    const T t0 = k < nl ? m[ikm + 1] + m[ikm + 3 * n3]  // (k; 1,1) + (k-1; 2,2)
+                        : m[ikm + 3 * n3];
+    const T sup = k < nl ? m[ikm + 1 + 2 * n3] : T(0);  // (k; 1,2)
+    const T pm = sub * tpk;
+    denom = T(1) / (t0 - pm);
+    const T xm = sub * xk;
+    tpk = sup * denom;
+    xk = (x[ij + k] - xm) * denom;
+    tpn[ij + k] = tpk;
+    y[ij + k] = xk;
+  }
+  T yk = xk;   // y(nl)
+  for (int k = nl - 1; k >= 0; --k) {
+    const T py = tpn[ij + k] * yk;
+    yk = y[ij + k] - py;
+    y[ij + k] = yk;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lfr_tri_solve(lfr_ctx* ctx, int prec, void* y, const void* x, const void* tri_0, const void* tri_plus,
+                  const void* tri_minus, const lfr_space* space) {
+  if (!ctx) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, prec == 4 || prec == 8, "lfr_tri_solve: prec must be 4 or 8, got %d", prec);
+  LFR_REQUIRE(ctx, y && x && tri_0 && tri_plus && tri_minus && space, "lfr_tri_solve: null argument");
+  LFR_REQUIRE(ctx, space->ndf == 1, "lfr_tri_solve: lowest-order W3 only (ndf %d, expected 1)", space->ndf);
+  LFR_REQUIRE(ctx, y != x && y != tri_0 && y != tri_plus && y != tri_minus,
+              "lfr_tri_solve: y must not alias an input");
+  const size_t sb = (size_t)space->undf * prec;
+  int rc = lfr_work_reserve(ctx, sb);
+  if (rc) return rc;
+  const unsigned nb = (unsigned)((space->ncell + 255) / 256);
+  if (prec == 4)
+    hipLaunchKernelGGL(k_tri_solve<float>, dim3(nb), dim3(256), 0, ctx->stream, (float*)y, (const float*)x,
+                       (const float*)tri_0, (const float*)tri_plus, (const float*)tri_minus, (float*)ctx->work,
+                       space->map0_d, space->ncell, space->ndf, space->nlayers);
+  else
+    hipLaunchKernelGGL(k_tri_solve<double>, dim3(nb), dim3(256), 0, ctx->stream, (double*)y, (const double*)x,
+                       (const double*)tri_0, (const double*)tri_plus, (const double*)tri_minus,
+                       (double*)ctx->work, space->map0_d, space->ncell, space->ndf, space->nlayers);
+  LFR_CHECK_LAUNCH(ctx);
+  return LFR_OK;
+}
+
+int lfr_operator_tri_solve(lfr_ctx* ctx, int prec, void* y, const void* x, const void* matrix,
+                           const lfr_space* space) {
+  if (!ctx) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, prec == 4 || prec == 8, "lfr_operator_tri_solve: prec must be 4 or 8, got %d", prec);
+  LFR_REQUIRE(ctx, y && x && matrix && space, "lfr_operator_tri_solve: null argument");
+  LFR_REQUIRE(ctx, y != x, "lfr_operator_tri_solve: y must not alias x");
+  LFR_REQUIRE(ctx, space->ndf == 2,
+              "lfr_operator_tri_solve: lowest-order Wtheta/W2v only (ndf %d, expected 2)", space->ndf);
+  for (int c = 0; c < space->ncell; ++c)
+    LFR_REQUIRE(ctx, space->map_h[(size_t)c * 2 + 1] == space->map_h[(size_t)c * 2] + 1,
+                "lfr_operator_tri_solve: cell %d: the top dof does not follow the bottom dof", c + 1);
+  const size_t sb = (size_t)space->undf * prec;
+  int rc = lfr_work_reserve(ctx, sb);
+  if (rc) return rc;
+  const unsigned nb = (unsigned)((space->ncell + 255) / 256);
+  if (prec == 4)
+    hipLaunchKernelGGL(k_operator_tri_solve<float>, dim3(nb), dim3(256), 0, ctx->stream, (float*)y,
+                       (const float*)x, (const float*)matrix, (float*)ctx->work, space->map0_d, space->ncell,
+                       space->ndf, space->nlayers);
+  else
+    hipLaunchKernelGGL(k_operator_tri_solve<double>, dim3(nb), dim3(256), 0, ctx->stream, (double*)y,
+                       (const double*)x, (const double*)matrix, (double*)ctx->work, space->map0_d, space->ncell,
+                       space->ndf, space->nlayers);
+  LFR_CHECK_LAUNCH(ctx);
+  return LFR_OK;
+}
+
+}  // extern "C"

@@ -241,6 +241,41 @@ def enforce_bc(field, dofmap, boundary, nlayers, use_ref=False):
     return field
 
 
+def tri_solve(y, x, tri_0, tri_plus, tri_minus, dofmap, nlayers):
+    """tri_solve_code over every W3 column (lowest order, ndf = 1); y is
+    overwritten.  Pinned by the closed form (a banded LU solve) only: the
+    reference module cannot be compiled here (fs_continuity_mod -> log_mod ->
+    MPI)."""
+    dm = np.ascontiguousarray(dofmap, dtype=np.int32)
+    ncell, ndf = dm.shape
+    arrs = [np.ascontiguousarray(a) for a in (x, tri_0, tri_plus, tri_minus)]
+    getattr(lib(), f"orc_invoke_tri_solve_{_suf(y.dtype)}")(
+        _I(ncell), _I(nlayers), y.ctypes.data_as(C.c_void_p),
+        *[a.ctypes.data_as(C.c_void_p) for a in arrs], _I(ndf), dm.ctypes.data_as(C.c_void_p))
+    return y
+
+
+def operator_tri_solve(y, x, matrix, dofmap, nlayers, use_ref=False):
+    """operator_tri_solve_code over every column of a 2-dof-per-cell space
+    (Wtheta/W2v lowest order); matrix is local_stencil(ncell_3d, 2, 2) as the
+    numpy array (2, 2, ncell_3d); y is overwritten."""
+    dm = np.ascontiguousarray(dofmap, dtype=np.int32)
+    ncell, ndf = dm.shape
+    m = np.ascontiguousarray(matrix)
+    xx = np.ascontiguousarray(x)
+    if use_ref:
+        if y.dtype != np.float64:
+            raise ValueError("the reference kernel is r_def (float64) only")
+        ref().ref_invoke_operator_tri_solve_r64(
+            _I(ncell), _I(nlayers), y.ctypes.data_as(C.c_void_p), xx.ctypes.data_as(C.c_void_p),
+            m.ctypes.data_as(C.c_void_p), _I(ndf), _I(y.size), dm.ctypes.data_as(C.c_void_p))
+    else:
+        getattr(lib(), f"orc_invoke_operator_tri_solve_{_suf(y.dtype)}")(
+            _I(ncell), _I(nlayers), y.ctypes.data_as(C.c_void_p), xx.ctypes.data_as(C.c_void_p),
+            m.ctypes.data_as(C.c_void_p), _I(ndf), dm.ctypes.data_as(C.c_void_p))
+    return y
+
+
 def dot(x, y, nthreads=None) -> float:
     nt = nthreads_default() if nthreads is None else nthreads
     fn = getattr(lib(), f"orc_dot_{_suf(x.dtype)}")

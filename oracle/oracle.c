@@ -220,6 +220,80 @@ DEF_CMAASM(double, r64)
 DEF_ENFORCE_BC(float, r32)
 DEF_ENFORCE_BC(double, r64)
 
+/* tri_solve_code (components/science/source/kernel/solver/
+ * sci_tri_solve_kernel_mod.f90:95-118) over all columns in cell order: Thomas
+ * solve of tri_minus y(k-1) + tri_0 y(k) + tri_plus y(k+1) = x on the nl dofs
+ * of one W3 column (map(1) = first dof); one reciprocal per row, multiplied
+ * into the row; y must not alias an input. */
+#define DEF_TRI_SOLVE(T, SUF)                                                           \
+  void orc_invoke_tri_solve_##SUF(int ncell, int nl, T* y, const T* x, const T* t0,    \
+                                  const T* tp, const T* tm, int ndf, const int* dofmap) { \
+    T tpn[4096], xn[4096];                                                             \
+    for (int c = 0; c < ncell; ++c) {                                                   \
+      const int64_t ij = (int64_t)dofmap[(int64_t)c * ndf] - 1;                         \
+      T denom = (T)1 / t0[ij];                                                          \
+      tpn[0] = tp[ij] * denom;                                                          \
+      xn[0] = x[ij] * denom;                                                            \
+      for (int k = 1; k < nl; ++k) {                                                    \
+        const T pm = tm[ij + k] * tpn[k - 1];                                           \
+        denom = (T)1 / (t0[ij + k] - pm);                                               \
+        tpn[k] = tp[ij + k] * denom;                                                    \
+        const T xm = tm[ij + k] * xn[k - 1];                                            \
+        xn[k] = (x[ij + k] - xm) * denom;                                               \
+      }                                                                                 \
+      y[ij + nl - 1] = xn[nl - 1];                                                      \
+      for (int k = nl - 2; k >= 0; --k) {                                               \
+        const T py = tpn[k] * y[ij + k + 1];                                            \
+        y[ij + k] = xn[k] - py;                                                         \
+      }                                                                                 \
+    }                                                                                   \
+  }
+DEF_TRI_SOLVE(float, r32)
+DEF_TRI_SOLVE(double, r64)
+
+/* operator_tri_solve_code (components/science/source/kernel/solver/
+ * sci_operator_tri_solve_kernel_mod.f90:65-120) over all columns in cell
+ * order: the tridiagonal of the nl+1 dofs of one Wtheta/W2v column taken
+ * from the 2x2 local matrices matrix(ncell_3d, 2, 2) (the diagonal of a
+ * shared dof is matrix(ik,1,1) + matrix(ik-1,2,2)), then the same Thomas
+ * solve. */
+#define DEF_OP_TRI_SOLVE(T, SUF)                                                        \
+  void orc_invoke_operator_tri_solve_##SUF(int ncell, int nl, T* y, const T* x,        \
+                                           const T* m, int ndf, const int* dofmap) {   \
+    T t0[4097], tp[4097], tm[4097], tpn[4097], xn[4097];                                \
+    const int64_t n3 = (int64_t)ncell * nl;                                             \
+    for (int c = 0; c < ncell; ++c) {                                                   \
+      const int64_t ij = (int64_t)dofmap[(int64_t)c * ndf] - 1;                         \
+      t0[0] = (T)0;                                                                     \
+      tm[0] = (T)0;                                                                     \
+      for (int k = 0; k < nl; ++k) {                                                    \
+        const int64_t ik = (int64_t)c * nl + k;                                         \
+        t0[k] = m[ik] + t0[k];                /* (ik,1,1) */                             \
+        tp[k] = m[ik + n3 * 2];               /* (ik,1,2) */                             \
+        tm[k + 1] = m[ik + n3 * 1];           /* (ik,2,1) */                             \
+        t0[k + 1] = m[ik + n3 * 3];           /* (ik,2,2) */                             \
+      }                                                                                 \
+      tp[nl] = (T)0;                                                                    \
+      T denom = (T)1 / t0[0];                                                           \
+      tpn[0] = tp[0] * denom;                                                           \
+      xn[0] = x[ij] * denom;                                                            \
+      for (int k = 1; k <= nl; ++k) {                                                   \
+        const T pm = tm[k] * tpn[k - 1];                                                \
+        denom = (T)1 / (t0[k] - pm);                                                    \
+        tpn[k] = tp[k] * denom;                                                         \
+        const T xm = tm[k] * xn[k - 1];                                                 \
+        xn[k] = (x[ij + k] - xm) * denom;                                               \
+      }                                                                                 \
+      y[ij + nl] = xn[nl];                                                              \
+      for (int k = nl - 1; k >= 0; --k) {                                               \
+        const T py = tpn[k] * y[ij + k + 1];                                            \
+        y[ij + k] = xn[k] - py;                                                         \
+      }                                                                                 \
+    }                                                                                   \
+  }
+DEF_OP_TRI_SOLVE(float, r32)
+DEF_OP_TRI_SOLVE(double, r64)
+
 /* ------------------------------------------------------------------------ */
 /* Invoke-level loops (what the generated PSy layer does around a kernel)   */
 /* ------------------------------------------------------------------------ */

@@ -167,3 +167,19 @@ void ref_invoke_enforce_bc_r64(int ncell, int nlayers, double* field, int ndf, i
     FK(sci_enforce_bc_kernel_mod, enforce_bc_code_r_double)(&nlayers, field, &ndf, &undf,
                                                             dofmap + (int64_t)c * ndf, bv);
 }
+
+/* operator_tri_solve_code (r_def = real64) per column in cell order; its
+ * `ncell` argument is the 3D cell count (the first extent of matrix). */
+extern void FK(sci_operator_tri_solve_kernel_mod, operator_tri_solve_code)(
+    const int* cell, const int* nlayers, double* y, const double* x, const int* ncell,
+    const double* matrix, const int* ndf, const int* undf, const int* map);
+
+void ref_invoke_operator_tri_solve_r64(int ncell, int nlayers, double* y, const double* x,
+                                       const double* matrix, int ndf, int undf, const int* dofmap) {
+  int n3 = ncell * nlayers;
+  for (int c = 0; c < ncell; ++c) {
+    int cell = c + 1;
+    FK(sci_operator_tri_solve_kernel_mod, operator_tri_solve_code)(&cell, &nlayers, y, x, &n3, matrix, &ndf,
+                                                                 &undf, dofmap + (int64_t)c * ndf);
+  }
+}
