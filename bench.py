@@ -273,6 +273,32 @@ def bench_multi(ctx, torch, edge_cells: int = 48, nl: int = 70, steps: int = 20)
     return out
 
 
+def bench_assembly(ctx, torch, edge_cells: int = 96, nl: int = 70, steps: int = 5):
+    """Device assembly of lowest-order W0 local matrices (mass, and mass +
+    stiffness) by 3-point Gauss quadrature at C96L70, from synthetic Cartesian
+    coordinate fields (the cost does not depend on the values)."""
+    from lfric_core_amd import api
+    from lfric_core_amd.function_space import build_dofmap
+    from lfric_core_amd.mesh import cubed_sphere
+    from lfric_core_amd.rng import TorchStream
+    fs = build_dofmap(cubed_sphere(edge_cells), "W0", nl)
+    sp = api.Space(ctx, fs)
+    chi = (1.0 + TorchStream(5, ctx.device).uniform(3 * fs.undf)).reshape(3, fs.undf).contiguous()
+    n3 = fs.mesh.ncells * nl
+    api.compute_mass_matrix_w0(chi, sp)
+    ms_m = time_events(lambda: api.compute_mass_matrix_w0(chi, sp), steps, torch)
+    ms_ms = time_events(lambda: api.compute_mass_matrix_w0(chi, sp, stiffness=True), steps, torch)
+    wb = n3 * 64 * 8
+    out = {"workload": f"W0 lowest order, C{edge_cells}L{nl}, 27 quadrature points, FP64",
+           "ncell_3d": n3, "mass_ms": round(ms_m, 3), "mass_and_stiffness_ms": round(ms_ms, 3),
+           "mass_write_gbs": round(wb / (ms_m * 1e-3) / 1e9, 1)}
+    log(f"W0 assembly C{edge_cells}L{nl}: mass {ms_m:.2f} ms, mass + stiffness {ms_ms:.2f} ms "
+        f"({n3} cell-layers)")
+    del chi
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_multigrid(ctx, torch, edge_cells=(192, 96, 48, 24), nl: int = 70, nrhs: int = 8,
                     cycles: int = 10):
     """Batched V-cycles (2 pre / 2 post line-Jacobi sweeps, 8 coarse sweeps) on the
@@ -324,6 +350,7 @@ def main():
     ap.add_argument("--no-solvers", action="store_true")
     ap.add_argument("--no-mg", action="store_true", help="skip the batched multigrid V-cycles")
     ap.add_argument("--no-multi", action="store_true", help="skip the 16-field MFMA apply")
+    ap.add_argument("--no-assembly", action="store_true", help="skip the device W0 assembly")
     ap.add_argument("--packed-only", action="store_true",
                     help="time only the packed applies (profiling runs: the reference-layout "
                          "kernels share the packed kernels' names)")
@@ -415,6 +442,8 @@ def main():
     if not args.no_mg:
         extra["multigrid"] = bench_multigrid(ctx, torch, (48, 24, 12, 6) if args.quick
                                              else (192, 96, 48, 24))
+    if not args.no_assembly:
+        extra["assembly_w0"] = bench_assembly(ctx, torch, 24 if args.quick else 96)
 
     # achievable-bandwidth reference points on this GPU (PyTorch's own kernels)
     src = torch.ones(1 << 28, dtype=torch.float32, device=dev)

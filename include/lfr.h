@@ -154,6 +154,19 @@ int lfr_matrix_vector_ndata(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_d
                             const lfr_space* lhs_space, const lfr_space* x_space, const lfr_plan* plan,
                             int ndata, int overwrite);
 
+/* compute_mass_matrix_w_scalar_code for w_scalar = chi space = lowest-order
+ * W0 (components/science/source/kernel/fem/sci_compute_mass_matrix_kernel_w_scalar_mod.F90:150-174)
+ * over every cell-layer, with Cartesian coordinates chi1/chi2/chi3 (FP64 W0
+ * fields on `space`) and the coordinate Jacobian of
+ * components/science/source/kernel/geometry/sci_coordinate_jacobian_mod.F90:330-345,430-440:
+ * nq-point Gauss-Legendre quadrature per direction (1 <= nq <= 6).  mass_d
+ * receives local_stencil(ncell_3d, 8, 8) in FP64.  stiff_d (may be NULL)
+ * receives the stiffness matrix sum_q w (J^-T grad phi_a).(J^-T grad phi_b)
+ * |detJ| (no reference kernel; the diffusion term of the assembled
+ * operator). */
+int lfr_compute_mass_matrix_w0(lfr_ctx* ctx, int nq, const void* chi1_d, const void* chi2_d, const void* chi3_d,
+                               const lfr_space* space, void* mass_d, void* stiff_d);
+
 /* Packed operator for repeated applies of one LMA operator (the
  * operator_type whose local_stencil, INF/operator/operator_mod.t90:31-122,
  * is assembled once and applied many times, as the mass matrices in the

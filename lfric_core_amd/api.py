@@ -541,6 +541,25 @@ def operator_tri_solve(y: torch.Tensor, x: torch.Tensor, matrix: torch.Tensor, s
     return y
 
 
+def compute_mass_matrix_w0(chi: torch.Tensor, space: Space, nq: int = 3, stiffness: bool = False):
+    """compute_mass_matrix_w_scalar for lowest-order W0 on the device: chi is
+    the (3, undf) FP64 coordinate field (chi1, chi2, chi3) on ``space``;
+    returns the local_stencil (8, 8, ncell_3d) mass matrix (and the stiffness
+    matrix with ``stiffness=True``)."""
+    if chi.dim() != 2 or chi.shape[0] != 3 or chi.shape[1] != space.undf or chi.dtype != torch.float64 \
+            or not chi.is_contiguous() or chi.device != space.ctx.device:
+        raise LfrError(1, f"chi must be a contiguous (3, {space.undf}) float64 tensor on {space.ctx.device}")
+    n3 = space.fs.mesh.ncells * space.fs.nlayers
+    mass = torch.empty((8, 8, n3), dtype=torch.float64, device=chi.device)
+    stiff = torch.empty_like(mass) if stiffness else None
+    ctx = space.ctx
+    ctx.sync_stream()
+    check(lib().lfr_compute_mass_matrix_w0(ctx.h, int(nq), _P(chi[0].data_ptr()), _P(chi[1].data_ptr()),
+                                           _P(chi[2].data_ptr()), space.h, _P(mass.data_ptr()),
+                                           _P(stiff.data_ptr()) if stiff is not None else None), ctx.h)
+    return (mass, stiff) if stiffness else mass
+
+
 class ColumnwiseGeometry:
     """Banded column-operator geometry between two horizontally discontinuous
     spaces (lfr_cma_create), from lfric_core_amd.cma.cma_geometry."""

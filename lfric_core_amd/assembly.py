@@ -115,3 +115,18 @@ def to_local_stencil(A):
     """(ncell, nlayers, ndf_to, ndf_from) -> local_stencil layout (ndf_from, ndf_to, ncell_3d)."""
     nc, nl, nt, nf = A.shape
     return np.ascontiguousarray(A.reshape(nc * nl, nt, nf).transpose(2, 1, 0))
+
+
+def w0_chi_fields(fs, chi):
+    """The coordinate field as LFRic holds it: chi1, chi2, chi3 on the
+    lowest-order W0 space ``fs`` ((3, undf) FP64), from the cell-layer
+    corners chi (ncell, nlayers, 8, 3) in the W0 local-dof order (corner j of
+    cell c at layer k is dof map(j, c) + k; shared vertices receive the same
+    values from every cell)."""
+    nc, nl = chi.shape[0], chi.shape[1]
+    out = np.zeros((3, fs.undf))
+    dm = fs.dofmap.astype(np.int64) - 1                   # (ncell, 8)
+    d = dm[:, None, :] + np.arange(nl)[None, :, None]     # (ncell, nl, 8)
+    for i in range(3):
+        out[i, d.ravel()] = chi[..., i].ravel()
+    return out

@@ -76,3 +76,21 @@ def test_shell_operator_invariants_and_volume():
     r = 6.371e6 + cfg.heights
     vol = area[:, None] * ((r[:-1] ** 2 + r[:-1] * r[1:] + r[1:] ** 2) / 3 * np.diff(r))[None, :]
     np.testing.assert_allclose(M.sum(axis=(0, 1)), vol.ravel(), rtol=1e-11)
+
+
+def test_w0_chi_fields_reproduce_the_corners():
+    """chi1/chi2/chi3 on W0 read back through the dofmap give every cell-layer
+    corner (shared vertices are consistent)."""
+    from lfric_core_amd.assembly import shell_corners, w0_chi_fields
+    from lfric_core_amd.function_space import build_dofmap
+    from lfric_core_amd.mesh import cubed_sphere
+    g = np.load(GOLDEN / "meshes.npz")
+    nl = 4
+    fs = build_dofmap(cubed_sphere(12), "W0", nl)
+    heights = np.linspace(0.0, 1.0e4, nl + 1)
+    chi = shell_corners(g["C12_node_x"], g["C12_node_y"], g["C12_face_nodes"], 6.371e6, heights)
+    f = w0_chi_fields(fs, chi)
+    dm = fs.dofmap.astype(np.int64) - 1
+    for k in range(nl):
+        for j in range(8):
+            np.testing.assert_array_equal(f[:, dm[:, j] + k].T, chi[:, k, j, :])
