@@ -167,6 +167,25 @@ int lfr_matrix_vector_ndata(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_d
 int lfr_compute_mass_matrix_w0(lfr_ctx* ctx, int nq, const void* chi1_d, const void* chi2_d, const void* chi3_d,
                                const lfr_space* space, void* mass_d, void* stiff_d);
 
+/* Mass matrices of lowest-order spaces from basis tables the caller
+ * evaluates at the quadrature points (as the PSy layer passes them):
+ *   kind 0, scalar space (W0 ndf 8, Wtheta / W2-vertical-scalar ndf 2, W3 ndf 1):
+ *     compute_mass_matrix_w_scalar_code
+ *     (components/science/source/kernel/fem/sci_compute_mass_matrix_kernel_w_scalar_mod.F90:150-174)
+ *   kind 1, vector space with the Piola map (W2 ndf 6, W2H ndf 4, W2V ndf 2):
+ *     compute_mass_matrix_w2_code
+ *     (components/science/source/kernel/fem/sci_compute_mass_matrix_kernel_w2_mod.F90:131-162)
+ * Host tables in Fortran order: basis(ncomp, ndf, nqp_h, nqp_v) with ncomp 1
+ * (kind 0) or 3 (kind 1), diff_basis_chi(3, 8, nqp_h, nqp_v) of the
+ * lowest-order W0 chi space, wqp_h(nqp_h), wqp_v(nqp_v).  chi1/2/3_d are FP64
+ * fields on chi_space (lowest-order W0, Cartesian coordinates); mass_d
+ * receives local_stencil(ncell_3d, ndf, ndf) in FP64.  Synchronous (the
+ * tables are read from host memory during the call). */
+int lfr_compute_mass_matrix(lfr_ctx* ctx, int kind, int ndf, int nqp_h, int nqp_v, const double* basis,
+                            const double* diff_basis_chi, const double* wqp_h, const double* wqp_v,
+                            const void* chi1_d, const void* chi2_d, const void* chi3_d,
+                            const lfr_space* chi_space, void* mass_d);
+
 /* Packed operator for repeated applies of one LMA operator (the
  * operator_type whose local_stencil, INF/operator/operator_mod.t90:31-122,
  * is assembled once and applied many times, as the mass matrices in the

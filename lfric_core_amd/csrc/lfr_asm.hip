@@ -207,3 +207,167 @@ extern "C" int lfr_compute_mass_matrix_w0(lfr_ctx* ctx, int nq, const void* chi1
   LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LFR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Mass matrices from basis tables given by the caller (as the PSy layer hands
+// them to the reference kernels): scalar spaces
+// (compute_mass_matrix_w_scalar_code, sci_compute_mass_matrix_kernel_w_scalar_mod.F90:150-174)
+//   mm(ik, df, df2) = sum (((w_h w_v) phi_df) phi_df2) detJ
+// and W2 with the Piola map (compute_mass_matrix_w2_code,
+// sci_compute_mass_matrix_kernel_w2_mod.F90:131-162)
+//   mm(ik, df, df2) = sum ((w_h w_v) (J v_df).(J v_df2)) / detJ,
+// df >= df2 computed and mirrored, qp_v outer and qp_h inner.  The chi
+// space is lowest-order W0 (its reference gradients in the table).
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int NDF, int NC>
+struct MassQp {
+  double b[NC][NDF];   // basis(c, df) at the point
+  double g[3][8];      // chi reference gradients
+  double wh, wv;
+};
+
+template <int NDF, int NC>
+__global__ void __launch_bounds__(256)
+k_mass_assemble(double* __restrict__ mass, const double* __restrict__ chi1, const double* __restrict__ chi2,
+                const double* __restrict__ chi3, const int32_t* __restrict__ map0,
+                const MassQp<NDF, NC>* __restrict__ tab, int nqp, int ncell, int nl) {
+  const int64_t n3 = (int64_t)ncell * nl;
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= NDF * n3) return;
+  const int a = (int)(t / n3);
+  const int64_t ik = t - (int64_t)a * n3;
+  const int c = (int)(ik / nl);
+  const int k = (int)(ik - (int64_t)c * nl);
+  double x1[8], x2[8], x3[8];
+#pragma unroll
+  for (int df = 0; df < 8; ++df) {
+    const int64_t d = (int64_t)map0[(int64_t)c * 8 + df] + k;
+    x1[df] = chi1[d];
+    x2[df] = chi2[d];
+    x3[df] = chi3[d];
+  }
+  double m[NDF];
+#pragma unroll
+  for (int b = 0; b < NDF; ++b) m[b] = 0.0;
+  for (int q = 0; q < nqp; ++q) {
+    const MassQp<NDF, NC>& T = tab[q];
+    double j[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int dir = 0; dir < 3; ++dir) j[i][dir] = 0.0;
+#pragma unroll
+    for (int df = 0; df < 8; ++df)
+#pragma unroll
+      for (int dir = 0; dir < 3; ++dir) {
+        const double gd = T.g[dir][df];
+        j[0][dir] = j[0][dir] + x1[df] * gd;
+        j[1][dir] = j[1][dir] + x2[df] * gd;
+        j[2][dir] = j[2][dir] + x3[df] * gd;
+      }
+    const double dj = j[0][0] * (j[1][1] * j[2][2] - j[1][2] * j[2][1]) -
+                      j[0][1] * (j[1][0] * j[2][2] - j[1][2] * j[2][0]) +
+                      j[0][2] * (j[1][0] * j[2][1] - j[1][1] * j[2][0]);
+    const double wq = T.wh * T.wv;
+    if (NC == 1) {
+#pragma unroll
+      for (int b = 0; b < NDF; ++b) {
+        const int df = a > b ? a : b, df2 = a > b ? b : a;
+        const double v = ((wq * T.b[0][df]) * T.b[0][df2]) * dj;
+        m[b] = m[b] + v;
+      }
+    } else {
+      // J v for every local dof, components in matmul order
+      double jv[NDF][3];
+#pragma unroll
+      for (int df = 0; df < NDF; ++df)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double s = 0.0;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) s = s + j[i][d] * T.b[d][df];
+          jv[df][i] = s;
+        }
+#pragma unroll
+      for (int b = 0; b < NDF; ++b) {
+        const int df = a > b ? a : b, df2 = a > b ? b : a;
+        double dot = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) dot = dot + jv[df][i] * jv[df2][i];
+        const double v = (wq * dot) / dj;
+        m[b] = m[b] + v;
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NDF; ++b) mass[ik + n3 * (a + NDF * b)] = m[b];
+}
+
+template <int NDF, int NC>
+int mass_assemble(lfr_ctx* ctx, int nqp_h, int nqp_v, const double* basis, const double* dchi, const double* wh,
+                  const double* wv, const void* chi1, const void* chi2, const void* chi3, const lfr_space* sp,
+                  void* mass) {
+  std::vector<MassQp<NDF, NC>> tab;
+  for (int qv = 0; qv < nqp_v; ++qv)
+    for (int qh = 0; qh < nqp_h; ++qh) {
+      MassQp<NDF, NC> T;
+      for (int c = 0; c < NC; ++c)
+        for (int df = 0; df < NDF; ++df) T.b[c][df] = basis[c + NC * (df + NDF * (qh + (size_t)nqp_h * qv))];
+      for (int dir = 0; dir < 3; ++dir)
+        for (int df = 0; df < 8; ++df) T.g[dir][df] = dchi[dir + 3 * (df + 8 * (qh + (size_t)nqp_h * qv))];
+      T.wh = wh[qh];
+      T.wv = wv[qv];
+      tab.push_back(T);
+    }
+  const size_t tb = tab.size() * sizeof(MassQp<NDF, NC>);
+  int rc = lfr_work_reserve(ctx, tb);
+  if (rc) return rc;
+  LFR_HIP(ctx, hipMemcpyAsync(ctx->work, tab.data(), tb, hipMemcpyHostToDevice, ctx->stream));
+  const int64_t n = (int64_t)NDF * sp->ncell * sp->nlayers;
+  hipLaunchKernelGGL((k_mass_assemble<NDF, NC>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (double*)mass, (const double*)chi1, (const double*)chi2, (const double*)chi3, sp->map0_d,
+                     (const MassQp<NDF, NC>*)ctx->work, (int)tab.size(), sp->ncell, sp->nlayers);
+  LFR_CHECK_LAUNCH(ctx);
+  LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));   // the table is this call's host memory
+  return LFR_OK;
+}
+
+}  // namespace
+
+extern "C" int lfr_compute_mass_matrix(lfr_ctx* ctx, int kind, int ndf, int nqp_h, int nqp_v, const double* basis,
+                                       const double* diff_basis_chi, const double* wqp_h, const double* wqp_v,
+                                       const void* chi1_d, const void* chi2_d, const void* chi3_d,
+                                       const lfr_space* chi_space, void* mass_d) {
+  if (!ctx) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, basis && diff_basis_chi && wqp_h && wqp_v && chi1_d && chi2_d && chi3_d && chi_space && mass_d,
+              "lfr_compute_mass_matrix: null argument");
+  LFR_REQUIRE(ctx, kind == 0 || kind == 1, "lfr_compute_mass_matrix: kind must be 0 (scalar) or 1 (W2), got %d",
+              kind);
+  LFR_REQUIRE(ctx, nqp_h >= 1 && nqp_v >= 1 && nqp_h * nqp_v <= 4096, "lfr_compute_mass_matrix: bad point counts");
+  LFR_REQUIRE(ctx, chi_space->ndf == 8, "lfr_compute_mass_matrix: the chi space must be lowest-order W0");
+  if (kind == 0) {
+    switch (ndf) {
+      case 1: return mass_assemble<1, 1>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
+                                         chi3_d, chi_space, mass_d);
+      case 2: return mass_assemble<2, 1>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
+                                         chi3_d, chi_space, mass_d);
+      case 8: return mass_assemble<8, 1>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
+                                         chi3_d, chi_space, mass_d);
+      default: break;
+    }
+  } else {
+    switch (ndf) {
+      case 2: return mass_assemble<2, 3>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
+                                         chi3_d, chi_space, mass_d);
+      case 4: return mass_assemble<4, 3>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
+                                         chi3_d, chi_space, mass_d);
+      case 6: return mass_assemble<6, 3>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
+                                         chi3_d, chi_space, mass_d);
+      default: break;
+    }
+  }
+  return lfr_set_error(ctx, LFR_ERR_ARG, "lfr_compute_mass_matrix: no kernel for kind %d with %d dofs per cell",
+                       kind, ndf);
+}
