@@ -274,9 +274,10 @@ def bench_multi(ctx, torch, edge_cells: int = 48, nl: int = 70, steps: int = 20)
 
 
 def bench_assembly(ctx, torch, edge_cells: int = 96, nl: int = 70, steps: int = 5):
-    """Device assembly of lowest-order W0 local matrices (mass, and mass +
-    stiffness) by 3-point Gauss quadrature at C96L70, from synthetic Cartesian
-    coordinate fields (the cost does not depend on the values)."""
+    """Device assembly by 3-point Gauss quadrature at C96L70, from synthetic
+    Cartesian coordinate fields (the cost does not depend on the values): the
+    lowest-order W0 mass (and mass + stiffness), the table-driven W2 Piola
+    mass and all ten de Rham outputs."""
     from lfric_core_amd import api
     from lfric_core_amd.function_space import build_dofmap
     from lfric_core_amd.mesh import cubed_sphere
@@ -296,12 +297,21 @@ def bench_assembly(ctx, torch, edge_cells: int = 96, nl: int = 70, steps: int = 
     wq = np.full(9, 1.0 / 9.0), np.full(3, 1.0 / 3.0)
     api.compute_mass_matrix(1, b2, dchi, *wq, chi, sp)
     ms_w2 = time_events(lambda: api.compute_mass_matrix(1, b2, dchi, *wq, chi, sp), steps, torch)
+    # all ten de Rham outputs (synthetic tables of the lowest-order shapes)
+    tabs = {"wqp_h": wq[0], "wqp_v": wq[1], "diff_basis_chi": dchi}
+    for n, nd in (("w0", 8), ("w1", 12), ("w2", 6), ("w2b", 6), ("w3", 1), ("wt", 2)):
+        tabs[f"basis_{n}"] = st.uniform(size=(1 if n in ("w0", "w3", "wt") else 3, nd, 9, 3))
+        tabs[f"diff_basis_{n}"] = st.uniform(size=(1 if n in ("w2", "w2b") else 3, nd, 9, 3))
+    dr = api.compute_derham_matrices(tabs, chi, sp)
+    del dr
+    ms_dr = time_events(lambda: api.compute_derham_matrices(tabs, chi, sp), 2, torch)
     wb = n3 * 64 * 8
     out = {"workload": f"W0 lowest order, C{edge_cells}L{nl}, 27 quadrature points, FP64",
            "ncell_3d": n3, "mass_ms": round(ms_m, 3), "mass_and_stiffness_ms": round(ms_ms, 3),
-           "mass_write_gbs": round(wb / (ms_m * 1e-3) / 1e9, 1), "w2_piola_mass_ms": round(ms_w2, 3)}
+           "mass_write_gbs": round(wb / (ms_m * 1e-3) / 1e9, 1), "w2_piola_mass_ms": round(ms_w2, 3),
+           "derham_all_ten_ms": round(ms_dr, 3)}
     log(f"W0 assembly C{edge_cells}L{nl}: mass {ms_m:.2f} ms, mass + stiffness {ms_ms:.2f} ms, "
-        f"W2 Piola mass {ms_w2:.2f} ms ({n3} cell-layers)")
+        f"W2 Piola mass {ms_w2:.2f} ms, de Rham (10 outputs) {ms_dr:.2f} ms ({n3} cell-layers)")
     del chi
     torch.cuda.empty_cache()
     return out
@@ -358,7 +368,7 @@ def main():
     ap.add_argument("--no-solvers", action="store_true")
     ap.add_argument("--no-mg", action="store_true", help="skip the batched multigrid V-cycles")
     ap.add_argument("--no-multi", action="store_true", help="skip the 16-field MFMA apply")
-    ap.add_argument("--no-assembly", action="store_true", help="skip the device W0 assembly")
+    ap.add_argument("--no-assembly", action="store_true", help="skip the device assembly timings")
     ap.add_argument("--packed-only", action="store_true",
                     help="time only the packed applies (profiling runs: the reference-layout "
                          "kernels share the packed kernels' names)")
@@ -451,7 +461,7 @@ def main():
         extra["multigrid"] = bench_multigrid(ctx, torch, (48, 24, 12, 6) if args.quick
                                              else (192, 96, 48, 24))
     if not args.no_assembly:
-        extra["assembly_w0"] = bench_assembly(ctx, torch, 24 if args.quick else 96)
+        extra["assembly"] = bench_assembly(ctx, torch, 24 if args.quick else 96)
 
     # achievable-bandwidth reference points on this GPU (PyTorch's own kernels)
     src = torch.ones(1 << 28, dtype=torch.float32, device=dev)

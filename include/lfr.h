@@ -186,6 +186,40 @@ int lfr_compute_mass_matrix(lfr_ctx* ctx, int kind, int ndf, int nqp_h, int nqp_
                             const void* chi1_d, const void* chi2_d, const void* chi3_d,
                             const lfr_space* chi_space, void* mass_d);
 
+/* Basis tables of the lowest-order de Rham complex at the quadrature points,
+ * host FP64, Fortran order (ncomp, ndf, nqp_h, nqp_v), the func_type
+ * arguments of compute_derham_matrices_code.  A table may be NULL when no
+ * requested output needs it. */
+typedef struct lfr_derham_tables {
+  int nqp_h, nqp_v;
+  const double* wqp_h;           /* (nqp_h) */
+  const double* wqp_v;           /* (nqp_v) */
+  const double* diff_basis_chi;  /* (3, 8, ...) lowest-order W0 gradients */
+  const double* basis_w0;        /* (1, 8, ...) */
+  const double* diff_basis_w0;   /* (3, 8, ...) gradient */
+  const double* basis_w1;        /* (3, 12, ...) */
+  const double* diff_basis_w1;   /* (3, 12, ...) curl */
+  const double* basis_w2;        /* (3, 6, ...) */
+  const double* diff_basis_w2;   /* (1, 6, ...) divergence */
+  const double* basis_w2b;       /* (3, 6, ...) W2broken */
+  const double* diff_basis_w2b;  /* (1, 6, ...) */
+  const double* basis_w3;        /* (1, 1, ...) */
+  const double* basis_wt;        /* (1, 2, ...) Wtheta */
+} lfr_derham_tables;
+
+/* compute_derham_matrices_code
+ * (components/science/source/kernel/fem/sci_compute_derham_matrices_kernel_mod.F90:134-387)
+ * over every cell-layer, lowest order, Cartesian chi (FP64 fields on the
+ * lowest-order W0 chi_space): the mass matrices of W0, W1 (covariant map),
+ * W2 and W2broken (Piola), W3 and Wtheta, the weak gradient (W1 x W0), curl
+ * (W2 x W1), divergence (W3 x W2) and broken divergence (W3 x W2broken)
+ * matrices, each local_stencil(ncell_3d, ndf_to, ndf_from) in FP64.  Any
+ * output may be NULL (not computed).  Synchronous. */
+int lfr_compute_derham_matrices(lfr_ctx* ctx, const lfr_derham_tables* tables, const void* chi1_d,
+                                const void* chi2_d, const void* chi3_d, const lfr_space* chi_space, void* mm0,
+                                void* mm1, void* mm2, void* mm2b, void* mm3, void* mmt, void* grad, void* curl,
+                                void* div, void* broken_div);
+
 /* Packed operator for repeated applies of one LMA operator (the
  * operator_type whose local_stencil, INF/operator/operator_mod.t90:31-122,
  * is assembled once and applied many times, as the mass matrices in the

@@ -53,6 +53,7 @@ SIGNATURES = {
     "lfr_matrix_vector_ndata": [_P, _I, _P, _P, _P, _P, _P, _P, _I, _I],
     "lfr_compute_mass_matrix_w0": [_P, _I, _P, _P, _P, _P, _P, _P],
     "lfr_compute_mass_matrix": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lfr_compute_derham_matrices": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "lfr_tri_solve": [_P, _I, _P, _P, _P, _P, _P, _P],
     "lfr_operator_tri_solve": [_P, _I, _P, _P, _P, _P],
     "lfr_lma_pack": [_P, _I, _P, _P, _P, C.POINTER(_P)],

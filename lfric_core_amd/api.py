@@ -595,6 +595,67 @@ def compute_mass_matrix(kind: int, basis, diff_basis_chi, wqp_h, wqp_v, chi: tor
     return mass
 
 
+_DERHAM_TABLES = ("diff_basis_chi", "basis_w0", "diff_basis_w0", "basis_w1", "diff_basis_w1", "basis_w2",
+                  "diff_basis_w2", "basis_w2b", "diff_basis_w2b", "basis_w3", "basis_wt")
+# (output, ndf_to, ndf_from, tables it needs)
+DERHAM_OUTPUTS = (("mm0", 8, 8, ("basis_w0",)), ("mm1", 12, 12, ("basis_w1",)), ("mm2", 6, 6, ("basis_w2",)),
+                  ("mm2b", 6, 6, ("basis_w2b",)), ("mm3", 1, 1, ("basis_w3",)), ("mmt", 2, 2, ("basis_wt",)),
+                  ("grad", 12, 8, ("basis_w1", "diff_basis_w0")), ("curl", 6, 12, ("basis_w2", "diff_basis_w1")),
+                  ("div", 1, 6, ("basis_w3", "diff_basis_w2")),
+                  ("broken_div", 1, 6, ("basis_w3", "diff_basis_w2b")))
+
+
+class _DerhamTables(C.Structure):
+    _fields_ = [("nqp_h", C.c_int), ("nqp_v", C.c_int), ("wqp_h", _P), ("wqp_v", _P)] + \
+               [(n, _P) for n in _DERHAM_TABLES]
+
+
+def compute_derham_matrices(tables: dict, chi: torch.Tensor, chi_space: Space, outputs=None):
+    """compute_derham_matrices on the device, lowest order: ``tables`` holds
+    wqp_h, wqp_v and the basis tables named like the kernel's arguments
+    (basis_w0 (1, 8, h, v), diff_basis_w0 (3, 8, h, v), basis_w1 / diff_basis_w1
+    (3, 12, h, v), basis_w2 / basis_w2b (3, 6, h, v), diff_basis_w2 /
+    diff_basis_w2b (1, 6, h, v), basis_w3 (1, 1, h, v), basis_wt (1, 2, h, v),
+    diff_basis_chi (3, 8, h, v)); ``outputs`` names which of mm0, mm1, mm2,
+    mm2b, mm3, mmt, grad, curl, div, broken_div to compute (all by default).
+    Returns {name: local_stencil (ndf_from, ndf_to, ncell_3d) FP64 tensor}."""
+    names = [o[0] for o in DERHAM_OUTPUTS] if outputs is None else list(outputs)
+    shapes = {o[0]: o[1:] for o in DERHAM_OUTPUTS}
+    bad = [n for n in names if n not in shapes]
+    if bad:
+        raise LfrError(1, f"unknown de Rham outputs {bad}")
+    if chi.dim() != 2 or chi.shape[0] != 3 or chi.shape[1] != chi_space.undf or chi.dtype != torch.float64 \
+            or not chi.is_contiguous() or chi.device != chi_space.ctx.device:
+        raise LfrError(1, f"chi must be a contiguous (3, {chi_space.undf}) float64 tensor on {chi_space.ctx.device}")
+    wh = np.ascontiguousarray(tables["wqp_h"], np.float64)
+    wv = np.ascontiguousarray(tables["wqp_v"], np.float64)
+    nh, nv = wh.size, wv.size
+    want = {"diff_basis_chi": (3, 8)}
+    for n in names:
+        for tname in shapes[n][2]:
+            nd = {"w0": 8, "w1": 12, "w2": 6, "w2b": 6, "w3": 1, "wt": 2}[tname.rsplit("_", 1)[1]]
+            nc = 1 if tname in ("basis_w0", "basis_w3", "basis_wt", "diff_basis_w2", "diff_basis_w2b") else 3
+            want[tname] = (nc, nd)
+    keep, st = [], _DerhamTables(nqp_h=nh, nqp_v=nv)
+    for tname, lead in want.items():
+        a = np.asarray(tables[tname], np.float64)
+        if a.shape != lead + (nh, nv):
+            raise LfrError(1, f"{tname} must have shape {lead + (nh, nv)}, got {a.shape}")
+        a = np.ascontiguousarray(a.transpose(3, 2, 1, 0))          # C order of the Fortran table
+        keep.append(a)
+        setattr(st, tname, a.ctypes.data_as(_P))
+    st.wqp_h, st.wqp_v = wh.ctypes.data_as(_P), wv.ctypes.data_as(_P)
+    n3 = chi_space.fs.mesh.ncells * chi_space.fs.nlayers
+    out = {n: torch.empty((shapes[n][1], shapes[n][0], n3), dtype=torch.float64, device=chi.device)
+           for n in names}
+    ptr = [_P(out[o[0]].data_ptr()) if o[0] in out else None for o in DERHAM_OUTPUTS]
+    ctx = chi_space.ctx
+    ctx.sync_stream()
+    check(lib().lfr_compute_derham_matrices(ctx.h, C.byref(st), _P(chi[0].data_ptr()), _P(chi[1].data_ptr()),
+                                            _P(chi[2].data_ptr()), chi_space.h, *ptr), ctx.h)
+    return out
+
+
 class ColumnwiseGeometry:
     """Banded column-operator geometry between two horizontally discontinuous
     spaces (lfr_cma_create), from lfric_core_amd.cma.cma_geometry."""

@@ -209,33 +209,53 @@ extern "C" int lfr_compute_mass_matrix_w0(lfr_ctx* ctx, int nq, const void* chi1
 }
 
 // ---------------------------------------------------------------------------
-// Mass matrices from basis tables given by the caller (as the PSy layer hands
-// them to the reference kernels): scalar spaces
-// (compute_mass_matrix_w_scalar_code, sci_compute_mass_matrix_kernel_w_scalar_mod.F90:150-174)
-//   mm(ik, df, df2) = sum (((w_h w_v) phi_df) phi_df2) detJ
-// and W2 with the Piola map (compute_mass_matrix_w2_code,
-// sci_compute_mass_matrix_kernel_w2_mod.F90:131-162)
-//   mm(ik, df, df2) = sum ((w_h w_v) (J v_df).(J v_df2)) / detJ,
-// df >= df2 computed and mirrored, qp_v outer and qp_h inner.  The chi
-// space is lowest-order W0 (its reference gradients in the table).
+// Bilinear forms of the lowest-order de Rham complex from basis tables given
+// by the caller (as the PSy layer hands them to the reference kernels):
+//
+//   out(ik, df, df2) = sum_{qp_v} sum_{qp_h} integrand(df, df2, qp)
+//
+// with, wt = w_h w_v, L the row table and R the column table,
+//   kMapDj      ((wt L_df) R_df2) detJ                  scalar masses
+//   kMapCov     (wt (Jt L_df).(Jt R_df2)) detJ          W1 mass, grad
+//   kMapContra  (wt (J L_df).(J R_df2)) / detJ          W2 mass, curl
+//   kMapRef     (wt L_df) R_df2                         div (no metric)
+// Jt = transpose of the cofactor inverse
+// (pointwise_coordinate_jacobian_inverse, sci_coordinate_jacobian_mod.F90:1140-1161),
+// J and detJ from the chi gradients (:993-1103, Cartesian).  The statements
+// follow compute_derham_matrices_code
+// (components/science/source/kernel/fem/sci_compute_derham_matrices_kernel_mod.F90:241-352),
+// compute_mass_matrix_w_scalar_code
+// (sci_compute_mass_matrix_kernel_w_scalar_mod.F90:150-174) and
+// compute_mass_matrix_w2_code (sci_compute_mass_matrix_kernel_w2_mod.F90:131-162).
+// Symmetric forms (the masses) are computed for df >= df2 and mirrored.
+// One thread per (row, cell-layer), cell-layer fastest; the transformed
+// column vectors of a point are formed once per thread.
 // ---------------------------------------------------------------------------
 namespace {
 
-template <int NDF, int NC>
-struct MassQp {
-  double b[NC][NDF];   // basis(c, df) at the point
-  double g[3][8];      // chi reference gradients
+enum { kMapDj = 0, kMapCov = 1, kMapContra = 2, kMapRef = 3 };
+
+template <int NR, int NC, int CL, int CR>
+struct FormQp {
+  double l[CL][NR];   // row basis (component, dof) at the point
+  double r[CR][NC];   // column basis
+  double g[3][8];     // chi reference gradients
   double wh, wv;
 };
 
-template <int NDF, int NC>
+template <int NR, int NC, int CL, int CR, int MAP, bool SYM>
 __global__ void __launch_bounds__(256)
-k_mass_assemble(double* __restrict__ mass, const double* __restrict__ chi1, const double* __restrict__ chi2,
-                const double* __restrict__ chi3, const int32_t* __restrict__ map0,
-                const MassQp<NDF, NC>* __restrict__ tab, int nqp, int ncell, int nl) {
+k_form(double* __restrict__ out, const double* __restrict__ chi1, const double* __restrict__ chi2,
+       const double* __restrict__ chi3, const int32_t* __restrict__ map0,
+       const FormQp<NR, NC, CL, CR>* __restrict__ tab, int nqp, int ncell, int nl) {
+  static_assert(!SYM || (NR == NC && CL == CR), "a symmetric form has one table");
+  constexpr bool kVec = MAP == kMapCov || MAP == kMapContra;
+  static_assert(!kVec || (CL == 3 && CR == 3), "vector maps take 3-component tables");
+  static_assert(kVec || (CL == 1 && CR == 1), "scalar maps take 1-component tables");
+  constexpr int KC = kVec ? 3 : 1;
   const int64_t n3 = (int64_t)ncell * nl;
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= NDF * n3) return;
+  if (t >= NR * n3) return;
   const int a = (int)(t / n3);
   const int64_t ik = t - (int64_t)a * n3;
   const int c = (int)(ik / nl);
@@ -248,11 +268,11 @@ k_mass_assemble(double* __restrict__ mass, const double* __restrict__ chi1, cons
     x2[df] = chi2[d];
     x3[df] = chi3[d];
   }
-  double m[NDF];
+  double acc[NC];
 #pragma unroll
-  for (int b = 0; b < NDF; ++b) m[b] = 0.0;
+  for (int b = 0; b < NC; ++b) acc[b] = 0.0;
   for (int q = 0; q < nqp; ++q) {
-    const MassQp<NDF, NC>& T = tab[q];
+    const FormQp<NR, NC, CL, CR>& T = tab[q];
     double j[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -270,67 +290,128 @@ k_mass_assemble(double* __restrict__ mass, const double* __restrict__ chi1, cons
     const double dj = j[0][0] * (j[1][1] * j[2][2] - j[1][2] * j[2][1]) -
                       j[0][1] * (j[1][0] * j[2][2] - j[1][2] * j[2][0]) +
                       j[0][2] * (j[1][0] * j[2][1] - j[1][1] * j[2][0]);
-    const double wq = T.wh * T.wv;
-    if (NC == 1) {
+    const double wt = T.wh * T.wv;
+    // m[i][d]: the matrix applied to basis vectors, (m v)_i = sum_d m[i][d] v_d
+    double m[3][3];
+    if (MAP == kMapContra) {
 #pragma unroll
-      for (int b = 0; b < NDF; ++b) {
-        const int df = a > b ? a : b, df2 = a > b ? b : a;
-        const double v = ((wq * T.b[0][df]) * T.b[0][df2]) * dj;
-        m[b] = m[b] + v;
-      }
-    } else {
-      // J v for every local dof, components in matmul order
-      double jv[NDF][3];
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int df = 0; df < NDF; ++df)
+        for (int d = 0; d < 3; ++d) m[i][d] = j[i][d];
+    } else if (MAP == kMapCov) {
+      const double idj = 1.0 / dj;
+      double inv[3][3];   // the reference's jac_inv(r, c) at inv[r][c]
+      inv[0][0] = (j[1][1] * j[2][2] - j[1][2] * j[2][1]) * idj;
+      inv[0][1] = -(j[0][1] * j[2][2] - j[0][2] * j[2][1]) * idj;
+      inv[0][2] = (j[0][1] * j[1][2] - j[0][2] * j[1][1]) * idj;
+      inv[1][0] = -(j[1][0] * j[2][2] - j[1][2] * j[2][0]) * idj;
+      inv[1][1] = (j[0][0] * j[2][2] - j[0][2] * j[2][0]) * idj;
+      inv[1][2] = -(j[0][0] * j[1][2] - j[0][2] * j[1][0]) * idj;
+      inv[2][0] = (j[1][0] * j[2][1] - j[1][1] * j[2][0]) * idj;
+      inv[2][1] = -(j[0][0] * j[2][1] - j[0][1] * j[2][0]) * idj;
+      inv[2][2] = (j[0][0] * j[1][1] - j[0][1] * j[1][0]) * idj;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) m[i][d] = inv[d][i];   // jac_t = transpose(jac_inv)
+    }
+    // transformed column vectors and this thread's row vector
+    double rv[NC][KC], lv[KC];
+#pragma unroll
+    for (int b = 0; b < NC; ++b) {
+#pragma unroll
+      for (int i = 0; i < KC; ++i) {
+        if (kVec) {
           double s = 0.0;
 #pragma unroll
-          for (int d = 0; d < 3; ++d) s = s + j[i][d] * T.b[d][df];
-          jv[df][i] = s;
+          for (int d = 0; d < 3; ++d) s = s + m[i][d] * T.r[d][b];
+          rv[b][i] = s;
+        } else {
+          rv[b][i] = T.r[0][b];
         }
+      }
+    }
 #pragma unroll
-      for (int b = 0; b < NDF; ++b) {
-        const int df = a > b ? a : b, df2 = a > b ? b : a;
+    for (int i = 0; i < KC; ++i) {
+      if (kVec) {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s = s + m[i][d] * T.l[d][a];
+        lv[i] = s;
+      } else {
+        lv[i] = T.l[0][a];
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NC; ++b) {
+      double v;
+      if (kVec) {
+        // the reference's dot of the row-df and column-df2 vectors; products
+        // commute exactly, so a mirrored entry needs no swap
         double dot = 0.0;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) dot = dot + jv[df][i] * jv[df2][i];
-        const double v = (wq * dot) / dj;
-        m[b] = m[b] + v;
+        for (int i = 0; i < 3; ++i) dot = dot + lv[i] * rv[b][i];
+        v = MAP == kMapCov ? (wt * dot) * dj : (wt * dot) / dj;
+      } else {
+        double l0 = lv[0], r0 = rv[b][0];
+        if (SYM && b > a) {   // mirrored entry: computed as (df, df2) = (b, a)
+          l0 = rv[b][0];
+          r0 = lv[0];
+        }
+        v = MAP == kMapDj ? ((wt * l0) * r0) * dj : (wt * l0) * r0;
       }
+      acc[b] = acc[b] + v;
     }
   }
 #pragma unroll
-  for (int b = 0; b < NDF; ++b) mass[ik + n3 * (a + NDF * b)] = m[b];
+  for (int b = 0; b < NC; ++b) out[ik + n3 * (a + (int64_t)NR * b)] = acc[b];
 }
 
-template <int NDF, int NC>
-int mass_assemble(lfr_ctx* ctx, int nqp_h, int nqp_v, const double* basis, const double* dchi, const double* wh,
-                  const double* wv, const void* chi1, const void* chi2, const void* chi3, const lfr_space* sp,
-                  void* mass) {
-  std::vector<MassQp<NDF, NC>> tab;
+// Fortran-order host table (ncomp, ndf, nqp_h, nqp_v) element
+inline double tab_at(const double* t, int ncomp, int ndf, int nqp_h, int c, int df, int qh, int qv) {
+  return t[c + ncomp * (df + (size_t)ndf * (qh + (size_t)nqp_h * qv))];
+}
+
+template <int NR, int NC, int CL, int CR, int MAP, bool SYM>
+int form_assemble(lfr_ctx* ctx, int nqp_h, int nqp_v, const double* L, const double* R, const double* dchi,
+                  const double* wh, const double* wv, const void* chi1, const void* chi2, const void* chi3,
+                  const lfr_space* sp, void* out) {
+  using Q = FormQp<NR, NC, CL, CR>;
+  std::vector<Q> tab;
   for (int qv = 0; qv < nqp_v; ++qv)
     for (int qh = 0; qh < nqp_h; ++qh) {
-      MassQp<NDF, NC> T;
-      for (int c = 0; c < NC; ++c)
-        for (int df = 0; df < NDF; ++df) T.b[c][df] = basis[c + NC * (df + NDF * (qh + (size_t)nqp_h * qv))];
+      Q T;
+      for (int c = 0; c < CL; ++c)
+        for (int df = 0; df < NR; ++df) T.l[c][df] = tab_at(L, CL, NR, nqp_h, c, df, qh, qv);
+      for (int c = 0; c < CR; ++c)
+        for (int df = 0; df < NC; ++df) T.r[c][df] = tab_at(R, CR, NC, nqp_h, c, df, qh, qv);
       for (int dir = 0; dir < 3; ++dir)
-        for (int df = 0; df < 8; ++df) T.g[dir][df] = dchi[dir + 3 * (df + 8 * (qh + (size_t)nqp_h * qv))];
+        for (int df = 0; df < 8; ++df) T.g[dir][df] = tab_at(dchi, 3, 8, nqp_h, dir, df, qh, qv);
       T.wh = wh[qh];
       T.wv = wv[qv];
       tab.push_back(T);
     }
-  const size_t tb = tab.size() * sizeof(MassQp<NDF, NC>);
+  const size_t tb = tab.size() * sizeof(Q);
   int rc = lfr_work_reserve(ctx, tb);
   if (rc) return rc;
   LFR_HIP(ctx, hipMemcpyAsync(ctx->work, tab.data(), tb, hipMemcpyHostToDevice, ctx->stream));
-  const int64_t n = (int64_t)NDF * sp->ncell * sp->nlayers;
-  hipLaunchKernelGGL((k_mass_assemble<NDF, NC>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                     (double*)mass, (const double*)chi1, (const double*)chi2, (const double*)chi3, sp->map0_d,
-                     (const MassQp<NDF, NC>*)ctx->work, (int)tab.size(), sp->ncell, sp->nlayers);
+  const int64_t n = (int64_t)NR * sp->ncell * sp->nlayers;
+  hipLaunchKernelGGL((k_form<NR, NC, CL, CR, MAP, SYM>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     ctx->stream, (double*)out, (const double*)chi1, (const double*)chi2, (const double*)chi3,
+                     sp->map0_d, (const Q*)ctx->work, (int)tab.size(), sp->ncell, sp->nlayers);
   LFR_CHECK_LAUNCH(ctx);
-  LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));   // the table is this call's host memory
+  // the table lives in ctx->work and comes from this call's host memory:
+  // finish before the next form reuses either
+  LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LFR_OK;
+}
+
+int check_chi(lfr_ctx* ctx, const lfr_space* sp, const char* who) {
+  LFR_REQUIRE(ctx, sp->ndf == 8, "%s: the chi space must be lowest-order W0 (ndf %d)", who, sp->ndf);
+  for (int c = 0; c < sp->ncell; ++c)
+    for (int v = 0; v < 4; ++v)
+      LFR_REQUIRE(ctx, sp->map_h[(size_t)c * 8 + 4 + v] == sp->map_h[(size_t)c * 8 + v] + 1,
+                  "%s: cell %d is not a lowest-order W0 dofmap", who, c + 1);
   return LFR_OK;
 }
 
@@ -346,28 +427,79 @@ extern "C" int lfr_compute_mass_matrix(lfr_ctx* ctx, int kind, int ndf, int nqp_
   LFR_REQUIRE(ctx, kind == 0 || kind == 1, "lfr_compute_mass_matrix: kind must be 0 (scalar) or 1 (W2), got %d",
               kind);
   LFR_REQUIRE(ctx, nqp_h >= 1 && nqp_v >= 1 && nqp_h * nqp_v <= 4096, "lfr_compute_mass_matrix: bad point counts");
-  LFR_REQUIRE(ctx, chi_space->ndf == 8, "lfr_compute_mass_matrix: the chi space must be lowest-order W0");
+  int rc = check_chi(ctx, chi_space, "lfr_compute_mass_matrix");
+  if (rc) return rc;
+  const double *B = basis, *G = diff_basis_chi;
+#define LFR_MASS(N, C, MAP)                                                                                   \
+  return form_assemble<N, N, C, C, MAP, true>(ctx, nqp_h, nqp_v, B, B, G, wqp_h, wqp_v, chi1_d, chi2_d, chi3_d, \
+                                              chi_space, mass_d)
   if (kind == 0) {
     switch (ndf) {
-      case 1: return mass_assemble<1, 1>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
-                                         chi3_d, chi_space, mass_d);
-      case 2: return mass_assemble<2, 1>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
-                                         chi3_d, chi_space, mass_d);
-      case 8: return mass_assemble<8, 1>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
-                                         chi3_d, chi_space, mass_d);
+      case 1: LFR_MASS(1, 1, kMapDj);
+      case 2: LFR_MASS(2, 1, kMapDj);
+      case 8: LFR_MASS(8, 1, kMapDj);
       default: break;
     }
   } else {
     switch (ndf) {
-      case 2: return mass_assemble<2, 3>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
-                                         chi3_d, chi_space, mass_d);
-      case 4: return mass_assemble<4, 3>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
-                                         chi3_d, chi_space, mass_d);
-      case 6: return mass_assemble<6, 3>(ctx, nqp_h, nqp_v, basis, diff_basis_chi, wqp_h, wqp_v, chi1_d, chi2_d,
-                                         chi3_d, chi_space, mass_d);
+      case 2: LFR_MASS(2, 3, kMapContra);
+      case 4: LFR_MASS(4, 3, kMapContra);
+      case 6: LFR_MASS(6, 3, kMapContra);
       default: break;
     }
   }
+#undef LFR_MASS
   return lfr_set_error(ctx, LFR_ERR_ARG, "lfr_compute_mass_matrix: no kernel for kind %d with %d dofs per cell",
                        kind, ndf);
+}
+
+extern "C" int lfr_compute_derham_matrices(lfr_ctx* ctx, const lfr_derham_tables* t, const void* chi1_d,
+                                           const void* chi2_d, const void* chi3_d, const lfr_space* chi_space,
+                                           void* mm0, void* mm1, void* mm2, void* mm2b, void* mm3, void* mmt,
+                                           void* grad, void* curl, void* div, void* broken_div) {
+  if (!ctx) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, t && chi1_d && chi2_d && chi3_d && chi_space, "lfr_compute_derham_matrices: null argument");
+  LFR_REQUIRE(ctx, t->nqp_h >= 1 && t->nqp_v >= 1 && t->nqp_h * t->nqp_v <= 4096 && t->wqp_h && t->wqp_v &&
+                       t->diff_basis_chi,
+              "lfr_compute_derham_matrices: bad quadrature");
+  int rc = check_chi(ctx, chi_space, "lfr_compute_derham_matrices");
+  if (rc) return rc;
+  const int h = t->nqp_h, v = t->nqp_v;
+  const double *G = t->diff_basis_chi, *wh = t->wqp_h, *wv = t->wqp_v;
+  struct Need {
+    void* out;
+    const double* a;
+    const double* b;
+    const char* what;
+  };
+  const Need need[10] = {{mm0, t->basis_w0, t->basis_w0, "basis_w0"},
+                         {mm1, t->basis_w1, t->basis_w1, "basis_w1"},
+                         {mm2, t->basis_w2, t->basis_w2, "basis_w2"},
+                         {mm2b, t->basis_w2b, t->basis_w2b, "basis_w2b"},
+                         {mm3, t->basis_w3, t->basis_w3, "basis_w3"},
+                         {mmt, t->basis_wt, t->basis_wt, "basis_wt"},
+                         {grad, t->basis_w1, t->diff_basis_w0, "basis_w1 and diff_basis_w0"},
+                         {curl, t->basis_w2, t->diff_basis_w1, "basis_w2 and diff_basis_w1"},
+                         {div, t->basis_w3, t->diff_basis_w2, "basis_w3 and diff_basis_w2"},
+                         {broken_div, t->basis_w3, t->diff_basis_w2b, "basis_w3 and diff_basis_w2b"}};
+  for (const Need& n : need)
+    LFR_REQUIRE(ctx, !n.out || (n.a && n.b), "lfr_compute_derham_matrices: an output needs %s", n.what);
+#define LFR_FORM(OUT, NR, NC, CL, CR, MAP, SYM, L, R)                                                          \
+  if (OUT) {                                                                                                  \
+    rc = form_assemble<NR, NC, CL, CR, MAP, SYM>(ctx, h, v, L, R, G, wh, wv, chi1_d, chi2_d, chi3_d, chi_space, \
+                                                 OUT);                                                        \
+    if (rc) return rc;                                                                                        \
+  }
+  LFR_FORM(mm0, 8, 8, 1, 1, kMapDj, true, t->basis_w0, t->basis_w0)
+  LFR_FORM(mm1, 12, 12, 3, 3, kMapCov, true, t->basis_w1, t->basis_w1)
+  LFR_FORM(mm2, 6, 6, 3, 3, kMapContra, true, t->basis_w2, t->basis_w2)
+  LFR_FORM(mm2b, 6, 6, 3, 3, kMapContra, true, t->basis_w2b, t->basis_w2b)
+  LFR_FORM(mm3, 1, 1, 1, 1, kMapDj, true, t->basis_w3, t->basis_w3)
+  LFR_FORM(mmt, 2, 2, 1, 1, kMapDj, true, t->basis_wt, t->basis_wt)
+  LFR_FORM(grad, 12, 8, 3, 3, kMapCov, false, t->basis_w1, t->diff_basis_w0)
+  LFR_FORM(curl, 6, 12, 3, 3, kMapContra, false, t->basis_w2, t->diff_basis_w1)
+  LFR_FORM(div, 1, 6, 1, 1, kMapRef, false, t->basis_w3, t->diff_basis_w2)
+  LFR_FORM(broken_div, 1, 6, 1, 1, kMapRef, false, t->basis_w3, t->diff_basis_w2b)
+#undef LFR_FORM
+  return LFR_OK;
 }

@@ -1,4 +1,5 @@
-"""lfr_compute_mass_matrix on the device against the mass-matrix oracle, on
+"""lfr_compute_mass_matrix and lfr_compute_derham_matrices on the device
+against the assembly oracle (oracle/assembly.py), on
 the C12 shell with 5 layers, with the reference's own basis tables
 (infrastructure/unit-test/support/fs_test_data_mod.F90) for W0, Wtheta, W3
 (scalar kernel) and W2, W2H, W2V (Piola kernel).  Same per-entry summation
@@ -94,3 +95,51 @@ def test_device_mass_bad_args(ctx, shell):
         w3 = api.Space(ctx, build_dofmap(cubed_sphere(12), "W3", NL))
         api.compute_mass_matrix(0, d["fs3k0_basis_function"], dchi, wh, w,
                                 torch.zeros((3, w3.undf), dtype=torch.float64, device=ctx.device), w3)
+
+
+def _derham_tables(d, wh, wv):
+    return {"wqp_h": wh, "wqp_v": wv, "diff_basis_chi": d["fs0k0_basis_diff_function"],
+            "basis_w0": d["fs0k0_basis_function"], "diff_basis_w0": d["fs0k0_basis_diff_function"],
+            "basis_w1": d["fs1k0_basis_function"], "diff_basis_w1": d["fs1k0_basis_diff_function"],
+            "basis_w2": d["fs2k0_basis_function"], "diff_basis_w2": d["fs2k0_basis_diff_function"],
+            "basis_w2b": d["fs2k0_basis_function"], "diff_basis_w2b": d["fs2k0_basis_diff_function"],
+            "basis_w3": d["fs3k0_basis_function"], "basis_wt": d["fstk0_basis_function"]}
+
+
+def test_device_derham_matrices_match_oracle(ctx, shell):
+    """All ten outputs of compute_derham_matrices on the C12 shell against the
+    oracle; the mass outputs also equal lfr_compute_mass_matrix bit for bit
+    (one kernel template)."""
+    from lfric_core_amd import api
+    from oracle.assembly import derham_matrices
+    print(f"case device de Rham matrices C12L{NL}", flush=True)
+    chi, fs = shell
+    d = np.load(GOLDEN / "fs_test_data.npz")
+    _, w = gauss_rule(3)
+    wh = np.outer(w, w).ravel()
+    t = _derham_tables(d, wh, w)
+    chi_e = chi.reshape(-1, 8, 3).transpose(0, 2, 1)
+    ref = derham_matrices(t, chi_e)
+    sp = api.Space(ctx, fs)
+    chi_f = torch.from_numpy(w0_chi_fields(fs, chi)).to(ctx.device)
+    out = api.compute_derham_matrices(t, chi_f, sp)
+    m2 = api.compute_mass_matrix(1, d["fs2k0_basis_function"], d["fs0k0_basis_diff_function"], wh, w, chi_f, sp)
+    torch.cuda.synchronize()
+    for name, r in ref.items():
+        got = out[name].cpu().numpy()
+        r_ls = np.ascontiguousarray(r.transpose(2, 1, 0))
+        assert got.shape == r_ls.shape, name
+        err = np.abs(got - r_ls).max() / np.abs(r_ls).max()
+        print(f"  {name}: max error / max entry {err:.2e}", flush=True)
+        assert err <= 1e-13, name
+    assert torch.equal(out["mm2"], m2)
+    assert torch.equal(out["mm2b"], m2)
+    # a subset of outputs, and the error when a needed table is missing
+    sub = api.compute_derham_matrices({k: v for k, v in t.items() if k not in ("basis_w1", "diff_basis_w1")},
+                                      chi_f, sp, outputs=("div", "mm3"))
+    torch.cuda.synchronize()
+    assert set(sub) == {"div", "mm3"} and torch.equal(sub["div"], out["div"])
+    from lfric_core_amd._ffi import LfrError
+    with pytest.raises((LfrError, KeyError)):
+        api.compute_derham_matrices({k: v for k, v in t.items() if k != "basis_w1"}, chi_f, sp,
+                                    outputs=("grad",))
