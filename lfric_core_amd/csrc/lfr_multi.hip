@@ -18,7 +18,12 @@
 // Matrix: repacked once (at handle creation) into
 //   Mp[cell][j/4][slot][k][j%4]   (slot = permuted output row, k = layer)
 // so that the 16 layer-lanes of one row read 16 contiguous 16-byte (FP32)
-// or 8-byte (BF16) pieces: 256 B / 128 B per row per instruction.
+// or 8-byte (BF16) pieces: 256 B / 128 B per row per instruction.  The
+// layer extent is padded to nlp (a multiple of 8 FP32 / 16 BF16 layers) so
+// that every slot's run, and every 16-layer piece of it, starts on a 128-byte
+// line: a piece never shares a line with the next block's piece, which
+// would otherwise have to survive in L2 until that block (and mostly does
+// not, so it was fetched twice).
 //
 // Continuous output (GH_INC): cells are processed colour by colour, one
 // launch per colour.  Inside a column the top-face/top-vertex rows of layer
@@ -41,7 +46,7 @@ struct lfr_multi {
   lfr_ctx* ctx = nullptr;
   int mprec = 4;                  // 4: FP32 matrix/MFMA, 2: BF16 matrix and inputs
   int nt = 0, nf = 0, ntp = 0, jq = 0, pig = 0, npair = 0;
-  int nl = 0, ncell = 0;
+  int nl = 0, nlp = 0, ncell = 0;  // nlp: packed layer extent (128-byte line multiple)
   const lfr_space* to = nullptr;
   const lfr_space* from = nullptr;
   void* mp_d = nullptr;           // packed matrix
@@ -62,19 +67,19 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
   return (unsigned short)(u >> 16);
 }
 
-// Mp[((c*JQ + q)*NTP + p)*nl + k][jr] = M[c*nl + k, row(p), 4q + jr]
+// Mp[((c*JQ + q)*NTP + p)*nlp + k][jr] = M[c*nl + k, row(p), 4q + jr]  (0 for k >= nl)
 template <typename TI, typename TO>
 __global__ void k_multi_pack(TO* __restrict__ mp, const TI* __restrict__ m, const int32_t* __restrict__ slot,
-                             int ncell, int nl, int nt, int nf, int ntp, int jqn) {
+                             int ncell, int nl, int nlp, int nt, int nf, int ntp, int jqn) {
   const int64_t n3 = (int64_t)ncell * nl;
-  const int64_t total = (int64_t)ncell * jqn * ntp * nl * 4;
+  const int64_t total = (int64_t)ncell * jqn * ntp * nlp * 4;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
     int64_t t = idx;
     const int jr = (int)(t & 3);
     t >>= 2;
-    const int k = (int)(t % nl);
-    t /= nl;
+    const int k = (int)(t % nlp);
+    t /= nlp;
     const int p = (int)(t % ntp);
     t /= ntp;
     const int q = (int)(t % jqn);
@@ -83,7 +88,7 @@ __global__ void k_multi_pack(TO* __restrict__ mp, const TI* __restrict__ m, cons
     const int row = slot[p] & 0xffff;
     const int kind = slot[p] >> 16;
     float v = 0.f;
-    if (kind != 0 && j < nf) v = (float)m[(c * nl + k) + n3 * ((int64_t)row + (int64_t)nt * j)];
+    if (kind != 0 && j < nf && k < nl) v = (float)m[(c * nl + k) + n3 * ((int64_t)row + (int64_t)nt * j)];
     if constexpr (sizeof(TO) == 2)
       mp[idx] = f2bf(v);
     else
@@ -98,12 +103,21 @@ template <>
 struct MTile<false> {
   lfr_v4f m;
   __device__ __forceinline__ void load(const void* p) { m = *(const lfr_v4f*)p; }
+  __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    m = __builtin_bit_cast(lfr_v4f, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  }
 };
 template <>
 struct MTile<true> {
   lfr_v4s m;
   __device__ __forceinline__ void load(const void* p) { m = *(const lfr_v4s*)p; }
+  __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    m = __builtin_bit_cast(lfr_v4s, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  }
 };
+
+// raw buffer resource word 3 for gfx9-family (CDNA) parts: 32-bit data, no swizzle
+constexpr int kBufWord3 = 0x00020000;
 
 template <bool BF>
 __device__ __forceinline__ lfr_v4f mfma_step(const lfr_v4f (&xa)[4], int v, const MTile<BF>& mt, lfr_v4f acc);
@@ -133,12 +147,16 @@ __device__ __forceinline__ lfr_v4f mfma_step<true>(const lfr_v4f (&xa)[4], int v
 // [w*IGW, w*IGW + IGW).  Lane l: layer b = l >> 2 of the current 16-layer
 // block, q = l & 3 (rhs sub-index for the A operand, row-in-tile for B/D).
 // Vertically paired tiles (0..2*PIG) all belong to wave 0.
-template <bool BF, int NTP, int NF, int PIG, int IGW, int UNR = 1>
+// PF: steps of operand prefetch.  PF = 1 issues step s+1's loads before
+// step s's MFMAs; PF = 2 keeps two steps in flight in a ring of three
+// operand buffers (the j loop fully unrolled so the ring index is a
+// constant; JQ a multiple of 3 so the ring continues across 16-layer blocks).
+template <bool BF, int NTP, int NF, int PIG, int IGW, int UNR = 1, int PF = 1, bool BUF = false>
 __global__ void __launch_bounds__(64 * ((NTP / 4 + IGW - 1) / IGW))
 k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __restrict__ mp_,
           const int32_t* __restrict__ cells, const int32_t* __restrict__ map_to,
           const int32_t* __restrict__ map_from, const int32_t* __restrict__ slot,
-          const uint64_t* __restrict__ first, int nl, int nt, int overwrite) {
+          const uint64_t* __restrict__ first, int nl, int nlp, int nt, int overwrite, uint32_t x_bytes) {
   constexpr int IG = NTP / 4;
   constexpr int JQ = (NF + 3) / 4;
   constexpr int ME = BF ? 2 : 4;   // bytes per matrix element
@@ -166,27 +184,62 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
   // matrix tile per row tile; the loads of the next step (the next jq, or
   // the first of the next 16-layer block) are issued before the MFMAs of
   // the current one
+  // BUF: raw buffer loads, a wave-uniform part of each address in an SGPR
+  // offset (the dof's base for x, the step and tile for the matrix) and one
+  // per-lane offset shared by every load of a step; x must span < 4 GiB
+  // (checked on the host), the matrix resource covers this column's block.
+  const int64_t cb = (int64_t)JQ * NTP * nlp * 4 * ME;   // bytes of one column's matrix block
+  const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc((void*)x, (short)0, (int)x_bytes, kBufWord3);
+  const __amdgpu_buffer_rsrc_t rm =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(mp + (int64_t)c * cb), (short)0, (int)cb, kBufWord3);
   auto load_step = [&](int k, int jq, lfr_v4f (&xa)[4], MTile<BF> (&tl)[IGW]) {
+      if constexpr (BUF) {
+        const unsigned vx = (unsigned)k * (kRhs * 4u) + 16u * q;
 #pragma unroll
-      for (int jr = 0; jr < 4; ++jr) {
-        const int j = 4 * jq + jr;
-        if (j < NF) {
-          const int64_t d = (int64_t)mf[j] + k;
-          xa[jr] = *(const lfr_v4f*)(x + d * kRhs + 4 * q);
-        } else {
-          xa[jr] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
+        for (int jr = 0; jr < 4; ++jr) {
+          const int j = 4 * jq + jr;
+          if (j < NF)
+            xa[jr] = __builtin_bit_cast(
+                lfr_v4f, __builtin_amdgcn_raw_buffer_load_b128(rx, vx, (unsigned)mf[j] * (kRhs * 4u), 0));
+          else
+            xa[jr] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
+        }
+        const unsigned vm = ((unsigned)q * nlp + k) * (4u * ME);
+#pragma unroll
+        for (int gi = 0; gi < IGW; ++gi) {
+          if (IG % IGW != 0 && g0 + gi >= IG) break;
+          tl[gi].load(rm, vm, (unsigned)((jq * NTP + 4 * (g0 + gi)) * nlp) * (4u * ME));
+        }
+      } else {
+#pragma unroll
+        for (int jr = 0; jr < 4; ++jr) {
+          const int j = 4 * jq + jr;
+          if (j < NF) {
+            const int64_t d = (int64_t)mf[j] + k;
+            xa[jr] = *(const lfr_v4f*)(x + d * kRhs + 4 * q);
+          } else {
+            xa[jr] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
+          }
+        }
+        const int64_t mrow = ((int64_t)c * JQ + jq) * NTP;
+#pragma unroll
+        for (int gi = 0; gi < IGW; ++gi) {
+          if (IG % IGW != 0 && g0 + gi >= IG) break;
+          tl[gi].load(mp + ((((mrow + 4 * (g0 + gi) + q) * nlp) + k) * 4) * ME);
         }
       }
-      const int64_t mrow = ((int64_t)c * JQ + jq) * NTP;
-#pragma unroll
-      for (int gi = 0; gi < IGW; ++gi) {
-        if (IG % IGW != 0 && g0 + gi >= IG) break;
-        tl[gi].load(mp + ((((mrow + 4 * (g0 + gi) + q) * nl) + k) * 4) * ME);
-      }
   };
+  static_assert(PF == 1 || (PF == 2 && JQ % 3 == 0), "the ring of three needs JQ % 3 == 0");
   lfr_v4f xa[4], xn[4];
   MTile<BF> ta[IGW], tn[IGW];
-  load_step(min(b, nl - 1), 0, xa, ta);
+  lfr_v4f xr[PF == 2 ? 3 : 1][4];
+  MTile<BF> tr[PF == 2 ? 3 : 1][IGW];
+  if constexpr (PF == 1) {
+    load_step(min(b, nl - 1), 0, xa, ta);
+  } else {
+    load_step(min(b, nl - 1), 0, xr[0], tr[0]);
+    load_step(min(b, nl - 1), 1, xr[1 % (PF == 2 ? 3 : 1)], tr[1 % (PF == 2 ? 3 : 1)]);
+  }
 
   for (int k0 = 0; k0 < nl; k0 += 16) {
     const int k = min(k0 + b, nl - 1);     // clamped: lanes past the top only load
@@ -197,20 +250,36 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[gi][v] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
 
+    if constexpr (PF == 1) {
 #pragma unroll UNR
-    for (int jq = 0; jq < JQ; ++jq) {
-      if (jq + 1 < JQ) load_step(k, jq + 1, xn, tn);
-      else if (k0 + 16 < nl) load_step(kn, 0, xn, tn);
+      for (int jq = 0; jq < JQ; ++jq) {
+        if (jq + 1 < JQ) load_step(k, jq + 1, xn, tn);
+        else if (k0 + 16 < nl) load_step(kn, 0, xn, tn);
 #pragma unroll
-      for (int gi = 0; gi < IGW; ++gi) {
-        if (IG % IGW != 0 && g0 + gi >= IG) break;
+        for (int gi = 0; gi < IGW; ++gi) {
+          if (IG % IGW != 0 && g0 + gi >= IG) break;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[gi][v] = mfma_step<BF>(xa, v, ta[gi], acc[gi][v]);
+          for (int v = 0; v < 4; ++v) acc[gi][v] = mfma_step<BF>(xa, v, ta[gi], acc[gi][v]);
+        }
+#pragma unroll
+        for (int jr = 0; jr < 4; ++jr) xa[jr] = xn[jr];
+#pragma unroll
+        for (int gi = 0; gi < IGW; ++gi) ta[gi] = tn[gi];
       }
+    } else {
+      constexpr int R = PF == 2 ? 3 : 1;
 #pragma unroll
-      for (int jr = 0; jr < 4; ++jr) xa[jr] = xn[jr];
+      for (int jq = 0; jq < JQ; ++jq) {
+        const int s2 = jq + 2;
+        if (s2 < JQ) load_step(k, s2, xr[s2 % R], tr[s2 % R]);
+        else if (k0 + 16 < nl) load_step(kn, s2 - JQ, xr[s2 % R], tr[s2 % R]);
 #pragma unroll
-      for (int gi = 0; gi < IGW; ++gi) ta[gi] = tn[gi];
+        for (int gi = 0; gi < IGW; ++gi) {
+          if (IG % IGW != 0 && g0 + gi >= IG) break;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[gi][v] = mfma_step<BF>(xr[jq % R], v, tr[jq % R][gi], acc[gi][v]);
+        }
+      }
     }
 
     // epilogue: lane (b, q) of tile g holds slot 4g+q of layer k0+b,
@@ -264,15 +333,16 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
   }
 }
 
-template <bool BF, int NTP, int NF, int PIG, int IGW, int UNR = 1>
+template <bool BF, int NTP, int NF, int PIG, int IGW, int UNR = 1, int PF = 1, bool BUF = false>
 void launch_multi(hipStream_t s, lfr_multi* h, float* y, const float* x, int overwrite) {
   constexpr int WS = (NTP / 4 + IGW - 1) / IGW;
   for (size_t i = 0; i + 1 < h->launch_off.size(); ++i) {
     const int n = h->launch_off[i + 1] - h->launch_off[i];
     if (n <= 0) continue;
-    hipLaunchKernelGGL((k_multi16<BF, NTP, NF, PIG, IGW, UNR>), dim3(n), dim3(64 * WS), 0, s, y, x,
+    hipLaunchKernelGGL((k_multi16<BF, NTP, NF, PIG, IGW, UNR, PF, BUF>), dim3(n), dim3(64 * WS), 0, s, y, x,
                        (const void*)h->mp_d, h->cells_d + h->launch_off[i], h->to->map0_d, h->from->map0_d,
-                       h->slot_d, h->first_d, h->nl, h->nt, overwrite);
+                       h->slot_d, h->first_d, h->nl, h->nlp, h->nt, overwrite,
+                       (uint32_t)((int64_t)h->from->undf * kRhs * 4));
   }
 }
 
@@ -399,11 +469,15 @@ int lfr_multi_create(lfr_ctx* ctx, int mprec, int prec_in, const void* matrix_d,
   h->pig = pig;
   h->npair = P;
   h->nl = nl;
+  {
+    const int lq = mprec == 4 ? 8 : 16;   // layers per 128-byte line (16 B / 8 B per layer)
+    h->nlp = (nl + lq - 1) / lq * lq;
+  }
   h->ncell = ncell;
   h->to = to;
   h->from = from;
   h->launch_off = off;
-  const int64_t nel = (int64_t)ncell * h->jq * ntp * nl * 4;
+  const int64_t nel = (int64_t)ncell * h->jq * ntp * h->nlp * 4;
   h->mp_bytes = nel * (mprec == 4 ? 4 : 2);
   if (hipMalloc(&h->mp_d, h->mp_bytes) != hipSuccess || hipMalloc(&h->slot_d, ntp * 4) != hipSuccess ||
       hipMalloc(&h->first_d, (size_t)ncell * 8) != hipSuccess ||
@@ -419,16 +493,16 @@ int lfr_multi_create(lfr_ctx* ctx, int mprec, int prec_in, const void* matrix_d,
   const unsigned g = lfr_blocks(nel, 256, 8192);
   if (prec_in == 4 && mprec == 4)
     hipLaunchKernelGGL((k_multi_pack<float, float>), dim3(g), dim3(256), 0, s, (float*)h->mp_d,
-                       (const float*)matrix_d, h->slot_d, ncell, nl, nt, nf, ntp, h->jq);
+                       (const float*)matrix_d, h->slot_d, ncell, nl, h->nlp, nt, nf, ntp, h->jq);
   else if (prec_in == 8 && mprec == 4)
     hipLaunchKernelGGL((k_multi_pack<double, float>), dim3(g), dim3(256), 0, s, (float*)h->mp_d,
-                       (const double*)matrix_d, h->slot_d, ncell, nl, nt, nf, ntp, h->jq);
+                       (const double*)matrix_d, h->slot_d, ncell, nl, h->nlp, nt, nf, ntp, h->jq);
   else if (prec_in == 4)
     hipLaunchKernelGGL((k_multi_pack<float, unsigned short>), dim3(g), dim3(256), 0, s,
-                       (unsigned short*)h->mp_d, (const float*)matrix_d, h->slot_d, ncell, nl, nt, nf, ntp, h->jq);
+                       (unsigned short*)h->mp_d, (const float*)matrix_d, h->slot_d, ncell, nl, h->nlp, nt, nf, ntp, h->jq);
   else
     hipLaunchKernelGGL((k_multi_pack<double, unsigned short>), dim3(g), dim3(256), 0, s,
-                       (unsigned short*)h->mp_d, (const double*)matrix_d, h->slot_d, ncell, nl, nt, nf, ntp,
+                       (unsigned short*)h->mp_d, (const double*)matrix_d, h->slot_d, ncell, nl, h->nlp, nt, nf, ntp,
                        h->jq);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -480,20 +554,35 @@ int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void*
     else                                                                \
       launch_multi<false, NTP, NF, PIG, IGW>(s, h, y, x, overwrite);    \
     break;
-  // W2 k=1 variants (row tiles per wave, j-quad loop unroll), selected by
-  // LFR_MULTI_VARIANT: 0 = 3 tiles (3 waves/column), 1 = 2 tiles (5 waves),
-  // 2 = 2 tiles with the j loop unrolled by 3, 3 = 3 tiles unrolled by 2
+  // W2 k=1 variants, selected by LFR_MULTI_VARIANT (row tiles per wave, j
+  // loop unroll, prefetch steps, buffer loads): 0 = (3, 1, 2, yes) when x
+  // spans < 4 GiB, else the switch below's (3, 1, 1, no); 1 = (2,
+  // 1, 1, no), 2 = (2, 3, 1, no), 3 = (3, 2, 1, no), 4 = (3, 1, 2, no), 5 =
+  // (2, 1, 2, no), 6 = (3, 1, 1, yes), 7 = (3, 1, 2, yes), 8 = (2, 1, 2,
+  // yes), 9 = (2, 1, 1, yes)
   static const int var36 = [] {
     const char* e = getenv("LFR_MULTI_VARIANT");
-    return (e && e[0] >= '0' && e[0] <= '3') ? e[0] - '0' : 0;
+    return (e && e[0] >= '0' && e[0] <= '9') ? e[0] - '0' : 0;
   }();
-  if (key == 36 * 1000 + 36 * 10 + 1 && var36 != 0) {
-#define LFR_V(IG, UN)                                                   \
-  if (bf) launch_multi<true, 36, 36, 1, IG, UN>(s, h, y, x, overwrite); \
-  else launch_multi<false, 36, 36, 1, IG, UN>(s, h, y, x, overwrite);
-    if (var36 == 1) { LFR_V(2, 1) }
-    else if (var36 == 2) { LFR_V(2, 3) }
-    else { LFR_V(3, 2) }
+  const bool buf_ok = (int64_t)h->from->undf * kRhs * 4 < (int64_t)1 << 32;
+  if (key == 36 * 1000 + 36 * 10 + 1 && (var36 != 0 || buf_ok)) {
+#define LFR_V(IG, UN, PF, BUF)                                                   \
+  if (bf) launch_multi<true, 36, 36, 1, IG, UN, PF, BUF>(s, h, y, x, overwrite); \
+  else launch_multi<false, 36, 36, 1, IG, UN, PF, BUF>(s, h, y, x, overwrite);
+    if ((var36 >= 6 || var36 == 0) && !buf_ok)
+      return lfr_set_error(ctx, LFR_ERR_ARG, "lfr_matrix_vector_multi: variant %d needs x below 4 GiB", var36);
+    switch (var36) {
+      case 0: LFR_V(3, 1, 2, true) break;   // default when x spans < 4 GiB
+      case 1: LFR_V(2, 1, 1, false) break;
+      case 2: LFR_V(2, 3, 1, false) break;
+      case 3: LFR_V(3, 2, 1, false) break;
+      case 4: LFR_V(3, 1, 2, false) break;
+      case 5: LFR_V(2, 1, 2, false) break;
+      case 6: LFR_V(3, 1, 1, true) break;
+      case 7: LFR_V(3, 1, 2, true) break;
+      case 8: LFR_V(2, 1, 2, true) break;
+      default: LFR_V(2, 1, 1, true) break;
+    }
 #undef LFR_V
     LFR_CHECK_LAUNCH(ctx);
     return LFR_OK;
