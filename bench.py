@@ -167,6 +167,145 @@ def bench_solvers(ctx, cfg, w2, torch, steps_cg: int = 50):
     return out
 
 
+def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
+    """The other operator kernels of the path at C96L70, each timed with HIP
+    events over `steps` cold-cache calls and set against its algorithmic HBM bytes:
+    dg_matrix_vector / dg_inc_matrix_vector (div, W3 <- W2), the transposed
+    apply onto W2 (D^T p), columnwise_op_appinv on W3 columns, CMA assembly
+    and apply on Wtheta, tri_solve (W3) and operator_tri_solve (Wtheta),
+    GMRES(10) on the W2 mass matrix (per Arnoldi step), and the vector
+    kernels (axpy, FP64-accumulated dot of FP32 fields)."""
+    from lfric_core_amd import api
+    from lfric_core_amd import function_space as fsm
+    from lfric_core_amd.rng import Stream
+    from lfric_core_amd.solvers import GMRES
+    dev = ctx.device
+    nl = cfg.nlayers
+    nc = cfg.mesh.ncells
+    n3 = nc * nl
+    u2, u3 = cfg.w2.undf, cfg.w3.undf
+    st = Stream(11)
+    out = {}
+
+    def rec(name, ms, nbytes, **kw):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"us": round(ms * 1e3, 2), "algorithmic_bytes": int(nbytes), "gbs": round(gbs, 1),
+                     "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), **kw}
+        log(f"row {name}: {ms*1e3:.1f} us, {gbs:.0f} GB/s ({100*gbs/HBM_PEAK_GBS:.1f} % of 8 TB/s)")
+
+    # Most of these working sets (tens to a few hundred MB) fit the 256 MB
+    # last-level cache, so back-to-back calls would be served from it: each
+    # timed call is preceded by a 1 GiB write that evicts it, and only the
+    # call itself is bracketed by events.
+    flush = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+
+    def timed(fn, n=steps):
+        for _ in range(3):
+            fn()
+        cur = torch.cuda.current_stream()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+        for e0, e1 in ev:
+            flush.zero_()
+            e0.record(cur)
+            fn()
+            e1.record(cur)
+        torch.cuda.synchronize()
+        return sum(e0.elapsed_time(e1) for e0, e1 in ev) / n
+
+    # a2 / a3 / a4: the divergence operator (local_stencil(ncell_3d, 1, 6))
+    for tag, dt, s in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
+        D = torch.from_numpy(cfg.div).to(dev).to(dt).contiguous()
+        op = api.LMAOperator(D, w3, w2, cfg.colouring)
+        x2 = torch.from_numpy(cfg.b).to(dev).to(dt)
+        y3 = torch.zeros(u3, dtype=dt, device=dev)
+        maps = nc * (6 + 1) * 4
+        rec(f"a2_dg_matrix_vector_{tag}", timed(lambda: api.dg_matrix_vector(y3, x2, op)),
+            n3 * 6 * s + u2 * s + u3 * s + maps)
+        rec(f"a3_dg_inc_matrix_vector_{tag}", timed(lambda: api.dg_inc_matrix_vector(y3, x2, op)),
+            n3 * 6 * s + u2 * s + 2 * u3 * s + maps)
+        y2 = torch.empty(u2, dtype=dt, device=dev)
+        rec(f"a4_transpose_div_{tag}", timed(lambda: api.transpose_matrix_vector(y2, y3, op, overwrite=True)),
+            n3 * 6 * s + u3 * s + u2 * s + maps)
+        del D, op, x2, y3, y2
+    # a5: columnwise_op_appinv on the W3 columns (tridiagonal, FP32 r_solver)
+    A = np.empty((nc, nl, 3), np.float32)
+    A[:, :, 1] = 4.0 + st.uniform(nc * nl).reshape(nc, nl)
+    A[:, :, 0] = -0.5 - 0.1 * st.uniform(nc * nl).reshape(nc, nl)
+    A[:, :, 2] = -0.5 - 0.1 * st.uniform(nc * nl).reshape(nc, nl)
+    cma3 = api.CMAOperator(torch.from_numpy(A).to(dev), w3, np.arange(1, nl + 1, dtype=np.int32))
+    x3 = torch.from_numpy(st.uniform(u3).astype(np.float32)).to(dev)
+    z3 = torch.empty_like(x3)
+    rec("a5_columnwise_op_appinv_fp32", timed(lambda: api.columnwise_op_appinv(z3, x3, cma3)),
+        nc * nl * (3 + 2) * 4 + nl * 4 + nc * 4)
+    del cma3
+    # the same solve on the C192 columns (the finest multigrid level)
+    from lfric_core_amd.mesh import cubed_sphere
+    w3b_fs = fsm.build_dofmap(cubed_sphere(192), "W3", nl)
+    w3b = api.Space(ctx, w3b_fs)
+    ncb = w3b_fs.mesh.ncells
+    Ab = torch.from_numpy(np.tile(A[:1], (ncb, 1, 1))).to(dev)
+    cmab = api.CMAOperator(Ab, w3b, np.arange(1, nl + 1, dtype=np.int32))
+    xb3 = torch.from_numpy(st.uniform(w3b_fs.undf).astype(np.float32)).to(dev)
+    zb3 = torch.empty_like(xb3)
+    rec("a5_columnwise_op_appinv_fp32_c192", timed(lambda: api.columnwise_op_appinv(zb3, xb3, cmab)),
+        ncb * nl * (3 + 2) * 4 + nl * 4 + ncb * 4)
+    del cmab, Ab, xb3, zb3, w3b
+    # f1: CMA assembly and apply on Wtheta (2x2 local matrices, bandwidth 3)
+    wt_fs = fsm.build_dofmap(cfg.mesh, "WTHETA", nl)
+    wt = api.Space(ctx, wt_fs)
+    ut = wt_fs.undf
+    Mt = np.empty((2, 2, n3))
+    Mt[0, 0] = 2.0 + st.uniform(n3)
+    Mt[1, 1] = 2.0 + st.uniform(n3)
+    Mt[0, 1] = 0.5 * st.uniform(n3)
+    Mt[1, 0] = 0.5 * st.uniform(n3)
+    Mt32 = torch.from_numpy(Mt.astype(np.float32)).to(dev)
+    lma_t = api.LMAOperator(Mt32, wt, wt, cfg.colouring)
+    geom = api.ColumnwiseGeometry(wt, wt)
+    cm = geom.empty_matrix(torch.float32)
+    g = geom.geom
+    rec("f1_columnwise_op_asm_fp32", timed(lambda: api.columnwise_op_asm(cm, lma_t, geom)),
+        n3 * 4 * 4 + nc * g.nrow * g.bandwidth * 4, bandwidth=int(g.bandwidth), nrow=int(g.nrow))
+    xt = torch.from_numpy(st.uniform(ut).astype(np.float32)).to(dev)
+    yt = torch.zeros_like(xt)
+    rec("f1_columnwise_op_app_fp32", timed(lambda: api.columnwise_op_app(yt, xt, cm, geom)),
+        nc * g.nrow * g.bandwidth * 4 + ut * 4 + 2 * ut * 4)
+    rec("f1_operator_tri_solve_fp32", timed(lambda: api.operator_tri_solve(yt, xt, Mt32, wt)),
+        n3 * 4 * 4 + 2 * ut * 4)
+    t0 = torch.from_numpy((4.0 + st.uniform(u3)).astype(np.float32)).to(dev)
+    tp = torch.from_numpy((0.5 * st.uniform(u3)).astype(np.float32)).to(dev)
+    tm = torch.from_numpy((0.5 * st.uniform(u3)).astype(np.float32)).to(dev)
+    rec("f1_tri_solve_fp32", timed(lambda: api.tri_solve(z3, x3, t0, tp, tm, w3)), 5 * u3 * 4)
+    geom.close()
+    del cm, lma_t, Mt32, xt, yt, t0, tp, tm, x3, z3
+    # a10: GMRES(10), two restarts (20 Arnoldi steps), FP64 W2 mass, packed operator
+    M64 = torch.from_numpy(cfg.mass).to(dev)
+    op64 = api.LMAOperator(M64, w2, w2, cfg.colouring)
+    op64.packed = api.PackedOperator(op64)
+    b = torch.from_numpy(cfg.b).to(dev)
+    xg = torch.zeros_like(b)
+    gm = GMRES(op64, gcrk=10, r_tol=1e-30, max_iter=2, monitor_convergence=False)
+    gm.apply(xg, b, x_is_zero=True)
+    torch.cuda.synchronize()
+    ms = timed(lambda: gm.apply(xg, b, x_is_zero=True), 5)
+    # per Arnoldi step j (1-based): apply + j dots + j axpys over W2; plus the
+    # explicit residual per restart; counted as apply + 2.5 (j+1) vector passes
+    step_bytes = n3 * 36 * 8 + 2 * u2 * 8
+    vec = sum(2.5 * (j + 1) for j in range(10)) * u2 * 8 / 10
+    rec("a10_gmres10_per_arnoldi_step_fp64", ms / 20, step_bytes + vec)
+    op64.packed.close()
+    del M64, op64, b, xg
+    # a11-a14: vector kernels on W2-sized FP32 fields
+    xa = torch.from_numpy(st.uniform(u2).astype(np.float32)).to(dev)
+    ya = torch.from_numpy(st.uniform(u2).astype(np.float32)).to(dev)
+    rec("a11_axpy_fp32", timed(lambda: api.axpy(ctx, 0.5, xa, ya)), 3 * u2 * 4)
+    rec("a14_dot_fp32_fp64acc", timed(lambda: api.dot(ctx, xa, ya)), 2 * u2 * 4,
+        note="includes the host read-back of the FP64 result")
+    del xa, ya, flush
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_multi(ctx, torch, edge_cells: int = 48, nl: int = 70, steps: int = 20):
     """16-field W2 (element order 1, 36x36) apply on MFMA, FP32 and BF16-input,
     each against 16 single-field FP64 applies of the same operator."""
@@ -369,6 +508,8 @@ def main():
     ap.add_argument("--no-mg", action="store_true", help="skip the batched multigrid V-cycles")
     ap.add_argument("--no-multi", action="store_true", help="skip the 16-field MFMA apply")
     ap.add_argument("--no-assembly", action="store_true", help="skip the device assembly timings")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the other operator kernels (dg applies, appinv, CMA, tri solves, GMRES, vectors)")
     ap.add_argument("--packed-only", action="store_true",
                     help="time only the packed applies (profiling runs: the reference-layout "
                          "kernels share the packed kernels' names)")
@@ -455,6 +596,8 @@ def main():
 
     if not args.no_solvers:
         extra["solvers"] = bench_solvers(ctx, cfg, w2, torch)
+    if not args.no_rows:
+        extra["rows"] = bench_rows(ctx, cfg, w2, w3, torch)
     if not args.no_multi:
         extra["multi_rhs"] = bench_multi(ctx, torch, 24 if args.quick else 48)
     if not args.no_mg:

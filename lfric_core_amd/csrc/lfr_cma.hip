@@ -25,7 +25,7 @@
 namespace {
 
 constexpr int kCols = 64;   // columns (lanes) per workgroup
-constexpr int kCh = 10;     // rows per LDS stage
+constexpr int kCh = 14;     // rows per LDS stage (<= 15: 4 loads per row stay under 63 in flight)
 
 template <typename T, typename C, int NROW>
 __global__ void __launch_bounds__(kCols)
@@ -38,10 +38,12 @@ k_appinv(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
   __shared__ T sb[kCols * PITCH_B];
   __shared__ T sx[kCols * PITCH_X];
   __shared__ int32_t sbase[kCols];
+  __shared__ int32_t sind[NROW];   // indirection_dofmap - 1, read once
   const int lane = threadIdx.x;
   const int c0 = blockIdx.x * kCols;
   const int ncols = min(kCols, ncell - c0);
   if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
+  for (int i = lane; i < NROW; i += kCols) sind[i] = ind[i] - 1;
   C cp[NROW], dp[NROW];
 #pragma unroll
   for (int s = 0; s < NST; ++s) {
@@ -59,7 +61,7 @@ k_appinv(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
     for (int e = lane; e < nx; e += kCols) {
       const int cc = e / nr;
       const int q = e - cc * nr;
-      sx[cc * PITCH_X + q] = x[sbase[cc] + ind[r0 + q] - 1];
+      sx[cc * PITCH_X + q] = x[sbase[cc] + sind[r0 + q]];
     }
     __syncthreads();
     if (lane < ncols) {
@@ -108,7 +110,7 @@ k_appinv(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
     for (int e = lane; e < nx; e += kCols) {
       const int cc = e / nr;
       const int q = e - cc * nr;
-      lhs[sbase[cc] + ind[r0 + q] - 1] = sx[cc * PITCH_X + q];
+      lhs[sbase[cc] + sind[r0 + q]] = sx[cc * PITCH_X + q];
     }
   }
 }
@@ -504,6 +506,135 @@ k_operator_tri_solve(T* __restrict__ y, const T* __restrict__ x, const T* __rest
   }
 }
 
+// The same two column solves, LDS-staged like k_appinv: one lane per
+// column, 64 columns per workgroup, the column's inputs brought in CH-row
+// chunks with coalesced loads, c' and d' in registers (NROW a compile-time
+// constant), the solution stored through LDS.  The arithmetic is the
+// recurrence of k_tri_solve / k_operator_tri_solve term for term (products
+// commute exactly), so the bits are the same; only the memory traffic moves
+// (no scratch array for c', no dependent global load per row).
+//   MODE 0: tri_solve, bands (tri_minus, tri_0, tri_plus) at dof map0 + k,
+//           NROW = nl.
+//   MODE 1: operator_tri_solve, bands from the 2x2 local matrices of the
+//           cell (k-1 and k), NROW = nl + 1.
+template <typename T, int NROW, int MODE>
+__global__ void __launch_bounds__(kCols)
+k_thomas(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ b0, const T* __restrict__ b1,
+         const T* __restrict__ b2, const int32_t* __restrict__ map0, int ndf, int ncell, int nl) {
+  constexpr int NST = (NROW + kCh - 1) / kCh;
+  constexpr int NV = MODE == 0 ? 3 : 4;            // staged values per row
+  constexpr int PITCH_B = kCh * NV + 1;            // odd pitch: conflict-free lane-strided reads
+  constexpr int PITCH_X = kCh + 1;
+  __shared__ T sb[kCols * PITCH_B];
+  __shared__ T sx[kCols * PITCH_X];
+  __shared__ int32_t sbase[kCols];
+  const int lane = threadIdx.x;
+  const int c0 = blockIdx.x * kCols;
+  const int ncols = min(kCols, ncell - c0);
+  const int64_t n3 = (int64_t)ncell * nl;
+  if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
+  T cp[NROW], dp[NROW];
+#pragma unroll
+  for (int s = 0; s < NST; ++s) {
+    const int r0 = s * kCh;
+    const int nr = (NROW - r0) < kCh ? (NROW - r0) : kCh;
+    __syncthreads();
+    const int nx = ncols * nr;
+    for (int e = lane; e < nx; e += kCols) {
+      const int cc = e / nr;
+      const int q = e - cc * nr;
+      const int r = r0 + q;
+      const int64_t d = (int64_t)sbase[cc] + r;
+      T* o = sb + cc * PITCH_B + NV * q;
+      if (MODE == 0) {
+        o[0] = b2[d];   // tri_minus
+        o[1] = b0[d];   // tri_0
+        o[2] = b1[d];   // tri_plus
+      } else {
+        // m(ik, i, j) at ik + n3 (i + 2 j), ik = cell*nl + layer
+        const int64_t ik = (int64_t)(c0 + cc) * nl + r;
+        o[0] = r < nl ? b0[ik] : T(0);                    // (k; 1,1)
+        o[1] = r >= 1 ? b0[ik - 1 + 3 * n3] : T(0);       // (k-1; 2,2)
+        o[2] = r >= 1 ? b0[ik - 1 + n3] : T(0);           // (k-1; 2,1)
+        o[3] = r < nl ? b0[ik + 2 * n3] : T(0);           // (k; 1,2)
+      }
+      sx[cc * PITCH_X + q] = x[d];
+    }
+    __syncthreads();
+    if (lane < ncols) {
+#pragma unroll
+      for (int q = 0; q < kCh; ++q) {
+        const int i = r0 + q;
+        if (i < NROW) {
+          const T* v = sb + lane * PITCH_B + NV * q;
+          T a, dg, u;
+          if (MODE == 0) {
+            a = v[0];
+            dg = v[1];
+            u = v[2];
+          } else {
+            a = v[2];
+            dg = i == 0 ? v[0] + T(0) : (i < nl ? v[0] + v[1] : v[1]);
+            u = i < nl ? v[3] : T(0);
+          }
+          const T xv = sx[lane * PITCH_X + q];
+          if (i == 0) {
+            const T inv = T(1) / dg;
+            cp[i] = u * inv;
+            dp[i] = xv * inv;
+          } else {
+            const T ac = a * cp[i - 1];
+            const T inv = T(1) / (dg - ac);
+            const T ad = a * dp[i - 1];
+            if (i < NROW - 1) cp[i] = u * inv;
+            dp[i] = (xv - ad) * inv;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = NROW - 2; i >= 0; --i) {
+    const T cd = cp[i] * dp[i + 1];
+    dp[i] = dp[i] - cd;
+  }
+#pragma unroll
+  for (int s = 0; s < NST; ++s) {
+    const int r0 = s * kCh;
+    const int nr = (NROW - r0) < kCh ? (NROW - r0) : kCh;
+    __syncthreads();
+    if (lane < ncols) {
+#pragma unroll
+      for (int q = 0; q < kCh; ++q)
+        if (r0 + q < NROW) sx[lane * PITCH_X + q] = dp[r0 + q];
+    }
+    __syncthreads();
+    const int nx = ncols * nr;
+    for (int e = lane; e < nx; e += kCols) {
+      const int cc = e / nr;
+      const int q = e - cc * nr;
+      y[(int64_t)sbase[cc] + r0 + q] = sx[cc * PITCH_X + q];
+    }
+  }
+}
+
+template <typename T, int MODE>
+bool launch_thomas(hipStream_t s, T* y, const T* x, const T* b0, const T* b1, const T* b2, const lfr_space* sp) {
+  const int nrow = MODE == 0 ? sp->nlayers : sp->nlayers + 1;
+  const unsigned nb = (unsigned)((sp->ncell + kCols - 1) / kCols);
+#define CASE(NR)                                                                                               \
+  case NR:                                                                                                     \
+    hipLaunchKernelGGL((k_thomas<T, NR, MODE>), dim3(nb), dim3(kCols), 0, s, y, x, b0, b1, b2, sp->map0_d, \
+                       sp->ndf, sp->ncell, sp->nlayers);                                                     \
+    return true;
+  if constexpr (MODE == 0) {
+    switch (nrow) { CASE(10) CASE(70) default: return false; }
+  } else {
+    switch (nrow) { CASE(71) default: return false; }
+  }
+#undef CASE
+}
+
 }  // namespace
 
 extern "C" {
@@ -516,6 +647,15 @@ int lfr_tri_solve(lfr_ctx* ctx, int prec, void* y, const void* x, const void* tr
   LFR_REQUIRE(ctx, space->ndf == 1, "lfr_tri_solve: lowest-order W3 only (ndf %d, expected 1)", space->ndf);
   LFR_REQUIRE(ctx, y != x && y != tri_0 && y != tri_plus && y != tri_minus,
               "lfr_tri_solve: y must not alias an input");
+  const bool staged =
+      prec == 4 ? launch_thomas<float, 0>(ctx->stream, (float*)y, (const float*)x, (const float*)tri_0,
+                                          (const float*)tri_plus, (const float*)tri_minus, space)
+                : launch_thomas<double, 0>(ctx->stream, (double*)y, (const double*)x, (const double*)tri_0,
+                                           (const double*)tri_plus, (const double*)tri_minus, space);
+  if (staged) {
+    LFR_CHECK_LAUNCH(ctx);
+    return LFR_OK;
+  }
   const size_t sb = (size_t)space->undf * prec;
   int rc = lfr_work_reserve(ctx, sb);
   if (rc) return rc;
@@ -540,9 +680,23 @@ int lfr_operator_tri_solve(lfr_ctx* ctx, int prec, void* y, const void* x, const
   LFR_REQUIRE(ctx, y != x, "lfr_operator_tri_solve: y must not alias x");
   LFR_REQUIRE(ctx, space->ndf == 2,
               "lfr_operator_tri_solve: lowest-order Wtheta/W2v only (ndf %d, expected 2)", space->ndf);
-  for (int c = 0; c < space->ncell; ++c)
-    LFR_REQUIRE(ctx, space->map_h[(size_t)c * 2 + 1] == space->map_h[(size_t)c * 2] + 1,
-                "lfr_operator_tri_solve: cell %d: the top dof does not follow the bottom dof", c + 1);
+  if (space->vstack < 0) {   // checked once per space
+    int ok = 1;
+    for (int c = 0; c < space->ncell && ok; ++c)
+      if (space->map_h[(size_t)c * 2 + 1] != space->map_h[(size_t)c * 2] + 1) ok = 0;
+    space->vstack = ok;
+  }
+  LFR_REQUIRE(ctx, space->vstack == 1,
+              "lfr_operator_tri_solve: the top dof of a cell does not follow its bottom dof");
+  const bool staged =
+      prec == 4 ? launch_thomas<float, 1>(ctx->stream, (float*)y, (const float*)x, (const float*)matrix, nullptr,
+                                          nullptr, space)
+                : launch_thomas<double, 1>(ctx->stream, (double*)y, (const double*)x, (const double*)matrix,
+                                           nullptr, nullptr, space);
+  if (staged) {
+    LFR_CHECK_LAUNCH(ctx);
+    return LFR_OK;
+  }
   const size_t sb = (size_t)space->undf * prec;
   int rc = lfr_work_reserve(ctx, sb);
   if (rc) return rc;
