@@ -10,13 +10,13 @@
 // = (sub, diag, super); one reciprocal of the pivot per row, then the products
 // inv*c and inv*(x - a*d') exactly as written there.
 //
-// Mapping: one lane per column, 64 columns per workgroup.  The band matrix
-// of the workgroup's 64 columns (contiguous in columnwise_matrix(bw, nrow,
-// ncell)) and their right-hand sides are staged through LDS in chunks of
-// CH rows with coalesced loads; each lane runs its recurrence out of LDS and
-// keeps c' and d' in registers (NROW is a compile-time constant and every
-// loop over rows is unrolled, so the arrays never spill to scratch).  The
-// solution goes back through LDS for a coalesced store.
+// Mapping: one lane per column, 64 columns per workgroup.  A lane reads its
+// column's bands (contiguous in columnwise_matrix(bw, nrow, ncell)) and
+// right-hand side straight from global memory, every load a constant offset
+// from one per-lane base, issued kPf rows ahead of the recurrence (rows are
+// unrolled, NROW a compile-time constant, and scheduling barriers keep each
+// group of loads one group ahead); c' goes to LDS, d' stays in registers,
+// and the solution goes back through LDS for a coalesced store.
 #include <cmath>
 #include <vector>
 
@@ -25,7 +25,8 @@
 namespace {
 
 constexpr int kCols = 64;   // columns (lanes) per workgroup
-constexpr int kCh = 14;     // rows per LDS stage (<= 15: 4 loads per row stay under 63 in flight)
+constexpr int kCh = 14;     // rows per LDS stage of the solution store
+constexpr int kPf = 10;     // rows of loads ahead of a column recurrence (<= 63 loads in flight)
 
 template <typename T, typename C, int NROW>
 __global__ void __launch_bounds__(kCols)
@@ -33,59 +34,56 @@ k_appinv(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
          const int32_t* __restrict__ map0, int ndf, const int32_t* __restrict__ ind,
          int ncell) {
   constexpr int NST = (NROW + kCh - 1) / kCh;
-  constexpr int PITCH_B = kCh * 3 + 1;     // odd pitch: conflict-free lane-strided reads
-  constexpr int PITCH_X = kCh + 1;
-  __shared__ T sb[kCols * PITCH_B];
+  constexpr int PITCH_X = kCh + 1;         // odd pitch: conflict-free lane-strided access
   __shared__ T sx[kCols * PITCH_X];
   __shared__ int32_t sbase[kCols];
   __shared__ int32_t sind[NROW];   // indirection_dofmap - 1, read once
+  constexpr int PC = NROW | 1;     // odd pitch of the c' rows
+  __shared__ C scp[kCols * PC];    // c' of every lane's column (registers hold d')
   const int lane = threadIdx.x;
   const int c0 = blockIdx.x * kCols;
   const int ncols = min(kCols, ncell - c0);
   if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
   for (int i = lane; i < NROW; i += kCols) sind[i] = ind[i] - 1;
-  C cp[NROW], dp[NROW];
+  C* cp = scp + lane * PC;
+  C dp[NROW];
+  // forward sweep straight from global memory: lane = column, its bands
+  // are contiguous (3 NROW values) and so are its W3 dofs, so each load is a
+  // constant offset from one per-lane base.  Loads run kPf rows ahead of the
+  // recurrence: scheduling region r holds the loads of rows [r kPf, (r+1) kPf)
+  // and the recurrence of the kPf rows before them, and no instruction
+  // crosses a region boundary, so about 4 kPf loads are in flight while the
+  // recurrence runs (a partial last workgroup clamps idle lanes to a real
+  // column).
+  {
+    const int cl = min(lane, ncols - 1);
+    const T* mb = cm + (int64_t)(c0 + cl) * 3 * NROW;
+    const T* xb = x + map0[(int64_t)(c0 + cl) * ndf];
+    T la[NROW], ld[NROW], lu[NROW], lx[NROW];
 #pragma unroll
-  for (int s = 0; s < NST; ++s) {
-    const int r0 = s * kCh;
-    const int nr = (NROW - r0) < kCh ? (NROW - r0) : kCh;
-    __syncthreads();
-    // coalesced staging: element q of column cc's chunk
-    const int nb = ncols * nr * 3;
-    for (int e = lane; e < nb; e += kCols) {
-      const int cc = e / (nr * 3);
-      const int q = e - cc * (nr * 3);
-      sb[cc * PITCH_B + q] = cm[(int64_t)(c0 + cc) * 3 * NROW + (int64_t)r0 * 3 + q];
-    }
-    const int nx = ncols * nr;
-    for (int e = lane; e < nx; e += kCols) {
-      const int cc = e / nr;
-      const int q = e - cc * nr;
-      sx[cc * PITCH_X + q] = x[sbase[cc] + sind[r0 + q]];
-    }
-    __syncthreads();
-    if (lane < ncols) {
-#pragma unroll
-      for (int q = 0; q < kCh; ++q) {
-        const int i = r0 + q;
-        if (i < NROW) {
-          const C a = (C)sb[lane * PITCH_B + 3 * q + 0];
-          const C d = (C)sb[lane * PITCH_B + 3 * q + 1];
-          const C u = (C)sb[lane * PITCH_B + 3 * q + 2];
-          const C xv = (C)sx[lane * PITCH_X + q];
-          if (i == 0) {
-            const C inv = C(1) / d;
-            cp[i] = inv * u;
-            dp[i] = inv * xv;
-          } else {
-            const C ac = a * cp[i - 1];
-            const C inv = C(1) / (d - ac);
-            if (i < NROW - 1) cp[i] = inv * u;
-            const C ad = a * dp[i - 1];
-            dp[i] = inv * (xv - ad);
-          }
+    for (int t = 0; t < NROW + kPf; ++t) {
+      if (t < NROW) {
+        la[t] = mb[3 * t + 0];
+        ld[t] = mb[3 * t + 1];
+        lu[t] = mb[3 * t + 2];
+        lx[t] = xb[ind[t] - 1];
+      }
+      if (t >= kPf) {
+        const int i = t - kPf;
+        const C a = (C)la[i], d = (C)ld[i], u = (C)lu[i], xv = (C)lx[i];
+        if (i == 0) {
+          const C inv = C(1) / d;
+          cp[i] = inv * u;
+          dp[i] = inv * xv;
+        } else {
+          const C ac = a * cp[i - 1];
+          const C inv = C(1) / (d - ac);
+          if (i < NROW - 1) cp[i] = inv * u;
+          const C ad = a * dp[i - 1];
+          dp[i] = inv * (xv - ad);
         }
       }
+      if ((t + 1) % kPf == 0) __builtin_amdgcn_sched_barrier(0);
     }
   }
   // backward substitution in registers
@@ -506,24 +504,99 @@ k_operator_tri_solve(T* __restrict__ y, const T* __restrict__ x, const T* __rest
   }
 }
 
-// The same two column solves, LDS-staged like k_appinv: one lane per
-// column, 64 columns per workgroup, the column's inputs brought in CH-row
-// chunks with coalesced loads, c' and d' in registers (NROW a compile-time
-// constant), the solution stored through LDS.  The arithmetic is the
-// recurrence of k_tri_solve / k_operator_tri_solve term for term (products
-// commute exactly), so the bits are the same; only the memory traffic moves
-// (no scratch array for c', no dependent global load per row).
-//   MODE 0: tri_solve, bands (tri_minus, tri_0, tri_plus) at dof map0 + k,
-//           NROW = nl.
-//   MODE 1: operator_tri_solve, bands from the 2x2 local matrices of the
-//           cell (k-1 and k), NROW = nl + 1.
-template <typename T, int NROW, int MODE>
+// tri_solve in the form of k_appinv: one lane per column, 64 columns per
+// workgroup, the column's bands and right-hand side loaded straight from
+// global memory (each a constant offset from one per-lane base: the W3 dofs
+// of a column are contiguous) kPf rows ahead of the recurrence, c' in LDS
+// and d' in registers (NROW a compile-time constant), the solution stored
+// through LDS.  The arithmetic is k_tri_solve's term for term (products
+// commute exactly), so the bits are the same.
+template <typename T, int NROW>
 __global__ void __launch_bounds__(kCols)
-k_thomas(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ b0, const T* __restrict__ b1,
-         const T* __restrict__ b2, const int32_t* __restrict__ map0, int ndf, int ncell, int nl) {
+k_thomas(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ t0, const T* __restrict__ tp,
+         const T* __restrict__ tm, const int32_t* __restrict__ map0, int ndf, int ncell) {
   constexpr int NST = (NROW + kCh - 1) / kCh;
-  constexpr int NV = MODE == 0 ? 3 : 4;            // staged values per row
-  constexpr int PITCH_B = kCh * NV + 1;            // odd pitch: conflict-free lane-strided reads
+  constexpr int PITCH_X = kCh + 1;                 // odd pitch: conflict-free lane-strided access
+  __shared__ T sx[kCols * PITCH_X];
+  __shared__ int32_t sbase[kCols];
+  constexpr int PC = NROW | 1;     // odd pitch of the c' rows
+  __shared__ T scp[kCols * PC];    // c' of every lane's column (registers hold d')
+  const int lane = threadIdx.x;
+  const int c0 = blockIdx.x * kCols;
+  const int ncols = min(kCols, ncell - c0);
+  if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
+  T* cp = scp + lane * PC;
+  T dp[NROW];
+  {
+    const int cl = min(lane, ncols - 1);   // idle lanes of a partial workgroup repeat a real column
+    const int64_t base = map0[(int64_t)(c0 + cl) * ndf];
+    T la[NROW], ld[NROW], lu[NROW], lx[NROW];
+#pragma unroll
+    for (int t = 0; t < NROW + kPf; ++t) {
+      if (t < NROW) {
+        la[t] = tm[base + t];
+        ld[t] = t0[base + t];
+        lu[t] = tp[base + t];
+        lx[t] = x[base + t];
+      }
+      if (t >= kPf) {
+        const int i = t - kPf;
+        const T a = la[i], dg = ld[i], u = lu[i], xv = lx[i];
+        if (i == 0) {
+          const T inv = T(1) / dg;
+          cp[i] = u * inv;
+          dp[i] = xv * inv;
+        } else {
+          const T ac = a * cp[i - 1];
+          const T inv = T(1) / (dg - ac);
+          const T ad = a * dp[i - 1];
+          if (i < NROW - 1) cp[i] = u * inv;
+          dp[i] = (xv - ad) * inv;
+        }
+      }
+      // loads of rows [r kPf, (r+1) kPf) share a scheduling region with the
+      // recurrence of the kPf rows before them
+      if ((t + 1) % kPf == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+#pragma unroll
+  for (int i = NROW - 2; i >= 0; --i) {
+    const T cd = cp[i] * dp[i + 1];
+    dp[i] = dp[i] - cd;
+  }
+#pragma unroll
+  for (int s = 0; s < NST; ++s) {
+    const int r0 = s * kCh;
+    const int nr = (NROW - r0) < kCh ? (NROW - r0) : kCh;
+    __syncthreads();
+    if (lane < ncols) {
+#pragma unroll
+      for (int q = 0; q < kCh; ++q)
+        if (r0 + q < NROW) sx[lane * PITCH_X + q] = dp[r0 + q];
+    }
+    __syncthreads();
+    const int nx = ncols * nr;
+    for (int e = lane; e < nx; e += kCols) {
+      const int cc = e / nr;
+      const int q = e - cc * nr;
+      y[(int64_t)sbase[cc] + r0 + q] = sx[cc * PITCH_X + q];
+    }
+  }
+}
+
+// operator_tri_solve with LDS staging: the 2x2 local matrices of the 64
+// columns of a workgroup come in kCh-row stages with coalesced, branch-free
+// loads (clamped addresses; slots past the last column or row hold
+// duplicates and are never read), the recurrence runs from LDS with c' and
+// d' in registers.  For these four strided planes this beats the per-lane
+// direct loads of k_thomas.  Same arithmetic as k_operator_tri_solve.
+template <typename T, int NROW>
+__global__ void __launch_bounds__(kCols)
+k_optri_staged(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ m,
+               const int32_t* __restrict__ map0, int ndf, int ncell, int nl) {
+  constexpr int NST = (NROW + kCh - 1) / kCh;
+  constexpr int NV = 4;
+  constexpr int PITCH_B = kCh * NV + 1;
   constexpr int PITCH_X = kCh + 1;
   __shared__ T sb[kCols * PITCH_B];
   __shared__ T sx[kCols * PITCH_X];
@@ -539,26 +612,26 @@ k_thomas(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ b0, c
     const int r0 = s * kCh;
     const int nr = (NROW - r0) < kCh ? (NROW - r0) : kCh;
     __syncthreads();
-    const int nx = ncols * nr;
-    for (int e = lane; e < nx; e += kCols) {
-      const int cc = e / nr;
-      const int q = e - cc * nr;
-      const int r = r0 + q;
+#pragma unroll
+    for (int it = 0; it < kCh; ++it) {
+      const int e = lane + it * kCols;
+      const int cs = e / kCh;
+      const int qs = e - cs * kCh;
+      const int cc = min(cs, ncols - 1);
+      const int r = r0 + min(qs, nr - 1);
       const int64_t d = (int64_t)sbase[cc] + r;
-      T* o = sb + cc * PITCH_B + NV * q;
-      if (MODE == 0) {
-        o[0] = b2[d];   // tri_minus
-        o[1] = b0[d];   // tri_0
-        o[2] = b1[d];   // tri_plus
-      } else {
-        // m(ik, i, j) at ik + n3 (i + 2 j), ik = cell*nl + layer
-        const int64_t ik = (int64_t)(c0 + cc) * nl + r;
-        o[0] = r < nl ? b0[ik] : T(0);                    // (k; 1,1)
-        o[1] = r >= 1 ? b0[ik - 1 + 3 * n3] : T(0);       // (k-1; 2,2)
-        o[2] = r >= 1 ? b0[ik - 1 + n3] : T(0);           // (k-1; 2,1)
-        o[3] = r < nl ? b0[ik + 2 * n3] : T(0);           // (k; 1,2)
-      }
-      sx[cc * PITCH_X + q] = x[d];
+      // m(ik, i, j) at ik + n3 (i + 2 j); rows 0 and nl read a clamped
+      // neighbour and select zero
+      const int64_t ik = (int64_t)(c0 + cc) * nl + r;
+      const int64_t ia = r < nl ? ik : ik - 1;
+      const int64_t ib = r >= 1 ? ik - 1 : ik;
+      const T va = m[ia], vb = m[ib + 3 * n3], vs = m[ib + n3], vu = m[ia + 2 * n3];
+      T* o = sb + cs * PITCH_B + NV * qs;
+      o[0] = r < nl ? va : T(0);    // (k; 1,1)
+      o[1] = r >= 1 ? vb : T(0);    // (k-1; 2,2)
+      o[2] = r >= 1 ? vs : T(0);    // (k-1; 2,1)
+      o[3] = r < nl ? vu : T(0);    // (k; 1,2)
+      sx[cs * PITCH_X + qs] = x[d];
     }
     __syncthreads();
     if (lane < ncols) {
@@ -567,16 +640,9 @@ k_thomas(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ b0, c
         const int i = r0 + q;
         if (i < NROW) {
           const T* v = sb + lane * PITCH_B + NV * q;
-          T a, dg, u;
-          if (MODE == 0) {
-            a = v[0];
-            dg = v[1];
-            u = v[2];
-          } else {
-            a = v[2];
-            dg = i == 0 ? v[0] + T(0) : (i < nl ? v[0] + v[1] : v[1]);
-            u = i < nl ? v[3] : T(0);
-          }
+          const T a = v[2];
+          const T dg = i == 0 ? v[0] + T(0) : (i < nl ? v[0] + v[1] : v[1]);
+          const T u = i < nl ? v[3] : T(0);
           const T xv = sx[lane * PITCH_X + q];
           if (i == 0) {
             const T inv = T(1) / dg;
@@ -620,19 +686,24 @@ k_thomas(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ b0, c
 
 template <typename T, int MODE>
 bool launch_thomas(hipStream_t s, T* y, const T* x, const T* b0, const T* b1, const T* b2, const lfr_space* sp) {
-  const int nrow = MODE == 0 ? sp->nlayers : sp->nlayers + 1;
   const unsigned nb = (unsigned)((sp->ncell + kCols - 1) / kCols);
-#define CASE(NR)                                                                                               \
-  case NR:                                                                                                     \
-    hipLaunchKernelGGL((k_thomas<T, NR, MODE>), dim3(nb), dim3(kCols), 0, s, y, x, b0, b1, b2, sp->map0_d, \
-                       sp->ndf, sp->ncell, sp->nlayers);                                                     \
+  if constexpr (MODE == 0) {   // tri_solve: b0 = tri_0, b1 = tri_plus, b2 = tri_minus
+    switch (sp->nlayers) {
+#define CASE(NR)                                                                                          \
+  case NR:                                                                                                \
+    hipLaunchKernelGGL((k_thomas<T, NR>), dim3(nb), dim3(kCols), 0, s, y, x, b0, b1, b2, sp->map0_d, sp->ndf, \
+                       sp->ncell);                                                                        \
     return true;
-  if constexpr (MODE == 0) {
-    switch (nrow) { CASE(10) CASE(70) default: return false; }
-  } else {
-    switch (nrow) { CASE(71) default: return false; }
-  }
+      CASE(10) CASE(70)
 #undef CASE
+      default: return false;
+    }
+  } else {                     // operator_tri_solve: b0 = the 2x2 local matrices
+    if (sp->nlayers != 70) return false;
+    hipLaunchKernelGGL((k_optri_staged<T, 71>), dim3(nb), dim3(kCols), 0, s, y, x, b0, sp->map0_d, sp->ndf,
+                       sp->ncell, sp->nlayers);
+    return true;
+  }
 }
 
 }  // namespace
