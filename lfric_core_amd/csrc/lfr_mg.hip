@@ -293,6 +293,79 @@ __global__ void __launch_bounds__(256) k_from_mg(T* __restrict__ out, const T* _
   }
 }
 
+// The same sweep for a compile-time layer count: the layers are unrolled,
+// every load (x of the layer, its four neighbours, b, the bands and the
+// Thomas factor) is issued PF layers ahead of the recurrence (scheduling
+// region g holds the loads of layers [g PF, (g+1) PF) and the sweep of the PF
+// layers before them), and d' stays in registers, so xo is written once
+// instead of written, re-read and rewritten.  Same operations in the same
+// order as k_mg_smooth.
+template <typename T, int NL, int PF = 4>
+__global__ void __launch_bounds__(256) k_mg_smooth_nl(T* __restrict__ xo, const T* __restrict__ xi,
+                                                      const T* __restrict__ b, const T* __restrict__ ba,
+                                                      const T* __restrict__ bd, const T* __restrict__ bu,
+                                                      const T* __restrict__ cp, const T* __restrict__ iv,
+                                                      const int32_t* __restrict__ nb, int ncell, int lgR, T sigma,
+                                                      T omega) {
+  const int R = 1 << lgR;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ncell * R) return;
+  const int c = (int)(t >> lgR), r = (int)(t & (R - 1));
+  const int64_t plane = (int64_t)ncell * R;
+  const int64_t o0 = (int64_t)nb[4 * c + 0] * R + r, o1 = (int64_t)nb[4 * c + 1] * R + r;
+  const int64_t o2 = (int64_t)nb[4 * c + 2] * R + r, o3 = (int64_t)nb[4 * c + 3] * R + r;
+  T lx[NL], lb[NL], la[NL], ld[NL], lu[NL], li[NL], h0[NL], h1[NL], h2[NL], h3[NL];
+  T dp[NL];
+  T dprev = 0;
+#pragma unroll
+  for (int q = 0; q < NL + PF; ++q) {
+    if (q < NL) {
+      const int k = q;
+      const T* xk = xi + (int64_t)k * plane;
+      const int64_t ik = (int64_t)k * ncell + c;
+      lx[k] = xk[t];
+      lb[k] = b[(int64_t)k * plane + t];
+      la[k] = ba[ik];
+      ld[k] = bd[ik];
+      lu[k] = bu[ik];
+      li[k] = iv[ik];
+      h0[k] = xk[o0];
+      h1[k] = xk[o1];
+      h2[k] = xk[o2];
+      h3[k] = xk[o3];
+    }
+    if (q >= PF) {
+      const int k = q - PF;
+      const T x0 = lx[k];
+      const T xm = k > 0 ? lx[k > 0 ? k - 1 : 0] : T(0);
+      const T xp = k + 1 < NL ? lx[k + 1 < NL ? k + 1 : k] : T(0);
+      const T ak = la[k];
+      T ax = ak * xm;
+      ax = ax + ld[k] * x0;
+      ax = ax + lu[k] * xp;
+      const T s4 = h0[k] + h1[k];
+      const T s5 = s4 + h2[k];
+      const T sn = s5 + h3[k];
+      const T four = (T)4 * x0;
+      ax = ax + sigma * (four - sn);
+      const T res = lb[k] - ax;
+      const T d = k == 0 ? li[k] * res : li[k] * (res - ak * dprev);
+      dp[k] = d;
+      dprev = d;
+    }
+    if ((q + 1) % PF == 0) __builtin_amdgcn_sched_barrier(0);
+  }
+  T dnext = 0;
+#pragma unroll
+  for (int k = NL - 1; k >= 0; --k) {
+    const int64_t i = (int64_t)k * plane + t;
+    T d = dp[k];
+    if (k < NL - 1) d = d - cp[(int64_t)k * ncell + c] * dnext;
+    xo[i] = xi[i] + omega * d;
+    dnext = d;
+  }
+}
+
 int lg2(int v) {
   int l = 0;
   while ((1 << l) < v) ++l;
@@ -303,6 +376,14 @@ template <typename T>
 void launch_smooth(hipStream_t s, lfr_mg* mg, lfr_mg_level& L, T* xo, const T* xi) {
   const int bs = L.block_smooth;
   const int64_t n = (int64_t)L.ncell * mg->nrhs;
+  if (mg->nl == 70) {
+    // FP64: two layers ahead (the registers of four would halve occupancy)
+    hipLaunchKernelGGL((k_mg_smooth_nl<T, 70, sizeof(T) == 4 ? 4 : 2>), dim3((unsigned)((n + bs - 1) / bs)), dim3(bs),
+                       0, s, xo, xi,
+                       (const T*)L.b_d, (const T*)L.a_d, (const T*)L.d_d, (const T*)L.u_d, (const T*)L.cp_d,
+                       (const T*)L.iv_d, L.cell_next_d, L.ncell, lg2(mg->nrhs), (T)L.sigma, (T)mg->omega);
+    return;
+  }
   hipLaunchKernelGGL(k_mg_smooth<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, xo, xi, (const T*)L.b_d,
                      (const T*)L.a_d, (const T*)L.d_d, (const T*)L.u_d, (const T*)L.cp_d, (const T*)L.iv_d,
                      L.cell_next_d, L.ncell, mg->nl, lg2(mg->nrhs), (T)L.sigma, (T)mg->omega);
