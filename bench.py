@@ -288,10 +288,13 @@ def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
     gm.apply(xg, b, x_is_zero=True)
     torch.cuda.synchronize()
     ms = timed(lambda: gm.apply(xg, b, x_is_zero=True), 5)
-    # per Arnoldi step j (1-based): apply + j dots + j axpys over W2; plus the
-    # explicit residual per restart; counted as apply + 2.5 (j+1) vector passes
+    # Arnoldi step iv (0-based) of modified Gram-Schmidt with each axpy fused
+    # to the next inner product: the apply (matrix, v_iv, w), <v_0, w> (2
+    # vector passes), iv fused steps reading w, v_j, v_j+1 and writing w (4),
+    # the last fused step with <w, w> (3) and the scaling into v_iv+1 (2);
+    # averaged over iv = 0..9 (the restart's residual and update excluded)
     step_bytes = n3 * 36 * 8 + 2 * u2 * 8
-    vec = sum(2.5 * (j + 1) for j in range(10)) * u2 * 8 / 10
+    vec = sum(2 + 4 * iv + 3 + 2 for iv in range(10)) * u2 * 8 / 10
     rec("a10_gmres10_per_arnoldi_step_fp64", ms / 20, step_bytes + vec)
     op64.packed.close()
     del M64, op64, b, xg

@@ -886,6 +886,30 @@ __global__ void k_scale_inv_dev(T* __restrict__ v, const T* __restrict__ s, int6
     v[i] = c * s[i];
 }
 
+// Modified Gram-Schmidt step fused with the next inner product: w += (T)(-a[0])
+// * vj (k_axpy_dev's arithmetic), then the first stage of <vn, w> (vn == w:
+// <w, w>) over the updated w, in the grid and order of the dot kernel
+// (red_blocks(n) blocks, interleaved, FP64 block sums), so both results
+// have the bits of the separate kernels while w is read once.
+template <typename T, bool SELF>
+__global__ void __launch_bounds__(lfr::kRedThreads)
+k_mgs_axpy_dot(T* __restrict__ w, const T* __restrict__ vj, const T* __restrict__ vn, int64_t n,
+               const double* __restrict__ a, double* __restrict__ part) {
+  __shared__ double lds[lfr::kRedThreads / 64];
+  const T c = (T)(-1.0 * a[0]);
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * lfr::kRedThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * lfr::kRedThreads) {
+    const T p = c * vj[i];
+    const T wn = w[i] + p;
+    w[i] = wn;
+    const double q = SELF ? (double)wn * (double)wn : (double)vn[i] * (double)wn;
+    s = s + q;
+  }
+  s = lfr::block_sum(s, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 __global__ void k_sqrt_slot(double* __restrict__ v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) v[0] = sqrt(v[0]);
 }
@@ -967,18 +991,32 @@ int gmres_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T
     hipLaunchKernelGGL(k_sqrt_slot, dim3(1), dim3(1), 0, ctx->stream, beta_d);
     hipLaunchKernelGGL(k_scale_inv_dev<T>, dim3(G), dim3(kThreads), 0, ctx->stream, Vv(0), s, n, beta_d);
     for (int iv = 0; iv < m; ++iv) {
-      rc = apply_op<T>(ctx, A, Vv(iv), s, (const T*)nullptr, nullptr, nullptr);
-      if (rc) return rc;
-      prec(w, s);
-      for (int j = 0; j <= iv; ++j) {
-        double* hij = H + j + (size_t)(m + 1) * iv;
-        rc = lfr_dot_queue(ctx, sizeof(T), n, Vv(j), w, part, hij);
+      // w = prec(A v_iv): with the null preconditioner the apply writes w
+      // directly (the reference's copy gives the same values)
+      if (diag) {
+        rc = apply_op<T>(ctx, A, Vv(iv), s, (const T*)nullptr, nullptr, nullptr);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_axpy_dev<T>, dim3(G), dim3(kThreads), 0, ctx->stream, w, Vv(j), n, hij, -1.0);
+        prec(w, s);
+      } else {
+        rc = apply_op<T>(ctx, A, Vv(iv), w, (const T*)nullptr, nullptr, nullptr);
+        if (rc) return rc;
       }
+      // h(0, iv) = <v_0, w>; then for each j: w -= h(j, iv) v_j fused with
+      // h(j+1, iv) = <v_{j+1}, w> (j < iv) or <w, w> (j == iv)
       double* hn = H + (iv + 1) + (size_t)(m + 1) * iv;
-      rc = lfr_dot_queue(ctx, sizeof(T), n, w, w, part, hn);
+      rc = lfr_dot_queue(ctx, sizeof(T), n, Vv(0), w, part, H + (size_t)(m + 1) * iv);
       if (rc) return rc;
+      for (int j = 0; j <= iv; ++j) {
+        const double* hij = H + j + (size_t)(m + 1) * iv;
+        double* out = j < iv ? H + (j + 1) + (size_t)(m + 1) * iv : hn;
+        if (j < iv)
+          hipLaunchKernelGGL((k_mgs_axpy_dot<T, false>), dim3(G), dim3(lfr::kRedThreads), 0, ctx->stream, w,
+                             Vv(j), Vv(j + 1), n, hij, part);
+        else
+          hipLaunchKernelGGL((k_mgs_axpy_dot<T, true>), dim3(G), dim3(lfr::kRedThreads), 0, ctx->stream, w,
+                             Vv(j), (const T*)nullptr, n, hij, part);
+        hipLaunchKernelGGL(lfr::k_reduce_final, dim3(1), dim3(lfr::kRedThreads), 0, ctx->stream, part, G, 0, out);
+      }
       hipLaunchKernelGGL(k_sqrt_slot, dim3(1), dim3(1), 0, ctx->stream, hn);
       if (iv < m - 1)
         hipLaunchKernelGGL(k_scale_inv_dev<T>, dim3(G), dim3(kThreads), 0, ctx->stream, Vv(iv + 1), w, n, hn);
