@@ -195,9 +195,10 @@ def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
 
     # Most of these working sets (tens to a few hundred MB) fit the 256 MB
     # last-level cache, so back-to-back calls would be served from it: each
-    # timed call is preceded by a 1 GiB write that evicts it, and only the
-    # call itself is bracketed by events.
-    flush = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+    # timed call is preceded by a 1 GiB read that evicts it (a read leaves
+    # clean lines; a write would leave dirty ones to be written back inside
+    # the timed call), and only the call itself is bracketed by events.
+    flush = torch.ones(1 << 28, dtype=torch.float32, device=dev)
 
     def timed(fn, n=steps):
         for _ in range(3):
@@ -205,13 +206,24 @@ def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
         cur = torch.cuda.current_stream()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
         for e0, e1 in ev:
-            flush.zero_()
+            flush.sum()
             e0.record(cur)
             fn()
             e1.record(cur)
         torch.cuda.synchronize()
         return sum(e0.elapsed_time(e1) for e0, e1 in ev) / n
 
+    # a1 under the same cold-cache protocol (the headline apply, 652 MB), to
+    # set the protocol's own cost against its back-to-back time
+    M32 = torch.from_numpy(cfg.mass.astype(np.float32)).to(dev)
+    op32 = api.LMAOperator(M32, w2, w2, cfg.colouring)
+    pk = api.PackedOperator(op32)
+    xw = torch.from_numpy(cfg.b.astype(np.float32)).to(dev)
+    yw = torch.empty_like(xw)
+    rec("a1_matrix_vector_packed_fp32_cold", timed(lambda: pk.apply(yw, xw, overwrite=True)),
+        apply_bytes(n3, 6, 6, u2, u2, nc, 4))
+    pk.close()
+    del M32, op32, xw, yw
     # a2 / a3 / a4: the divergence operator (local_stencil(ncell_3d, 1, 6))
     for tag, dt, s in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
         D = torch.from_numpy(cfg.div).to(dev).to(dt).contiguous()
