@@ -227,9 +227,9 @@ extern "C" int lfr_compute_mass_matrix_w0(lfr_ctx* ctx, int nq, const void* chi1
 // compute_mass_matrix_w_scalar_code
 // (sci_compute_mass_matrix_kernel_w_scalar_mod.F90:150-174) and
 // compute_mass_matrix_w2_code (sci_compute_mass_matrix_kernel_w2_mod.F90:131-162).
-// Symmetric forms (the masses) are computed for df >= df2 and mirrored.
-// One thread per (row, cell-layer), cell-layer fastest; the transformed
-// column vectors of a point are formed once per thread.
+// Symmetric forms (the masses) are computed for df >= df2 and mirrored
+// (for the vector maps the dot product has the same bits either way:
+// products commute exactly).
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -243,119 +243,146 @@ struct FormQp {
   double wh, wv;
 };
 
+// Kernel: a workgroup holds CPB = 256 / NR cell-layers, thread (li, a) =
+// (cell-layer, row), li fastest, so output stores are coalesced.  Per
+// quadrature point the row-0 thread of a cell-layer forms J, detJ and the
+// map (J or Jt) into LDS; then every thread maps its share of the column
+// vectors (columns a, a + NR, ...) into LDS and its own row vector into
+// registers; then each thread sums its row.  The Jacobian is formed once
+// per cell-layer and each mapped column vector once, not once per row (71 ms
+// -> 45 ms for all ten outputs at C96L70).  LDS is double-buffered by point
+// parity.
 template <int NR, int NC, int CL, int CR, int MAP, bool SYM>
 __global__ void __launch_bounds__(256)
 k_form(double* __restrict__ out, const double* __restrict__ chi1, const double* __restrict__ chi2,
-       const double* __restrict__ chi3, const int32_t* __restrict__ map0,
-       const FormQp<NR, NC, CL, CR>* __restrict__ tab, int nqp, int ncell, int nl) {
-  static_assert(!SYM || (NR == NC && CL == CR), "a symmetric form has one table");
+              const double* __restrict__ chi3, const int32_t* __restrict__ map0,
+              const FormQp<NR, NC, CL, CR>* __restrict__ tab, int nqp, int ncell, int nl) {
   constexpr bool kVec = MAP == kMapCov || MAP == kMapContra;
-  static_assert(!kVec || (CL == 3 && CR == 3), "vector maps take 3-component tables");
-  static_assert(kVec || (CL == 1 && CR == 1), "scalar maps take 1-component tables");
   constexpr int KC = kVec ? 3 : 1;
+  constexpr int CPB = 256 / NR;
+  constexpr int NCV = kVec ? NC : 1;            // LDS column vectors (vector maps only)
+  __shared__ double sm[2][CPB][10];             // map (9) and detJ
+  __shared__ double srv[2][CPB][NCV][KC];
   const int64_t n3 = (int64_t)ncell * nl;
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= NR * n3) return;
-  const int a = (int)(t / n3);
-  const int64_t ik = t - (int64_t)a * n3;
-  const int c = (int)(ik / nl);
-  const int k = (int)(ik - (int64_t)c * nl);
+  const int li = threadIdx.x % CPB;
+  const int a = threadIdx.x / CPB;              // row; a == NR: idle thread
+  const int64_t ik = (int64_t)blockIdx.x * CPB + li;
+  const bool valid = a < NR && ik < n3;
+  const int64_t ikc = ik < n3 ? ik : n3 - 1;    // idle lanes work on a real cell-layer
   double x1[8], x2[8], x3[8];
+  if (a == 0) {
+    const int c = (int)(ikc / nl);
+    const int k = (int)(ikc - (int64_t)c * nl);
 #pragma unroll
-  for (int df = 0; df < 8; ++df) {
-    const int64_t d = (int64_t)map0[(int64_t)c * 8 + df] + k;
-    x1[df] = chi1[d];
-    x2[df] = chi2[d];
-    x3[df] = chi3[d];
+    for (int df = 0; df < 8; ++df) {
+      const int64_t d = (int64_t)map0[(int64_t)c * 8 + df] + k;
+      x1[df] = chi1[d];
+      x2[df] = chi2[d];
+      x3[df] = chi3[d];
+    }
   }
   double acc[NC];
 #pragma unroll
   for (int b = 0; b < NC; ++b) acc[b] = 0.0;
+  const int ar = a < NR ? a : 0;
   for (int q = 0; q < nqp; ++q) {
     const FormQp<NR, NC, CL, CR>& T = tab[q];
-    double j[3][3];
+    const int pb = q & 1;
+    if (MAP != kMapRef && a == 0) {   // the divergence form has no metric term
+      double j[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int dir = 0; dir < 3; ++dir) j[i][dir] = 0.0;
+#pragma unroll
+      for (int df = 0; df < 8; ++df)
+#pragma unroll
+        for (int dir = 0; dir < 3; ++dir) {
+          const double gd = T.g[dir][df];
+          j[0][dir] = j[0][dir] + x1[df] * gd;
+          j[1][dir] = j[1][dir] + x2[df] * gd;
+          j[2][dir] = j[2][dir] + x3[df] * gd;
+        }
+      const double dj = j[0][0] * (j[1][1] * j[2][2] - j[1][2] * j[2][1]) -
+                        j[0][1] * (j[1][0] * j[2][2] - j[1][2] * j[2][0]) +
+                        j[0][2] * (j[1][0] * j[2][1] - j[1][1] * j[2][0]);
+      double m[3][3];
+      if (MAP == kMapContra) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) m[i][d] = j[i][d];
+      } else if (MAP == kMapCov) {
+        const double idj = 1.0 / dj;
+        double inv[3][3];
+        inv[0][0] = (j[1][1] * j[2][2] - j[1][2] * j[2][1]) * idj;
+        inv[0][1] = -(j[0][1] * j[2][2] - j[0][2] * j[2][1]) * idj;
+        inv[0][2] = (j[0][1] * j[1][2] - j[0][2] * j[1][1]) * idj;
+        inv[1][0] = -(j[1][0] * j[2][2] - j[1][2] * j[2][0]) * idj;
+        inv[1][1] = (j[0][0] * j[2][2] - j[0][2] * j[2][0]) * idj;
+        inv[1][2] = -(j[0][0] * j[1][2] - j[0][2] * j[1][0]) * idj;
+        inv[2][0] = (j[1][0] * j[2][1] - j[1][1] * j[2][0]) * idj;
+        inv[2][1] = -(j[0][0] * j[2][1] - j[0][1] * j[2][0]) * idj;
+        inv[2][2] = (j[0][0] * j[1][1] - j[0][1] * j[1][0]) * idj;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) m[i][d] = inv[d][i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) m[i][d] = 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) sm[pb][li][3 * i + d] = m[i][d];
+      sm[pb][li][9] = dj;
+    }
+    __syncthreads();
+    double m[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int dir = 0; dir < 3; ++dir) j[i][dir] = 0.0;
+      for (int d = 0; d < 3; ++d) m[i][d] = sm[pb][li][3 * i + d];
+    const double dj = sm[pb][li][9];
+    double lv[KC];
+    if (kVec) {
+      // this thread's share of the column vectors, and its row vector
+      for (int b = ar; b < NC && a < NR; b += NR) {
 #pragma unroll
-    for (int df = 0; df < 8; ++df)
+        for (int i = 0; i < KC; ++i) {
+          double sacc = 0.0;
 #pragma unroll
-      for (int dir = 0; dir < 3; ++dir) {
-        const double gd = T.g[dir][df];
-        j[0][dir] = j[0][dir] + x1[df] * gd;
-        j[1][dir] = j[1][dir] + x2[df] * gd;
-        j[2][dir] = j[2][dir] + x3[df] * gd;
-      }
-    const double dj = j[0][0] * (j[1][1] * j[2][2] - j[1][2] * j[2][1]) -
-                      j[0][1] * (j[1][0] * j[2][2] - j[1][2] * j[2][0]) +
-                      j[0][2] * (j[1][0] * j[2][1] - j[1][1] * j[2][0]);
-    const double wt = T.wh * T.wv;
-    // m[i][d]: the matrix applied to basis vectors, (m v)_i = sum_d m[i][d] v_d
-    double m[3][3];
-    if (MAP == kMapContra) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) m[i][d] = j[i][d];
-    } else if (MAP == kMapCov) {
-      const double idj = 1.0 / dj;
-      double inv[3][3];   // the reference's jac_inv(r, c) at inv[r][c]
-      inv[0][0] = (j[1][1] * j[2][2] - j[1][2] * j[2][1]) * idj;
-      inv[0][1] = -(j[0][1] * j[2][2] - j[0][2] * j[2][1]) * idj;
-      inv[0][2] = (j[0][1] * j[1][2] - j[0][2] * j[1][1]) * idj;
-      inv[1][0] = -(j[1][0] * j[2][2] - j[1][2] * j[2][0]) * idj;
-      inv[1][1] = (j[0][0] * j[2][2] - j[0][2] * j[2][0]) * idj;
-      inv[1][2] = -(j[0][0] * j[1][2] - j[0][2] * j[1][0]) * idj;
-      inv[2][0] = (j[1][0] * j[2][1] - j[1][1] * j[2][0]) * idj;
-      inv[2][1] = -(j[0][0] * j[2][1] - j[0][1] * j[2][0]) * idj;
-      inv[2][2] = (j[0][0] * j[1][1] - j[0][1] * j[1][0]) * idj;
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) m[i][d] = inv[d][i];   // jac_t = transpose(jac_inv)
-    }
-    // transformed column vectors and this thread's row vector
-    double rv[NC][KC], lv[KC];
-#pragma unroll
-    for (int b = 0; b < NC; ++b) {
-#pragma unroll
-      for (int i = 0; i < KC; ++i) {
-        if (kVec) {
-          double s = 0.0;
-#pragma unroll
-          for (int d = 0; d < 3; ++d) s = s + m[i][d] * T.r[d][b];
-          rv[b][i] = s;
-        } else {
-          rv[b][i] = T.r[0][b];
+          for (int d = 0; d < 3; ++d) sacc = sacc + m[i][d] * T.r[d][b];
+          srv[pb][li][b < NCV ? b : 0][i] = sacc;
         }
       }
-    }
 #pragma unroll
-    for (int i = 0; i < KC; ++i) {
-      if (kVec) {
-        double s = 0.0;
+      for (int i = 0; i < KC; ++i) {
+        double sacc = 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) s = s + m[i][d] * T.l[d][a];
-        lv[i] = s;
-      } else {
-        lv[i] = T.l[0][a];
+        for (int d = 0; d < 3; ++d) sacc = sacc + m[i][d] * T.l[d][ar];
+        lv[i] = sacc;
       }
+      __syncthreads();
+    } else {
+      lv[0] = T.l[0][ar];
     }
+    const double wt = T.wh * T.wv;
 #pragma unroll
     for (int b = 0; b < NC; ++b) {
       double v;
       if (kVec) {
-        // the reference's dot of the row-df and column-df2 vectors; products
-        // commute exactly, so a mirrored entry needs no swap
         double dot = 0.0;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) dot = dot + lv[i] * rv[b][i];
+        for (int i = 0; i < 3; ++i) dot = dot + lv[i] * srv[pb][li][b < NCV ? b : 0][i < KC ? i : 0];
         v = MAP == kMapCov ? (wt * dot) * dj : (wt * dot) / dj;
       } else {
-        double l0 = lv[0], r0 = rv[b][0];
-        if (SYM && b > a) {   // mirrored entry: computed as (df, df2) = (b, a)
-          l0 = rv[b][0];
+        double l0 = lv[0], r0 = T.r[0][b];
+        if (SYM && b > ar) {   // mirrored entry: computed as (df, df2) = (b, a)
+          l0 = T.r[0][b];
           r0 = lv[0];
         }
         v = MAP == kMapDj ? ((wt * l0) * r0) * dj : (wt * l0) * r0;
@@ -363,8 +390,10 @@ k_form(double* __restrict__ out, const double* __restrict__ chi1, const double* 
       acc[b] = acc[b] + v;
     }
   }
+  if (valid) {
 #pragma unroll
-  for (int b = 0; b < NC; ++b) out[ik + n3 * (a + (int64_t)NR * b)] = acc[b];
+    for (int b = 0; b < NC; ++b) out[ik + n3 * (a + (int64_t)NR * b)] = acc[b];
+  }
 }
 
 // Fortran-order host table (ncomp, ndf, nqp_h, nqp_v) element
@@ -395,9 +424,10 @@ int form_assemble(lfr_ctx* ctx, int nqp_h, int nqp_v, const double* L, const dou
   int rc = lfr_work_reserve(ctx, tb);
   if (rc) return rc;
   LFR_HIP(ctx, hipMemcpyAsync(ctx->work, tab.data(), tb, hipMemcpyHostToDevice, ctx->stream));
-  const int64_t n = (int64_t)NR * sp->ncell * sp->nlayers;
-  hipLaunchKernelGGL((k_form<NR, NC, CL, CR, MAP, SYM>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     ctx->stream, (double*)out, (const double*)chi1, (const double*)chi2, (const double*)chi3,
+  const int64_t n3 = (int64_t)sp->ncell * sp->nlayers;
+  constexpr int CPB = 256 / NR;
+  hipLaunchKernelGGL((k_form<NR, NC, CL, CR, MAP, SYM>), dim3((unsigned)((n3 + CPB - 1) / CPB)), dim3(256),
+                     0, ctx->stream, (double*)out, (const double*)chi1, (const double*)chi2, (const double*)chi3,
                      sp->map0_d, (const Q*)ctx->work, (int)tab.size(), sp->ncell, sp->nlayers);
   LFR_CHECK_LAUNCH(ctx);
   // the table lives in ctx->work and comes from this call's host memory:
