@@ -173,8 +173,9 @@ def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
     dg_matrix_vector / dg_inc_matrix_vector (div, W3 <- W2), the transposed
     apply onto W2 (D^T p), columnwise_op_appinv on W3 columns, CMA assembly
     and apply on Wtheta, tri_solve (W3) and operator_tri_solve (Wtheta),
-    GMRES(10) on the W2 mass matrix (per Arnoldi step), and the vector
-    kernels (axpy, FP64-accumulated dot of FP32 fields)."""
+    the W3 intergrid restriction and prolongation (C192 <-> C96), GMRES(10)
+    on the W2 mass matrix (per Arnoldi step), and the vector kernels (axpy,
+    FP64-accumulated dot of FP32 fields)."""
     from lfric_core_amd import api
     from lfric_core_amd import function_space as fsm
     from lfric_core_amd.rng import Stream
@@ -261,7 +262,20 @@ def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
     zb3 = torch.empty_like(xb3)
     rec("a5_columnwise_op_appinv_fp32_c192", timed(lambda: api.columnwise_op_appinv(zb3, xb3, cmab)),
         ncb * nl * (3 + 2) * 4 + nl * 4 + ncb * 4)
-    del cmab, Ab, xb3, zb3, w3b
+    del cmab, Ab, xb3, zb3
+    # f3: intergrid transfers of the W3 field between C192 and C96 (2x2 children)
+    from lfric_core_amd.mesh import intergrid_map
+    ig = api.Intergrid(w3, w3b, intergrid_map(96, 192))
+    xf3 = torch.from_numpy(st.uniform(w3b_fs.undf).astype(np.float32)).to(dev)
+    xc3 = torch.empty(u3, dtype=torch.float32, device=dev)
+    rec("f3_restrict_scalar_w3_c192_to_c96_fp32",
+        timed(lambda: api.restrict_scalar(xc3, xf3, ig)),
+        w3b_fs.undf * 4 + u3 * 4 + nc * 4 * 4 + nc * 4)
+    rec("f3_prolong_scalar_w3_c96_to_c192_fp32",
+        timed(lambda: api.prolong_scalar(xf3, xc3, ig)),
+        2 * w3b_fs.undf * 4 + u3 * 4 + nc * 4 * 4 + nc * 4)
+    ig.close()
+    del xf3, xc3, w3b
     # f1: CMA assembly and apply on Wtheta (2x2 local matrices, bandwidth 3)
     wt_fs = fsm.build_dofmap(cfg.mesh, "WTHETA", nl)
     wt = api.Space(ctx, wt_fs)

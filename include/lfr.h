@@ -220,6 +220,26 @@ int lfr_compute_derham_matrices(lfr_ctx* ctx, const lfr_derham_tables* tables, c
                                 void* mm1, void* mm2, void* mm2b, void* mm3, void* mmt, void* grad, void* curl,
                                 void* div, void* broken_div);
 
+/* Intergrid transfers of lowest-order scalar fields (W3: ndf 1, Wtheta:
+ * ndf 2) between a coarse and a fine mesh of a multigrid hierarchy.
+ * lfr_intergrid_create checks and uploads the intergrid map once: cell_map_h
+ * (host, 1-based fine cells) is cell_map(nfx, nfy, ncell_coarse) in Fortran
+ * order, the mesh-to-mesh connectivity of the two meshes.  Over every coarse
+ * cell:
+ *   lfr_restrict_scalar: restrict_scalar_unweighted_code
+ *     (components/science/source/kernel/inter_mesh/sci_restrict_scalar_unweighted_kernel_mod.F90:71-125),
+ *     coarse = the mean of the children (each child times 1/(nfx nfy),
+ *     summed y outer, x inner); coarse_d is overwritten on the coarse columns.
+ *   lfr_prolong_scalar: prolong_scalar_unweighted_code
+ *     (components/science/source/kernel/inter_mesh/sci_prolong_scalar_unweighted_kernel_mod.F90),
+ *     fine += coarse on every child. */
+typedef struct lfr_intergrid lfr_intergrid;
+int lfr_intergrid_create(lfr_ctx* ctx, const lfr_space* coarse_space, const lfr_space* fine_space,
+                         const int32_t* cell_map_h, int nfx, int nfy, lfr_intergrid** intergrid);
+int lfr_intergrid_destroy(lfr_intergrid* intergrid);
+int lfr_restrict_scalar(lfr_ctx* ctx, int prec, void* coarse_d, const void* fine_d, const lfr_intergrid* intergrid);
+int lfr_prolong_scalar(lfr_ctx* ctx, int prec, void* fine_d, const void* coarse_d, const lfr_intergrid* intergrid);
+
 /* Packed operator for repeated applies of one LMA operator (the
  * operator_type whose local_stencil, INF/operator/operator_mod.t90:31-122,
  * is assembled once and applied many times, as the mass matrices in the

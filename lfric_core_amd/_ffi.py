@@ -54,6 +54,10 @@ SIGNATURES = {
     "lfr_compute_mass_matrix_w0": [_P, _I, _P, _P, _P, _P, _P, _P],
     "lfr_compute_mass_matrix": [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "lfr_compute_derham_matrices": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lfr_intergrid_create": [_P, _P, _P, _P, _I, _I, C.POINTER(_P)],
+    "lfr_intergrid_destroy": [_P],
+    "lfr_restrict_scalar": [_P, _I, _P, _P, _P],
+    "lfr_prolong_scalar": [_P, _I, _P, _P, _P],
     "lfr_tri_solve": [_P, _I, _P, _P, _P, _P, _P, _P],
     "lfr_operator_tri_solve": [_P, _I, _P, _P, _P, _P],
     "lfr_lma_pack": [_P, _I, _P, _P, _P, C.POINTER(_P)],

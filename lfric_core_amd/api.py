@@ -595,6 +595,58 @@ def compute_mass_matrix(kind: int, basis, diff_basis_chi, wqp_h, wqp_v, chi: tor
     return mass
 
 
+class Intergrid:
+    """The intergrid map of a coarse and a fine lowest-order scalar space
+    (lfr_intergrid_create): cell_map (ncell_coarse, nfy, nfx), 1-based fine
+    cells, as the reference's mesh-to-mesh connectivity."""
+
+    def __init__(self, coarse_space: Space, fine_space: Space, cell_map):
+        cm = np.ascontiguousarray(cell_map, dtype=np.int32)
+        if cm.ndim != 3 or cm.shape[0] != coarse_space.fs.mesh.ncells:
+            raise LfrError(1, f"cell_map must be (ncell_coarse, nfy, nfx), got {cm.shape}")
+        self.coarse_space, self.fine_space = coarse_space, fine_space
+        h = _P()
+        ctx = coarse_space.ctx
+        check(lib().lfr_intergrid_create(ctx.h, coarse_space.h, fine_space.h, cm.ctypes.data_as(_P),
+                                         int(cm.shape[2]), int(cm.shape[1]), C.byref(h)), ctx.h)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lfr_intergrid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _transfer(name, out, inp, ig: Intergrid):
+    if out.dtype != inp.dtype:
+        raise LfrError(1, "the two fields must have one dtype")
+    fo, fi = (ig.coarse_space, ig.fine_space) if name == "restrict" else (ig.fine_space, ig.coarse_space)
+    _field(out, fo, "output", inp.dtype)
+    _field(inp, fi, "input", inp.dtype)
+    ctx = ig.coarse_space.ctx
+    ctx.sync_stream()
+    check(getattr(lib(), f"lfr_{name}_scalar")(ctx.h, _prec(out), _P(out.data_ptr()), _P(inp.data_ptr()), ig.h),
+          ctx.h)
+    return out
+
+
+def restrict_scalar(coarse: torch.Tensor, fine: torch.Tensor, ig: Intergrid):
+    """restrict_scalar_unweighted over every coarse cell (W3 or Wtheta, lowest
+    order): coarse = the mean of the children."""
+    return _transfer("restrict", coarse, fine, ig)
+
+
+def prolong_scalar(fine: torch.Tensor, coarse: torch.Tensor, ig: Intergrid):
+    """prolong_scalar_unweighted: fine += coarse on every child."""
+    return _transfer("prolong", fine, coarse, ig)
+
+
 _DERHAM_TABLES = ("diff_basis_chi", "basis_w0", "diff_basis_w0", "basis_w1", "diff_basis_w1", "basis_w2",
                   "diff_basis_w2", "basis_w2b", "diff_basis_w2b", "basis_w3", "basis_wt")
 # (output, ndf_to, ndf_from, tables it needs)

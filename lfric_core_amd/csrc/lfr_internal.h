@@ -36,7 +36,7 @@ struct lfr_space {
   int32_t* map0_d = nullptr;        // map(ndf, ncell) converted to 0-based, device
   std::vector<int32_t> map_h;       // 1-based host copy (plan construction)
   std::vector<int32_t> boundary_h;  // boundary_value(ndf, 2) for enforce_bc (empty: not set)
-  mutable int vstack = -1;          // ndf 2: top dof = bottom dof + 1 in every cell (-1: not checked)
+  mutable int vstack = -1;          // every cell's dof j = its dof 1 + j - 1 (one column; -1: not checked)
 };
 
 // Queue enforce_bc over all cells of `sp` on `field` (sp must have its

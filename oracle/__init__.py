@@ -276,6 +276,43 @@ def operator_tri_solve(y, x, matrix, dofmap, nlayers, use_ref=False):
     return y
 
 
+def _transfer(name, out, inp, map_coarse, map_fine, cell_map, nlayers, use_ref):
+    mc = np.ascontiguousarray(map_coarse, dtype=np.int32)
+    mf = np.ascontiguousarray(map_fine, dtype=np.int32)
+    cm = np.ascontiguousarray(cell_map, dtype=np.int32)       # (ncell_coarse, nfy, nfx)
+    ncc, ndf = mc.shape
+    ncf = mf.shape[0]
+    nfy, nfx = cm.shape[1:]
+    src = np.ascontiguousarray(inp)
+    if use_ref:
+        sfx = "r_single" if out.dtype == np.float32 else "r_double"
+        undf_c = out.size if name == "restrict" else src.size
+        undf_f = src.size if name == "restrict" else out.size
+        getattr(ref(), f"ref_invoke_{name}_scalar_{sfx}")(
+            _I(ncc), _I(ncf), _I(nlayers), _I(ndf), out.ctypes.data_as(C.c_void_p),
+            src.ctypes.data_as(C.c_void_p), _I(undf_c), mc.ctypes.data_as(C.c_void_p), _I(undf_f),
+            mf.ctypes.data_as(C.c_void_p), cm.ctypes.data_as(C.c_void_p), _I(nfx), _I(nfy))
+    else:
+        getattr(lib(), f"orc_{name}_scalar_{_suf(out.dtype)}")(
+            _I(ncc), _I(nlayers), _I(ndf), out.ctypes.data_as(C.c_void_p), src.ctypes.data_as(C.c_void_p),
+            mc.ctypes.data_as(C.c_void_p), mf.ctypes.data_as(C.c_void_p), cm.ctypes.data_as(C.c_void_p),
+            _I(nfx), _I(nfy))
+    return out
+
+
+def restrict_scalar(coarse, fine, map_coarse, map_fine, cell_map, nlayers, use_ref=False):
+    """restrict_scalar_unweighted_code over every coarse cell of a lowest-order
+    W3 (ndf 1) or Wtheta (ndf 2) space: coarse = the mean of the children.
+    map_* are the (ncell, ndf) 1-based dofmaps, cell_map the
+    (ncell_coarse, nfy, nfx) 1-based intergrid map (Fortran cell_map(nfx, nfy))."""
+    return _transfer("restrict", coarse, fine, map_coarse, map_fine, cell_map, nlayers, use_ref)
+
+
+def prolong_scalar(fine, coarse, map_coarse, map_fine, cell_map, nlayers, use_ref=False):
+    """prolong_scalar_unweighted_code: fine += coarse on every child (injection)."""
+    return _transfer("prolong", fine, coarse, map_coarse, map_fine, cell_map, nlayers, use_ref)
+
+
 def dot(x, y, nthreads=None) -> float:
     nt = nthreads_default() if nthreads is None else nthreads
     fn = getattr(lib(), f"orc_dot_{_suf(x.dtype)}")

@@ -421,3 +421,51 @@ int orc_omp_max_threads(void) {
   return 1;
 #endif
 }
+
+/* Intergrid transfers of lowest-order scalar fields (W3: ndf 1, levels
+ * 0..nl-1; Wtheta: ndf 2, levels 0..nl), one coarse cell at a time:
+ *   restrict_scalar_unweighted_code
+ *   (components/science/source/kernel/inter_mesh/sci_restrict_scalar_unweighted_kernel_mod.F90:71-125):
+ *     coarse(k) = sum over y outer, x inner of fine(child(x, y), k) * denom,
+ *     denom = 1 / (nfx nfy), from zero;
+ *   prolong_scalar_unweighted_code
+ *   (.../sci_prolong_scalar_unweighted_kernel_mod.F90): fine(child, k) += coarse(k).
+ * cell_map(nfx, nfy, ncell_coarse) holds 1-based fine cells. */
+#define DEF_TRANSFER(T, SUF)                                                                     \
+  void orc_restrict_scalar_##SUF(int ncell_coarse, int nl, int ndf, T* coarse, const T* fine,   \
+                                 const int* map_coarse, const int* map_fine, const int* cell_map, \
+                                 int nfx, int nfy) {                                              \
+    const T denom = (T)1 / (T)(nfx * nfy);                                                       \
+    const int top = nl - 2 + ndf;                                                                \
+    for (int c = 0; c < ncell_coarse; ++c) {                                                     \
+      const int* cm = cell_map + (int64_t)c * nfx * nfy;                                         \
+      const int64_t cb = (int64_t)map_coarse[(int64_t)c * ndf] - 1;                              \
+      for (int k = 0; k <= top; ++k) {                                                           \
+        T v = (T)0;                                                                              \
+        for (int y = 0; y < nfy; ++y)                                                            \
+          for (int x = 0; x < nfx; ++x) {                                                        \
+            const int f = cm[x + nfx * y] - 1;                                                   \
+            const T p = fine[(int64_t)map_fine[(int64_t)f * ndf] - 1 + k] * denom;              \
+            v = v + p;                                                                           \
+          }                                                                                      \
+        coarse[cb + k] = v;                                                                      \
+      }                                                                                          \
+    }                                                                                            \
+  }                                                                                              \
+  void orc_prolong_scalar_##SUF(int ncell_coarse, int nl, int ndf, T* fine, const T* coarse,    \
+                                const int* map_coarse, const int* map_fine, const int* cell_map, \
+                                int nfx, int nfy) {                                               \
+    const int top = nl - 2 + ndf;                                                                \
+    for (int c = 0; c < ncell_coarse; ++c) {                                                     \
+      const int* cm = cell_map + (int64_t)c * nfx * nfy;                                         \
+      const int64_t cb = (int64_t)map_coarse[(int64_t)c * ndf] - 1;                              \
+      for (int y = 0; y < nfy; ++y)                                                              \
+        for (int x = 0; x < nfx; ++x) {                                                          \
+          const int f = cm[x + nfx * y] - 1;                                                     \
+          const int64_t fb = (int64_t)map_fine[(int64_t)f * ndf] - 1;                            \
+          for (int k = 0; k <= top; ++k) fine[fb + k] = fine[fb + k] + coarse[cb + k];          \
+        }                                                                                        \
+    }                                                                                            \
+  }
+DEF_TRANSFER(float, r32)
+DEF_TRANSFER(double, r64)

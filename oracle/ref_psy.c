@@ -183,3 +183,36 @@ void ref_invoke_operator_tri_solve_r64(int ncell, int nlayers, double* y, const 
                                                                  &undf, dofmap + (int64_t)c * ndf);
   }
 }
+
+/* Intergrid transfers of lowest-order scalar (W3 / Wtheta) fields, one call
+ * per coarse cell with its slice of the intergrid map
+ * cell_map(ncell_fine_per_coarse_x, ncell_fine_per_coarse_y, ncell_coarse). */
+#define REF_TRANSFER(T, SUF)                                                                            \
+  extern void FK(sci_restrict_scalar_unweighted_kernel_mod, restrict_scalar_unweighted_code_##SUF)(     \
+      const int* nlayers, const int* cell_map, const int* nfx, const int* nfy, const int* ncell_fine,   \
+      T* coarse, const T* fine, const int* undf_coarse, const int* map_coarse, const int* ndf,          \
+      const int* undf_fine, const int* map_fine);                                                        \
+  extern void FK(sci_prolong_scalar_unweighted_kernel_mod, prolong_scalar_unweighted_code_##SUF)(       \
+      const int* nlayers, const int* cell_map, const int* nfx, const int* nfy, const int* ncell_fine,   \
+      T* fine, const T* coarse, const int* ndf, const int* undf_fine, const int* map_fine,              \
+      const int* undf_coarse, const int* map_coarse);                                                    \
+  void ref_invoke_restrict_scalar_##SUF(int ncell_coarse, int ncell_fine, int nlayers, int ndf, T* coarse, \
+                                        const T* fine, int undf_coarse, const int* map_coarse,          \
+                                        int undf_fine, const int* map_fine, const int* cell_map, int nfx, \
+                                        int nfy) {                                                        \
+    for (int c = 0; c < ncell_coarse; ++c)                                                              \
+      FK(sci_restrict_scalar_unweighted_kernel_mod, restrict_scalar_unweighted_code_##SUF)(            \
+          &nlayers, cell_map + (int64_t)c * nfx * nfy, &nfx, &nfy, &ncell_fine, coarse, fine,          \
+          &undf_coarse, map_coarse + (int64_t)c * ndf, &ndf, &undf_fine, map_fine);                     \
+  }                                                                                                     \
+  void ref_invoke_prolong_scalar_##SUF(int ncell_coarse, int ncell_fine, int nlayers, int ndf, T* fine,  \
+                                       const T* coarse, int undf_coarse, const int* map_coarse,          \
+                                       int undf_fine, const int* map_fine, const int* cell_map, int nfx,  \
+                                       int nfy) {                                                         \
+    for (int c = 0; c < ncell_coarse; ++c)                                                              \
+      FK(sci_prolong_scalar_unweighted_kernel_mod, prolong_scalar_unweighted_code_##SUF)(              \
+          &nlayers, cell_map + (int64_t)c * nfx * nfy, &nfx, &nfy, &ncell_fine, fine, coarse, &ndf,    \
+          &undf_fine, map_fine, &undf_coarse, map_coarse + (int64_t)c * ndf);                           \
+  }
+REF_TRANSFER(float, r_single)
+REF_TRANSFER(double, r_double)

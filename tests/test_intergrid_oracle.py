@@ -1,0 +1,65 @@
+"""Intergrid transfers of lowest-order scalar fields in the oracle, bit for
+bit against the reference kernels compiled by flang
+(sci_restrict_scalar_unweighted_kernel_mod.F90, sci_prolong_scalar_unweighted_kernel_mod.F90),
+with the reference's own intergrid maps (mesh_C32_MG.nc: C8 -> C16 2x2,
+and the C4 -> C12 3x3 map of the C12/C4 pair), and against what the
+operation fixes: restriction of a layer-constant field is that constant,
+prolongation then restriction adds the coarse field exactly once."""
+
+import numpy as np
+import pytest
+
+import oracle
+from lfric_core_amd.function_space import build_dofmap
+from lfric_core_amd.mesh import cubed_sphere
+from lfric_core_amd.rng import Stream
+
+from conftest import GOLDEN
+
+CASES = [("C8_C16_map", 8, 16), ("C4_C12_map", 4, 12)]
+
+
+def _spaces(space, nc, nf, nl):
+    return build_dofmap(cubed_sphere(nc), space, nl), build_dofmap(cubed_sphere(nf), space, nl)
+
+
+@pytest.mark.skipif(oracle.ref() is None, reason="flang-built reference kernels not present")
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("space", ["W3", "WTHETA"])
+@pytest.mark.parametrize("key,nc,nf", CASES)
+def test_transfers_bitexact_vs_reference_kernels(key, nc, nf, space, dtype):
+    cm = np.load(GOLDEN / "meshes.npz")[key]
+    fc, ff = _spaces(space, nc, nf, 5)
+    st = Stream(61)
+    fine = st.uniform(ff.undf).astype(dtype)
+    coarse = st.uniform(fc.undf).astype(dtype)
+    a = oracle.restrict_scalar(np.full(fc.undf, np.nan, dtype), fine, fc.dofmap, ff.dofmap, cm, 5)
+    b = oracle.restrict_scalar(np.full(fc.undf, np.nan, dtype), fine, fc.dofmap, ff.dofmap, cm, 5, use_ref=True)
+    np.testing.assert_array_equal(a, b)
+    assert not np.isnan(a).any()
+    c = oracle.prolong_scalar(fine.copy(), coarse, fc.dofmap, ff.dofmap, cm, 5)
+    d = oracle.prolong_scalar(fine.copy(), coarse, fc.dofmap, ff.dofmap, cm, 5, use_ref=True)
+    np.testing.assert_array_equal(c, d)
+
+
+@pytest.mark.parametrize("space", ["W3", "WTHETA"])
+def test_transfer_identities(space):
+    cm = np.load(GOLDEN / "meshes.npz")["C8_C16_map"]
+    nl = 4
+    fc, ff = _spaces(space, 8, 16, nl)
+    levels = nl if space == "W3" else nl + 1
+    # a field that depends on the level only restricts to itself (1/4 exact)
+    fine = np.empty(ff.undf)
+    for c in range(ff.mesh.ncells):
+        b = ff.dofmap[c, 0] - 1
+        fine[b:b + levels] = 0.25 * np.arange(1, levels + 1)
+    coarse = oracle.restrict_scalar(np.zeros(fc.undf), fine, fc.dofmap, ff.dofmap, cm, nl)
+    for c in range(fc.mesh.ncells):
+        b = fc.dofmap[c, 0] - 1
+        np.testing.assert_array_equal(coarse[b:b + levels], 0.25 * np.arange(1, levels + 1))
+    # restrict(prolong(0, xc)) == xc, and every fine dof is touched once
+    xc = Stream(62).uniform(fc.undf)
+    xf = oracle.prolong_scalar(np.zeros(ff.undf), xc, fc.dofmap, ff.dofmap, cm, nl)
+    back = oracle.restrict_scalar(np.zeros(fc.undf), xf, fc.dofmap, ff.dofmap, cm, nl)
+    np.testing.assert_allclose(back, xc, rtol=1e-15, atol=0)
+    assert np.count_nonzero(xf) == ff.undf
