@@ -731,7 +731,9 @@ k_gather_uni(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y
           LVec<T, V> o;
 #pragma unroll
           for (int u = 0; u < V; ++u) o.v[u] = buf[q - wlo + u];
-          *reinterpret_cast<LVec<T, V>*>(y + q) = o;
+          // streaming (non-temporal) stores: y is written once per apply
+#pragma unroll
+          for (int u = 0; u < V; ++u) __builtin_nontemporal_store(o.v[u], y + q + u);
         }
       }
     }
