@@ -276,6 +276,16 @@ def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
         2 * w3b_fs.undf * 4 + u3 * 4 + nc * 4 * 4 + nc * 4)
     ig.close()
     del xf3, xc3, w3b
+    # W2 restriction C192 -> C96: side faces (2 fine faces each) and B/T columns (4)
+    w2c192 = fsm.build_dofmap(w3b_fs.mesh, "W2", nl)
+    w2b = api.Space(ctx, w2c192)
+    ig2 = api.Intergrid(w2, w2b, intergrid_map(96, 192))
+    xf2 = torch.from_numpy(st.uniform(w2c192.undf).astype(np.float32)).to(dev)
+    xc2 = torch.empty(u2, dtype=torch.float32, device=dev)
+    rec("f3_restrict_w2_c192_to_c96_fp32", timed(lambda: api.restrict_w2(xc2, xf2, ig2)),
+        w2c192.undf * 4 + u2 * 4 + nc * 4 * 5 * 4 + nc * 6 * 4 + nc)
+    ig2.close()
+    del xf2, xc2, w2b
     # f1: CMA assembly and apply on Wtheta (2x2 local matrices, bandwidth 3)
     wt_fs = fsm.build_dofmap(cfg.mesh, "WTHETA", nl)
     wt = api.Space(ctx, wt_fs)

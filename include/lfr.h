@@ -220,8 +220,9 @@ int lfr_compute_derham_matrices(lfr_ctx* ctx, const lfr_derham_tables* tables, c
                                 void* mm1, void* mm2, void* mm2b, void* mm3, void* mmt, void* grad, void* curl,
                                 void* div, void* broken_div);
 
-/* Intergrid transfers of lowest-order scalar fields (W3: ndf 1, Wtheta:
- * ndf 2) between a coarse and a fine mesh of a multigrid hierarchy.
+/* Intergrid transfers between a coarse and a fine mesh of a multigrid
+ * hierarchy: lowest-order scalar fields (W3: ndf 1, Wtheta: ndf 2) and W2
+ * fluxes (ndf 6).
  * lfr_intergrid_create checks and uploads the intergrid map once: cell_map_h
  * (host, 1-based fine cells) is cell_map(nfx, nfy, ncell_coarse) in Fortran
  * order, the mesh-to-mesh connectivity of the two meshes.  Over every coarse
@@ -239,6 +240,16 @@ int lfr_intergrid_create(lfr_ctx* ctx, const lfr_space* coarse_space, const lfr_
 int lfr_intergrid_destroy(lfr_intergrid* intergrid);
 int lfr_restrict_scalar(lfr_ctx* ctx, int prec, void* coarse_d, const void* fine_d, const lfr_intergrid* intergrid);
 int lfr_prolong_scalar(lfr_ctx* ctx, int prec, void* fine_d, const void* coarse_d, const lfr_intergrid* intergrid);
+/* restrict_w2_code
+ * (components/science/source/kernel/inter_mesh/sci_restrict_w2_kernel_mod.F90)
+ * for an intergrid map between lowest-order W2 spaces: every coarse side
+ * face receives the sum of the fine faces on it, every B face column the sum
+ * over the children; the faces a coarse cell writes follow the face
+ * selectors of face_selector_code
+ * (components/science/source/kernel/algebra/sci_face_selector_kernel_mod.F90,
+ * computed when the handle is made), and a face written by two cells keeps
+ * the later cell's value, as the reference's cell loop leaves it. */
+int lfr_restrict_w2(lfr_ctx* ctx, int prec, void* coarse_d, const void* fine_d, const lfr_intergrid* intergrid);
 
 /* Packed operator for repeated applies of one LMA operator (the
  * operator_type whose local_stencil, INF/operator/operator_mod.t90:31-122,

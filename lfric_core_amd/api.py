@@ -596,9 +596,9 @@ def compute_mass_matrix(kind: int, basis, diff_basis_chi, wqp_h, wqp_v, chi: tor
 
 
 class Intergrid:
-    """The intergrid map of a coarse and a fine lowest-order scalar space
-    (lfr_intergrid_create): cell_map (ncell_coarse, nfy, nfx), 1-based fine
-    cells, as the reference's mesh-to-mesh connectivity."""
+    """The intergrid map of a coarse and a fine lowest-order space (W3,
+    Wtheta or W2; lfr_intergrid_create): cell_map (ncell_coarse, nfy, nfx),
+    1-based fine cells, as the reference's mesh-to-mesh connectivity."""
 
     def __init__(self, coarse_space: Space, fine_space: Space, cell_map):
         cm = np.ascontiguousarray(cell_map, dtype=np.int32)
@@ -645,6 +645,19 @@ def restrict_scalar(coarse: torch.Tensor, fine: torch.Tensor, ig: Intergrid):
 def prolong_scalar(fine: torch.Tensor, coarse: torch.Tensor, ig: Intergrid):
     """prolong_scalar_unweighted: fine += coarse on every child."""
     return _transfer("prolong", fine, coarse, ig)
+
+
+def restrict_w2(coarse: torch.Tensor, fine: torch.Tensor, ig: Intergrid):
+    """restrict_w2 over an intergrid map between lowest-order W2 spaces: each
+    coarse face receives the sum of the fine faces on it."""
+    if coarse.dtype != fine.dtype:
+        raise LfrError(1, "the two fields must have one dtype")
+    _field(coarse, ig.coarse_space, "coarse", fine.dtype)
+    _field(fine, ig.fine_space, "fine", fine.dtype)
+    ctx = ig.coarse_space.ctx
+    ctx.sync_stream()
+    check(lib().lfr_restrict_w2(ctx.h, _prec(coarse), _P(coarse.data_ptr()), _P(fine.data_ptr()), ig.h), ctx.h)
+    return coarse
 
 
 _DERHAM_TABLES = ("diff_basis_chi", "basis_w0", "diff_basis_w0", "basis_w1", "diff_basis_w1", "basis_w2",

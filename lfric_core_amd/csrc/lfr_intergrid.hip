@@ -54,6 +54,48 @@ k_prolong_scalar(T* __restrict__ fine, const T* __restrict__ coarse, const int32
   }
 }
 
+// restrict_w2_code
+// (components/science/source/kernel/inter_mesh/sci_restrict_w2_kernel_mod.F90):
+// a coarse side face (W, S, E, N) is the sum of the fine faces on it (nl
+// levels), the B face column the sum over every child (nl + 1 levels), each
+// summed y outer, x inner.  One thread per (coarse cell, level); wmask bit
+// f says this cell writes face f (the last writer in cell order of the
+// reference's loop, whose faces come from the face selectors), and
+// fface(child, face, coarse cell) is the first dof (0-based) of the fine
+// face column.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_restrict_w2(T* __restrict__ coarse, const T* __restrict__ fine, const int32_t* __restrict__ map_c,
+              const int32_t* __restrict__ fface, const uint8_t* __restrict__ wmask, int ncc, int nl, int nfx,
+              int nfy) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)ncc * (nl + 1)) return;
+  const int c = (int)(t / (nl + 1));
+  const int k = (int)(t - (int64_t)c * (nl + 1));
+  const int nch = nfx * nfy;
+  const int32_t* ff = fface + (int64_t)c * nch * 5;
+  const int32_t* mc = map_c + (int64_t)c * 6;
+  const unsigned m = wmask[c];
+  if (k < nl) {
+#pragma unroll
+    for (int df = 0; df < 4; ++df) {   // W, S, E, N
+      if (!((m >> df) & 1u)) continue;
+      int x0 = 0, x1 = nfx - 1, y0 = 0, y1 = nfy - 1;
+      if (df == 3) y1 = 0;              // N: first row
+      else if (df == 1) y0 = nfy - 1;   // S: last row
+      else if (df == 0) x1 = 0;         // W: first column
+      else x0 = nfx - 1;                // E: last column
+      T v = T(0);
+      for (int y = y0; y <= y1; ++y)
+        for (int x = x0; x <= x1; ++x) v = v + fine[(int64_t)ff[(x + nfx * y) * 5 + df] + k];
+      coarse[(int64_t)mc[df] + k] = v;
+    }
+  }
+  T v = T(0);
+  for (int i = 0; i < nch; ++i) v = v + fine[(int64_t)ff[i * 5 + 4] + k];
+  coarse[(int64_t)mc[4] + k] = v;
+}
+
 // every cell's dofs 1..ndf are one column (dof j = dof 1 + j - 1)
 bool column_dofs(const lfr_space* sp) {
   if (sp->vstack >= 0) return sp->vstack == 1;
@@ -74,7 +116,9 @@ struct lfr_intergrid {
   const lfr_space* coarse = nullptr;
   const lfr_space* fine = nullptr;
   int nfx = 0, nfy = 0;
-  int32_t* fbase_d = nullptr;   // (nfx nfy, ncell_coarse): first dof (0-based) of each child's column
+  int32_t* fbase_d = nullptr;   // scalar: (nfx nfy, ncell_coarse) first dof (0-based) of each child's column;
+                                // W2: (5, nfx nfy, ncell_coarse) first dof of each child's W, S, E, N, B face column
+  uint8_t* wmask_d = nullptr;   // W2: per coarse cell, the faces (bits W, S, E, N) it writes
 };
 
 namespace {
@@ -86,8 +130,8 @@ int transfer(lfr_ctx* ctx, bool restrict_, int prec, void* out, const void* in, 
   LFR_REQUIRE(ctx, out && in && ig, "%s: null argument", who);
   LFR_REQUIRE(ctx, out != in, "%s: the output must not alias the input", who);
   const lfr_space* cs = ig->coarse;
-  const lfr_space* fs = ig->fine;
   const int nfx = ig->nfx, nfy = ig->nfy;
+  LFR_REQUIRE(ctx, cs->ndf != 6, "%s: a W2 intergrid map takes lfr_restrict_w2", who);
   const int nlev = cs->nlayers - 1 + cs->ndf;   // top_df + 1
   const int64_t n = (int64_t)cs->ncell * nlev;
   const unsigned nb = (unsigned)((n + 255) / 256);
@@ -122,14 +166,24 @@ extern "C" int lfr_intergrid_create(lfr_ctx* ctx, const lfr_space* coarse_space,
   const lfr_space* cs = coarse_space;
   const lfr_space* fs = fine_space;
   LFR_REQUIRE(ctx, cs && fs && cell_map_h, "%s: null argument", who);
-  LFR_REQUIRE(ctx, cs->ndf == fs->ndf && (cs->ndf == 1 || cs->ndf == 2),
-              "%s: lowest-order W3 (ndf 1) or Wtheta (ndf 2) spaces expected (ndf %d and %d)", who, cs->ndf,
-              fs->ndf);
+  LFR_REQUIRE(ctx, cs->ndf == fs->ndf && (cs->ndf == 1 || cs->ndf == 2 || cs->ndf == 6),
+              "%s: lowest-order W3 (ndf 1), Wtheta (ndf 2) or W2 (ndf 6) spaces expected (ndf %d and %d)", who,
+              cs->ndf, fs->ndf);
   LFR_REQUIRE(ctx, cs->nlayers == fs->nlayers, "%s: the meshes have different layer counts", who);
   LFR_REQUIRE(ctx, nfx >= 1 && nfy >= 1 && (int64_t)cs->ncell * nfx * nfy == fs->ncell,
               "%s: %d coarse cells x %d x %d children do not cover %d fine cells", who, cs->ncell, nfx, nfy,
               fs->ncell);
-  LFR_REQUIRE(ctx, column_dofs(cs) && column_dofs(fs), "%s: a cell's dofs are not one column", who);
+  const bool w2 = cs->ndf == 6;
+  if (w2) {
+    // lowest-order W2: local dofs W, S, E, N, B, T with T = B + 1 (the B
+    // column holds every horizontal face of the cell's column)
+    for (const lfr_space* sp : {cs, fs})
+      for (int c = 0; c < sp->ncell; ++c)
+        LFR_REQUIRE(ctx, sp->map_h[(size_t)c * 6 + 5] == sp->map_h[(size_t)c * 6 + 4] + 1,
+                    "%s: cell %d is not a lowest-order W2 dofmap", who, c + 1);
+  } else {
+    LFR_REQUIRE(ctx, column_dofs(cs) && column_dofs(fs), "%s: a cell's dofs are not one column", who);
+  }
   const int64_t nmap = (int64_t)cs->ncell * nfx * nfy;
   for (int64_t i = 0; i < nmap; ++i)
     LFR_REQUIRE(ctx, cell_map_h[i] >= 1 && cell_map_h[i] <= fs->ncell, "%s: cell_map entry %lld is %d (1..%d)",
@@ -140,16 +194,61 @@ extern "C" int lfr_intergrid_create(lfr_ctx* ctx, const lfr_space* coarse_space,
   ig->fine = fs;
   ig->nfx = nfx;
   ig->nfy = nfy;
-  std::vector<int32_t> fb((size_t)nmap);
-  for (int64_t i = 0; i < nmap; ++i) fb[(size_t)i] = fs->map_h[(size_t)(cell_map_h[i] - 1) * fs->ndf] - 1;
-  if (hipMalloc(&ig->fbase_d, (size_t)nmap * sizeof(int32_t)) != hipSuccess) {
+  std::vector<int32_t> fb;
+  std::vector<uint8_t> wm;
+  if (!w2) {
+    fb.resize((size_t)nmap);
+    for (int64_t i = 0; i < nmap; ++i) fb[(size_t)i] = fs->map_h[(size_t)(cell_map_h[i] - 1) * fs->ndf] - 1;
+  } else {
+    fb.resize((size_t)nmap * 5);
+    for (int64_t i = 0; i < nmap; ++i)
+      for (int df = 0; df < 5; ++df)
+        fb[(size_t)i * 5 + df] = fs->map_h[(size_t)(cell_map_h[i] - 1) * 6 + df] - 1;
+    // face selectors of the coarse mesh, as face_selector_code
+    // (components/science/source/kernel/algebra/sci_face_selector_kernel_mod.F90)
+    // does over the cells in order: a cell takes its E (N) face unless an
+    // earlier cell took that face as its E or N face
+    const int ncc = cs->ncell;
+    std::vector<char> taken((size_t)cs->undf + 1, 0);
+    std::vector<int> ew(ncc, 1), ns(ncc, 1);
+    for (int c = 0; c < ncc; ++c) {
+      const int32_t* m = &cs->map_h[(size_t)c * 6];
+      if (!taken[m[2]]) { ew[c] = 2; taken[m[2]] = 1; }
+      if (!taken[m[3]]) { ns[c] = 2; taken[m[3]] = 1; }
+    }
+    // faces each cell writes (W, S, then E and/or N), and the last writer
+    // of every face in cell order (the value the reference's loop leaves)
+    std::vector<int> last((size_t)cs->undf + 1, -1);
+    std::vector<uint8_t> writes(ncc, 0);
+    for (int c = 0; c < ncc; ++c) {
+      uint8_t w = 0x3;                          // W, S
+      const int nface = ew[c] + ns[c];
+      if (nface >= 3) w |= (ew[c] == 2) ? 0x4 : 0x8;   // face 3: E, or N when only N is selected
+      if (nface == 4) w |= 0x8;                        // face 4: N
+      writes[c] = w;
+      for (int df = 0; df < 4; ++df)
+        if ((w >> df) & 1) last[cs->map_h[(size_t)c * 6 + df]] = c;
+    }
+    wm.resize(ncc);
+    for (int c = 0; c < ncc; ++c) {
+      uint8_t w = 0;
+      for (int df = 0; df < 4; ++df)
+        if (((writes[c] >> df) & 1) && last[cs->map_h[(size_t)c * 6 + df]] == c) w |= (uint8_t)(1u << df);
+      wm[c] = w;
+    }
+  }
+  if (hipMalloc(&ig->fbase_d, fb.size() * sizeof(int32_t)) != hipSuccess ||
+      (w2 && hipMalloc(&ig->wmask_d, wm.size()) != hipSuccess)) {
     (void)hipGetLastError();
+    if (ig->fbase_d) (void)hipFree(ig->fbase_d);
     delete ig;
     return lfr_set_error(ctx, LFR_ERR_NOMEM, "%s: cannot allocate the map", who);
   }
-  const hipError_t e = hipMemcpy(ig->fbase_d, fb.data(), (size_t)nmap * sizeof(int32_t), hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpy(ig->fbase_d, fb.data(), fb.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && w2) e = hipMemcpy(ig->wmask_d, wm.data(), wm.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     (void)hipFree(ig->fbase_d);
+    if (ig->wmask_d) (void)hipFree(ig->wmask_d);
     delete ig;
     return lfr_set_error(ctx, LFR_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
   }
@@ -160,6 +259,7 @@ extern "C" int lfr_intergrid_create(lfr_ctx* ctx, const lfr_space* coarse_space,
 extern "C" int lfr_intergrid_destroy(lfr_intergrid* ig) {
   if (!ig) return LFR_OK;
   if (ig->fbase_d) (void)hipFree(ig->fbase_d);
+  if (ig->wmask_d) (void)hipFree(ig->wmask_d);
   delete ig;
   return LFR_OK;
 }
@@ -172,4 +272,26 @@ extern "C" int lfr_restrict_scalar(lfr_ctx* ctx, int prec, void* coarse_d, const
 extern "C" int lfr_prolong_scalar(lfr_ctx* ctx, int prec, void* fine_d, const void* coarse_d,
                                   const lfr_intergrid* ig) {
   return transfer(ctx, false, prec, fine_d, coarse_d, ig, "lfr_prolong_scalar");
+}
+
+extern "C" int lfr_restrict_w2(lfr_ctx* ctx, int prec, void* coarse_d, const void* fine_d, const lfr_intergrid* ig) {
+  const char* who = "lfr_restrict_w2";
+  if (!ctx) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, prec == 4 || prec == 8, "%s: prec must be 4 or 8, got %d", who, prec);
+  LFR_REQUIRE(ctx, coarse_d && fine_d && ig, "%s: null argument", who);
+  LFR_REQUIRE(ctx, coarse_d != fine_d, "%s: the output must not alias the input", who);
+  LFR_REQUIRE(ctx, ig->coarse->ndf == 6, "%s: the intergrid map is not between W2 spaces", who);
+  const lfr_space* cs = ig->coarse;
+  const int nl = cs->nlayers;
+  const int64_t n = (int64_t)cs->ncell * (nl + 1);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (prec == 4)
+    hipLaunchKernelGGL(k_restrict_w2<float>, dim3(nb), dim3(256), 0, ctx->stream, (float*)coarse_d,
+                       (const float*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, ig->nfx, ig->nfy);
+  else
+    hipLaunchKernelGGL(k_restrict_w2<double>, dim3(nb), dim3(256), 0, ctx->stream, (double*)coarse_d,
+                       (const double*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, ig->nfx,
+                       ig->nfy);
+  LFR_CHECK_LAUNCH(ctx);
+  return LFR_OK;
 }

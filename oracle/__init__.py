@@ -313,6 +313,34 @@ def prolong_scalar(fine, coarse, map_coarse, map_fine, cell_map, nlayers, use_re
     return _transfer("prolong", fine, coarse, map_coarse, map_fine, cell_map, nlayers, use_ref)
 
 
+def face_selector(dofmap, undf):
+    """face_selector_code over the cells in order on a lowest-order W2
+    dofmap (ncell, 6): (sel_ew, sel_ns) per cell."""
+    dm = np.ascontiguousarray(dofmap, dtype=np.int32)
+    n = dm.shape[0]
+    ew = np.zeros(n, np.int32)
+    ns = np.zeros(n, np.int32)
+    lib().orc_face_selector(_I(n), dm.ctypes.data_as(C.c_void_p), _I(undf), ew.ctypes.data_as(C.c_void_p),
+                            ns.ctypes.data_as(C.c_void_p))
+    return ew, ns
+
+
+def restrict_w2(coarse, fine, map_coarse, map_fine, cell_map, nlayers, undf_coarse):
+    """restrict_w2_code over the coarse cells in order, with the coarse mesh's
+    face selectors; coarse receives the restricted fluxes."""
+    mc = np.ascontiguousarray(map_coarse, dtype=np.int32)
+    mf = np.ascontiguousarray(map_fine, dtype=np.int32)
+    cm = np.ascontiguousarray(cell_map, dtype=np.int32)
+    ew, ns = face_selector(mc, undf_coarse)
+    nfy, nfx = cm.shape[1:]
+    src = np.ascontiguousarray(fine)
+    getattr(lib(), f"orc_restrict_w2_{_suf(coarse.dtype)}")(
+        _I(mc.shape[0]), _I(nlayers), coarse.ctypes.data_as(C.c_void_p), src.ctypes.data_as(C.c_void_p),
+        mc.ctypes.data_as(C.c_void_p), mf.ctypes.data_as(C.c_void_p), cm.ctypes.data_as(C.c_void_p),
+        _I(nfx), _I(nfy), ew.ctypes.data_as(C.c_void_p), ns.ctypes.data_as(C.c_void_p))
+    return coarse
+
+
 def dot(x, y, nthreads=None) -> float:
     nt = nthreads_default() if nthreads is None else nthreads
     fn = getattr(lib(), f"orc_dot_{_suf(x.dtype)}")

@@ -469,3 +469,67 @@ int orc_omp_max_threads(void) {
   }
 DEF_TRANSFER(float, r32)
 DEF_TRANSFER(double, r64)
+
+/* face_selector_code (components/science/source/kernel/algebra/sci_face_selector_kernel_mod.F90)
+ * over the cells in order, selectors starting at 1: a cell takes its E (N)
+ * face, selector 2, unless an earlier cell took that face as its E or N
+ * face.  dofmap is the lowest-order W2 map (ncell, 6), local dofs
+ * W, S, E, N, B, T; the side-face dofs identify the faces. */
+void orc_face_selector(int ncell, const int* dofmap, int undf, int* sel_ew, int* sel_ns) {
+  char* taken = (char*)calloc((size_t)undf + 1, 1);
+  for (int c = 0; c < ncell; ++c) {
+    const int* m = dofmap + (int64_t)c * 6;
+    sel_ew[c] = 1;
+    sel_ns[c] = 1;
+    if (!taken[m[2]]) { sel_ew[c] = 2; taken[m[2]] = 1; }
+    if (!taken[m[3]]) { sel_ns[c] = 2; taken[m[3]] = 1; }
+  }
+  free(taken);
+}
+
+/* restrict_w2_code (components/science/source/kernel/inter_mesh/sci_restrict_w2_kernel_mod.F90)
+ * over the coarse cells in order: faces 1 .. sel_ew + sel_ns (W, S, then E
+ * or N; face 3 is N when only N is selected) take the sum, y outer and x
+ * inner, of the fine faces on them (nl levels), then the B face takes the
+ * sum over every child (nl + 1 levels: every horizontal face of the
+ * column).  A face written by two cells keeps the later cell's value. */
+#define DEF_RESTRICT_W2(T, SUF)                                                                   \
+  void orc_restrict_w2_##SUF(int ncell_coarse, int nl, T* coarse, const T* fine, const int* map_c,  \
+                             const int* map_f, const int* cell_map, int nfx, int nfy,              \
+                             const int* sel_ew, const int* sel_ns) {                               \
+    enum { W = 1, S = 2, E = 3, N = 4, B = 5 };                                                    \
+    for (int c = 0; c < ncell_coarse; ++c) {                                                       \
+      const int* cm = cell_map + (int64_t)c * nfx * nfy;                                           \
+      const int* mc = map_c + (int64_t)c * 6;                                                      \
+      const int nface = sel_ew[c] + sel_ns[c];                                                     \
+      for (int face = 1; face <= nface; ++face) {                                                  \
+        int df = face;                                                                             \
+        if (face == 3 && sel_ns[c] == 2 && sel_ew[c] == 1) df = N;                                 \
+        int x0 = 1, x1 = nfx, y0 = 1, y1 = nfy;                                                    \
+        if (df == N) { y1 = 1; }                                                                   \
+        else if (df == S) { y0 = nfy; }                                                            \
+        else if (df == W) { x1 = 1; }                                                              \
+        else if (df == E) { x0 = nfx; }                                                            \
+        for (int k = 0; k < nl; ++k) {                                                             \
+          T v = (T)0;                                                                              \
+          for (int y = y0; y <= y1; ++y)                                                           \
+            for (int x = x0; x <= x1; ++x) {                                                       \
+              const int f = cm[(x - 1) + nfx * (y - 1)] - 1;                                       \
+              v = v + fine[(int64_t)map_f[(int64_t)f * 6 + df - 1] - 1 + k];                      \
+            }                                                                                      \
+          coarse[(int64_t)mc[df - 1] - 1 + k] = v;                                                 \
+        }                                                                                          \
+      }                                                                                            \
+      for (int k = 0; k <= nl; ++k) {                                                              \
+        T v = (T)0;                                                                                \
+        for (int y = 1; y <= nfy; ++y)                                                             \
+          for (int x = 1; x <= nfx; ++x) {                                                         \
+            const int f = cm[(x - 1) + nfx * (y - 1)] - 1;                                         \
+            v = v + fine[(int64_t)map_f[(int64_t)f * 6 + B - 1] - 1 + k];                         \
+          }                                                                                        \
+        coarse[(int64_t)mc[B - 1] - 1 + k] = v;                                                    \
+      }                                                                                            \
+    }                                                                                              \
+  }
+DEF_RESTRICT_W2(float, r32)
+DEF_RESTRICT_W2(double, r64)

@@ -63,3 +63,23 @@ def test_transfer_bad_args(ctx):
     ig = api.Intergrid(sc, sf, cm)
     with pytest.raises(LfrError):   # field on the wrong space
         api.restrict_scalar(z(ff.undf), z(ff.undf), ig)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("key,nc,nf,nl", [("C8_C16_map", 8, 16, 70), ("C4_C12_map", 4, 12, 5)])
+def test_restrict_w2_bitexact(ctx, key, nc, nf, nl, dtype):
+    from lfric_core_amd import api
+    print(f"case restrict_w2 {key} nl={nl} {dtype.__name__}", flush=True)
+    cm = np.load(GOLDEN / "meshes.npz")[key]
+    fc = build_dofmap(cubed_sphere(nc), "W2", nl)
+    ff = build_dofmap(cubed_sphere(nf), "W2", nl)
+    fine = Stream(65).uniform(ff.undf).astype(dtype)
+    ref = oracle.restrict_w2(np.full(fc.undf, np.nan, dtype), fine, fc.dofmap, ff.dofmap, cm, nl, fc.undf)
+    ig = api.Intergrid(api.Space(ctx, fc), api.Space(ctx, ff), cm)
+    y = torch.full((fc.undf,), float("nan"), dtype=torch.from_numpy(fine).dtype, device=ctx.device)
+    api.restrict_w2(y, torch.from_numpy(fine).to(ctx.device), ig)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(y.cpu().numpy(), ref)
+    from lfric_core_amd._ffi import LfrError
+    with pytest.raises(LfrError):   # the scalar transfer on a W2 map
+        api.restrict_scalar(y, torch.from_numpy(fine).to(ctx.device), ig)

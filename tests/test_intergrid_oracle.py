@@ -63,3 +63,64 @@ def test_transfer_identities(space):
     back = oracle.restrict_scalar(np.zeros(fc.undf), xf, fc.dofmap, ff.dofmap, cm, nl)
     np.testing.assert_allclose(back, xc, rtol=1e-15, atol=0)
     assert np.count_nonzero(xf) == ff.undf
+
+
+def _parents(cm, nf_cells):
+    par = np.empty(nf_cells, np.int64)
+    for c in range(cm.shape[0]):
+        par[cm[c].ravel() - 1] = c
+    return par
+
+
+@pytest.mark.parametrize("key,nc,nf", CASES)
+def test_restrict_w2_sums_the_fine_faces_on_each_coarse_face(key, nc, nf):
+    """Independent of the restriction's face and child indexing: a fine side
+    face lies on the coarse face between the parents of its two cells when
+    those differ, and every fine B/T face on the coarse B/T face of its
+    cell's parent.  Integer-valued data, so every sum is exact."""
+    cm = np.load(GOLDEN / "meshes.npz")[key]
+    nl = 3
+    fc = build_dofmap(cubed_sphere(nc), "W2", nl)
+    ff = build_dofmap(cubed_sphere(nf), "W2", nl)
+    fine = np.floor(Stream(64).uniform(ff.undf) * 1000.0)
+    got = oracle.restrict_w2(np.full(fc.undf, np.nan), fine, fc.dofmap, ff.dofmap, cm, nl, fc.undf)
+    par = _parents(cm, ff.mesh.ncells)
+    want = np.zeros(fc.undf)
+    # side faces: coarse face dof shared by coarse cells (A, B)
+    owner = {}
+    for c in range(fc.mesh.ncells):
+        for df in range(4):
+            owner.setdefault(int(fc.dofmap[c, df]), set()).add(c)
+    face_of_pair = {}
+    for d, cells in owner.items():
+        assert len(cells) == 2
+        face_of_pair[frozenset(cells)] = d
+    fowner = {}
+    for f in range(ff.mesh.ncells):
+        for df in range(4):
+            fowner.setdefault(int(ff.dofmap[f, df]), set()).add(f)
+    for d, cells in fowner.items():
+        ps = frozenset(int(par[f]) for f in cells)
+        if len(ps) == 2:
+            dc = face_of_pair[ps]
+            want[dc - 1:dc - 1 + nl] += fine[d - 1:d - 1 + nl]
+    # B/T faces: the column of nl + 1 horizontal faces
+    for f in range(ff.mesh.ncells):
+        db, dc = int(ff.dofmap[f, 4]), int(fc.dofmap[par[f], 4])
+        want[dc - 1:dc + nl] += fine[db - 1:db + nl]
+    np.testing.assert_array_equal(got, want)
+
+
+def test_face_selector_claims_every_face_once():
+    fc = build_dofmap(cubed_sphere(8), "W2", 2)
+    ew, ns = oracle.face_selector(fc.dofmap, fc.undf)
+    claimed = {}
+    for c in range(fc.mesh.ncells):
+        if ew[c] == 2:
+            claimed[int(fc.dofmap[c, 2])] = claimed.get(int(fc.dofmap[c, 2]), 0) + 1
+        if ns[c] == 2:
+            claimed[int(fc.dofmap[c, 3])] = claimed.get(int(fc.dofmap[c, 3]), 0) + 1
+    assert set(claimed.values()) == {1}
+    # together with every cell's W and S faces, all side faces are written
+    written = set(claimed) | {int(d) for d in fc.dofmap[:, :2].ravel()}
+    assert written == {int(d) for d in fc.dofmap[:, :4].ravel()}
