@@ -284,6 +284,8 @@ def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
     xc2 = torch.empty(u2, dtype=torch.float32, device=dev)
     rec("f3_restrict_w2_c192_to_c96_fp32", timed(lambda: api.restrict_w2(xc2, xf2, ig2)),
         w2c192.undf * 4 + u2 * 4 + nc * 4 * 5 * 4 + nc * 6 * 4 + nc)
+    rec("f3_prolong_w2_c96_to_c192_fp32", timed(lambda: api.prolong_w2(xf2, xc2, ig2)),
+        w2c192.undf * 4 + u2 * 4 + nc * 4 * 5 * 4 + nc * 6 * 4)
     ig2.close()
     del xf2, xc2, w2b
     # f1: CMA assembly and apply on Wtheta (2x2 local matrices, bandwidth 3)

@@ -250,6 +250,12 @@ int lfr_prolong_scalar(lfr_ctx* ctx, int prec, void* fine_d, const void* coarse_
  * computed when the handle is made), and a face written by two cells keeps
  * the later cell's value, as the reference's cell loop leaves it. */
 int lfr_restrict_w2(lfr_ctx* ctx, int prec, void* coarse_d, const void* fine_d, const lfr_intergrid* intergrid);
+/* prolong_w2_kernel_code
+ * (components/science/source/kernel/inter_mesh/sci_prolong_w2_kernel_mod.F90)
+ * with the weights weights_prolong_w2_kernel_code
+ * (.../sci_weights_prolong_w2_kernel_mod.F90) computes, for square children
+ * (nfx = nfy): every fine face of the coarse cells is overwritten. */
+int lfr_prolong_w2(lfr_ctx* ctx, int prec, void* fine_d, const void* coarse_d, const lfr_intergrid* intergrid);
 
 /* Packed operator for repeated applies of one LMA operator (the
  * operator_type whose local_stencil, INF/operator/operator_mod.t90:31-122,

@@ -59,6 +59,7 @@ SIGNATURES = {
     "lfr_restrict_scalar": [_P, _I, _P, _P, _P],
     "lfr_prolong_scalar": [_P, _I, _P, _P, _P],
     "lfr_restrict_w2": [_P, _I, _P, _P, _P],
+    "lfr_prolong_w2": [_P, _I, _P, _P, _P],
     "lfr_tri_solve": [_P, _I, _P, _P, _P, _P, _P, _P],
     "lfr_operator_tri_solve": [_P, _I, _P, _P, _P, _P],
     "lfr_lma_pack": [_P, _I, _P, _P, _P, C.POINTER(_P)],

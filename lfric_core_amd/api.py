@@ -660,6 +660,19 @@ def restrict_w2(coarse: torch.Tensor, fine: torch.Tensor, ig: Intergrid):
     return coarse
 
 
+def prolong_w2(fine: torch.Tensor, coarse: torch.Tensor, ig: Intergrid):
+    """prolong_w2 with its weights (square children): every fine face of the
+    coarse cells is overwritten with its share of the coarse fluxes."""
+    if coarse.dtype != fine.dtype:
+        raise LfrError(1, "the two fields must have one dtype")
+    _field(fine, ig.fine_space, "fine", coarse.dtype)
+    _field(coarse, ig.coarse_space, "coarse", coarse.dtype)
+    ctx = ig.coarse_space.ctx
+    ctx.sync_stream()
+    check(lib().lfr_prolong_w2(ctx.h, _prec(fine), _P(fine.data_ptr()), _P(coarse.data_ptr()), ig.h), ctx.h)
+    return fine
+
+
 _DERHAM_TABLES = ("diff_basis_chi", "basis_w0", "diff_basis_w0", "basis_w1", "diff_basis_w1", "basis_w2",
                   "diff_basis_w2", "basis_w2b", "diff_basis_w2b", "basis_w3", "basis_wt")
 # (output, ndf_to, ndf_from, tables it needs)

@@ -96,6 +96,60 @@ k_restrict_w2(T* __restrict__ coarse, const T* __restrict__ fine, const int32_t*
   coarse[(int64_t)mc[4] + k] = v;
 }
 
+// prolong_w2_kernel_code
+// (components/science/source/kernel/inter_mesh/sci_prolong_w2_kernel_mod.F90)
+// with the weights of weights_prolong_w2_kernel_code
+// (.../sci_weights_prolong_w2_kernel_mod.F90) formed in place: a fine face on
+// a coarse side face takes 1/n of the coarse flux (n fine faces per coarse
+// face), an interior fine W (N) face at column x (row y) the linear blend
+// (1/n) (w a + (1 - w) b) of the coarse W and E (N and S) fluxes with
+// w = (n + 1 - x) / n, a fine B/T face 1/n^2 of the coarse one.  Square
+// children only (nfx = nfy = n): a fine face shared by two coarse cells then
+// gets the same weight, and the same value, from both.  One thread per
+// (coarse cell, level).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_prolong_w2(T* __restrict__ fine, const T* __restrict__ coarse, const int32_t* __restrict__ map_c,
+             const int32_t* __restrict__ fface, int ncc, int nl, int n) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)ncc * (nl + 1)) return;
+  const int c = (int)(t / (nl + 1));
+  const int k = (int)(t - (int64_t)c * (nl + 1));
+  const int32_t* ff = fface + (int64_t)c * n * n * 5;
+  const int32_t* mc = map_c + (int64_t)c * 6;
+  auto fd = [&](int x, int y, int df) { return (int64_t)ff[(x + n * y) * 5 + df] + k; };   // 0-based x, y
+  if (k < nl) {
+    const T w1 = T(1) / T(n);
+    const T cw = coarse[(int64_t)mc[0] + k], cs = coarse[(int64_t)mc[1] + k];
+    const T ce = coarse[(int64_t)mc[2] + k], cn = coarse[(int64_t)mc[3] + k];
+    for (int x = 0; x < n; ++x) {
+      fine[fd(x, 0, 3)] = w1 * cn;         // N faces of the first row
+      fine[fd(x, n - 1, 1)] = w1 * cs;     // S faces of the last row
+    }
+    for (int y = 0; y < n; ++y) {
+      fine[fd(0, y, 0)] = w1 * cw;         // W faces of the first column
+      fine[fd(n - 1, y, 2)] = w1 * ce;     // E faces of the last column
+    }
+    for (int y = 0; y < n; ++y)
+      for (int x = 1; x < n; ++x) {        // interior W faces
+        const T iw = T(n - x) / T(n);      // (n + 1 - x_1based) / n
+        const T a = iw * cw;
+        const T b = (T(1) - iw) * ce;
+        fine[fd(x, y, 0)] = w1 * (a + b);
+      }
+    for (int x = 0; x < n; ++x)
+      for (int y = 1; y < n; ++y) {        // interior N faces
+        const T iw = T(n - y) / T(n);
+        const T a = iw * cn;
+        const T b = (T(1) - iw) * cs;
+        fine[fd(x, y, 3)] = w1 * (a + b);
+      }
+  }
+  const T wb = T(1) / T(n * n);
+  const T cb = coarse[(int64_t)mc[4] + k];
+  for (int i = 0; i < n * n; ++i) fine[(int64_t)ff[i * 5 + 4] + k] = wb * cb;
+}
+
 // every cell's dofs 1..ndf are one column (dof j = dof 1 + j - 1)
 bool column_dofs(const lfr_space* sp) {
   if (sp->vstack >= 0) return sp->vstack == 1;
@@ -292,6 +346,28 @@ extern "C" int lfr_restrict_w2(lfr_ctx* ctx, int prec, void* coarse_d, const voi
     hipLaunchKernelGGL(k_restrict_w2<double>, dim3(nb), dim3(256), 0, ctx->stream, (double*)coarse_d,
                        (const double*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, ig->nfx,
                        ig->nfy);
+  LFR_CHECK_LAUNCH(ctx);
+  return LFR_OK;
+}
+
+extern "C" int lfr_prolong_w2(lfr_ctx* ctx, int prec, void* fine_d, const void* coarse_d, const lfr_intergrid* ig) {
+  const char* who = "lfr_prolong_w2";
+  if (!ctx) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, prec == 4 || prec == 8, "%s: prec must be 4 or 8, got %d", who, prec);
+  LFR_REQUIRE(ctx, coarse_d && fine_d && ig, "%s: null argument", who);
+  LFR_REQUIRE(ctx, coarse_d != fine_d, "%s: the output must not alias the input", who);
+  LFR_REQUIRE(ctx, ig->coarse->ndf == 6, "%s: the intergrid map is not between W2 spaces", who);
+  LFR_REQUIRE(ctx, ig->nfx == ig->nfy, "%s: square children only (%d x %d)", who, ig->nfx, ig->nfy);
+  const lfr_space* cs = ig->coarse;
+  const int nl = cs->nlayers;
+  const int64_t n = (int64_t)cs->ncell * (nl + 1);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (prec == 4)
+    hipLaunchKernelGGL(k_prolong_w2<float>, dim3(nb), dim3(256), 0, ctx->stream, (float*)fine_d,
+                       (const float*)coarse_d, cs->map0_d, ig->fbase_d, cs->ncell, nl, ig->nfx);
+  else
+    hipLaunchKernelGGL(k_prolong_w2<double>, dim3(nb), dim3(256), 0, ctx->stream, (double*)fine_d,
+                       (const double*)coarse_d, cs->map0_d, ig->fbase_d, cs->ncell, nl, ig->nfx);
   LFR_CHECK_LAUNCH(ctx);
   return LFR_OK;
 }

@@ -341,6 +341,26 @@ def restrict_w2(coarse, fine, map_coarse, map_fine, cell_map, nlayers, undf_coar
     return coarse
 
 
+def prolong_w2(fine, coarse, map_coarse, map_fine, cell_map, nlayers):
+    """weights_prolong_w2_kernel_code then prolong_w2_kernel_code over the
+    coarse cells in order (in the precision of the fields)."""
+    mc = np.ascontiguousarray(map_coarse, dtype=np.int32)
+    mf = np.ascontiguousarray(map_fine, dtype=np.int32)
+    cm = np.ascontiguousarray(cell_map, dtype=np.int32)
+    nfy, nfx = cm.shape[1:]
+    w = np.zeros(fine.size, fine.dtype)
+    sfx = _suf(fine.dtype)
+    getattr(lib(), f"orc_weights_prolong_w2_{sfx}")(
+        _I(mc.shape[0]), _I(nlayers), w.ctypes.data_as(C.c_void_p), mf.ctypes.data_as(C.c_void_p),
+        cm.ctypes.data_as(C.c_void_p), _I(nfx), _I(nfy))
+    src = np.ascontiguousarray(coarse)
+    getattr(lib(), f"orc_prolong_w2_{sfx}")(
+        _I(mc.shape[0]), _I(nlayers), fine.ctypes.data_as(C.c_void_p), src.ctypes.data_as(C.c_void_p),
+        w.ctypes.data_as(C.c_void_p), mc.ctypes.data_as(C.c_void_p), mf.ctypes.data_as(C.c_void_p),
+        cm.ctypes.data_as(C.c_void_p), _I(nfx), _I(nfy))
+    return fine
+
+
 def dot(x, y, nthreads=None) -> float:
     nt = nthreads_default() if nthreads is None else nthreads
     fn = getattr(lib(), f"orc_dot_{_suf(x.dtype)}")

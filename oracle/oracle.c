@@ -533,3 +533,102 @@ void orc_face_selector(int ncell, const int* dofmap, int undf, int* sel_ew, int*
   }
 DEF_RESTRICT_W2(float, r32)
 DEF_RESTRICT_W2(double, r64)
+
+/* weights_prolong_w2_kernel_code
+ * (components/science/source/kernel/inter_mesh/sci_weights_prolong_w2_kernel_mod.F90)
+ * and prolong_w2_kernel_code (.../sci_prolong_w2_kernel_mod.F90) over the
+ * coarse cells in order (a fine face written by two coarse cells keeps the
+ * later value).  dofmaps are the lowest-order W2 maps (ncell, 6). */
+#define DEF_PROLONG_W2(T, SUF)                                                                    \
+  void orc_weights_prolong_w2_##SUF(int ncell_coarse, int nl, T* w, const int* map_f,             \
+                                    const int* cell_map, int nfx, int nfy) {                      \
+    enum { W = 0, S = 1, E = 2, N = 3, B = 4 };                                                   \
+    for (int c = 0; c < ncell_coarse; ++c) {                                                      \
+      const int* cm = cell_map + (int64_t)c * nfx * nfy;                                          \
+      for (int k = 0; k < nl; ++k) {                                                              \
+        for (int x = 1; x <= nfx; ++x) {                                                          \
+          int f = cm[(x - 1) + nfx * 0] - 1;                                                      \
+          w[(int64_t)map_f[(int64_t)f * 6 + N] - 1 + k] = (T)1 / (T)nfx;                          \
+          f = cm[(x - 1) + nfx * (nfy - 1)] - 1;                                                  \
+          w[(int64_t)map_f[(int64_t)f * 6 + S] - 1 + k] = (T)1 / (T)nfx;                          \
+        }                                                                                         \
+        for (int y = 1; y <= nfy; ++y) {                                                          \
+          int f = cm[0 + nfx * (y - 1)] - 1;                                                      \
+          w[(int64_t)map_f[(int64_t)f * 6 + W] - 1 + k] = (T)1 / (T)nfy;                          \
+          f = cm[(nfx - 1) + nfx * (y - 1)] - 1;                                                  \
+          w[(int64_t)map_f[(int64_t)f * 6 + E] - 1 + k] = (T)1 / (T)nfy;                          \
+        }                                                                                         \
+        for (int y = 1; y <= nfy; ++y)                                                            \
+          for (int x = 2; x <= nfx; ++x) {                                                        \
+            const int f = cm[(x - 1) + nfx * (y - 1)] - 1;                                        \
+            w[(int64_t)map_f[(int64_t)f * 6 + W] - 1 + k] = (T)(nfx + 1 - x) / (T)nfx;            \
+          }                                                                                       \
+        for (int x = 1; x <= nfx; ++x)                                                            \
+          for (int y = 2; y <= nfy; ++y) {                                                        \
+            const int f = cm[(x - 1) + nfx * (y - 1)] - 1;                                        \
+            w[(int64_t)map_f[(int64_t)f * 6 + N] - 1 + k] = (T)(nfy + 1 - y) / (T)nfy;            \
+          }                                                                                       \
+      }                                                                                           \
+      for (int k = 0; k <= nl; ++k)                                                               \
+        for (int y = 1; y <= nfy; ++y)                                                            \
+          for (int x = 1; x <= nfx; ++x) {                                                        \
+            const int f = cm[(x - 1) + nfx * (y - 1)] - 1;                                        \
+            w[(int64_t)map_f[(int64_t)f * 6 + B] - 1 + k] = (T)1 / (T)(nfx * nfy);                \
+          }                                                                                       \
+    }                                                                                             \
+  }                                                                                               \
+  void orc_prolong_w2_##SUF(int ncell_coarse, int nl, T* fine, const T* coarse, const T* w,       \
+                            const int* map_c, const int* map_f, const int* cell_map, int nfx,     \
+                            int nfy) {                                                            \
+    enum { W = 0, S = 1, E = 2, N = 3, B = 4 };                                                   \
+    for (int c = 0; c < ncell_coarse; ++c) {                                                      \
+      const int* cm = cell_map + (int64_t)c * nfx * nfy;                                          \
+      const int* mc = map_c + (int64_t)c * 6;                                                     \
+      for (int k = 0; k < nl; ++k) {                                                              \
+        for (int x = 1; x <= nfx; ++x) {                                                          \
+          int f = cm[(x - 1)] - 1;                                                                \
+          int64_t d = (int64_t)map_f[(int64_t)f * 6 + N] - 1 + k;                                 \
+          fine[d] = w[d] * coarse[(int64_t)mc[N] - 1 + k];                                        \
+          f = cm[(x - 1) + nfx * (nfy - 1)] - 1;                                                  \
+          d = (int64_t)map_f[(int64_t)f * 6 + S] - 1 + k;                                         \
+          fine[d] = w[d] * coarse[(int64_t)mc[S] - 1 + k];                                        \
+        }                                                                                         \
+        for (int y = 1; y <= nfy; ++y) {                                                          \
+          int f = cm[nfx * (y - 1)] - 1;                                                          \
+          int64_t d = (int64_t)map_f[(int64_t)f * 6 + W] - 1 + k;                                 \
+          fine[d] = w[d] * coarse[(int64_t)mc[W] - 1 + k];                                        \
+          f = cm[(nfx - 1) + nfx * (y - 1)] - 1;                                                  \
+          d = (int64_t)map_f[(int64_t)f * 6 + E] - 1 + k;                                         \
+          fine[d] = w[d] * coarse[(int64_t)mc[E] - 1 + k];                                        \
+        }                                                                                         \
+        for (int y = 1; y <= nfy; ++y) {                                                          \
+          const T ew = w[(int64_t)map_f[(int64_t)(cm[nfx * (y - 1)] - 1) * 6 + W] - 1 + k];       \
+          for (int x = 2; x <= nfx; ++x) {                                                        \
+            const int64_t d = (int64_t)map_f[(int64_t)(cm[(x - 1) + nfx * (y - 1)] - 1) * 6 + W] - 1 + k; \
+            const T iw = w[d];                                                                    \
+            const T a = iw * coarse[(int64_t)mc[W] - 1 + k];                                      \
+            const T b = ((T)1 - iw) * coarse[(int64_t)mc[E] - 1 + k];                             \
+            fine[d] = ew * (a + b);                                                               \
+          }                                                                                       \
+        }                                                                                         \
+        for (int x = 1; x <= nfx; ++x) {                                                          \
+          const T ew = w[(int64_t)map_f[(int64_t)(cm[x - 1] - 1) * 6 + N] - 1 + k];               \
+          for (int y = 2; y <= nfy; ++y) {                                                        \
+            const int64_t d = (int64_t)map_f[(int64_t)(cm[(x - 1) + nfx * (y - 1)] - 1) * 6 + N] - 1 + k; \
+            const T iw = w[d];                                                                    \
+            const T a = iw * coarse[(int64_t)mc[N] - 1 + k];                                      \
+            const T b = ((T)1 - iw) * coarse[(int64_t)mc[S] - 1 + k];                             \
+            fine[d] = ew * (a + b);                                                               \
+          }                                                                                       \
+        }                                                                                         \
+      }                                                                                           \
+      for (int k = 0; k <= nl; ++k)                                                               \
+        for (int y = 1; y <= nfy; ++y)                                                            \
+          for (int x = 1; x <= nfx; ++x) {                                                        \
+            const int64_t d = (int64_t)map_f[(int64_t)(cm[(x - 1) + nfx * (y - 1)] - 1) * 6 + B] - 1 + k; \
+            fine[d] = w[d] * coarse[(int64_t)mc[B] - 1 + k];                                      \
+          }                                                                                       \
+    }                                                                                             \
+  }
+DEF_PROLONG_W2(float, r32)
+DEF_PROLONG_W2(double, r64)

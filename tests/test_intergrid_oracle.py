@@ -124,3 +124,51 @@ def test_face_selector_claims_every_face_once():
     # together with every cell's W and S faces, all side faces are written
     written = set(claimed) | {int(d) for d in fc.dofmap[:, :2].ravel()}
     assert written == {int(d) for d in fc.dofmap[:, :4].ravel()}
+
+
+@pytest.mark.parametrize("key,nc,nf,tol", [("C8_C16_map", 8, 16, 0.0), ("C4_C12_map", 4, 12, 1e-15)])
+def test_prolong_w2_then_restrict_w2_is_identity(key, nc, nf, tol):
+    """The prolongation splits each coarse flux over the fine faces on its
+    face (weights 1/n, 1/n^2) and restriction sums them back."""
+    cm = np.load(GOLDEN / "meshes.npz")[key]
+    nl = 3
+    fc = build_dofmap(cubed_sphere(nc), "W2", nl)
+    ff = build_dofmap(cubed_sphere(nf), "W2", nl)
+    u = np.floor(Stream(66).uniform(fc.undf) * 64.0) / 8.0
+    xf = oracle.prolong_w2(np.full(ff.undf, np.nan), u, fc.dofmap, ff.dofmap, cm, nl)
+    assert not np.isnan(xf).any()
+    back = oracle.restrict_w2(np.full(fc.undf, np.nan), xf, fc.dofmap, ff.dofmap, cm, nl, fc.undf)
+    np.testing.assert_allclose(back, u, rtol=tol, atol=tol)
+
+
+def test_prolong_w2_interpolates_interior_faces_linearly():
+    """3x3 children: the interior fine W face of column x lies (x-1)/3 of the
+    way from the coarse W face to the E face (it is the E face of column
+    x-1), and receives 1/3 of the linear blend of the two coarse fluxes;
+    likewise N faces between the coarse N and S faces."""
+    cm = np.load(GOLDEN / "meshes.npz")["C4_C12_map"]
+    nl, n = 2, 3
+    fc = build_dofmap(cubed_sphere(4), "W2", nl)
+    ff = build_dofmap(cubed_sphere(12), "W2", nl)
+    u = Stream(67).uniform(fc.undf)
+    xf = oracle.prolong_w2(np.zeros(ff.undf), u, fc.dofmap, ff.dofmap, cm, nl)
+    W, S, E, N = 0, 1, 2, 3
+    for c in range(fc.mesh.ncells):
+        cw, ce = fc.dofmap[c, W] - 1, fc.dofmap[c, E] - 1
+        cn, cs = fc.dofmap[c, N] - 1, fc.dofmap[c, S] - 1
+        for y in range(n):
+            for x in range(1, n):
+                f, g = cm[c, y, x] - 1, cm[c, y, x - 1] - 1
+                assert ff.dofmap[f, W] == ff.dofmap[g, E]
+                p = x / n
+                d = ff.dofmap[f, W] - 1
+                np.testing.assert_allclose(xf[d:d + nl], ((1 - p) * u[cw:cw + nl] + p * u[ce:ce + nl]) / n,
+                                           rtol=1e-13, atol=1e-15)
+        for x in range(n):
+            for y in range(1, n):
+                f, g = cm[c, y, x] - 1, cm[c, y - 1, x] - 1
+                assert ff.dofmap[f, N] == ff.dofmap[g, S]
+                p = y / n
+                d = ff.dofmap[f, N] - 1
+                np.testing.assert_allclose(xf[d:d + nl], ((1 - p) * u[cn:cn + nl] + p * u[cs:cs + nl]) / n,
+                                           rtol=1e-13, atol=1e-15)
