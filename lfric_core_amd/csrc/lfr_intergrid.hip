@@ -8,11 +8,15 @@
 //   prolong_scalar_unweighted_code
 //   (components/science/source/kernel/inter_mesh/sci_prolong_scalar_unweighted_kernel_mod.F90):
 //     fine(child, k) += coarse(k).
-// One thread per (coarse cell, level), level fastest, so the reads of a
-// child's column and the writes of the coarse column are contiguous across
-// a wavefront; each output is one thread's sum in the reference order.  The
-// children's first dofs are resolved once, when the intergrid handle is
-// made, so a thread's loads do not wait on the map.
+// One thread per coarse cell and LPT of its levels, strided by S =
+// ceil(levels / LPT) (thread j of a column takes levels j, j + S, ...), so
+// the loads of one instruction are contiguous runs of a column across a
+// wavefront, a thread's map entries are read once for LPT levels and LPT
+// loads per child are in flight; each output is one thread's sum in the
+// reference order.  The children's first dofs are resolved once, when the
+// intergrid handle is made, so a thread's loads do not wait on the map.
+#include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "lfr_internal.h"
@@ -21,36 +25,55 @@ namespace {
 
 // fbase(child, coarse cell): first (0-based) dof of the child's column, in
 // the map's order (x fastest, then y)
-template <typename T>
+template <typename T, int LPT>
 __global__ void __launch_bounds__(256)
 k_restrict_scalar(T* __restrict__ coarse, const T* __restrict__ fine, const int32_t* __restrict__ map_c,
-                  const int32_t* __restrict__ fbase, int ncc, int ndf, int nlev, int nch, T denom) {
+                  const int32_t* __restrict__ fbase, int ncc, int ndf, int nlev, int S, int nch, T denom) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (int64_t)ncc * nlev) return;
-  const int c = (int)(t / nlev);
-  const int k = (int)(t - (int64_t)c * nlev);
+  if (t >= (int64_t)ncc * S) return;
+  const int c = (int)(t / S);
+  const int j = (int)(t - (int64_t)c * S);
   const int32_t* fb = fbase + (int64_t)c * nch;
-  T v = T(0);
+  T v[LPT];
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) v[q] = T(0);
   for (int i = 0; i < nch; ++i) {   // y outer, x inner
-    const T p = fine[(int64_t)fb[i] + k] * denom;
-    v = v + p;
+    const T* f = fine + fb[i];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+      const int k = j + q * S;
+      if (k < nlev) {
+        const T p = f[k] * denom;
+        v[q] = v[q] + p;
+      }
+    }
   }
-  coarse[(int64_t)map_c[(int64_t)c * ndf] + k] = v;
+  T* o = coarse + map_c[(int64_t)c * ndf];
+#pragma unroll
+  for (int q = 0; q < LPT; ++q)
+    if (j + q * S < nlev) o[j + q * S] = v[q];
 }
 
-template <typename T>
+template <typename T, int LPT>
 __global__ void __launch_bounds__(256)
 k_prolong_scalar(T* __restrict__ fine, const T* __restrict__ coarse, const int32_t* __restrict__ map_c,
-                 const int32_t* __restrict__ fbase, int ncc, int ndf, int nlev, int nch) {
+                 const int32_t* __restrict__ fbase, int ncc, int ndf, int nlev, int S, int nch) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (int64_t)ncc * nlev) return;
-  const int c = (int)(t / nlev);
-  const int k = (int)(t - (int64_t)c * nlev);
+  if (t >= (int64_t)ncc * S) return;
+  const int c = (int)(t / S);
+  const int j = (int)(t - (int64_t)c * S);
   const int32_t* fb = fbase + (int64_t)c * nch;
-  const T cv = coarse[(int64_t)map_c[(int64_t)c * ndf] + k];
+  const T* cc = coarse + map_c[(int64_t)c * ndf];
+  T cv[LPT];
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) cv[q] = (j + q * S < nlev) ? cc[j + q * S] : T(0);
   for (int i = 0; i < nch; ++i) {
-    const int64_t d = (int64_t)fb[i] + k;
-    fine[d] = fine[d] + cv;
+    T* f = fine + fb[i];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+      const int k = j + q * S;
+      if (k < nlev) f[k] = f[k] + cv[q];
+    }
   }
 }
 
@@ -58,42 +81,59 @@ k_prolong_scalar(T* __restrict__ fine, const T* __restrict__ coarse, const int32
 // (components/science/source/kernel/inter_mesh/sci_restrict_w2_kernel_mod.F90):
 // a coarse side face (W, S, E, N) is the sum of the fine faces on it (nl
 // levels), the B face column the sum over every child (nl + 1 levels), each
-// summed y outer, x inner.  One thread per (coarse cell, level); wmask bit
-// f says this cell writes face f (the last writer in cell order of the
-// reference's loop, whose faces come from the face selectors), and
-// fface(child, face, coarse cell) is the first dof (0-based) of the fine
-// face column.
-template <typename T>
+// summed y outer, x inner.  One thread per coarse cell and LPT of its nl + 1
+// levels (strided by S); wmask bit f says this cell writes face f (the last
+// writer in cell order of the reference's loop, whose faces come from the
+// face selectors), and fface(child, face, coarse cell) is the first dof
+// (0-based) of the fine face column.
+template <typename T, int LPT>
 __global__ void __launch_bounds__(256)
 k_restrict_w2(T* __restrict__ coarse, const T* __restrict__ fine, const int32_t* __restrict__ map_c,
-              const int32_t* __restrict__ fface, const uint8_t* __restrict__ wmask, int ncc, int nl, int nfx,
-              int nfy) {
+              const int32_t* __restrict__ fface, const uint8_t* __restrict__ wmask, int ncc, int nl, int S,
+              int nfx, int nfy) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (int64_t)ncc * (nl + 1)) return;
-  const int c = (int)(t / (nl + 1));
-  const int k = (int)(t - (int64_t)c * (nl + 1));
+  if (t >= (int64_t)ncc * S) return;
+  const int c = (int)(t / S);
+  const int j = (int)(t - (int64_t)c * S);
   const int nch = nfx * nfy;
   const int32_t* ff = fface + (int64_t)c * nch * 5;
   const int32_t* mc = map_c + (int64_t)c * 6;
   const unsigned m = wmask[c];
-  if (k < nl) {
+  T v[LPT];
 #pragma unroll
-    for (int df = 0; df < 4; ++df) {   // W, S, E, N
-      if (!((m >> df) & 1u)) continue;
-      int x0 = 0, x1 = nfx - 1, y0 = 0, y1 = nfy - 1;
-      if (df == 3) y1 = 0;              // N: first row
-      else if (df == 1) y0 = nfy - 1;   // S: last row
-      else if (df == 0) x1 = 0;         // W: first column
-      else x0 = nfx - 1;                // E: last column
-      T v = T(0);
-      for (int y = y0; y <= y1; ++y)
-        for (int x = x0; x <= x1; ++x) v = v + fine[(int64_t)ff[(x + nfx * y) * 5 + df] + k];
-      coarse[(int64_t)mc[df] + k] = v;
-    }
+  for (int df = 0; df < 4; ++df) {   // W, S, E, N
+    if (!((m >> df) & 1u)) continue;
+    int x0 = 0, x1 = nfx - 1, y0 = 0, y1 = nfy - 1;
+    if (df == 3) y1 = 0;              // N: first row
+    else if (df == 1) y0 = nfy - 1;   // S: last row
+    else if (df == 0) x1 = 0;         // W: first column
+    else x0 = nfx - 1;                // E: last column
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) v[q] = T(0);
+    for (int y = y0; y <= y1; ++y)
+      for (int x = x0; x <= x1; ++x) {
+        const T* f = fine + ff[(x + nfx * y) * 5 + df];
+#pragma unroll
+        for (int q = 0; q < LPT; ++q)
+          if (j + q * S < nl) v[q] = v[q] + f[j + q * S];
+      }
+    T* o = coarse + mc[df];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q)
+      if (j + q * S < nl) o[j + q * S] = v[q];
   }
-  T v = T(0);
-  for (int i = 0; i < nch; ++i) v = v + fine[(int64_t)ff[i * 5 + 4] + k];
-  coarse[(int64_t)mc[4] + k] = v;
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) v[q] = T(0);
+  for (int i = 0; i < nch; ++i) {
+    const T* f = fine + ff[i * 5 + 4];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q)
+      if (j + q * S <= nl) v[q] = v[q] + f[j + q * S];
+  }
+  T* o = coarse + mc[4];
+#pragma unroll
+  for (int q = 0; q < LPT; ++q)
+    if (j + q * S <= nl) o[j + q * S] = v[q];
 }
 
 // prolong_w2_kernel_code
@@ -106,48 +146,79 @@ k_restrict_w2(T* __restrict__ coarse, const T* __restrict__ fine, const int32_t*
 // w = (n + 1 - x) / n, a fine B/T face 1/n^2 of the coarse one.  Square
 // children only (nfx = nfy = n): a fine face shared by two coarse cells then
 // gets the same weight, and the same value, from both.  One thread per
-// (coarse cell, level).
-template <typename T>
+// coarse cell and LPT of its nl + 1 levels (strided by S).
+template <typename T, int LPT>
 __global__ void __launch_bounds__(256)
 k_prolong_w2(T* __restrict__ fine, const T* __restrict__ coarse, const int32_t* __restrict__ map_c,
-             const int32_t* __restrict__ fface, int ncc, int nl, int n) {
+             const int32_t* __restrict__ fface, int ncc, int nl, int S, int n) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (int64_t)ncc * (nl + 1)) return;
-  const int c = (int)(t / (nl + 1));
-  const int k = (int)(t - (int64_t)c * (nl + 1));
+  if (t >= (int64_t)ncc * S) return;
+  const int c = (int)(t / S);
+  const int j = (int)(t - (int64_t)c * S);
   const int32_t* ff = fface + (int64_t)c * n * n * 5;
   const int32_t* mc = map_c + (int64_t)c * 6;
-  auto fd = [&](int x, int y, int df) { return (int64_t)ff[(x + n * y) * 5 + df] + k; };   // 0-based x, y
-  if (k < nl) {
-    const T w1 = T(1) / T(n);
-    const T cw = coarse[(int64_t)mc[0] + k], cs = coarse[(int64_t)mc[1] + k];
-    const T ce = coarse[(int64_t)mc[2] + k], cn = coarse[(int64_t)mc[3] + k];
-    for (int x = 0; x < n; ++x) {
-      fine[fd(x, 0, 3)] = w1 * cn;         // N faces of the first row
-      fine[fd(x, n - 1, 1)] = w1 * cs;     // S faces of the last row
+  // fine face column of child (x, y) (0-based), face df
+  auto fcol = [&](int x, int y, int df) { return fine + ff[(x + n * y) * 5 + df]; };
+  auto side = [&](T* f, const T* val) {
+#pragma unroll
+    for (int q = 0; q < LPT; ++q)
+      if (j + q * S < nl) f[j + q * S] = val[q];
+  };
+  const T w1 = T(1) / T(n);
+  T cw[LPT], cs[LPT], ce[LPT], cn[LPT], tmp[LPT];
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) {
+    const int k = j + q * S;
+    const bool in = k < nl;
+    cw[q] = in ? coarse[mc[0] + k] : T(0);
+    cs[q] = in ? coarse[mc[1] + k] : T(0);
+    ce[q] = in ? coarse[mc[2] + k] : T(0);
+    cn[q] = in ? coarse[mc[3] + k] : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) tmp[q] = w1 * cn[q];
+  for (int x = 0; x < n; ++x) side(fcol(x, 0, 3), tmp);         // N faces of the first row
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) tmp[q] = w1 * cs[q];
+  for (int x = 0; x < n; ++x) side(fcol(x, n - 1, 1), tmp);     // S faces of the last row
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) tmp[q] = w1 * cw[q];
+  for (int y = 0; y < n; ++y) side(fcol(0, y, 0), tmp);         // W faces of the first column
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) tmp[q] = w1 * ce[q];
+  for (int y = 0; y < n; ++y) side(fcol(n - 1, y, 2), tmp);     // E faces of the last column
+  for (int x = 1; x < n; ++x) {                                  // interior W faces
+    const T iw = T(n - x) / T(n);                                // (n + 1 - x_1based) / n
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+      const T a = iw * cw[q];
+      const T b = (T(1) - iw) * ce[q];
+      tmp[q] = w1 * (a + b);
     }
-    for (int y = 0; y < n; ++y) {
-      fine[fd(0, y, 0)] = w1 * cw;         // W faces of the first column
-      fine[fd(n - 1, y, 2)] = w1 * ce;     // E faces of the last column
+    for (int y = 0; y < n; ++y) side(fcol(x, y, 0), tmp);
+  }
+  for (int y = 1; y < n; ++y) {                                  // interior N faces
+    const T iw = T(n - y) / T(n);
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+      const T a = iw * cn[q];
+      const T b = (T(1) - iw) * cs[q];
+      tmp[q] = w1 * (a + b);
     }
-    for (int y = 0; y < n; ++y)
-      for (int x = 1; x < n; ++x) {        // interior W faces
-        const T iw = T(n - x) / T(n);      // (n + 1 - x_1based) / n
-        const T a = iw * cw;
-        const T b = (T(1) - iw) * ce;
-        fine[fd(x, y, 0)] = w1 * (a + b);
-      }
-    for (int x = 0; x < n; ++x)
-      for (int y = 1; y < n; ++y) {        // interior N faces
-        const T iw = T(n - y) / T(n);
-        const T a = iw * cn;
-        const T b = (T(1) - iw) * cs;
-        fine[fd(x, y, 3)] = w1 * (a + b);
-      }
+    for (int x = 0; x < n; ++x) side(fcol(x, y, 3), tmp);
   }
   const T wb = T(1) / T(n * n);
-  const T cb = coarse[(int64_t)mc[4] + k];
-  for (int i = 0; i < n * n; ++i) fine[(int64_t)ff[i * 5 + 4] + k] = wb * cb;
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) {
+    const int k = j + q * S;
+    tmp[q] = (k <= nl) ? wb * coarse[mc[4] + k] : T(0);
+  }
+  for (int i = 0; i < n * n; ++i) {
+    T* f = fine + ff[i * 5 + 4];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q)
+      if (j + q * S <= nl) f[j + q * S] = tmp[q];
+  }
 }
 
 // every cell's dofs 1..ndf are one column (dof j = dof 1 + j - 1)
@@ -177,6 +248,28 @@ struct lfr_intergrid {
 
 namespace {
 
+// Levels per thread of the transfer kernels: 4 for the restrictions (more
+// loads in flight per map entry), 2 for the prolongations (their strided
+// stores touch more lines per instruction as LPT grows; measured 42.6 µs
+// against 44.8 µs at 4 for the C96 to C192 W3 prolongation, 88.4 against
+// 105.7 µs for W2); LFR_TRANSFER_LPT=1, 2 or 4 forces one value.
+int transfer_lpt(bool restrict_) {
+  static const int forced = [] {
+    const char* e = getenv("LFR_TRANSFER_LPT");
+    if (e && (e[0] == '1' || e[0] == '2' || e[0] == '4')) return e[0] - '0';
+    return 0;
+  }();
+  return forced ? forced : (restrict_ ? 4 : 2);
+}
+
+// f(std::integral_constant<int, LPT>) for lpt 1, 2 or 4
+template <typename F>
+void with_lpt(int lpt, F&& f) {
+  if (lpt == 1) f(std::integral_constant<int, 1>{});
+  else if (lpt == 2) f(std::integral_constant<int, 2>{});
+  else f(std::integral_constant<int, 4>{});
+}
+
 int transfer(lfr_ctx* ctx, bool restrict_, int prec, void* out, const void* in, const lfr_intergrid* ig,
              const char* who) {
   if (!ctx) return LFR_ERR_ARG;
@@ -187,25 +280,30 @@ int transfer(lfr_ctx* ctx, bool restrict_, int prec, void* out, const void* in, 
   const int nfx = ig->nfx, nfy = ig->nfy;
   LFR_REQUIRE(ctx, cs->ndf != 6, "%s: a W2 intergrid map takes lfr_restrict_w2", who);
   const int nlev = cs->nlayers - 1 + cs->ndf;   // top_df + 1
-  const int64_t n = (int64_t)cs->ncell * nlev;
-  const unsigned nb = (unsigned)((n + 255) / 256);
+  const int lpt = transfer_lpt(restrict_);
+  const int S = (nlev + lpt - 1) / lpt;
+  const unsigned nb = (unsigned)(((int64_t)cs->ncell * S + 255) / 256);
   const int32_t* fb = ig->fbase_d;
   const int nch = nfx * nfy;
-  if (restrict_) {
-    if (prec == 4)
-      hipLaunchKernelGGL(k_restrict_scalar<float>, dim3(nb), dim3(256), 0, ctx->stream, (float*)out,
-                         (const float*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, nch, 1.0f / (float)nch);
-    else
-      hipLaunchKernelGGL(k_restrict_scalar<double>, dim3(nb), dim3(256), 0, ctx->stream, (double*)out,
-                         (const double*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, nch, 1.0 / (double)nch);
-  } else {
-    if (prec == 4)
-      hipLaunchKernelGGL(k_prolong_scalar<float>, dim3(nb), dim3(256), 0, ctx->stream, (float*)out,
-                         (const float*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, nch);
-    else
-      hipLaunchKernelGGL(k_prolong_scalar<double>, dim3(nb), dim3(256), 0, ctx->stream, (double*)out,
-                         (const double*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, nch);
-  }
+  with_lpt(lpt, [&](auto L) {
+    constexpr int LPT = decltype(L)::value;
+    if (restrict_) {
+      if (prec == 4)
+        hipLaunchKernelGGL((k_restrict_scalar<float, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (float*)out,
+                           (const float*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch, 1.0f / (float)nch);
+      else
+        hipLaunchKernelGGL((k_restrict_scalar<double, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (double*)out,
+                           (const double*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch,
+                           1.0 / (double)nch);
+    } else {
+      if (prec == 4)
+        hipLaunchKernelGGL((k_prolong_scalar<float, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (float*)out,
+                           (const float*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch);
+      else
+        hipLaunchKernelGGL((k_prolong_scalar<double, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (double*)out,
+                           (const double*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch);
+    }
+  });
   LFR_CHECK_LAUNCH(ctx);
   return LFR_OK;
 }
@@ -337,15 +435,20 @@ extern "C" int lfr_restrict_w2(lfr_ctx* ctx, int prec, void* coarse_d, const voi
   LFR_REQUIRE(ctx, ig->coarse->ndf == 6, "%s: the intergrid map is not between W2 spaces", who);
   const lfr_space* cs = ig->coarse;
   const int nl = cs->nlayers;
-  const int64_t n = (int64_t)cs->ncell * (nl + 1);
-  const unsigned nb = (unsigned)((n + 255) / 256);
-  if (prec == 4)
-    hipLaunchKernelGGL(k_restrict_w2<float>, dim3(nb), dim3(256), 0, ctx->stream, (float*)coarse_d,
-                       (const float*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, ig->nfx, ig->nfy);
-  else
-    hipLaunchKernelGGL(k_restrict_w2<double>, dim3(nb), dim3(256), 0, ctx->stream, (double*)coarse_d,
-                       (const double*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, ig->nfx,
-                       ig->nfy);
+  const int lpt = transfer_lpt(true);
+  const int S = (nl + 1 + lpt - 1) / lpt;
+  const unsigned nb = (unsigned)(((int64_t)cs->ncell * S + 255) / 256);
+  with_lpt(lpt, [&](auto L) {
+    constexpr int LPT = decltype(L)::value;
+    if (prec == 4)
+      hipLaunchKernelGGL((k_restrict_w2<float, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (float*)coarse_d,
+                         (const float*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, S, ig->nfx,
+                         ig->nfy);
+    else
+      hipLaunchKernelGGL((k_restrict_w2<double, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (double*)coarse_d,
+                         (const double*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, S, ig->nfx,
+                         ig->nfy);
+  });
   LFR_CHECK_LAUNCH(ctx);
   return LFR_OK;
 }
@@ -360,14 +463,18 @@ extern "C" int lfr_prolong_w2(lfr_ctx* ctx, int prec, void* fine_d, const void* 
   LFR_REQUIRE(ctx, ig->nfx == ig->nfy, "%s: square children only (%d x %d)", who, ig->nfx, ig->nfy);
   const lfr_space* cs = ig->coarse;
   const int nl = cs->nlayers;
-  const int64_t n = (int64_t)cs->ncell * (nl + 1);
-  const unsigned nb = (unsigned)((n + 255) / 256);
-  if (prec == 4)
-    hipLaunchKernelGGL(k_prolong_w2<float>, dim3(nb), dim3(256), 0, ctx->stream, (float*)fine_d,
-                       (const float*)coarse_d, cs->map0_d, ig->fbase_d, cs->ncell, nl, ig->nfx);
-  else
-    hipLaunchKernelGGL(k_prolong_w2<double>, dim3(nb), dim3(256), 0, ctx->stream, (double*)fine_d,
-                       (const double*)coarse_d, cs->map0_d, ig->fbase_d, cs->ncell, nl, ig->nfx);
+  const int lpt = transfer_lpt(false);
+  const int S = (nl + 1 + lpt - 1) / lpt;
+  const unsigned nb = (unsigned)(((int64_t)cs->ncell * S + 255) / 256);
+  with_lpt(lpt, [&](auto L) {
+    constexpr int LPT = decltype(L)::value;
+    if (prec == 4)
+      hipLaunchKernelGGL((k_prolong_w2<float, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (float*)fine_d,
+                         (const float*)coarse_d, cs->map0_d, ig->fbase_d, cs->ncell, nl, S, ig->nfx);
+    else
+      hipLaunchKernelGGL((k_prolong_w2<double, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (double*)fine_d,
+                         (const double*)coarse_d, cs->map0_d, ig->fbase_d, cs->ncell, nl, S, ig->nfx);
+  });
   LFR_CHECK_LAUNCH(ctx);
   return LFR_OK;
 }
