@@ -113,6 +113,123 @@ k_appinv(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
   }
 }
 
+// columnwise_op_appinv with LDS-staged loads (FP32 throughout): the bands
+// of a workgroup's 64 columns are one contiguous block (3 NROW values per
+// column), so each kCh-row stage is read by instructions covering a few
+// columns' contiguous runs (the per-lane loads of k_appinv touch 64 lines
+// per instruction), all of a stage's loads issued before its LDS writes; the
+// recurrence is k_appinv's term for term with c' and d' in registers.
+template <typename T, int NROW>
+__global__ void __launch_bounds__(kCols)
+k_appinv_staged(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
+                const int32_t* __restrict__ map0, int ndf, const int32_t* __restrict__ ind, int ncell) {
+  constexpr int KCH = kCh;
+  constexpr int NST = (NROW + KCH - 1) / KCH;
+  constexpr int NB = 3 * KCH;              // band values of one column per stage
+  constexpr int PITCH_B = NB + 1;          // odd
+  constexpr int PITCH_X = KCH + 1;         // odd
+  __shared__ T sb[kCols * PITCH_B];
+  __shared__ T sx[kCols * PITCH_X];
+  __shared__ int32_t sbase[kCols];
+  __shared__ int32_t sind[NROW];   // indirection_dofmap - 1, read once
+  const int lane = threadIdx.x;
+  const int c0 = blockIdx.x * kCols;
+  const int ncols = min(kCols, ncell - c0);
+  if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
+  for (int i = lane; i < NROW; i += kCols) sind[i] = ind[i] - 1;
+  T cp[NROW], dp[NROW];
+#pragma unroll
+  for (int s = 0; s < NST; ++s) {
+    const int r0 = s * KCH;
+    const int nr = (NROW - r0) < KCH ? (NROW - r0) : KCH;
+    __syncthreads();
+    // band slot it of this lane: value qs of column cs's stage (e = lane +
+    // it kCols = cs NB + qs); x slot: row qs of column cs (e = cs KCH + qs);
+    // both clamped into the workgroup's columns and the stage's rows
+    T rb[NB], rx[KCH];
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {   // kCols NB slots: NB per lane
+      const int e = lane + it * kCols;
+      const int cs = e / NB;
+      const int qs = e - cs * NB;
+      rb[it] = cm[(int64_t)(c0 + min(cs, ncols - 1)) * 3 * NROW + 3 * r0 + min(qs, 3 * nr - 1)];
+    }
+#pragma unroll
+    for (int it = 0; it < KCH; ++it) {
+      const int e = lane + it * kCols;
+      const int cs = e / KCH;
+      const int qs = e - cs * KCH;
+      rx[it] = x[sbase[min(cs, ncols - 1)] + sind[r0 + min(qs, nr - 1)]];
+    }
+#pragma unroll
+    for (int it = 0; it < NB; ++it) {
+      const int e = lane + it * kCols;
+      const int cs = e / NB;
+      sb[cs * PITCH_B + (e - cs * NB)] = rb[it];
+    }
+#pragma unroll
+    for (int it = 0; it < KCH; ++it) {
+      const int e = lane + it * kCols;
+      const int cs = e / KCH;
+      sx[cs * PITCH_X + (e - cs * KCH)] = rx[it];
+    }
+    __syncthreads();
+    if (lane < ncols) {
+#pragma unroll
+      for (int q = 0; q < KCH; ++q) {
+        const int i = r0 + q;
+        if (i < NROW) {
+          const T* v = sb + lane * PITCH_B + 3 * q;
+          const T a = v[0], d = v[1], u = v[2], xv = sx[lane * PITCH_X + q];
+          if (i == 0) {
+            const T inv = T(1) / d;
+            cp[i] = inv * u;
+            dp[i] = inv * xv;
+          } else {
+            const T ac = a * cp[i - 1];
+            const T inv = T(1) / (d - ac);
+            if (i < NROW - 1) cp[i] = inv * u;
+            const T ad = a * dp[i - 1];
+            dp[i] = inv * (xv - ad);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = NROW - 2; i >= 0; --i) {
+    const T cd = cp[i] * dp[i + 1];
+    dp[i] = dp[i] - cd;
+  }
+#pragma unroll
+  for (int s = 0; s < NST; ++s) {
+    const int r0 = s * KCH;
+    const int nr = (NROW - r0) < KCH ? (NROW - r0) : KCH;
+    __syncthreads();
+    if (lane < ncols) {
+#pragma unroll
+      for (int q = 0; q < KCH; ++q)
+        if (r0 + q < NROW) sx[lane * PITCH_X + q] = dp[r0 + q];
+    }
+    __syncthreads();
+    const int nx = ncols * nr;
+    for (int e = lane; e < nx; e += kCols) {
+      const int cc = e / nr;
+      const int q = e - cc * nr;
+      lhs[sbase[cc] + sind[r0 + q]] = sx[cc * PITCH_X + q];
+    }
+  }
+}
+
+// FP32 appinv at 70 rows takes the staged loads unless LFR_APPINV_STAGED=0.
+bool appinv_staged() {
+  static const bool on = [] {
+    const char* e = getenv("LFR_APPINV_STAGED");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 // Generic row count: same recurrence with c' and d' in a global scratch
 // array of the compute kind (2*nrow per column).
 template <typename T, typename C>
@@ -154,6 +271,14 @@ template <typename T, typename C>
 int launch_appinv(lfr_ctx* ctx, T* lhs, const T* x, const T* cm, const lfr_space* sp,
                   const int32_t* ind, int nrow, int ncell) {
   const unsigned nb = (unsigned)((ncell + kCols - 1) / kCols);
+  if constexpr (sizeof(T) == 4 && sizeof(C) == 4) {
+    if (nrow == 70 && appinv_staged()) {
+      hipLaunchKernelGGL((k_appinv_staged<T, 70>), dim3(nb), dim3(kCols), 0, ctx->stream, lhs, x, cm, sp->map0_d,
+                         sp->ndf, ind, ncell);
+      LFR_CHECK_LAUNCH(ctx);
+      return LFR_OK;
+    }
+  }
   switch (nrow) {
 #define CASE(NR)                                                                              \
   case NR:                                                                                    \

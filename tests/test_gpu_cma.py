@@ -25,7 +25,7 @@ def _helmholtz_columns(st, ncell, nrow, h2=1.0):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("nc,nl", [(6, 70), (4, 10), (3, 13)])
+@pytest.mark.parametrize("nc,nl", [(6, 70), (3, 70), (4, 10), (3, 13)])
 def test_appinv_bitexact(ctx, dtype, nc, nl):
     from lfric_core_amd import api
     print(f"case appinv C{nc} nl={nl} {dtype.__name__}", flush=True)

@@ -19,7 +19,7 @@ def _t(a, ctx):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("nc,nl", [(3, 10), (8, 70)])
+@pytest.mark.parametrize("nc,nl", [(3, 10), (8, 70), (9, 70)])   # C9: a partial workgroup of columns
 def test_tri_solve_bitexact(ctx, dtype, nc, nl):
     from lfric_core_amd import api
     print(f"case tri_solve nc={nc} nl={nl} {dtype.__name__}", flush=True)
