@@ -20,6 +20,7 @@
 #include <map>
 #include <numeric>
 #include <tuple>
+#include <type_traits>
 
 #include "lfr_internal.h"
 #include "lfr_lma_kernels.h"
@@ -302,12 +303,34 @@ int run_gather(lfr_ctx* ctx, const lfr_plan* plan, int kind, const T* M, const T
   return LFR_OK;
 }
 
+// Levels per thread of the cell-centric apply: 1 (C96L70 div W3 <- W2, FP32:
+// 31.7 µs at 1, 56.9 µs at 2, 60.9 µs at 4, the strided levels splitting
+// each of the six matrix-plane loads into short runs); LFR_CELLS_LPT=2 or 4
+// forces another value.
+int cells_lpt(int) {
+  static const int forced = [] {
+    const char* e = getenv("LFR_CELLS_LPT");
+    if (e && (e[0] == '1' || e[0] == '2' || e[0] == '4')) return e[0] - '0';
+    return 0;
+  }();
+  return forced ? forced : 1;
+}
+
 template <typename T, int NO, int NI>
 void launch_cells_t(hipStream_t s, const T* M, const T* x, T* y, const lfr_space* out,
                     const lfr_space* in, int64_t n3, int inc) {
-  const unsigned nb = pad8(lfr_blocks((int64_t)out->ncell * out->nlayers, kThreads));
-  hipLaunchKernelGGL((k_cells<T, NO, NI>), dim3(nb), dim3(kThreads), 0, s, M, x, y, out->map0_d,
-                     in->map0_d, out->ncell, out->nlayers, n3, out->ndf, in->ndf, inc, nb);
+  const int nl = out->nlayers;
+  // the runtime-ndf kernel (NO = NI = 0) runs one level per thread
+  const int lpt = (NO > 0 && NI > 0) ? cells_lpt((int)sizeof(T)) : 1;
+  auto go = [&](auto L) {
+    constexpr int LPT = decltype(L)::value;
+    const unsigned nb = pad8(lfr_blocks((int64_t)out->ncell * ((nl + LPT - 1) / LPT), kThreads));
+    hipLaunchKernelGGL((k_cells<T, NO, NI, LPT>), dim3(nb), dim3(kThreads), 0, s, M, x, y, out->map0_d,
+                       in->map0_d, out->ncell, nl, n3, out->ndf, in->ndf, inc, nb);
+  };
+  if (lpt == 4) go(std::integral_constant<int, 4>{});
+  else if (lpt == 2) go(std::integral_constant<int, 2>{});
+  else go(std::integral_constant<int, 1>{});
 }
 
 template <typename T>

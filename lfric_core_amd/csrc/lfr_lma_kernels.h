@@ -781,37 +781,59 @@ k_gather_uni(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y
 // sum over df2 in ascending order is the reference order
 // (dg_matrix_vector_kernel_mod.F90:94-107, dg_inc_matrix_vector_kernel_mod.F90:134-141).
 // ---------------------------------------------------------------------------
-template <typename T, int NDF_OUT, int NDF_IN>
+template <typename T, int NDF_OUT, int NDF_IN, int LPT = 1>
 __global__ void __launch_bounds__(kThreads)
 k_cells(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
         const int32_t* __restrict__ map_out, const int32_t* __restrict__ map_in, int ncell,
         int nl, int64_t n3, int ndf_out_rt, int ndf_in_rt, int inc, unsigned nb_pad) {
   const unsigned lb = lfr_xcd_block(blockIdx.x, nb_pad);
   const int64_t t = (int64_t)lb * kThreads + threadIdx.x;
-  if (t >= (int64_t)ncell * nl) return;
-  const int c = (int)(t / nl);
-  const int k = (int)(t - (int64_t)c * nl);
-  const int64_t ik = t;
   const int ndf_in = NDF_IN > 0 ? NDF_IN : ndf_in_rt;
   const int ndf_out = NDF_OUT > 0 ? NDF_OUT : ndf_out_rt;
-  const int32_t* mo = map_out + (int64_t)c * ndf_out;
-  const int32_t* mi = map_in + (int64_t)c * ndf_in;
   if constexpr (NDF_OUT > 0 && NDF_IN > 0) {
-    T acc[NDF_OUT];
+    // thread <-> (cell, j): levels j, j + S, ... (S = ceil(nl / LPT)), so one
+    // load instruction still reads contiguous runs of a column across the
+    // wavefront and the cell's map entries are read once for LPT levels
+    const int S = (nl + LPT - 1) / LPT;
+    if (t >= (int64_t)ncell * S) return;
+    const int c = (int)(t / S);
+    const int j = (int)(t - (int64_t)c * S);
+    const int32_t* mo = map_out + (int64_t)c * ndf_out;
+    const int32_t* mi = map_in + (int64_t)c * ndf_in;
+    const T* Mc = M + (int64_t)c * nl + j;
+    T acc[LPT][NDF_OUT];
 #pragma unroll
-    for (int i = 0; i < NDF_OUT; ++i) acc[i] = inc ? y[mo[i] + k] : T(0);
+    for (int q = 0; q < LPT; ++q)
 #pragma unroll
-    for (int j = 0; j < NDF_IN; ++j) {
-      const T xv = x[mi[j] + k];
+      for (int i = 0; i < NDF_OUT; ++i)
+        acc[q][i] = (inc && j + q * S < nl) ? y[mo[i] + j + q * S] : T(0);
 #pragma unroll
-      for (int i = 0; i < NDF_OUT; ++i) {
-        const T p = M[ik + n3 * ((int64_t)i + (int64_t)NDF_OUT * j)] * xv;
-        acc[i] = acc[i] + p;
+    for (int jj = 0; jj < NDF_IN; ++jj) {
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) {
+        const int k = j + q * S;
+        if (k < nl) {
+          const T xv = x[mi[jj] + k];
+#pragma unroll
+          for (int i = 0; i < NDF_OUT; ++i) {
+            const T p = Mc[q * S + n3 * ((int64_t)i + (int64_t)NDF_OUT * jj)] * xv;
+            acc[q][i] = acc[q][i] + p;
+          }
+        }
       }
     }
 #pragma unroll
-    for (int i = 0; i < NDF_OUT; ++i) y[mo[i] + k] = acc[i];
+    for (int q = 0; q < LPT; ++q)
+      if (j + q * S < nl)
+#pragma unroll
+        for (int i = 0; i < NDF_OUT; ++i) y[mo[i] + j + q * S] = acc[q][i];
   } else {
+    if (t >= (int64_t)ncell * nl) return;
+    const int c = (int)(t / nl);
+    const int k = (int)(t - (int64_t)c * nl);
+    const int64_t ik = t;
+    const int32_t* mo = map_out + (int64_t)c * ndf_out;
+    const int32_t* mi = map_in + (int64_t)c * ndf_in;
     for (int i = 0; i < ndf_out; ++i) {
       T acc = inc ? y[mo[i] + k] : T(0);
       for (int j = 0; j < ndf_in; ++j) {

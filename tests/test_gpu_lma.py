@@ -99,6 +99,7 @@ def test_transpose_bitexact(ctx, dtype, nc, nl, k, to, frm):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("nc,nl,k,to,frm", [(6, 10, 0, "W3", "W2"), (6, 10, 0, "W3", "W3"),
+                                            (4, 7, 0, "W3", "W2"), (4, 7, 0, "W3", "W3"),   # odd nl
                                             (3, 4, 1, "W3", "W2"), (3, 4, 1, "W3", "W3")])
 def test_dg_bitexact(ctx, dtype, nc, nl, k, to, frm):
     from lfric_core_amd import api
