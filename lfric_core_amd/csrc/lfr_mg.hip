@@ -209,7 +209,19 @@ __global__ void __launch_bounds__(256) k_mg_prolong(T* __restrict__ xf, const T*
   const int f = (int)(t >> lgR), r = (int)(t & (R - 1));
   const int64_t src = (int64_t)parent[f] * R + r;
   const int64_t pf = (int64_t)ncell_f * R, pc = (int64_t)ncell_c * R;
-  for (int k = 0; k < nl; ++k) xf[(int64_t)k * pf + t] += xc[(int64_t)k * pc + src];
+  // levels in groups of 8: the group's loads are issued before its stores
+  int k = 0;
+  for (; k + 8 <= nl; k += 8) {
+    T a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] = xf[(int64_t)(k + u) * pf + t];
+      b[u] = xc[(int64_t)(k + u) * pc + src];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xf[(int64_t)(k + u) * pf + t] = a[u] + b[u];
+  }
+  for (; k < nl; ++k) xf[(int64_t)k * pf + t] += xc[(int64_t)k * pc + src];
 }
 
 // per-rhs partial sums of r^2, r = b - A x; part[blk*R + r]
