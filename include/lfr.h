@@ -417,7 +417,10 @@ int lfr_enforce_bc(lfr_ctx* ctx, int prec, void* field_d, const lfr_space* space
  * columnwise_operators/sci_columnwise_op_appinv_kernel_mod.F90:77-145; the
  * kernel's doc says lhs += A^{-1} x but the code assigns).  cm is
  * columnwise_matrix(bandwidth=3, nrow, ncell_2d); indirection_d is the
- * device copy of indirection_dofmap(nrow); space gives map(1) of each column.
+ * device copy of indirection_dofmap(nrow); space gives map(1) of each column,
+ * and map(1) + nrow - 1 must not pass undf for any column.  The indirection
+ * is not read on the host: entries outside 1..nrow are clamped into the
+ * column (lfr_cma_create validates its own copy).
  * fp64_pivots != 0 runs the FP32 recurrence in FP64 (pivots, c', d') and
  * rounds the solution to FP32 (for ill-conditioned columns). */
 int lfr_columnwise_op_appinv(lfr_ctx* ctx, int prec, void* lhs_d, const void* x_d,
@@ -547,10 +550,166 @@ int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void*
  * rel_res_h (optional) receives the 16 final relative residuals.  The inner
  * iteration is conjugate_gradient_type%cg_solve
  * (components/science/source/solver/sci_iterative_solver_mod.f90:495-592) per
- * field; the batching has no counterpart in the reference. */
+ * field; the batching has no counterpart in the reference.  A64 must be a
+ * kind-0 FP64 operator without apply_bc.  Returns LFR_ERR_NOT_CONVERGED (with
+ * stats filled) when some field has not reached r_tol after max_outer
+ * corrections, as lfr_refined_cg does. */
 int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_op* A64, double r_tol,
                          int max_outer, double inner_r_tol, int inner_max_iter, void* x64_d,
                          const void* b64_d, int nrhs, lfr_refine_stats* stats, double* rel_res_h);
+
+
+/* ---------------------------------------------------------------------- */
+/* n-field vectors, abstract operators and preconditioners, and the        */
+/* iterative solvers of sci_iterative_solver_mod over them                 */
+/* ---------------------------------------------------------------------- */
+
+#define LFR_MAX_FIELDS 8
+
+/* Device image of a field_vector_type (r_def, prec 8) or
+ * r_solver_field_vector_type (prec 4) of nfield fields
+ * (infrastructure/source/field/vector_mod.F90:17-190;
+ * components/science/source/algorithm/solver/sci_field_vector_mod.x90:20-56):
+ * the fields lie end to end in one device buffer, field f (0-based) starting
+ * at element undf[0] + ... + undf[f-1].  The elementwise methods (axpy, aypx,
+ * axpby, scale, set_scalar, copy) are the lfr_vec_* calls over the whole
+ * buffer; the reductions below combine per-field inner products in field
+ * order, as field_vector_type does. */
+typedef struct lfr_vector_layout {
+  int prec;                        /* 4 (r_solver) or 8 (r_def)              */
+  int nfield;                      /* 1 .. LFR_MAX_FIELDS                    */
+  int64_t undf[LFR_MAX_FIELDS];    /* length of each field                   */
+} lfr_vector_layout;
+
+/* vector_size: the number of fields (sci_field_vector_mod.x90:341-347). */
+int lfr_vector_size(const lfr_vector_layout* layout);
+/* Total number of elements (the length of the device buffer). */
+int64_t lfr_vector_length(const lfr_vector_layout* layout);
+/* dot: sum over fields, in field order, of X_innerproduct_Y of each field
+ * (sci_field_vector_mod.x90:85-107); FP64 accumulation. */
+int lfr_vector_dot(lfr_ctx* ctx, const lfr_vector_layout* layout, const void* x_d, const void* y_d,
+                   double* out_h);
+/* norm = sqrt(sum over fields of X_innerproduct_X) (:59-73). */
+int lfr_vector_norm(lfr_ctx* ctx, const lfr_vector_layout* layout, const void* x_d, double* out_h);
+/* field_norm(n): sqrt(X_innerproduct_X) of field n, 1-based (:75-83). */
+int lfr_vector_field_norm(lfr_ctx* ctx, const lfr_vector_layout* layout, const void* x_d, int n,
+                          double* out_h);
+
+/* A caller's operator or preconditioner: y = A x (or y = P^{-1} x) on the
+ * vectors of the given layout, queued on `hip_stream` (or completed before
+ * returning).  Returns 0 on success; any other value aborts the solve with
+ * LFR_ERR_ARG.  It may call any lfr_* entry point on the same context except
+ * lfr_iterative_solve. */
+typedef int (*lfr_apply_fn)(void* user, const void* x_d, void* y_d, void* hip_stream);
+
+/* abstract_linear_operator_type (components/science/source/solver/
+ * sci_linear_operator_mod.f90:18-35): an object with apply(x, y), y = A x. */
+typedef struct lfr_linop lfr_linop;
+
+/* mass_matrix_operator_type (components/science/source/algorithm/solver/
+ * sci_mass_matrix_operator_alg_mod.x90:72-119): y = M x on one field, with
+ * enforce_bc after the apply when A->apply_bc; packed copies are used when
+ * A->packed is set. */
+int lfr_linop_lma(lfr_ctx* ctx, const lfr_lma_op* A, lfr_linop** out);
+
+/* One block of a block operator: y_row += A x_col. */
+typedef struct lfr_block_entry {
+  int row, col;                    /* 0-based fields of y and x              */
+  const void* matrix_d;            /* local_stencil(ncell_3d, ndf_to, ndf_from)
+                                      of kind layout->prec                   */
+  const lfr_space* to_space;       /* space of field `row`                   */
+  const lfr_space* from_space;     /* space of field `col`                   */
+  const lfr_plan* plan;            /* continuous output: a plan of to_space,
+                                      kind 0 (kind 1 with transpose); NULL:
+                                      discontinuous output (dg_inc_matrix_vector) */
+  int transpose;                   /* 1: y_row += A^T x_col
+                                      (transpose_matrix_vector; the matrix is
+                                      stored (from_space, to_space)^T, i.e.
+                                      local_stencil(ncell_3d, ndf_from, ndf_to)) */
+} lfr_block_entry;
+
+/* Block operator on n-field vectors: y = 0, then every entry adds its
+ * invoke in the order given (matrix_vector for a continuous output,
+ * dg_inc_matrix_vector for a discontinuous one, transpose_matrix_vector with
+ * transpose).  dense_operator_type (components/science/source/algorithm/
+ * solver/sci_dense_operator_alg_mod.x90:147-182) is the two-entry case
+ * {(0, 0, W0 mass, plan), (1, 1, W3 mass, NULL)}.  A field of y with a single
+ * entry is written by that entry's overwrite form (same values, y not read). */
+int lfr_linop_block(lfr_ctx* ctx, const lfr_vector_layout* layout, int nentry,
+                    const lfr_block_entry* entries, lfr_linop** out);
+
+/* An operator supplied by the caller (the op_cb of a solver call). */
+int lfr_linop_callback(lfr_ctx* ctx, const lfr_vector_layout* layout, lfr_apply_fn fn, void* user,
+                       lfr_linop** out);
+int lfr_linop_destroy(lfr_linop* op);
+/* y = A x (queued). */
+int lfr_linop_apply(lfr_ctx* ctx, const lfr_linop* op, const void* x_d, void* y_d);
+
+/* abstract_preconditioner_type (components/science/source/solver/
+ * sci_preconditioner_mod.f90:19-37): apply(x, y), y = P^{-1} x. */
+typedef struct lfr_precon lfr_precon;
+/* null_preconditioner_type: y = x (sci_null_preconditioner_alg_mod.x90:73-95). */
+int lfr_precon_null(lfr_ctx* ctx, const lfr_vector_layout* layout, lfr_precon** out);
+/* diagonal_preconditioner_type: y = x / diag, elementwise over the whole
+ * vector (sci_diagonal_preconditioner_alg_mod.x90:71-106); diag_d has the
+ * vector's layout and must stay valid while the preconditioner is used. */
+int lfr_precon_diagonal(lfr_ctx* ctx, const lfr_vector_layout* layout, const void* diag_d,
+                        lfr_precon** out);
+/* Column solve: y = columnwise_op_appinv(x) on a one-field vector of `space`
+ * (as lfr_columnwise_op_appinv: bandwidth 3, nrow rows per column). */
+int lfr_precon_columnwise(lfr_ctx* ctx, const lfr_vector_layout* layout, const void* cm_d, int nrow,
+                          const lfr_space* space, const int32_t* indirection_d, int fp64_pivots,
+                          lfr_precon** out);
+int lfr_precon_callback(lfr_ctx* ctx, const lfr_vector_layout* layout, lfr_apply_fn fn, void* user,
+                        lfr_precon** out);
+int lfr_precon_destroy(lfr_precon* p);
+/* y = P^{-1} x (queued). */
+int lfr_precon_apply(lfr_ctx* ctx, const lfr_precon* p, const void* x_d, void* y_d);
+
+/* The solver types of components/science/source/solver/sci_iterative_solver_mod.f90. */
+enum {
+  LFR_SOLVER_CG = 0,               /* conjugate_gradient_type  (:495-592)   */
+  LFR_SOLVER_BICGSTAB = 1,         /* BiCGstab_type            (:645-779)   */
+  LFR_SOLVER_GMRES = 2,            /* gmres_type               (:830-1003)  */
+  LFR_SOLVER_FGMRES = 3,           /* fgmres_type              (:1055-1237) */
+  LFR_SOLVER_GCR = 4,              /* gcr_type                 (:1288-1408) */
+  LFR_SOLVER_PRECONDITION_ONLY = 5,/* precondition_only_type   (:1664-1694) */
+  LFR_SOLVER_JACOBI = 6,           /* jacobi_type              (:1745-1815) */
+  LFR_SOLVER_CHEBYSHEV = 7         /* chebyshev_type           (:1868-1940) */
+};
+
+/* Constructor arguments of those types. */
+typedef struct lfr_iterative_solver {
+  int method;                      /* LFR_SOLVER_*                           */
+  double r_tol, a_tol;
+  int max_iter;                    /* iterations (GMRES, FGMRES, GCR: restarts) */
+  int monitor_convergence;
+  int fail_on_non_converged;
+  int gcrk;                        /* Krylov vectors per restart (GMRES, FGMRES, GCR) */
+  double rho_relax;                /* Jacobi relaxation factor               */
+  double lmin, lmax;               /* Chebyshev eigenvalue bounds            */
+} lfr_iterative_solver;
+
+/* solver%apply(x, b) with solver = the type named by s->method over lin_op A
+ * and preconditioner P; x, b: vectors of A's layout, x holds the initial
+ * guess (FGMRES and GCR add their correction to x; Chebyshev starts from
+ * zero; precondition_only overwrites x), exactly as the reference routines.
+ * Scalars are FP64 and stay on the device; they are cast to the vector kind
+ * before each update.  With monitor_convergence the residual norm is read
+ * back once per check.  stats->iterations counts iterations (GMRES, FGMRES:
+ * restarts; GCR: inner steps); stats->r_nrm is the last residual norm the
+ * method computed.  Returns LFR_ERR_NOT_CONVERGED when monitoring,
+ * fail_on_non_converged and not converged, LFR_ERR_NAN on a NaN residual. */
+int lfr_iterative_solve(lfr_ctx* ctx, const lfr_iterative_solver* s, const lfr_linop* A,
+                        const lfr_precon* P, void* x_d, const void* b_d, lfr_solver_stats* stats);
+
+/* invoke(mm_diagonal_kernel_type(diag, mass_matrix)): diag(map(df)+k) +=
+ * mass_matrix(ik, df, df) over all cells (components/science/source/kernel/
+ * algebra/sci_mm_diagonal_kernel_mod.F90), in the colour order of `plan`
+ * (any plan of `space`) for a continuous space, or plan = NULL for a
+ * discontinuous one; as used by dense_operator_type%extract_diagonal. */
+int lfr_mm_diagonal(lfr_ctx* ctx, int prec, void* diag_d, const void* matrix_d, const lfr_space* space,
+                    const lfr_plan* plan);
 
 #ifdef __cplusplus
 }

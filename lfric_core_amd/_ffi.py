@@ -99,6 +99,26 @@ SIGNATURES = {
     "lfr_refined_cg": None,
     "lfr_refined_cg_multi": None,
     "lfr_gmres": None,
+    # n-field vectors, abstract operators, preconditioners and the generic
+    # solvers: typed in lfric_core_amd.solvers
+    "lfr_vector_size": None,
+    "lfr_vector_length": None,
+    "lfr_vector_dot": None,
+    "lfr_vector_norm": None,
+    "lfr_vector_field_norm": None,
+    "lfr_linop_lma": None,
+    "lfr_linop_block": None,
+    "lfr_linop_callback": None,
+    "lfr_linop_destroy": [_P],
+    "lfr_linop_apply": [_P, _P, _P, _P],
+    "lfr_precon_null": None,
+    "lfr_precon_diagonal": None,
+    "lfr_precon_columnwise": None,
+    "lfr_precon_callback": None,
+    "lfr_precon_destroy": [_P],
+    "lfr_precon_apply": [_P, _P, _P, _P],
+    "lfr_iterative_solve": None,
+    "lfr_mm_diagonal": [_P, _I, _P, _P, _P, _P],
 }
 _RESTYPE = {"lfr_last_error": C.c_char_p, "lfr_get_stream": _P}
 

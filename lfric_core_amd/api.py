@@ -121,6 +121,8 @@ class Plan:
 
     def __init__(self, space: Space, colouring: Colouring, kind: int):
         self.space, self.kind = space, kind
+        # held so the colouring's id (the Space's plan cache key) stays unique
+        self.colouring = colouring
         cic = np.asfortranarray(colouring.cells_in_colour.astype(np.int32))
         npc = np.ascontiguousarray(colouring.ncells_per_colour, dtype=np.int32)
         self._keep = (cic, npc)

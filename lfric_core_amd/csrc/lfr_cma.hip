@@ -28,6 +28,14 @@ constexpr int kCols = 64;   // columns (lanes) per workgroup
 constexpr int kCh = 14;     // rows per LDS stage of the solution store
 constexpr int kPf = 10;     // rows of loads ahead of a column recurrence (<= 63 loads in flight)
 
+// indirection_dofmap(i) - 1, clamped into the column's rows 0..nrow-1 so that
+// no access leaves the column (lfr_cma_create validates its maps; the raw
+// pointer given to lfr_columnwise_op_appinv is not read on the host)
+__device__ __forceinline__ int ind0(const int32_t* __restrict__ ind, int i, int nrow) {
+  const int v = ind[i] - 1;
+  return v < 0 ? 0 : (v >= nrow ? nrow - 1 : v);
+}
+
 template <typename T, typename C, int NROW>
 __global__ void __launch_bounds__(kCols)
 k_appinv(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
@@ -44,7 +52,7 @@ k_appinv(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
   const int c0 = blockIdx.x * kCols;
   const int ncols = min(kCols, ncell - c0);
   if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
-  for (int i = lane; i < NROW; i += kCols) sind[i] = ind[i] - 1;
+  for (int i = lane; i < NROW; i += kCols) sind[i] = ind0(ind, i, NROW);
   C* cp = scp + lane * PC;
   C dp[NROW];
   // forward sweep straight from global memory: lane = column, its bands
@@ -66,7 +74,7 @@ k_appinv(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
         la[t] = mb[3 * t + 0];
         ld[t] = mb[3 * t + 1];
         lu[t] = mb[3 * t + 2];
-        lx[t] = xb[ind[t] - 1];
+        lx[t] = xb[ind0(ind, t, NROW)];
       }
       if (t >= kPf) {
         const int i = t - kPf;
@@ -136,7 +144,7 @@ k_appinv_staged(T* __restrict__ lhs, const T* __restrict__ x, const T* __restric
   const int c0 = blockIdx.x * kCols;
   const int ncols = min(kCols, ncell - c0);
   if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
-  for (int i = lane; i < NROW; i += kCols) sind[i] = ind[i] - 1;
+  for (int i = lane; i < NROW; i += kCols) sind[i] = ind0(ind, i, NROW);
   T cp[NROW], dp[NROW];
 #pragma unroll
   for (int s = 0; s < NST; ++s) {
@@ -245,7 +253,7 @@ k_appinv_generic(T* __restrict__ lhs, const T* __restrict__ x, const T* __restri
   const int base = map0[(int64_t)c * ndf];
   for (int i = 0; i < nrow; ++i) {
     const C a = (C)m[3 * i], d = (C)m[3 * i + 1], u = (C)m[3 * i + 2];
-    const C xv = (C)x[base + ind[i] - 1];
+    const C xv = (C)x[base + ind0(ind, i, nrow)];
     if (i == 0) {
       const C inv = C(1) / d;
       cpc[i] = inv * u;
@@ -263,7 +271,7 @@ k_appinv_generic(T* __restrict__ lhs, const T* __restrict__ x, const T* __restri
       const C cd = cpc[i] * dpc[i + 1];
       dpc[i] = dpc[i] - cd;
     }
-    lhs[base + ind[i] - 1] = (T)dpc[i];
+    lhs[base + ind0(ind, i, nrow)] = (T)dpc[i];
   }
 }
 
@@ -313,6 +321,10 @@ extern "C" int lfr_columnwise_op_appinv(lfr_ctx* ctx, int prec, void* lhs, const
   LFR_REQUIRE(ctx, bandwidth == 3, "lfr_columnwise_op_appinv: tridiagonal only (bandwidth %d)", bandwidth);
   LFR_REQUIRE(ctx, ncell_2d == space->ncell && nrow > 0,
               "lfr_columnwise_op_appinv: ncell_2d %d vs space %d, nrow %d", ncell_2d, space->ncell, nrow);
+  // every column's rows map(1) .. map(1)+nrow-1 (1-based) must be dofs of the space
+  LFR_REQUIRE(ctx, (int64_t)space->max_first + nrow - 1 <= space->undf,
+              "lfr_columnwise_op_appinv: nrow %d runs past undf %d from a column starting at dof %d", nrow,
+              space->undf, space->max_first);
   if (prec == 4) {
     if (fp64_pivots)
       return launch_appinv<float, double>(ctx, (float*)lhs, (const float*)x, (const float*)cm,

@@ -1,6 +1,10 @@
 // Context, memory and function-space handles of liblfr.so.
+#include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
+#include <unordered_set>
 
 #include "lfr_internal.h"
 
@@ -28,6 +32,35 @@ int lfr_work_reserve(lfr_ctx* ctx, size_t bytes) {
     return lfr_set_error(ctx, LFR_ERR_NOMEM, "workspace allocation of %zu bytes failed", want);
   ctx->work_bytes = want;
   return LFR_OK;
+}
+
+uint64_t lfr_next_uid() {
+  static std::atomic<uint64_t> next{0};
+  return ++next;
+}
+
+// Contexts not yet finalised: a handle may outlive its context (a caller that
+// finalises first), and then has no graph cache to evict from.
+static std::mutex g_live_mu;
+static std::unordered_set<const lfr_ctx*> g_live;
+
+void lfr_graphs_evict(lfr_ctx* ctx, uint64_t uid) {
+  if (!ctx || !uid) return;
+  {
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    if (!g_live.count(ctx)) return;
+  }
+  for (size_t i = 0; i < ctx->graphs.size();) {
+    const auto& d = ctx->graphs[i].deps;
+    bool hit = false;
+    for (uint64_t u : d) hit = hit || u == uid;
+    if (hit) {
+      (void)hipGraphExecDestroy(ctx->graphs[i].exec);
+      ctx->graphs.erase(ctx->graphs.begin() + i);
+    } else {
+      ++i;
+    }
+  }
 }
 
 extern "C" {
@@ -81,17 +114,26 @@ int lfr_init(int device, lfr_ctx** out) {
     delete ctx;
     return rc;
   }
+  {
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    g_live.insert(ctx);
+  }
   *out = ctx;
   return LFR_OK;
 }
 
 int lfr_finalise(lfr_ctx* ctx) {
   if (!ctx) return LFR_OK;
+  {
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    g_live.erase(ctx);
+  }
   (void)hipStreamSynchronize(ctx->stream);
-  for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->work) (void)hipFree(ctx->work);
+  if (ctx->kwork) (void)hipFree(ctx->kwork);
   if (ctx->host_scalars) (void)hipHostFree(ctx->host_scalars);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
@@ -159,15 +201,15 @@ int lfr_space_create(lfr_ctx* ctx, const int32_t* map, int ndf, int ncell, int n
                            (int)(q % ndf) + 1, (int)(q / ndf) + 1, v, undf);
     m0[q] = v - 1;
   }
-  static uint64_t next_uid = 0;
   lfr_space* s = new lfr_space();
   s->ctx = ctx;
-  s->uid = ++next_uid;
+  s->uid = lfr_next_uid();
   s->ndf = ndf;
   s->ncell = ncell;
   s->nlayers = nlayers;
   s->undf = undf;
   s->map_h.assign(map, map + n);
+  for (int c = 0; c < ncell; ++c) s->max_first = std::max(s->max_first, (int)map[(int64_t)c * ndf]);
   if (hipMalloc(&s->map0_d, n * sizeof(int32_t)) != hipSuccess) {
     delete s;
     return lfr_set_error(ctx, LFR_ERR_NOMEM, "dofmap allocation failed");
@@ -183,6 +225,7 @@ int lfr_space_create(lfr_ctx* ctx, const int32_t* map, int ndf, int ncell, int n
 
 int lfr_space_destroy(lfr_space* s) {
   if (!s) return LFR_OK;
+  lfr_graphs_evict(s->ctx, s->uid);
   if (s->map0_d) (void)hipFree(s->map0_d);
   delete s;
   return LFR_OK;

@@ -11,6 +11,21 @@
 
 #include "../../include/lfr.h"
 
+// An instantiated solver graph.  `key` holds every pointer and scalar the
+// captured kernels were given, with library handles named by their uid;
+// `deps` lists the uids of the handles whose device buffers the graph reads
+// (plans, packed operators, spaces), so destroying one of them evicts it.
+struct lfr_graph_entry {
+  std::vector<uint64_t> key;
+  std::vector<uint64_t> deps;
+  hipGraphExec_t exec = nullptr;
+};
+
+// Process-wide identifier for library handles; never reused.
+uint64_t lfr_next_uid();
+// Destroy every cached graph of ctx that depends on the handle `uid`.
+void lfr_graphs_evict(lfr_ctx* ctx, uint64_t uid);
+
 struct lfr_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
@@ -24,15 +39,20 @@ struct lfr_ctx {
   int cu_count = 256;
   // events marking scalar read-backs (solver chunks in flight)
   hipEvent_t ev[2] = {nullptr, nullptr};
-  // instantiated solver graphs, most recently used last; the key holds
-  // every pointer and scalar the captured kernels were given
-  std::vector<std::pair<std::vector<uint64_t>, hipGraphExec_t>> graphs;
+  // instantiated solver graphs, most recently used last
+  std::vector<lfr_graph_entry> graphs;
+  // grow-only device workspace of the generic iterative solvers (their
+  // Krylov vectors, scalars and partial sums), apart from `work` so that an
+  // operator or preconditioner applied inside a solve may use `work`
+  void* kwork = nullptr;
+  size_t kwork_bytes = 0;
 };
 
 struct lfr_space {
   lfr_ctx* ctx = nullptr;
   uint64_t uid = 0;                 // unique per created space (never reused)
   int ndf = 0, ncell = 0, nlayers = 0, undf = 0;
+  int max_first = 0;                // largest map(1, cell) (1-based) over the cells
   int32_t* map0_d = nullptr;        // map(ndf, ncell) converted to 0-based, device
   std::vector<int32_t> map_h;       // 1-based host copy (plan construction)
   std::vector<int32_t> boundary_h;  // boundary_value(ndf, 2) for enforce_bc (empty: not set)
@@ -59,6 +79,7 @@ struct lfr_plan_class {
 
 struct lfr_plan {
   lfr_ctx* ctx = nullptr;
+  uint64_t uid = 0;                 // unique per created plan (never reused)
   const lfr_space* space = nullptr;
   int kind = 0;
   int ncolumns = 0;
@@ -104,6 +125,7 @@ unsigned lfr_class_blocks(const lfr_plan_class& c, int lpt);
 // level (class heights), zero where an entry has no cell layer.
 struct lfr_lma_packed {
   lfr_ctx* ctx = nullptr;
+  uint64_t uid = 0;                 // unique per packed operator (never reused)
   int prec = 0;
   int kind = 0;
   const lfr_plan* plan = nullptr;
@@ -119,6 +141,13 @@ struct lfr_lma_packed {
 };
 
 int lfr_set_error(lfr_ctx* ctx, int code, const char* fmt, ...);
+
+// Validate a square solver operator (kind-0 plan of its space, packed copy
+// of the same plan) and build its column records outside any graph capture.
+int lfr_check_op(lfr_ctx* ctx, const lfr_lma_op* A, const char* who);
+// Queue y = A x for a checked square operator (enforce_bc after the apply
+// when A->apply_bc), as mass_matrix_operator_type%apply.
+int lfr_lma_apply_queue(lfr_ctx* ctx, const lfr_lma_op* A, const void* x, void* y);
 
 // Output and input field sizes (undf) of a multi-field operator handle.
 extern "C" int lfr_multi_space_undf(const lfr_multi* h, int* undf_to, int* undf_from);

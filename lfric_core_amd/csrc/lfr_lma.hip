@@ -459,6 +459,7 @@ extern "C" int lfr_plan_create(lfr_ctx* ctx, const lfr_space* sp, const int32_t*
   }
 
   lfr_plan* plan = new lfr_plan();
+  plan->uid = lfr_next_uid();
   plan->ctx = ctx;
   plan->space = sp;
   plan->kind = kind;
@@ -672,6 +673,7 @@ const int32_t* lfr_plan_urecords(lfr_ctx* ctx, const lfr_plan* plan, const lfr_s
 
 extern "C" int lfr_plan_destroy(lfr_plan* plan) {
   if (!plan) return LFR_OK;
+  lfr_graphs_evict(plan->ctx, plan->uid);
   for (auto& kv : plan->records)
     for (int32_t* p : kv.second)
       if (p) (void)hipFree(p);
@@ -776,6 +778,7 @@ extern "C" int lfr_lma_pack(lfr_ctx* ctx, int prec, const void* M, const lfr_spa
     LFR_REQUIRE(ctx, c.ncol == 0 || lfr_record_shape(in->ndf, c.ngroup, c.gsize),
                 "lfr_lma_pack: plan class (%d cells x %d dofs) has no packed kernel", c.ngroup, c.gsize);
   auto* pk = new lfr_lma_packed();
+  pk->uid = lfr_next_uid();
   pk->ctx = ctx;
   pk->prec = prec;
   pk->kind = plan->kind;
@@ -877,6 +880,7 @@ extern "C" int lfr_lma_pack(lfr_ctx* ctx, int prec, const void* M, const lfr_spa
 
 extern "C" int lfr_lma_packed_destroy(lfr_lma_packed* pk) {
   if (!pk) return LFR_OK;
+  lfr_graphs_evict(pk->ctx, pk->uid);
   if (pk->data_d) (void)hipFree(pk->data_d);
   delete pk;
   return LFR_OK;

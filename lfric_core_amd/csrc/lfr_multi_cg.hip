@@ -169,8 +169,12 @@ extern "C" int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_
                                     double* rel_res_h) {
   if (!ctx || !A32 || !A64) return LFR_ERR_ARG;
   LFR_REQUIRE(ctx, nrhs == kR, "lfr_refined_cg_multi: nrhs must be %d", kR);
-  LFR_REQUIRE(ctx, A64->prec == 8 && A64->space && A64->plan && A64->matrix_d,
-              "lfr_refined_cg_multi: A64 must be an FP64 operator with its plan");
+  LFR_REQUIRE(ctx, A64->prec == 8, "lfr_refined_cg_multi: A64 must be an FP64 operator");
+  int rc0 = lfr_check_op(ctx, A64, "lfr_refined_cg_multi(A64)");
+  if (rc0) return rc0;
+  // the FP64 residual is a multidata apply of A64's matrix in the reference
+  // layout; enforce_bc has no multidata form here
+  LFR_REQUIRE(ctx, !A64->apply_bc, "lfr_refined_cg_multi: apply_bc is not supported for the 16-field residual");
   int ut = 0, uf = 0;
   lfr_multi_space_undf(A32, &ut, &uf);
   LFR_REQUIRE(ctx, ut == uf && ut == A64->space->undf,
@@ -280,5 +284,9 @@ extern "C" int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_
     stats->b_nrm = bmax;
     stats->r_nrm = worst;
   }
+  if (!conv)
+    return lfr_set_error(ctx, LFR_ERR_NOT_CONVERGED,
+                         "lfr_refined_cg_multi: not converged after %d corrections (worst ||r||/||b|| = %g)", outer,
+                         worst);
   return LFR_OK;
 }

@@ -543,7 +543,14 @@ int apply_op(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv,
   return LFR_OK;
 }
 
-int check_op(lfr_ctx* ctx, const lfr_lma_op* A, const char* who) {
+}  // namespace
+
+int lfr_lma_apply_queue(lfr_ctx* ctx, const lfr_lma_op* A, const void* x, void* y) {
+  if (A->prec == 4) return apply_op<float>(ctx, A, (const float*)x, (float*)y, nullptr, nullptr, nullptr);
+  return apply_op<double>(ctx, A, (const double*)x, (double*)y, nullptr, nullptr, nullptr);
+}
+
+int lfr_check_op(lfr_ctx* ctx, const lfr_lma_op* A, const char* who) {
   LFR_REQUIRE(ctx, A && A->matrix_d && A->space && A->plan, "%s: incomplete operator", who);
   LFR_REQUIRE(ctx, A->prec == 4 || A->prec == 8, "%s: prec %d", who, A->prec);
   LFR_REQUIRE(ctx, A->plan->space == A->space && A->plan->kind == 0,
@@ -569,6 +576,8 @@ int check_op(lfr_ctx* ctx, const lfr_lma_op* A, const char* who) {
   }
   return LFR_OK;
 }
+
+namespace {
 
 bool use_graphs() {
   const char* e = getenv("LFR_NO_GRAPH");
@@ -612,8 +621,8 @@ int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P,
     memcpy(&rt, &P->r_tol, 8);
     memcpy(&at, &P->a_tol, 8);
     std::vector<uint64_t> key = {
-        0xC6ull, sizeof(T), (uint64_t)(uintptr_t)A->matrix_d, (uint64_t)(uintptr_t)A->packed,
-        (uint64_t)(uintptr_t)A->plan, (uint64_t)(uintptr_t)A->space, (uint64_t)A->apply_bc,
+        0xC6ull, sizeof(T), (uint64_t)(uintptr_t)A->matrix_d, A->packed ? A->packed->uid : 0,
+        A->plan->uid, A->space->uid, (uint64_t)A->apply_bc,
         (uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)r, (uint64_t)(uintptr_t)p, (uint64_t)(uintptr_t)z,
         (uint64_t)(uintptr_t)w.part_apply, (uint64_t)(uintptr_t)w.part_rr, (uint64_t)(uintptr_t)w.part_rz,
         (uint64_t)(uintptr_t)w.S, (uint64_t)(uintptr_t)dg, (uint64_t)P->max_iter,
@@ -622,11 +631,11 @@ int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P,
     if (A->apply_bc)
       for (int32_t v : A->space->boundary_h) key.push_back((uint64_t)(uint32_t)v);
     for (size_t i = 0; i < ctx->graphs.size(); ++i)
-      if (ctx->graphs[i].first == key) {
-        auto e = ctx->graphs[i];
+      if (ctx->graphs[i].key == key) {
+        lfr_graph_entry e = ctx->graphs[i];
         ctx->graphs.erase(ctx->graphs.begin() + i);
         ctx->graphs.push_back(e);
-        exec = e.second;
+        exec = e.exec;
         break;
       }
     if (!exec) {
@@ -648,10 +657,14 @@ int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P,
       if (exec) {
         constexpr size_t kKeep = 8;
         if (ctx->graphs.size() >= kKeep) {
-          (void)hipGraphExecDestroy(ctx->graphs.front().second);
+          (void)hipGraphExecDestroy(ctx->graphs.front().exec);
           ctx->graphs.erase(ctx->graphs.begin());
         }
-        ctx->graphs.emplace_back(key, exec);
+        lfr_graph_entry e;
+        e.key = key;
+        e.deps = {A->plan->uid, A->space->uid, A->packed ? A->packed->uid : 0};
+        e.exec = exec;
+        ctx->graphs.push_back(e);
       }
     }
   }
@@ -763,7 +776,7 @@ int cg_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x
 extern "C" int lfr_cg(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, void* x,
                       const void* b, lfr_solver_stats* st) {
   if (!ctx) return LFR_ERR_ARG;
-  int rc = check_op(ctx, A, "lfr_cg");
+  int rc = lfr_check_op(ctx, A, "lfr_cg");
   if (rc) return rc;
   LFR_REQUIRE(ctx, P && x && b && st, "lfr_cg: null argument");
   LFR_REQUIRE(ctx, P->preconditioner == 0 || (P->preconditioner == 1 && P->diag_d),
@@ -782,9 +795,9 @@ extern "C" int lfr_refined_cg(lfr_ctx* ctx, const lfr_lma_op* A32, const lfr_lma
                               double r_tol, int max_outer, const lfr_solver_params* inner,
                               void* x64_v, const void* b64_v, lfr_refine_stats* st) {
   if (!ctx) return LFR_ERR_ARG;
-  int rc = check_op(ctx, A32, "lfr_refined_cg(A32)");
+  int rc = lfr_check_op(ctx, A32, "lfr_refined_cg(A32)");
   if (rc) return rc;
-  rc = check_op(ctx, A64, "lfr_refined_cg(A64)");
+  rc = lfr_check_op(ctx, A64, "lfr_refined_cg(A64)");
   if (rc) return rc;
   LFR_REQUIRE(ctx, A32->prec == 4 && A64->prec == 8, "lfr_refined_cg: need fp32 and fp64 operators");
   LFR_REQUIRE(ctx, A32->space->undf == A64->space->undf, "lfr_refined_cg: operator sizes differ");
@@ -1083,7 +1096,7 @@ int gmres_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T
 extern "C" int lfr_gmres(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, void* x,
                          const void* b, lfr_solver_stats* st) {
   if (!ctx) return LFR_ERR_ARG;
-  int rc = check_op(ctx, A, "lfr_gmres");
+  int rc = lfr_check_op(ctx, A, "lfr_gmres");
   if (rc) return rc;
   LFR_REQUIRE(ctx, P && x && b && st && P->gcrk > 0 && P->gcrk <= 512, "lfr_gmres: bad arguments");
   LFR_REQUIRE(ctx, P->preconditioner == 0 || (P->preconditioner == 1 && P->diag_d),

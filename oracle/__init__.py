@@ -361,6 +361,34 @@ def prolong_w2(fine, coarse, map_coarse, map_fine, cell_map, nlayers):
     return fine
 
 
+def mm_diagonal(diag, matrix, dofmap, nlayers, colouring=None, use_ref=False):
+    """invoke(mm_diagonal_kernel_type(diag, matrix)): diag += the local
+    diagonals, colour by colour (colouring given) or cell by cell."""
+    dt = diag.dtype
+    n3 = matrix.shape[2]
+    ncell, ndf = dofmap.shape
+    dm = np.ascontiguousarray(dofmap, dtype=np.int32)
+    if colouring is not None:
+        cic = np.asfortranarray(colouring.cells_in_colour).ravel(order="F").astype(np.int32)
+        npc = colouring.ncells_per_colour.astype(np.int32)
+        ncol = colouring.ncolours
+    else:
+        cic = np.zeros(1, np.int32)
+        npc = np.zeros(1, np.int32)
+        ncol = 0
+    vp = C.c_void_p
+    if use_ref:
+        fn = getattr(ref(), f"ref_invoke_mm_diagonal_{_suf(dt)}")
+        fn(_I(ncol), npc.ctypes.data_as(vp), cic.ctypes.data_as(vp), _I(ncell), _I(nlayers),
+           diag.ctypes.data_as(vp), _I(n3), matrix.ctypes.data_as(vp), _I(ndf), _I(diag.size),
+           dm.ctypes.data_as(vp))
+        return diag
+    fn = getattr(lib(), f"orc_mm_diagonal_{_suf(dt)}")
+    fn(_I(ncol), npc.ctypes.data_as(vp), cic.ctypes.data_as(vp), _I(ncell), _I(nlayers),
+       diag.ctypes.data_as(vp), C.c_int64(n3), matrix.ctypes.data_as(vp), _I(ndf), dm.ctypes.data_as(vp))
+    return diag
+
+
 def dot(x, y, nthreads=None) -> float:
     nt = nthreads_default() if nthreads is None else nthreads
     fn = getattr(lib(), f"orc_dot_{_suf(x.dtype)}")

@@ -72,8 +72,13 @@ def cg_reverse(apply_t, b, hist, xbar, want_outer=None):
     return rbar
 
 
-def loss_and_grad(cfg, p, w, iters=20, M=None, D=None, use_ref=None, dtype=np.float64):
-    """L = <w, CG_iters(M, D^T p)> and dL/dp, dL/dM on the CPU."""
+def loss_and_grad(cfg, p, w, iters=20, M=None, D=None, use_ref=None, dtype=np.float64, M_dirs=None):
+    """L = <w, CG_iters(M, D^T p)> and dL/dp, dL/dM on the CPU.
+
+    With ``M_dirs`` (a list of arrays shaped like M) the third result is the
+    list of directional derivatives <dL/dM, dM> instead of dL/dM itself,
+    formed as sum_i <qbar_i, dM p_i> with one apply of dM per iteration (the
+    matrix gradient is sum_i qbar_i (x) p_i per cell-layer)."""
     use_ref = oracle.ref() is not None if use_ref is None else use_ref
     M = (cfg.mass if M is None else M).astype(dtype)
     D = (cfg.div if D is None else D).astype(dtype)
@@ -93,6 +98,18 @@ def loss_and_grad(cfg, p, w, iters=20, M=None, D=None, use_ref=None, dtype=np.fl
                                use_ref=use_ref)
     u, hist = cg_forward_record(A, b, iters)
     L = _dot(w, u)
+    if M_dirs is not None:
+        proj = [0.0] * len(M_dirs)
+
+        def outer(qbar, p_i):
+            for q, dM in enumerate(M_dirs):
+                dMp = oracle.invoke_coloured(0, col, nl, np.zeros_like(p_i), p_i, dM.astype(dtype), dm2, dm2,
+                                             use_ref=use_ref)
+                proj[q] += _dot(qbar, dMp)
+
+        bbar = cg_reverse(AT, b, hist, w, outer)
+        pbar = oracle.invoke_dg(0, nl, np.zeros(cfg.w3.undf, dtype), bbar, D, dm3, dm2, use_ref=use_ref)
+        return L, pbar, proj
     Mbar = np.zeros_like(M)
     # dof of (cell-layer ik, local dof) on W2
     idx = (dm2[:, None, :] - 1 + np.arange(nl)[None, :, None]).reshape(-1, dm2.shape[1])

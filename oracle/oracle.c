@@ -632,3 +632,32 @@ DEF_RESTRICT_W2(double, r64)
   }
 DEF_PROLONG_W2(float, r32)
 DEF_PROLONG_W2(double, r64)
+
+/* mm_diagonal_kernel_code (components/science/source/kernel/algebra/
+ * sci_mm_diagonal_kernel_mod.F90): diag(map(df)+k) += M(ik, df, df), k outer,
+ * df inner, cells colour by colour (ncolour > 0) or in order (ncolour = 0). */
+#define ORC_MMDIAG(T, SUF)                                                                   \
+  void orc_mm_diagonal_##SUF(int ncolour, const int* npc, const int* cic, int ncell, int nl,  \
+                             T* diag, int64_t n3, const T* m, int ndf, const int* dm) {      \
+    int64_t total = ncolour > 0 ? 0 : ncell;                                                 \
+    for (int col = 0; col < ncolour; ++col) total += npc[col];                               \
+    int col = 0, idx = 0;                                                                    \
+    for (int64_t q = 0; q < total; ++q) {                                                    \
+      int cell;                                                                              \
+      if (ncolour == 0) {                                                                    \
+        cell = (int)q + 1;                                                                   \
+      } else {                                                                               \
+        while (idx >= npc[col]) { ++col; idx = 0; }                                          \
+        cell = cic[col + (int64_t)ncolour * idx];                                            \
+        ++idx;                                                                               \
+      }                                                                                      \
+      const int* map = dm + (int64_t)(cell - 1) * ndf;                                       \
+      for (int k = 0; k < nl; ++k) {                                                         \
+        const int64_t ik = (int64_t)(cell - 1) * nl + k;                                     \
+        for (int df = 0; df < ndf; ++df)                                                     \
+          diag[map[df] - 1 + k] = diag[map[df] - 1 + k] + m[ik + n3 * (df + (int64_t)ndf * df)]; \
+      }                                                                                      \
+    }                                                                                        \
+  }
+ORC_MMDIAG(float, r32)
+ORC_MMDIAG(double, r64)

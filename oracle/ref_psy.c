@@ -216,3 +216,32 @@ void ref_invoke_operator_tri_solve_r64(int ncell, int nlayers, double* y, const 
   }
 REF_TRANSFER(float, r_single)
 REF_TRANSFER(double, r_double)
+
+/* mm_diagonal_kernel_code over all cells: colour by colour for a continuous
+ * space (ncolour > 0, cells_in_colour(ncolour, maxc)), a plain cell loop for
+ * a discontinuous one (ncolour = 0). */
+#define DEF_MMDIAG(T, SUF, S1)                                                              \
+  extern void FK(sci_mm_diagonal_kernel_mod, mm_diagonal_kernel_code_##S1)(                \
+      const int* cell, const int* nlayers, T* diag, const int* ncell_3d, const T* m,         \
+      const int* ndf, const int* undf, const int* map);                                     \
+  void ref_invoke_mm_diagonal_##SUF(int ncolour, const int* npc, const int* cic, int ncell, \
+                                    int nl, T* diag, int n3, const T* m, int ndf, int undf, \
+                                    const int* dm) {                                        \
+    if (ncolour == 0) {                                                                     \
+      for (int c = 0; c < ncell; ++c) {                                                     \
+        int cell = c + 1;                                                                   \
+        FK(sci_mm_diagonal_kernel_mod, mm_diagonal_kernel_code_##S1)(                      \
+            &cell, &nl, diag, &n3, m, &ndf, &undf, dm + (int64_t)c * ndf);                   \
+      }                                                                                     \
+      return;                                                                               \
+    }                                                                                       \
+    for (int col = 0; col < ncolour; ++col)                                                 \
+      for (int idx = 0; idx < npc[col]; ++idx) {                                            \
+        int cell = cic[col + (int64_t)ncolour * idx];                                       \
+        FK(sci_mm_diagonal_kernel_mod, mm_diagonal_kernel_code_##S1)(                      \
+            &cell, &nl, diag, &n3, m, &ndf, &undf, dm + (int64_t)(cell - 1) * ndf);          \
+      }                                                                                     \
+  }
+
+DEF_MMDIAG(float, r32, r_single)
+DEF_MMDIAG(double, r64, r_double)

@@ -15,7 +15,7 @@ def declared():
     names = set()
     for h in HEADERS:
         text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
-        names.update(re.findall(r"^\s*(?:int|const char\*|void\*)\s+(lfr_\w+)\s*\(", text, flags=re.M))
+        names.update(re.findall(r"^\s*(?:int|int64_t|const char\*|void\*)\s+(lfr_\w+)\s*\(", text, flags=re.M))
     return sorted(names)
 
 

@@ -28,6 +28,39 @@ def test_panel_pattern_even():
     assert c.colour_map[:4].tolist() == [1, 2, 3, 4]
 
 
+# The reference's pattern diagram for a cube with 2x2 cells per panel
+# (mesh_colouring_mod.F90:215-242): the cell-number diagram places every
+# global cell id, the even-pattern diagram gives the colour at each place;
+# read together, the colour of cells 1..24 is:
+C2_DIAGRAM_COLOURS = [
+    1, 2, 3, 4,        # panel 1:  |1 2| over |3 4|, colours |1 2| over |3 4|
+    3, 2, 1, 4,        # panel 2:  |5 6| over |7 8|, colours |3 2| over |1 4|
+    3, 4, 1, 2,        # panel 3:  |9 10| over |11 12|, colours |3 4| over |1 2|
+    1, 4, 3, 2,        # panel 4:  |13 14| over |15 16|, colours |1 4| over |3 2|
+    3, 2, 4, 1,        # panel 5:  |18 20| over |17 19|, colours |2 1| over |3 4|
+    2, 3, 1, 4,        # panel 6:  |23 21| over |24 22|, colours |1 2| over |4 3|
+]
+
+
+def test_colour_map_c2_equals_the_reference_diagram():
+    c = colour_cells(cubed_sphere(2))
+    assert c.colour_map.tolist() == C2_DIAGRAM_COLOURS
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_colour_map_even_tiles_the_c2_pattern_on_every_panel(n):
+    """Each panel repeats its 2x2 pattern: the cell at (ix, iy) of a panel
+    (global id (p-1) n^2 + iy n + ix + 1, ix fastest) takes the colour of the C2
+    cell at (ix mod 2, iy mod 2) of the same panel."""
+    c = colour_cells(cubed_sphere(n))
+    for p in range(6):
+        for iy in range(n):
+            for ix in range(n):
+                gid = p * n * n + iy * n + ix + 1
+                want = C2_DIAGRAM_COLOURS[p * 4 + (iy % 2) * 2 + (ix % 2)]
+                assert c.colour_map[gid - 1] == want, (n, p + 1, ix, iy)
+
+
 def test_conflict_detected():
     m = cubed_sphere(4)
     bad = np.ones(m.ncells, dtype=np.int32)

@@ -1,0 +1,157 @@
+"""The reference's own kernel test cases and the smallest column heights on the
+GPU: the pFUnit 4x2 single-cell, one-layer cases of
+components/science/unit-test/kernel/algebra/{matrix_vector,dg_matrix_vector,
+dg_inc_matrix_vector,transpose_matrix_vector}_mod_test.pf through the lfr_*
+invokes, and nlayers = 1, 2, 3 on cubed spheres (below every kernel's
+levels-per-thread, so the one-level paths run) for every apply kind, FP32
+and FP64, reference layout and packed, bit-exact against the oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from lfric_core_amd.configs import normal_lma
+from lfric_core_amd.function_space import FunctionSpace, build_dofmap, colour_cells
+from lfric_core_amd.mesh import cubed_sphere
+from lfric_core_amd.rng import Stream
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+# matrix_vector_mod_test.pf:66-72 and transpose_matrix_vector_mod_test.pf:54
+D = np.array([[2.0, -2.0], [3.0, -1.0], [0.4, 1.3], [-4.5, 7.9]])
+X2 = np.array([0.25, 1.5])
+X4 = np.array([0.25, 1.5, -0.35, 5.0])
+
+
+def _t(ctx, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(ctx.device)
+
+
+class _Cell:
+    """One cell, one layer, ndf dofs numbered 1..ndf (the pFUnit set-up)."""
+
+    def __init__(self, ctx, ndf):
+        from lfric_core_amd._ffi import check, lib
+        self.map = np.arange(1, ndf + 1, dtype=np.int32).reshape(1, ndf)
+        h = C.c_void_p()
+        check(lib().lfr_space_create(ctx.h, self.map.ctypes.data_as(C.c_void_p), ndf, 1, 1, ndf,
+                                     C.byref(h)), ctx.h)
+        self.h, self.ndf = h, ndf
+
+    def plan(self, ctx, kind):
+        from lfric_core_amd._ffi import check, lib
+        cic = np.array([[1]], dtype=np.int32)
+        npc = np.array([1], dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().lfr_plan_create(ctx.h, self.h, cic.ctypes.data_as(C.c_void_p),
+                                    npc.ctypes.data_as(C.c_void_p), 1, 1, kind, C.byref(h)), ctx.h)
+        return h
+
+
+@pytest.mark.parametrize("dtype,prec", [(np.float64, 8), (np.float32, 4)])
+def test_pfunit_cases_through_the_invokes(ctx, dtype, prec):
+    from lfric_core_amd._ffi import check, lib
+    L = lib()
+    P = C.c_void_p
+    print(f"case pfunit 4x2 nlayers=1 prec={prec}", flush=True)
+    to, fr = _Cell(ctx, 4), _Cell(ctx, 2)
+    m = np.ascontiguousarray(D.T.reshape(2, 4, 1)).astype(dtype)   # local_stencil(1, 4, 2)
+    md = _t(ctx, m)
+    ans = D @ X2
+    for name in ("matrix_vector", "dg_matrix_vector", "dg_inc_matrix_vector"):
+        y0 = np.full(4, 0.0 if name != "dg_matrix_vector" else 99.0, dtype=dtype)
+        ref = oracle.kernel(name, 1, 1, y0.copy(), X2.astype(dtype), m, [1, 2, 3, 4], [1, 2])
+        y = _t(ctx, y0)
+        x = _t(ctx, X2.astype(dtype))
+        if name == "matrix_vector":
+            check(L.lfr_matrix_vector(ctx.h, prec, P(y.data_ptr()), P(x.data_ptr()), P(md.data_ptr()), to.h,
+                                      fr.h, to.plan(ctx, 0), 0), ctx.h)
+        else:
+            fn = getattr(L, "lfr_" + name)
+            check(fn(ctx.h, prec, P(y.data_ptr()), P(x.data_ptr()), P(md.data_ptr()), to.h, fr.h), ctx.h)
+        got = y.cpu().numpy()
+        np.testing.assert_array_equal(got, ref, err_msg=name)
+        np.testing.assert_allclose(got, ans, rtol=0, atol=1e-5 if prec == 4 else 1e-13)
+    ref = oracle.kernel("transpose_matrix_vector", 1, 1, np.zeros(2, dtype), X4.astype(dtype), m, [1, 2],
+                        [1, 2, 3, 4])
+    y = torch.zeros(2, dtype=md.dtype, device=ctx.device)
+    x = _t(ctx, X4.astype(dtype))
+    check(L.lfr_transpose_matrix_vector(ctx.h, prec, P(y.data_ptr()), P(x.data_ptr()), P(md.data_ptr()), fr.h,
+                                        to.h, fr.plan(ctx, 1), 0), ctx.h)
+    np.testing.assert_array_equal(y.cpu().numpy(), ref)
+    np.testing.assert_allclose(y.cpu().numpy(), D.T @ X4, rtol=0, atol=1e-5 if prec == 4 else 1e-13)
+
+
+CASES = [("W2", "W2", 0), ("W0", "W0", 0), ("W3", "W2", 0), ("W2", "W3", 0), ("W2", "W2", 1), ("W3", "W3", 0)]
+
+
+@pytest.mark.parametrize("nl", [1, 2, 3])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_thin_columns_bitexact(ctx, nl, dtype):
+    from lfric_core_amd import api
+    print(f"case thin columns nl={nl} {dtype.__name__}", flush=True)
+    m = cubed_sphere(3)
+    col = colour_cells(m)
+    st = Stream(70 + nl)
+    for to, frm, k in CASES:
+        fo, fi = build_dofmap(m, to, nl, k), build_dofmap(m, frm, nl, k)
+        so = api.Space(ctx, fo)
+        si = so if (to, k) == (frm, k) and to == frm else api.Space(ctx, fi)
+        M = normal_lma(st, m.ncells * nl, fo.ndf, fi.ndf).astype(dtype)
+        x = st.uniform(fi.undf).astype(dtype)
+        y0 = st.uniform(fo.undf).astype(dtype)
+        Md = _t(ctx, M)
+        tag = f"{to}<-{frm} k={k} nl={nl}"
+        if fo.is_discontinuous:
+            for mode, fn in ((0, api.dg_matrix_vector), (1, api.dg_inc_matrix_vector)):
+                ref = oracle.invoke_dg(mode, nl, y0.copy(), x, M, fo.dofmap, fi.dofmap)
+                y = _t(ctx, y0)
+                fn(y, _t(ctx, x), api.LMAOperator(Md, so, si, None))
+                np.testing.assert_array_equal(y.cpu().numpy(), ref, err_msg=f"{tag} dg mode {mode}")
+        else:
+            op = api.LMAOperator(Md, so, si, col)
+            ref = oracle.invoke_coloured(0, col, nl, y0.copy(), x, M, fo.dofmap, fi.dofmap)
+            y = _t(ctx, y0)
+            api.matrix_vector(y, _t(ctx, x), op)
+            np.testing.assert_array_equal(y.cpu().numpy(), ref, err_msg=f"{tag} inc")
+            ref0 = oracle.invoke_coloured(0, col, nl, np.zeros_like(y0), x, M, fo.dofmap, fi.dofmap)
+            pk = api.PackedOperator(op)
+            y = torch.full((fo.undf,), float("nan"), dtype=Md.dtype, device=ctx.device)
+            pk.apply(y, _t(ctx, x), overwrite=True)
+            np.testing.assert_array_equal(y.cpu().numpy(), ref0, err_msg=f"{tag} packed")
+            pk.close()
+        if not fi.is_discontinuous:
+            # adjoint into the continuous input space
+            yt = st.uniform(fo.undf).astype(dtype)
+            l0 = st.uniform(fi.undf).astype(dtype)
+            ref = oracle.invoke_coloured(1, col, nl, l0.copy(), yt, M, fi.dofmap, fo.dofmap)
+            lt = _t(ctx, l0)
+            api.transpose_matrix_vector(lt, _t(ctx, yt), api.LMAOperator(Md, so, si, col))
+            np.testing.assert_array_equal(lt.cpu().numpy(), ref, err_msg=f"{tag} transpose")
+
+
+@pytest.mark.parametrize("nl", [1, 2, 3])
+def test_thin_column_solves(ctx, nl):
+    """appinv and tri_solve on 1-3 layer columns (the generic row-count kernels)."""
+    from lfric_core_amd import api
+    m = cubed_sphere(3)
+    w3 = build_dofmap(m, "W3", nl)
+    st = Stream(80 + nl)
+    cm = np.empty((m.ncells, nl, 3))
+    cm[:, :, 1] = 3.0 + st.uniform(m.ncells * nl).reshape(m.ncells, nl)
+    cm[:, :, 0] = st.uniform(m.ncells * nl).reshape(m.ncells, nl)
+    cm[:, :, 2] = st.uniform(m.ncells * nl).reshape(m.ncells, nl)
+    cm[:, 0, 0] = 0.0
+    cm[:, -1, 2] = 0.0
+    ind = np.arange(1, nl + 1, dtype=np.int32)
+    sp = api.Space(ctx, w3)
+    for dt in (np.float32, np.float64):
+        cmd = cm.astype(dt)
+        x = st.uniform(w3.undf).astype(dt)
+        ref = oracle.appinv(np.zeros_like(x), x, cmd, w3.dofmap, ind)
+        op = api.CMAOperator(_t(ctx, cmd), sp, ind)
+        y = torch.full((w3.undf,), float("nan"), dtype=op.matrix.dtype, device=ctx.device)
+        api.columnwise_op_appinv(y, _t(ctx, x), op)
+        np.testing.assert_array_equal(y.cpu().numpy(), ref, err_msg=f"appinv nl={nl} {dt.__name__}")

@@ -80,3 +80,23 @@ def test_vcycle_converges_to_dense_solution():
     bl = omg.to_layer_major(b, lv.ncell, nl)
     want = np.linalg.solve(A, bl[0].ravel())
     assert np.linalg.norm(xl[0].ravel() - want) <= 1e-6 * np.linalg.norm(want)
+
+
+def test_galerkin_identity_full_hierarchy():
+    """R A_f P v = A_c v on random coarse fields for every level pair of the
+    C192/C96/C48/C24, 70-layer hierarchy (matrix-free, at the workload's size)."""
+    H = build_hierarchy((192, 96, 48, 24), nlayers=70)
+    nl = H.nlayers
+    st = Stream(9)
+    for l in range(len(H.levels) - 1):
+        f, c = H.levels[l], H.levels[l + 1]
+        v = st.uniform(nl * c.ncell).reshape(1, nl, c.ncell)
+        parent = np.empty(f.ncell, dtype=np.int64)
+        parent[c.children.ravel()] = np.repeat(np.arange(c.ncell), c.children.shape[1])
+        Pv = v[..., parent]                                    # injection
+        AfPv = omg.apply_A(Pv, f.bands, f.cell_next, f.sigma)
+        RAfPv = np.zeros_like(v)
+        for q in range(c.children.shape[1]):                   # cell average
+            RAfPv = RAfPv + AfPv[..., c.children[:, q]] * 0.25
+        Acv = omg.apply_A(v, c.bands, c.cell_next, c.sigma)
+        assert np.abs(RAfPv - Acv).max() <= 1e-12 * np.abs(Acv).max(), l
