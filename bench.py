@@ -167,6 +167,73 @@ def bench_solvers(ctx, cfg, w2, torch, steps_cg: int = 50):
     return out
 
 
+def bench_adjoint(ctx, cfg, w2, w3, torch, iters: int = 20, steps: int = 5):
+    """The differentiation configuration at C96L70: L = <w, CG_20(M, D^T p)>.
+    Forward: D^T p (transpose apply), 20 CG iterations recorded on the device
+    tape, <w, x>.  Backward: the tape's reverse sweep (dL/db, optionally
+    dL/dM) and D bbar (dg_matrix_vector) for dL/dp.  Both timed with HIP
+    events on the library's stream, operators packed once."""
+    from lfric_core_amd import api
+    from lfric_core_amd.autograd import CGTape
+    from lfric_core_amd.rng import Stream
+    dev = ctx.device
+    st = Stream(2)
+    p_h, w_h = st.uniform(cfg.w3.undf), st.normal(cfg.w2.undf)
+    out = {}
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        M = torch.from_numpy(cfg.mass).to(dev, dt).contiguous()
+        D = torch.from_numpy(cfg.div).to(dev, dt).contiguous()
+        mop = api.LMAOperator(M, w2, w2, cfg.colouring)
+        dop = api.LMAOperator(D, w3, w2, cfg.colouring)
+        pk, pkt = api.PackedOperator(mop, kind=0), api.PackedOperator(mop, kind=1)
+        dpk = api.PackedOperator(dop, kind=1)
+        tape = CGTape(M, mop, iters, 0, True, pk, pkt)
+        p = torch.from_numpy(p_h).to(dev, dt)
+        w = torch.from_numpy(w_h).to(dev, dt)
+        b = torch.empty(cfg.w2.undf, dtype=dt, device=dev)
+        pbar = torch.empty(cfg.w3.undf, dtype=dt, device=dev)
+
+        def forward():
+            dpk.apply(b, p, overwrite=True)
+            x = tape.forward(b)
+            return torch.dot(w, x)
+
+        def backward(want_M):
+            bbar, Mbar = tape.reverse(b, w, want_M)
+            api.dg_matrix_vector(pbar, bbar, dop)
+            return Mbar
+
+        for _ in range(2):
+            forward()
+            backward(True)
+        t_f = time_events(forward, steps, torch)
+        forward()
+        t_b = time_events(lambda: backward(False), steps, torch)
+        t_bm = time_events(lambda: backward(True), steps, torch)
+        nb, seg = tape_info(tape)
+        out[name] = {"forward_ms": round(t_f, 4), "backward_rhs_ms": round(t_b, 4),
+                     "backward_rhs_and_matrix_ms": round(t_bm, 4),
+                     "ratio_rhs": round(t_b / t_f, 3), "ratio_rhs_and_matrix": round(t_bm / t_f, 3),
+                     "tape_bytes": nb, "segment": seg}
+        log(f"adjoint {name}: forward {t_f:.2f} ms, backward dL/dp {t_b:.2f} ms ({t_b/t_f:.2f}x), "
+            f"with dL/dM {t_bm:.2f} ms ({t_bm/t_f:.2f}x); tape {nb/2**30:.2f} GiB, segment {seg}")
+        tape.close()
+        pk.close()
+        pkt.close()
+        dpk.close()
+        del M, D, mop, dop, p, w, b, pbar
+        torch.cuda.empty_cache()
+    return out
+
+
+def tape_info(tape):
+    import ctypes as C
+    from lfric_core_amd._ffi import lib
+    nb, seg = C.c_int64(), C.c_int()
+    lib().lfr_cg_tape_info(tape.h, C.byref(nb), C.byref(seg))
+    return nb.value, seg.value
+
+
 def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
     """The other operator kernels of the path at C96L70, each timed with HIP
     events over `steps` cold-cache calls and set against its algorithmic HBM bytes:
@@ -546,6 +613,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="C24L70 instead of C96L70 (debug)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solvers", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the CG20 forward/adjoint timing")
     ap.add_argument("--no-mg", action="store_true", help="skip the batched multigrid V-cycles")
     ap.add_argument("--no-multi", action="store_true", help="skip the 16-field MFMA apply")
     ap.add_argument("--no-assembly", action="store_true", help="skip the device assembly timings")
@@ -637,6 +705,8 @@ def main():
 
     if not args.no_solvers:
         extra["solvers"] = bench_solvers(ctx, cfg, w2, torch)
+    if not args.no_adjoint:
+        extra["adjoint_cg20"] = bench_adjoint(ctx, cfg, w2, w3, torch)
     if not args.no_rows:
         extra["rows"] = bench_rows(ctx, cfg, w2, w3, torch)
     if not args.no_multi:
@@ -705,7 +775,10 @@ def main():
                  "gbs": round(res["fp64"]["gbs"], 1),
                  "roofline_frac": round(res["fp64"]["gbs"] / HBM_PEAK_GBS, 4)},
         "adjoint_fwd_ratio": {"fp32": round(r32["adj_ratio"], 3),
-                              "fp64": round(res["fp64"]["adj_ratio"], 3)},
+                              "fp64": round(res["fp64"]["adj_ratio"], 3),
+                              **({"cg20": extra["adjoint_cg20"]["fp64"]["ratio_rhs_and_matrix"],
+                                  "cg20_rhs_only": extra["adjoint_cg20"]["fp64"]["ratio_rhs"]}
+                                 if "adjoint_cg20" in extra else {})},
         "reference_layout": {k: {"gdofs": round(res[k]["ref_gdofs"], 3), "ms": round(res[k]["ref_ms"], 6),
                                  "gbs": round(res[k]["ref_gbs"], 1),
                                  "roofline_frac": round(res[k]["ref_gbs"] / HBM_PEAK_GBS, 4),

@@ -559,6 +559,35 @@ int lfr_refined_cg_multi(lfr_ctx* ctx, lfr_multi* A32, const lfr_lma_op* A64, do
                          const void* b64_d, int nrhs, lfr_refine_stats* stats, double* rel_res_h);
 
 
+/* ---- differentiable K-iteration CG -------------------------------------
+ * x = CG_K(A, b): K unmonitored iterations of conjugate_gradient_type%cg_solve
+ * (components/science/source/solver/sci_iterative_solver_mod.f90:495-592) from
+ * x = 0 with the null preconditioner, and the exact reverse mode of that map
+ * (alpha_i and beta_i depend on b, so it is nonlinear in b).  The forward
+ * records the iteration's scalars and the operator products q_i = A p_i, and
+ * keeps r_i, p_i only at the start of every `segment` iterations (default
+ * ceil(sqrt(K)), at most 64) and through the last segment; the reverse sweep
+ * recomputes the iterates of each segment from its checkpoint (vector work
+ * only, the same bits) and walks it backwards with one transpose apply per
+ * step.  Scalars stay on the device; no host synchronisation.  No
+ * counterpart in the reference (which has no adjoint of its solvers).
+ *   A        square kind-0 operator without apply_bc (packed copy optional)
+ *   plan_t   kind-1 plan of A's space for A^T (or packed_t, a kind-1 pack)
+ *   record   0: forward only (no tape memory kept for a reverse sweep) */
+typedef struct lfr_cg_tape lfr_cg_tape;
+int lfr_cg_tape_create(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_plan* plan_t,
+                       const lfr_lma_packed* packed_t, int iters, int segment, int record,
+                       lfr_cg_tape** tape);
+int lfr_cg_tape_destroy(lfr_cg_tape* tape);
+int lfr_cg_tape_info(const lfr_cg_tape* tape, int64_t* bytes, int* segment);
+/* x = CG_K(A, b), recording the tape. */
+int lfr_cg_tape_forward(lfr_ctx* ctx, lfr_cg_tape* tape, const void* b_d, void* x_d);
+/* bbar = (dx_K/db)^T xbar for the last forward's b; with Mbar_d != NULL also
+ * the gradient with respect to local_stencil, sum_i qbar_i (x) p_i per
+ * cell-layer (accumulate != 0: added to Mbar_d). */
+int lfr_cg_tape_reverse(lfr_ctx* ctx, lfr_cg_tape* tape, const void* b_d, const void* xbar_d,
+                        void* bbar_d, void* Mbar_d, int accumulate);
+
 /* ---------------------------------------------------------------------- */
 /* n-field vectors, abstract operators and preconditioners, and the        */
 /* iterative solvers of sci_iterative_solver_mod over them                 */

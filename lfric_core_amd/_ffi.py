@@ -119,6 +119,11 @@ SIGNATURES = {
     "lfr_precon_apply": [_P, _P, _P, _P],
     "lfr_iterative_solve": None,
     "lfr_mm_diagonal": [_P, _I, _P, _P, _P, _P],
+    "lfr_cg_tape_create": None,
+    "lfr_cg_tape_destroy": [_P],
+    "lfr_cg_tape_info": [_P, C.POINTER(_I64), C.POINTER(_I)],
+    "lfr_cg_tape_forward": [_P, _P, _P, _P],
+    "lfr_cg_tape_reverse": [_P, _P, _P, _P, _P, _P, _I],
 }
 _RESTYPE = {"lfr_last_error": C.c_char_p, "lfr_get_stream": _P}
 

@@ -15,11 +15,12 @@ Backward passes:
   Mbar(ik,i,j) = ybar(map_to(i)+k) * x(map_from(j)+k) (``lfr_lma_outer``).
 * CG: exact reverse mode of the K-iteration algorithm (alpha and beta depend
   on b, so the map b -> x_K is nonlinear and its derivative is not the
-  linear-solve adjoint unless CG has converged).  Iterates are not stored:
-  the forward pass keeps checkpoints (r, p, rz) every ``C = ceil(sqrt(K))``
-  iterations; the reverse sweep recomputes one segment at a time from its
-  checkpoint, then walks it backwards.  Memory is O(sqrt(K)) vectors; cost is
-  one extra forward pass.
+  linear-solve adjoint unless CG has converged), on the device
+  (lfr_cg_tape_*).  The iterates r, p are not stored: the forward keeps them
+  every ``C = ceil(sqrt(K))`` iterations plus the operator products A p_i
+  and the scalars; the reverse sweep recomputes each segment's iterates from
+  its checkpoint by vector updates alone and walks it backwards, one
+  transpose apply per step, with no host synchronisation.
 """
 
 from __future__ import annotations
@@ -112,106 +113,92 @@ def lma_apply_transpose(M: torch.Tensor, p: torch.Tensor, op: "api.LMAOperator")
 
 
 # ---------------------------------------------------------------------------
-# K-iteration CG with checkpointed reverse mode
+# K-iteration CG with checkpointed reverse mode (lfr_cg_tape_*)
 # ---------------------------------------------------------------------------
 
-class _CGRecorder:
-    """Forward CG from x0 = 0 in the reference's statement order, using the
-    library's apply, vector and FP64 inner-product entry points."""
+class CGTape:
+    """A device tape of K CG iterations on op (lfr_cg_tape_create): forward
+    records the scalars and operator products, reverse recomputes the
+    iterates segment by segment and returns the gradients."""
 
-    def __init__(self, M, op):
-        self.M, self.op, self.ctx = M, op, op.ctx
+    def __init__(self, M: torch.Tensor, op: "api.LMAOperator", iters: int, segment: int = 0,
+                 record: bool = True, packed: "api.PackedOperator | None" = None,
+                 packed_t: "api.PackedOperator | None" = None):
+        from ._ffi import check, lib
+        from .solvers import LmaOp
+        import ctypes as C
+        L = lib()
+        L.lfr_cg_tape_create.argtypes = [C.c_void_p, C.POINTER(LmaOp), C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                         C.c_int, C.POINTER(C.c_void_p)]
+        L.lfr_cg_tape_create.restype = C.c_int
+        sp = op.to_space
+        self.ctx, self.op, self.M = op.ctx, op, M
+        plan = sp.plan(op.colouring, 0)
+        plan_t = sp.plan(op.colouring, 1)
+        # optional packed copies of M (kind 0) and of its transpose (kind 1)
+        self._a = LmaOp(api._prec(M), C.c_void_p(M.data_ptr()), sp.h, plan.h, 0,
+                        packed.h if packed is not None else None)
+        self._keep = (plan, plan_t, packed, packed_t)
+        h = C.c_void_p()
+        self.ctx.sync_stream()
+        check(L.lfr_cg_tape_create(self.ctx.h, C.byref(self._a), plan_t.h,
+                                   packed_t.h if packed_t is not None else None, int(iters), int(segment),
+                                   int(record), C.byref(h)), self.ctx.h)
+        self.h = h
 
-    def start(self, b):
-        ctx = self.ctx
-        r = b.clone()                       # r = b - A*0 = b
-        p = r.clone()                       # null preconditioner: z = r, p = z
-        rz = api.dot(ctx, r, r)
-        return r, p, rz
+    def forward(self, b: torch.Tensor) -> torch.Tensor:
+        from ._ffi import check, lib
+        import ctypes as C
+        x = torch.empty_like(b)
+        self.ctx.sync_stream()
+        check(lib().lfr_cg_tape_forward(self.ctx.h, self.h, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr())),
+              self.ctx.h)
+        return x
 
-    def step(self, x, r, p, rz):
-        """One iteration; returns (q, pq, alpha, rz_new, beta) and updates x, r, p in place."""
-        ctx = self.ctx
-        q = _apply(self.M, p, self.op)
-        pq = api.dot(ctx, p, q)
-        alpha = rz / pq
-        if x is not None:
-            api.axpy(ctx, alpha, p, x)
-        api.axpy(ctx, -alpha, q, r)
-        rz_new = api.dot(ctx, r, r)
-        beta = rz_new / rz
-        api.aypx(ctx, beta, r, p)           # p = beta p + r
-        return q, pq, alpha, rz_new, beta
+    def reverse(self, b: torch.Tensor, xbar: torch.Tensor, want_M: bool):
+        from ._ffi import check, lib
+        import ctypes as C
+        bbar = torch.empty_like(b)
+        Mbar = torch.empty_like(self.M) if want_M else None
+        self.ctx.sync_stream()
+        check(lib().lfr_cg_tape_reverse(self.ctx.h, self.h, C.c_void_p(b.data_ptr()), C.c_void_p(xbar.data_ptr()),
+                                        C.c_void_p(bbar.data_ptr()),
+                                        C.c_void_p(Mbar.data_ptr()) if want_M else None, 0), self.ctx.h)
+        return bbar, Mbar
+
+    def close(self):
+        from ._ffi import lib
+        if getattr(self, "h", None):
+            lib().lfr_cg_tape_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class _CGSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, M, b, op, iters, seg):
-        rec = _CGRecorder(M, op)
-        x = torch.zeros_like(b)
-        r, p, rz = rec.start(b)
-        ckpt = {0: (r.clone(), p.clone(), rz)}
-        for i in range(iters):
-            rz = rec.step(x, r, p, rz)[3]
-            if (i + 1) % seg == 0 and i + 1 < iters:
-                ckpt[i + 1] = (r.clone(), p.clone(), rz)
-        ctx.op, ctx.iters, ctx.seg, ctx.ckpt = op, iters, seg, ckpt
-        ctx.save_for_backward(M, b)
+        record = bool(ctx.needs_input_grad[0] or ctx.needs_input_grad[1])
+        tape = CGTape(M, op, iters, seg, record)
+        x = tape.forward(b.contiguous())
+        if record:
+            ctx.tape = tape
+            ctx.save_for_backward(M, b)
+        else:
+            tape.close()
         return x
 
     @staticmethod
     def backward(ctx, xbar):
         M, b = ctx.saved_tensors
-        op, K, seg, ckpt = ctx.op, ctx.iters, ctx.seg, ctx.ckpt
-        actx = op.ctx
-        rec = _CGRecorder(M, op)
-        need_M = ctx.needs_input_grad[0]
-        xbar = xbar.contiguous().clone()          # constant through the sweep
-        rbar = torch.zeros_like(b)
-        pbar = torch.zeros_like(b)
-        rzbar = 0.0                               # adjoint of rz_{i+1} carried down
-        Mbar = torch.zeros_like(M) if need_M else None
-        starts = sorted(ckpt)
-        for s0 in reversed(starts):
-            s1 = min(s0 + seg, K)
-            # recompute the segment from its checkpoint
-            r, p, rz = (t.clone() if torch.is_tensor(t) else t for t in ckpt[s0])
-            hist = []
-            for i in range(s0, s1):
-                p_i = p.clone()
-                q, pq, alpha, rz_new, beta = rec.step(None, r, p, rz)
-                hist.append((p_i, q, r.clone(), pq, alpha, rz, rz_new, beta))
-                rz = rz_new
-            for (p_i, q_i, r_next, pq, alpha, rz_i, rz_next, beta) in reversed(hist):
-                # p_{i+1} = r_{i+1} + beta_i p_i
-                api.axpy(actx, 1.0, pbar, rbar)
-                betabar = api.dot(actx, pbar, p_i)
-                api.scale(actx, beta, pbar)
-                # beta_i = rz_{i+1} / rz_i
-                rzbar_next = rzbar + betabar / rz_i
-                rzbar_i = -betabar * rz_next / (rz_i * rz_i)
-                # rz_{i+1} = <r_{i+1}, r_{i+1}>
-                api.axpy(actx, 2.0 * rzbar_next, r_next, rbar)
-                # r_{i+1} = r_i - alpha q_i ; x_{i+1} = x_i + alpha p_i
-                alphabar = -api.dot(actx, rbar, q_i) + api.dot(actx, xbar, p_i)
-                qbar = torch.zeros_like(b)
-                api.axpy(actx, -alpha, rbar, qbar)
-                api.axpy(actx, alpha, xbar, pbar)
-                # alpha_i = rz_i / pq_i
-                rzbar_i += alphabar / pq
-                pqbar = -alphabar * rz_i / (pq * pq)
-                # pq_i = <p_i, q_i>
-                api.axpy(actx, pqbar, q_i, pbar)
-                api.axpy(actx, pqbar, p_i, qbar)
-                # q_i = A p_i
-                api.axpy(actx, 1.0, _apply_t(M, qbar, op), pbar)
-                if need_M:
-                    _outer(qbar, p_i, op, out=Mbar)
-                rzbar = rzbar_i
-        # p_0 = r_0, rz_0 = <r_0, r_0>, r_0 = b
-        api.axpy(actx, 1.0, pbar, rbar)
-        api.axpy(actx, 2.0 * rzbar, b, rbar)
-        return Mbar, rbar, None, None, None
+        bbar, Mbar = ctx.tape.reverse(b.contiguous(), xbar.contiguous(), bool(ctx.needs_input_grad[0]))
+        ctx.tape.close()
+        ctx.tape = None
+        return Mbar, bbar, None, None, None
 
 
 def cg_solve(M: torch.Tensor, b: torch.Tensor, op: "api.LMAOperator", iters: int = 20,

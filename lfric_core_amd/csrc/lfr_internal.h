@@ -148,6 +148,12 @@ int lfr_check_op(lfr_ctx* ctx, const lfr_lma_op* A, const char* who);
 // Queue y = A x for a checked square operator (enforce_bc after the apply
 // when A->apply_bc), as mass_matrix_operator_type%apply.
 int lfr_lma_apply_queue(lfr_ctx* ctx, const lfr_lma_op* A, const void* x, void* y);
+// The same with the partial sums of <x, A x> (one per apply workgroup, in a
+// fixed order) written to part[0 .. *npart); part must hold
+// lfr_lma_apply_partials_max(A->plan) doubles.
+int lfr_lma_apply_dot_queue(lfr_ctx* ctx, const lfr_lma_op* A, const void* x, void* y, double* part,
+                            int* npart);
+int lfr_lma_apply_partials_max(const lfr_plan* plan);
 
 // Output and input field sizes (undf) of a multi-field operator handle.
 extern "C" int lfr_multi_space_undf(const lfr_multi* h, int* undf_to, int* undf_from);

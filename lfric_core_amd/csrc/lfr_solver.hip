@@ -545,6 +545,15 @@ int apply_op(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* dotv,
 
 }  // namespace
 
+int lfr_lma_apply_dot_queue(lfr_ctx* ctx, const lfr_lma_op* A, const void* x, void* y, double* part,
+                            int* npart) {
+  *npart = apply_partials(A->plan, A->prec, A->packed != nullptr);
+  if (A->prec == 4) return apply_op<float>(ctx, A, (const float*)x, (float*)y, (const float*)x, part, nullptr);
+  return apply_op<double>(ctx, A, (const double*)x, (double*)y, (const double*)x, part, nullptr);
+}
+
+int lfr_lma_apply_partials_max(const lfr_plan* plan) { return apply_partials_max(plan); }
+
 int lfr_lma_apply_queue(lfr_ctx* ctx, const lfr_lma_op* A, const void* x, void* y) {
   if (A->prec == 4) return apply_op<float>(ctx, A, (const float*)x, (float*)y, nullptr, nullptr, nullptr);
   return apply_op<double>(ctx, A, (const double*)x, (double*)y, nullptr, nullptr, nullptr);

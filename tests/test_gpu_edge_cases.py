@@ -56,6 +56,7 @@ def test_pfunit_cases_through_the_invokes(ctx, dtype, prec):
     L = lib()
     P = C.c_void_p
     print(f"case pfunit 4x2 nlayers=1 prec={prec}", flush=True)
+    ctx.sync_stream()          # the raw calls below queue on PyTorch's current stream
     to, fr = _Cell(ctx, 4), _Cell(ctx, 2)
     m = np.ascontiguousarray(D.T.reshape(2, 4, 1)).astype(dtype)   # local_stencil(1, 4, 2)
     md = _t(ctx, m)
@@ -116,12 +117,13 @@ def test_thin_columns_bitexact(ctx, nl, dtype):
             y = _t(ctx, y0)
             api.matrix_vector(y, _t(ctx, x), op)
             np.testing.assert_array_equal(y.cpu().numpy(), ref, err_msg=f"{tag} inc")
-            ref0 = oracle.invoke_coloured(0, col, nl, np.zeros_like(y0), x, M, fo.dofmap, fi.dofmap)
-            pk = api.PackedOperator(op)
-            y = torch.full((fo.undf,), float("nan"), dtype=Md.dtype, device=ctx.device)
-            pk.apply(y, _t(ctx, x), overwrite=True)
-            np.testing.assert_array_equal(y.cpu().numpy(), ref0, err_msg=f"{tag} packed")
-            pk.close()
+            if k == 0:    # packed copies exist for the lowest-order shapes
+                ref0 = oracle.invoke_coloured(0, col, nl, np.zeros_like(y0), x, M, fo.dofmap, fi.dofmap)
+                pk = api.PackedOperator(op)
+                y = torch.full((fo.undf,), float("nan"), dtype=Md.dtype, device=ctx.device)
+                pk.apply(y, _t(ctx, x), overwrite=True)
+                np.testing.assert_array_equal(y.cpu().numpy(), ref0, err_msg=f"{tag} packed")
+                pk.close()
         if not fi.is_discontinuous:
             # adjoint into the continuous input space
             yt = st.uniform(fo.undf).astype(dtype)

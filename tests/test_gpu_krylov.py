@@ -313,6 +313,7 @@ def test_block_operator_with_transpose_entry(ctx, prob):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_mm_diagonal_gpu_bitexact(ctx, dtype):
     from lfric_core_amd import api
+    ctx.sync_stream()          # the raw calls below queue on PyTorch's current stream
     m = cubed_sphere(5)
     nl = 7
     col = colour_cells(m)
