@@ -320,6 +320,16 @@ int lfr_vec_convert(lfr_ctx* ctx, int prec_out, void* y_d, int prec_in, const vo
  * device memory (no host synchronisation). */
 int lfr_vec_dot(lfr_ctx* ctx, int prec, int64_t n, const void* x_d, const void* y_d, double* out_h);
 int lfr_vec_dot_dev(lfr_ctx* ctx, int prec, int64_t n, const void* x_d, const void* y_d, double* out_d);
+/* lfr_vec_dot_dev without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same grid, order and bits as
+ * lfr_vec_dot_dev.  The partial sums go to a fixed scratch array of
+ * 1024 doubles (8 KiB) per GPU, held until the process ends and shared by
+ * every call on that GPU: calls that overlap on one GPU (two streams, or two
+ * contexts on one device) race.  Any n > 0 fits (the reduction grid never
+ * exceeds 1024 blocks); the result is left in out_d. */
+int lfr_vec_dot_dev_noalloc(lfr_ctx* ctx, int prec, int64_t n, const void* x_d, const void* y_d,
+                            double* out_d);
 
 /* ---------------------------------------------------------------------- */
 /* Solvers                                                                 */

@@ -78,6 +78,7 @@ SIGNATURES = {
     "lfr_vec_convert": [_P, _I, _P, _I, _P, _I64],
     "lfr_vec_dot": [_P, _I, _I64, _P, _P, C.POINTER(_D)],
     "lfr_vec_dot_dev": [_P, _I, _I64, _P, _P, _P],
+    "lfr_vec_dot_dev_noalloc": [_P, _I, _I64, _P, _P, _P],
     # solver entry points take structure pointers; lfric_core_amd.solvers
     # installs the typed signatures before calling them
     "lfr_columnwise_op_appinv": [_P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I],

@@ -107,7 +107,10 @@ int lfr_init(int device, lfr_ctx** out) {
       return lfr_set_error(nullptr, LFR_ERR_HIP, "event creation failed");
     }
   int rc = lfr_work_reserve(ctx, 1u << 20);
+  if (rc == LFR_OK) rc = lfr_noalloc_setup(ctx);
   if (rc != LFR_OK) {
+    if (ctx->work) (void)hipFree(ctx->work);
+    fprintf(stderr, "lfr_init: %s\n", ctx->err.c_str());
     drop_events();
     (void)hipHostFree(ctx->host_scalars);
     (void)hipStreamDestroy(ctx->own_stream);

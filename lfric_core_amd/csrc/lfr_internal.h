@@ -173,6 +173,10 @@ extern "C" int lfr_multi_space_undf(const lfr_multi* h, int* undf_to, int* undf_
     if (!(cond)) return lfr_set_error((ctx), LFR_ERR_ARG, __VA_ARGS__); \
   } while (0)
 
+// Resolve the device address of the allocation-free entry points' scratch
+// for ctx->device (lfr_init calls it with that device current).
+int lfr_noalloc_setup(lfr_ctx* ctx);
+
 // Grow the context workspace to at least `bytes`.
 int lfr_work_reserve(lfr_ctx* ctx, size_t bytes);
 

@@ -7,8 +7,43 @@
 // Scalars arrive as double (r_def) and are cast to the field's kind before
 // the update, as r_solver_field_vector_type does (:189-195).  Inner products
 // of r_solver data accumulate real(x,r_double)*real(y,r_double) in FP64.
+#include <mutex>
+
 #include "lfr_internal.h"
 #include "lfr_reduce.h"
+
+// Scratch of the allocation-free inner product lfr_vec_dot_dev_noalloc: the
+// first-stage partial sums, one slot per reduction block (the grid never has
+// more than lfr::kRedMaxBlocks blocks, so 8 KiB serve every length).  Each GPU
+// holds its own copy for the life of the process.  Host code never passes
+// this name to a kernel: it uses the device address that lfr_noalloc_setup
+// resolves once per device.
+constexpr int kScratchSlots = lfr::kRedMaxBlocks;
+static __device__ __align__(256) double scratch[kScratchSlots];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+}  // namespace
+
+// Resolve this device's copy of `scratch` (called by lfr_init with the
+// context's device current).
+int lfr_noalloc_setup(lfr_ctx* ctx) {
+  LFR_REQUIRE(ctx, ctx->device >= 0 && ctx->device < kMaxDevices, "device %d beyond the %d the library tracks",
+              ctx->device, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[ctx->device]) return LFR_OK;
+  double* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return lfr_set_error(ctx, LFR_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", ctx->device,
+                         hipGetErrorString(e));
+  g_scratch[ctx->device] = p;
+  return LFR_OK;
+}
 
 namespace {
 
@@ -229,6 +264,23 @@ int lfr_vec_dot(lfr_ctx* ctx, int prec, int64_t n, const void* x, const void* y,
   LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *out = ctx->host_scalars[0];
   return LFR_OK;
+}
+
+int lfr_vec_dot_dev_noalloc(lfr_ctx* ctx, int prec, int64_t n, const void* x, const void* y, double* out_d) {
+  if (!ctx) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, prec_ok(prec) && x && y && out_d && n > 0, "lfr_vec_dot_dev_noalloc: bad arguments");
+  const int nb = lfr::red_blocks(n);
+  LFR_REQUIRE(ctx, nb <= kScratchSlots, "lfr_vec_dot_dev_noalloc: %d partial sums exceed the %d-slot scratch", nb,
+              kScratchSlots);
+  double* part = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    part = g_scratch[ctx->device];
+  }
+  LFR_REQUIRE(ctx, part, "lfr_vec_dot_dev_noalloc: scratch of device %d not set up", ctx->device);
+  // every slot the final stage sums (0 .. nb-1) is written by the first
+  // stage of this call, so nothing left by an earlier call is read
+  return lfr_dot_queue(ctx, prec, n, x, y, part, out_d);
 }
 
 int lfr_vec_dot_dev(lfr_ctx* ctx, int prec, int64_t n, const void* x, const void* y,
