@@ -73,13 +73,114 @@ def time_events(fn, steps: int, torch) -> float:
     return e0.elapsed_time(e1) / steps
 
 
+def cpu_threads() -> int:
+    """Threads of the CPU baselines: OMP_NUM_THREADS (the host's CPU share for
+    one GPU; 16 on the GPU machines), at most os.cpu_count()."""
+    nt = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    return max(1, min(nt, os.cpu_count() or 1))
+
+
+def host_description() -> dict:
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "threads_used": cpu_threads()}
+
+
+def cpu_baselines(cfg, ctx, w2, torch, seconds: float = 10.0) -> dict:
+    """The solver path on the host cores, beside the GPU: FP64 CG to 1e-10 on
+    the C96L70 W2 mass matrix and columnwise_op_appinv on its W3 columns
+    (reference kernels via oracle/_ref, vector algebra as parallel loops),
+    and the plain-method case (C12L10 W0 assembled operator: one matvec and
+    50 FP64 CG iterations) on the CPU and on the GPU."""
+    import oracle
+    from lfric_core_amd import api
+    from lfric_core_amd.configs import diffusion_config
+    from lfric_core_amd.solvers import ConjugateGradient
+    out = {}
+    nt = cpu_threads()
+    if oracle.ref() is None:
+        return {"note": "reference kernels not built (oracle/_ref)"}
+    # FP64 CG to 1e-10 at C96L70
+    x = np.zeros(cfg.w2.undf)
+    t = time.perf_counter()
+    it, r0, rn = oracle.ref_cg(cfg.colouring, cfg.nlayers, cfg.mass, cfg.w2.dofmap, x, cfg.b, r_tol=1e-10,
+                               max_iter=1000, nthreads=nt)
+    t_cg = time.perf_counter() - t
+    out["cg_1e10_fp64_c96l70"] = {"seconds": round(t_cg, 3), "iterations": it, "rel_residual": rn / r0,
+                                  "kind": "reference", "cores": nt}
+    log(f"CPU FP64 CG to 1e-10 C96L70: {t_cg:.2f} s, {it} iterations ({nt} threads)")
+    # appinv on the W3 columns (FP32, flang-built kernel, OpenMP over columns)
+    nl, nc = cfg.nlayers, cfg.mesh.ncells
+    st_ = np.random.default_rng(4)
+    A = np.empty((nc, nl, 3), np.float32)
+    A[:, :, 1] = 4.0 + st_.random((nc, nl))
+    A[:, :, 0] = -0.5 - 0.1 * st_.random((nc, nl))
+    A[:, :, 2] = -0.5 - 0.1 * st_.random((nc, nl))
+    xv = st_.random(cfg.w3.undf).astype(np.float32)
+    ind = np.arange(1, nl + 1, dtype=np.int32)
+    yv = np.zeros_like(xv)
+    oracle.appinv(yv, xv, A, cfg.w3.dofmap, ind, nthreads=nt, use_ref=True)
+    n = 0
+    t = time.perf_counter()
+    while time.perf_counter() - t < min(seconds, 3.0):
+        oracle.appinv(yv, xv, A, cfg.w3.dofmap, ind, nthreads=nt, use_ref=True)
+        n += 1
+    dt = (time.perf_counter() - t) / n
+    nb = nc * nl * (3 + 2) * 4
+    out["appinv_fp32_c96l70"] = {"us": round(dt * 1e6, 1), "gbs": round(nb / dt / 1e9, 1), "samples": n,
+                                 "kind": "reference", "cores": nt}
+    log(f"CPU appinv C96L70: {dt*1e6:.0f} us ({nb/dt/1e9:.1f} GB/s, {nt} threads)")
+    # plain method, C12L10 W0: one matvec + 50 CG iterations, FP64, CPU and GPU
+    g = np.load(ROOT / "tests" / "golden" / "meshes.npz")
+    dc = diffusion_config(g["C12_node_x"], g["C12_node_y"], g["C12_face_nodes"])
+    A0 = dc.matrix
+    b0 = oracle.invoke_coloured(0, dc.colouring, dc.nlayers, np.zeros(dc.w0.undf), dc.x_true, A0, dc.w0.dofmap,
+                                dc.w0.dofmap, nthreads=nt, use_ref=True)
+    reps = 0
+    t = time.perf_counter()
+    while time.perf_counter() - t < 1.0:
+        bb = oracle.invoke_coloured(0, dc.colouring, dc.nlayers, np.zeros(dc.w0.undf), dc.x_true, A0,
+                                    dc.w0.dofmap, dc.w0.dofmap, nthreads=nt, use_ref=True)
+        x0 = np.zeros(dc.w0.undf)
+        oracle.ref_cg(dc.colouring, dc.nlayers, A0, dc.w0.dofmap, x0, bb, max_iter=50, monitor=False, nthreads=nt)
+        reps += 1
+    t_cpu = (time.perf_counter() - t) / reps
+    w0 = api.Space(ctx, dc.w0)
+    op = api.LMAOperator(torch.from_numpy(A0).to(ctx.device), w0, w0, dc.colouring)
+    xt = torch.from_numpy(dc.x_true).to(ctx.device)
+    bd = torch.empty_like(xt)
+    xg = torch.zeros_like(xt)
+    cg = ConjugateGradient(op, max_iter=50, monitor_convergence=False)
+
+    def gpu_run():
+        api.matrix_vector(bd, xt, op, overwrite=True)
+        cg.apply(xg, bd, x_is_zero=True)
+
+    gpu_run()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(10):
+        gpu_run()
+    torch.cuda.synchronize()
+    t_gpu = (time.perf_counter() - t) / 10
+    out["plain_c12l10_w0_matvec_cg50_fp64"] = {"cpu_seconds": round(t_cpu, 5), "gpu_seconds": round(t_gpu, 5),
+                                               "undf": dc.w0.undf, "kind": "reference", "cores": nt}
+    log(f"C12L10 W0 matvec + CG50 FP64: CPU {t_cpu*1e3:.2f} ms, GPU {t_gpu*1e3:.2f} ms")
+    return out
+
+
 def cpu_baseline(cfg, seconds: float = 15.0):
     """FP32 W2 mass apply by the reference Fortran kernel (oracle/_ref, flang
     build) in the coloured PSy loop with OpenMP over the cells of a colour;
     falls back to the C restatement when the reference build is absent."""
     import oracle
-    nt = int(os.environ.get("OMP_NUM_THREADS", "16"))
-    nt = max(1, min(nt, os.cpu_count() or 1))
+    nt = cpu_threads()
     kind = "reference" if oracle.ref() is not None else "port"
     M = cfg.mass.astype(np.float32)
     x = cfg.b.astype(np.float32)
@@ -99,7 +200,8 @@ def cpu_baseline(cfg, seconds: float = 15.0):
     dt = (time.perf_counter() - t0) / n
     return {"value": cfg.w2.undf / dt / 1e9, "unit": "GDOF/s", "cores": nt, "kind": kind,
             "sample": f"{n} FP32 W2 mass applies at C96L70 (setval_c + coloured matrix_vector "
-                      f"loop, {nt} OpenMP threads), {dt*1e3:.1f} ms each"}
+                      f"loop, {nt} OpenMP threads), {dt*1e3:.1f} ms each",
+            **host_description()}
 
 
 def bench_solvers(ctx, cfg, w2, torch, steps_cg: int = 50):
@@ -148,6 +250,16 @@ def bench_solvers(ctx, cfg, w2, torch, steps_cg: int = 50):
     out["refined_cg"] = {"seconds": round(tr, 6), "outer": str_.outer,
                          "inner_iterations": str_.inner_total, "converged": bool(str_.converged),
                          "rel_residual": true_rel_residual(xr), "rel_err_vs_fp64_cg": err}
+    # roofline of the refinement: inner FP32 iterations (matrix + 10 vector
+    # passes each) and FP64 residuals (FP64 matrix + x, b read, r written)
+    it_bytes32 = n3 * 36 * 4 + 10 * n * 4
+    res_bytes64 = n3 * 36 * 8 + 3 * n * 8
+    rbytes = str_.inner_total * it_bytes32 + (str_.outer + 1) * res_bytes64
+    out["refined_cg"].update({"algorithmic_bytes": int(rbytes), "gbs": round(rbytes / tr / 1e9, 1),
+                              "roofline_frac": round(rbytes / tr / 1e9 / HBM_PEAK_GBS, 4)})
+    cgbytes = st64.iterations * (n3 * 36 * 8 + 10 * n * 8) + res_bytes64
+    out["fp64_cg"].update({"algorithmic_bytes": int(cgbytes), "gbs": round(cgbytes / t64 / 1e9, 1),
+                           "roofline_frac": round(cgbytes / t64 / 1e9 / HBM_PEAK_GBS, 4)})
     out["refined_speedup_vs_fp64"] = round(t64 / tr, 3)
     log(f"FP64 CG to 1e-10: {t64*1e3:.1f} ms, {st64.iterations} its; refined: {tr*1e3:.1f} ms, "
         f"{str_.outer} outer / {str_.inner_total} inner its; x err vs FP64 {err:.2e}")
@@ -564,6 +676,28 @@ def bench_assembly(ctx, torch, edge_cells: int = 96, nl: int = 70, steps: int = 
     return out
 
 
+def vcycle_bytes(H, nrhs: int, prec: int, nu1: int = 2, nu2: int = 2, ncoarse: int = 8) -> int:
+    """Algorithmic HBM bytes of one V-cycle over nrhs right-hand sides: per
+    line-Jacobi sweep x, b read and x written for every right-hand side plus
+    the three bands and the two Thomas factors of each column (shared by the
+    right-hand sides); per restriction the fine x and b read, the coarse b
+    written and the coarse x zeroed, plus the fine bands; per prolongation the
+    fine x read and written and the coarse x read.  Neighbour and child maps
+    (integer, a few MB) are not counted."""
+    s, R = prec, nrhs
+    tot = 0
+    L = H.levels
+    for l, lv in enumerate(L):
+        N = lv.ncell * H.nlayers
+        sweeps = ncoarse if l == len(L) - 1 else nu1 + nu2
+        tot += sweeps * (3 * R + 5) * N * s
+        if l + 1 < len(L):
+            Nc = L[l + 1].ncell * H.nlayers
+            tot += (2 * R + 3) * N * s + 2 * R * Nc * s          # residual and restriction
+            tot += 2 * R * N * s + R * Nc * s                      # prolongation
+    return tot
+
+
 def bench_multigrid(ctx, torch, edge_cells=(192, 96, 48, 24), nl: int = 70, nrhs: int = 8,
                     cycles: int = 10):
     """Batched V-cycles (2 pre / 2 post line-Jacobi sweeps, 8 coarse sweeps) on the
@@ -590,10 +724,13 @@ def bench_multigrid(ctx, torch, edge_cells=(192, 96, 48, 24), nl: int = 70, nrhs
         mg.solve(x, bb, cycles, record=False)
         torch.cuda.synchronize()
         per = (time.perf_counter() - t) / cycles
+        nbytes = vcycle_bytes(H, nrhs, prec)
         out[name] = {"convergence_factor": round(factor, 4),
                      "residual_reduction": [float(v) for v in (res[-1] / res[0])],
                      "ms_per_vcycle": round(per * 1e3, 4),
                      "gdofs": round(nrhs * H.fine_undf / per / 1e9, 3),
+                     "algorithmic_bytes": nbytes, "gbs": round(nbytes / per / 1e9, 1),
+                     "roofline_frac": round(nbytes / per / 1e9 / HBM_PEAK_GBS, 4),
                      "block_sizes": mg.block_sizes}
         log(f"MG {name}: {per*1e3:.2f} ms/V-cycle ({nrhs} rhs), {nrhs*H.fine_undf/per/1e9:.2f} "
             f"GDOF/s, convergence factor {factor:.3f}, blocks {mg.block_sizes}")
@@ -791,6 +928,7 @@ def main():
     out.update(extra)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+        out["cpu_baselines"] = cpu_baselines(cfg, ctx, w2, torch)
     print(json.dumps(out), flush=True)
 
 

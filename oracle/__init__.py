@@ -389,6 +389,26 @@ def mm_diagonal(diag, matrix, dofmap, nlayers, colouring=None, use_ref=False):
     return diag
 
 
+def ref_cg(colouring, nlayers, matrix, dofmap, x, b, r_tol=1e-8, max_iter=100, monitor=True, nthreads=None):
+    """The reference-path CPU CG (oracle/_ref ref_cg_r64: the flang-built
+    matrix_vector in the coloured loop, vector built-ins as parallel loops,
+    FP64).  Solves in place; returns (iterations, r_nrm0, r_nrm)."""
+    nt = nthreads_default() if nthreads is None else nthreads
+    cic = np.asfortranarray(colouring.cells_in_colour).ravel(order="F").astype(np.int32)
+    npc = colouring.ncells_per_colour.astype(np.int32)
+    dm = np.ascontiguousarray(dofmap, dtype=np.int32)
+    m = np.ascontiguousarray(matrix, dtype=np.float64)
+    vp = C.c_void_p
+    r0, rn = C.c_double(), C.c_double()
+    fn = ref().ref_cg_r64
+    fn.restype = C.c_int
+    it = fn(_I(colouring.ncolours), npc.ctypes.data_as(vp), cic.ctypes.data_as(vp), _I(nlayers), _I(m.shape[2]),
+            m.ctypes.data_as(vp), _I(dm.shape[1]), _I(x.size), dm.ctypes.data_as(vp), x.ctypes.data_as(vp),
+            b.ctypes.data_as(vp), C.c_double(r_tol), _I(max_iter), _I(int(monitor)), _I(nt), C.byref(r0),
+            C.byref(rn))
+    return it, r0.value, rn.value
+
+
 def dot(x, y, nthreads=None) -> float:
     nt = nthreads_default() if nthreads is None else nthreads
     fn = getattr(lib(), f"orc_dot_{_suf(x.dtype)}")

@@ -245,3 +245,92 @@ REF_TRANSFER(double, r_double)
 
 DEF_MMDIAG(float, r32, r_single)
 DEF_MMDIAG(double, r64, r_double)
+
+/* ---- CPU baseline of the solver path ------------------------------------
+ * conjugate_gradient_type%cg_solve (components/science/source/solver/
+ * sci_iterative_solver_mod.f90:495-592) with the null preconditioner, as the
+ * reference runs it on the CPU: each vector method is its own parallel loop
+ * over the dofs (the PSyclone built-ins: setval_c, setval_X, inc_X_plus_bY,
+ * inc_aX_plus_Y, inc_a_times_X), the inner products take per-thread FP64
+ * partials over a static schedule summed in thread order
+ * (components/science/source/psy/sci_psykal_light_mod.f90:121-134), and the
+ * operator apply is setval_c(y, 0) followed by the flang-built
+ * matrix_vector_code in the coloured PSy loop.  Returns the iterations done;
+ * r_nrm receives the last residual norm (and r_nrm0 the first). */
+#include <math.h>
+#include <omp.h>
+#include <stdlib.h>
+
+static double psy_dot(const double* x, const double* y, int n, int nt) {
+  double part[256];
+  int nth = nt > 256 ? 256 : (nt > 0 ? nt : 1);
+  for (int t = 0; t < nth; ++t) part[t] = 0.0;
+#pragma omp parallel num_threads(nth)
+  {
+    const int t = omp_get_thread_num(), T = omp_get_num_threads();
+    const int lo = (int)((int64_t)n * t / T), hi = (int)((int64_t)n * (t + 1) / T);
+    double s = 0.0;
+    for (int i = lo; i < hi; ++i) s += x[i] * y[i];
+    part[t] = s;
+  }
+  double tot = 0.0;
+  for (int t = 0; t < nth; ++t) tot += part[t];
+  return tot;
+}
+
+static void psy_apply(int ncolour, const int* npc, const int* cic, int nl, double* y, const double* x, int n3,
+                      const double* m, int ndf, int undf, const int* dm, int nt) {
+#pragma omp parallel for schedule(static) num_threads(nt > 0 ? nt : 1)
+  for (int i = 0; i < undf; ++i) y[i] = 0.0;
+  ref_invoke_coloured_r64(0, ncolour, npc, cic, nl, y, x, n3, m, ndf, undf, dm, ndf, undf, dm, nt);
+}
+
+int ref_cg_r64(int ncolour, const int* npc, const int* cic, int nl, int n3, const double* m, int ndf,
+               int undf, const int* dm, double* x, const double* b, double r_tol, int max_iter, int monitor,
+               int nt, double* r_nrm0_out, double* r_nrm_out) {
+  double* r = (double*)malloc(sizeof(double) * undf);
+  double* p = (double*)malloc(sizeof(double) * undf);
+  double* z = (double*)malloc(sizeof(double) * undf);
+  const int nth = nt > 0 ? nt : 1;
+  psy_apply(ncolour, npc, cic, nl, r, x, n3, m, ndf, undf, dm, nth);            /* r = A x */
+#pragma omp parallel for schedule(static) num_threads(nth)
+  for (int i = 0; i < undf; ++i) r[i] = -1.0 * r[i];                             /* r = -r */
+#pragma omp parallel for schedule(static) num_threads(nth)
+  for (int i = 0; i < undf; ++i) r[i] = r[i] + 1.0 * b[i];                       /* r += b */
+  const double r0 = sqrt(psy_dot(r, r, undf, nth));
+  double rn = r0;
+  int it = 0;
+  if (r0 >= 2.220446049250313e-16) {
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int i = 0; i < undf; ++i) z[i] = r[i];                                  /* z = P^{-1} r (copy) */
+    double rz = psy_dot(r, z, undf, nth);
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int i = 0; i < undf; ++i) p[i] = z[i];
+    for (it = 1; it <= max_iter; ++it) {
+      psy_apply(ncolour, npc, cic, nl, z, p, n3, m, ndf, undf, dm, nth);        /* z = A p */
+      const double alpha = rz / psy_dot(p, z, undf, nth);
+#pragma omp parallel for schedule(static) num_threads(nth)
+      for (int i = 0; i < undf; ++i) x[i] = x[i] + alpha * p[i];
+#pragma omp parallel for schedule(static) num_threads(nth)
+      for (int i = 0; i < undf; ++i) r[i] = r[i] + (-alpha) * z[i];
+      if (monitor) {
+        rn = sqrt(psy_dot(r, r, undf, nth));
+        if (rn / r0 <= r_tol) break;
+      }
+#pragma omp parallel for schedule(static) num_threads(nth)
+      for (int i = 0; i < undf; ++i) z[i] = r[i];
+      const double rz_new = psy_dot(r, z, undf, nth);
+      const double beta = rz_new / rz;
+#pragma omp parallel for schedule(static) num_threads(nth)
+      for (int i = 0; i < undf; ++i) p[i] = beta * p[i] + z[i];
+      rz = rz_new;
+    }
+    if (it > max_iter) it = max_iter;
+  }
+  if (r_nrm0_out) *r_nrm0_out = r0;
+  if (r_nrm_out) *r_nrm_out = rn;
+  free(r);
+  free(p);
+  free(z);
+  return it;
+}

@@ -195,3 +195,27 @@ def test_mm_diagonal_bitexact_vs_reference(dtype):
                     dd[fs.dofmap[cell, df] - 1 + k] += M[df, df, cell * nl + k]
         np.testing.assert_allclose(a.astype(np.float64) - d0, dd, rtol=1e-5 if dtype == np.float32 else 1e-12,
                                    atol=1e-5 if dtype == np.float32 else 1e-12)
+
+
+def test_reference_cpu_cg_matches_the_restatement():
+    """ref_cg_r64 (the CPU baseline of the solver path: the flang-built kernel in
+    the coloured loop, built-ins as parallel loops) against oracle.solvers.cg_solve:
+    same iteration count, iterates within the rounding of the dot order."""
+    if oracle.ref() is None:
+        pytest.skip("reference kernels not built here (oracle/_ref)")
+    from oracle.solvers import cg_solve
+    from lfric_core_amd.configs import precision_config
+    cfg = precision_config(6, 8, with_div_grad=False)
+    A = lambda v: oracle.invoke_coloured(0, cfg.colouring, cfg.nlayers, np.zeros_like(v), v, cfg.mass,  # noqa: E731
+                                         cfg.w2.dofmap, cfg.w2.dofmap)
+    xo = np.zeros(cfg.w2.undf)
+    it_o, conv_o, r0_o, rn_o = cg_solve(A, xo, cfg.b, r_tol=1e-10, max_iter=500)
+    for nt in (1, 4):
+        x = np.zeros(cfg.w2.undf)
+        it, r0, rn = oracle.ref_cg(cfg.colouring, cfg.nlayers, cfg.mass, cfg.w2.dofmap, x, cfg.b, r_tol=1e-10,
+                                   max_iter=500, nthreads=nt)
+        assert it == it_o and abs(r0 - r0_o) <= 1e-13 * r0_o
+        assert np.linalg.norm(x - xo) <= 1e-11 * np.linalg.norm(xo)
+    x = np.zeros(cfg.w2.undf)
+    it, _, _ = oracle.ref_cg(cfg.colouring, cfg.nlayers, cfg.mass, cfg.w2.dofmap, x, cfg.b, max_iter=7, monitor=False)
+    assert it == 7
