@@ -18,6 +18,7 @@
 // group of loads one group ahead); c' goes to LDS, d' stays in registers,
 // and the solution goes back through LDS for a coalesced store.
 #include <cmath>
+#include <cstdint>
 #include <vector>
 
 #include "lfr_internal.h"
@@ -229,13 +230,139 @@ k_appinv_staged(T* __restrict__ lhs, const T* __restrict__ x, const T* __restric
   }
 }
 
-// FP32 appinv at 70 rows takes the staged loads unless LFR_APPINV_STAGED=0.
-bool appinv_staged() {
-  static const bool on = [] {
+// columnwise_op_appinv, FP32, every load issued at entry.  A C96 W3 solve
+// has about one wave per SIMD, so a wave that loads a stage, waits, solves
+// it and loads the next spends its life on memory latency (measured: each
+// stage's loads cost ~2 us, the whole kernel ran as long as one wave's chain).
+// Here a wave owns COLS columns (COLS of its 64 lanes run the recurrence) and
+// issues all its loads up front: the bands of every KCH-row stage into
+// registers as 8-byte pairs (NST * SLOTS per lane; a column's bands are
+// 3 NROW contiguous values, so stage s of column cs is the run
+// [3 KCH s, 3 KCH (s+1)) of it) and the right-hand side of all NROW rows of
+// its columns in flat order (64 consecutive dofs per instruction when the
+// columns are contiguous).  The right-hand side goes to LDS, each band stage
+// goes to LDS before its rows are solved (odd pitches: conflict-free per-lane
+// reads), the solution goes back through LDS and out in the same flat order
+// with streaming stores.  The recurrence is k_appinv's term for term, so the
+// bits are the same.
+template <int NROW, int KCH, int COLS>
+__global__ void __launch_bounds__(kCols)
+k_appinv_deep(float* __restrict__ lhs, const float* __restrict__ x, const float* __restrict__ cm,
+              const int32_t* __restrict__ map0, int ndf, const int32_t* __restrict__ ind, int ncell) {
+  static_assert(NROW % KCH == 0 && (3 * KCH) % 2 == 0 && COLS <= kCols, "stage shape");
+  constexpr int NST = NROW / KCH;
+  constexpr int NB = 3 * KCH;                        // band values of one column per stage
+  constexpr int NBV = NB / 2;                        // 8-byte pairs of them
+  constexpr int PB = NB + 1;                         // odd pitches
+  constexpr int PX = NROW | 1;
+  constexpr int SLOTS = (COLS * NBV + kCols - 1) / kCols;
+  constexpr int NXF = (COLS * NROW + kCols - 1) / kCols;
+  __shared__ float sb[COLS * PB];
+  __shared__ float sxa[COLS * PX];
+  __shared__ int32_t sbase[COLS];
+  __shared__ int32_t sind[NROW];
+  const int lane = threadIdx.x;
+  const int c0 = blockIdx.x * COLS;
+  const int ncols = min(COLS, ncell - c0);
+  // bands: pair slot it of stage s is pair qv of column cs (lane + it kCols =
+  // cs NBV + qv), clamped into the workgroup's columns
+  const float* cmw = cm + (int64_t)c0 * 3 * NROW;
+  float2 rb[NST][SLOTS];
+#pragma unroll
+  for (int s = 0; s < NST; ++s)
+#pragma unroll
+    for (int it = 0; it < SLOTS; ++it) {
+      const int e = min(lane + it * kCols, COLS * NBV - 1);
+      const int cs = e / NBV, qv = e - cs * NBV;
+      rb[s][it] = *reinterpret_cast<const float2*>(cmw + min(cs, ncols - 1) * 3 * NROW + s * NB + 2 * qv);
+    }
+  if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
+  for (int i = lane; i < NROW; i += kCols) sind[i] = ind0(ind, i, NROW);
+  __syncthreads();
+  // right-hand side, flat slot e = column c row r (e = c NROW + r)
+  const int nflat = ncols * NROW;
+  {
+    float rf[NXF];
+#pragma unroll
+    for (int it = 0; it < NXF; ++it) {
+      const int e = min(lane + it * kCols, nflat - 1);
+      const int c = e / NROW, r = e - c * NROW;
+      rf[it] = x[sbase[c] + sind[r]];
+    }
+#pragma unroll
+    for (int it = 0; it < NXF; ++it) {
+      const int e = lane + it * kCols;
+      if (e < nflat) {
+        const int c = e / NROW;
+        sxa[c * PX + (e - c * NROW)] = rf[it];
+      }
+    }
+  }
+  float cp[NROW], dp[NROW];
+#pragma unroll
+  for (int s = 0; s < NST; ++s) {
+    __syncthreads();   // the previous stage's reads are done (and, first time, the rhs is in)
+#pragma unroll
+    for (int it = 0; it < SLOTS; ++it) {
+      const int e = lane + it * kCols;
+      if (e < COLS * NBV) {
+        const int cs = e / NBV, qv = e - cs * NBV;
+        sb[cs * PB + 2 * qv] = rb[s][it].x;
+        sb[cs * PB + 2 * qv + 1] = rb[s][it].y;
+      }
+    }
+    __syncthreads();
+    if (lane < COLS) {
+#pragma unroll
+      for (int q = 0; q < KCH; ++q) {
+        const int i = s * KCH + q;
+        const float* v = sb + lane * PB + 3 * q;
+        const float a = v[0], d = v[1], u = v[2], xv = sxa[lane * PX + i];
+        if (i == 0) {
+          const float inv = 1.0f / d;
+          cp[i] = inv * u;
+          dp[i] = inv * xv;
+        } else {
+          const float ac = a * cp[i - 1];
+          const float inv = 1.0f / (d - ac);
+          if (i < NROW - 1) cp[i] = inv * u;
+          const float ad = a * dp[i - 1];
+          dp[i] = inv * (xv - ad);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = NROW - 2; i >= 0; --i) {
+    const float cd = cp[i] * dp[i + 1];
+    dp[i] = dp[i] - cd;
+  }
+  __syncthreads();
+  if (lane < COLS) {
+#pragma unroll
+    for (int i = 0; i < NROW; ++i) sxa[lane * PX + i] = dp[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NXF; ++it) {
+    const int e = lane + it * kCols;
+    if (e < nflat) {
+      const int c = e / NROW, r = e - c * NROW;
+      __builtin_nontemporal_store(sxa[c * PX + r], lhs + sbase[c] + sind[r]);
+    }
+  }
+}
+
+// FP32 appinv kernel choice: 1 the unpipelined staged kernel, 0 the per-lane
+// k_appinv (LFR_APPINV_STAGED=0/1); otherwise the up-front-load kernel.
+int appinv_variant() {
+  static const int v = [] {
     const char* e = getenv("LFR_APPINV_STAGED");
-    return !(e && e[0] == '0');
+    if (e && e[0] == '0' && !e[1]) return 0;
+    if (e && e[0] == '1' && !e[1]) return 1;
+    return 2;
   }();
-  return on;
+  return v;
 }
 
 // Generic row count: same recurrence with c' and d' in a global scratch
@@ -280,9 +407,24 @@ int launch_appinv(lfr_ctx* ctx, T* lhs, const T* x, const T* cm, const lfr_space
                   const int32_t* ind, int nrow, int ncell) {
   const unsigned nb = (unsigned)((ncell + kCols - 1) / kCols);
   if constexpr (sizeof(T) == 4 && sizeof(C) == 4) {
-    if (nrow == 70 && appinv_staged()) {
+    const int v = nrow == 70 ? appinv_variant() : 0;
+    if (v == 1) {
       hipLaunchKernelGGL((k_appinv_staged<T, 70>), dim3(nb), dim3(kCols), 0, ctx->stream, lhs, x, cm, sp->map0_d,
                          sp->ndf, ind, ncell);
+      LFR_CHECK_LAUNCH(ctx);
+      return LFR_OK;
+    }
+    // 8-byte band pairs need an 8-byte aligned matrix (columns are 840 bytes apart)
+    if (v == 2 && ((uintptr_t)cm & 7) == 0) {
+      // 32 columns a wave while the grid fits two waves per SIMD (all resident
+      // at once); beyond that, 16 columns a wave (three waves per SIMD)
+      if ((int64_t)(ncell + 31) / 32 <= 2048) {
+        hipLaunchKernelGGL((k_appinv_deep<70, 14, 32>), dim3((ncell + 31) / 32), dim3(kCols), 0, ctx->stream, lhs,
+                           x, cm, sp->map0_d, sp->ndf, ind, ncell);
+      } else {
+        hipLaunchKernelGGL((k_appinv_deep<70, 10, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, ctx->stream, lhs,
+                           x, cm, sp->map0_d, sp->ndf, ind, ncell);
+      }
       LFR_CHECK_LAUNCH(ctx);
       return LFR_OK;
     }

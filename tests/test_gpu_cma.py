@@ -63,3 +63,26 @@ def test_appinv_fp64_pivots(ctx):
     got = y.cpu().numpy().astype(np.float64)
     # FP64 recurrence rounded once to FP32: within one FP32 ulp of the FP64 solve
     np.testing.assert_allclose(got, ref64.astype(np.float32).astype(np.float64), rtol=2e-7, atol=0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("nc", [3, 6])
+def test_appinv_permuted_indirection(ctx, dtype, nc):
+    """A non-identity indirection_dofmap (rows solved in a permuted dof order)
+    at 70 rows, where the FP32 solve reads and writes through the indirection
+    table in flat order; 54 and 216 columns leave partial workgroups."""
+    from lfric_core_amd import api
+    print(f"case appinv permuted indirection C{nc} nl=70 {dtype.__name__}", flush=True)
+    m = cubed_sphere(nc)
+    w3 = build_dofmap(m, "W3", 70)
+    st = Stream(37)
+    cm = _helmholtz_columns(st, m.ncells, 70).astype(dtype)
+    x = st.uniform(w3.undf).astype(dtype)
+    ind = (np.random.default_rng(5).permutation(70) + 1).astype(np.int32)
+    ref = oracle.appinv(np.zeros_like(x), x, cm, w3.dofmap, ind)
+    sp = api.Space(ctx, w3)
+    op = api.CMAOperator(torch.from_numpy(cm).to(ctx.device), sp, ind)
+    y = torch.full((w3.undf,), float("nan"), dtype=op.matrix.dtype, device=ctx.device)
+    api.columnwise_op_appinv(y, torch.from_numpy(x).to(ctx.device), op)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(y.cpu().numpy(), ref)
