@@ -741,6 +741,78 @@ def bench_multigrid(ctx, torch, edge_cells=(192, 96, 48, 24), nl: int = 70, nrhs
     return out
 
 
+def profile_workload(kind: str, reps: int = 5) -> None:
+    """Short fixed workloads for rocprofv3 runs (no JSON line): `appinv`
+    (columnwise_op_appinv FP32 on W3 columns, C96L70 then C192L70, each call
+    after a 1 GiB cache-evicting read), `tri` (tri_solve FP32, same shapes),
+    `cg` (refined CG to 1e-10 and CG50 FP32 on the C96L70 W2 mass matrix,
+    packed operators), `copy` (a plain FP32 copy of the appinv C96 and C192
+    byte counts, for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+    from lfric_core_amd import api
+    from lfric_core_amd.function_space import build_dofmap
+    from lfric_core_amd.mesh import cubed_sphere
+    ctx = api.Context(0)
+    dev = ctx.device
+    nl = 70
+    flush = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    if kind in ("appinv", "tri", "copy"):
+        for nc in (96, 192):
+            fs = build_dofmap(cubed_sphere(nc), "W3", nl)
+            sp = api.Space(ctx, fs)
+            rng = np.random.default_rng(1)
+            u = fs.undf
+            if kind == "appinv":
+                A = np.empty((fs.mesh.ncells, nl, 3), np.float32)
+                A[:, :, 1] = 4.0 + rng.random((fs.mesh.ncells, nl))
+                A[:, :, 0] = -0.5 - 0.1 * rng.random((fs.mesh.ncells, nl))
+                A[:, :, 2] = -0.5 - 0.1 * rng.random((fs.mesh.ncells, nl))
+                op = api.CMAOperator(torch.from_numpy(A).to(dev), sp, np.arange(1, nl + 1, dtype=np.int32))
+                x = torch.from_numpy(rng.random(u, dtype=np.float32)).to(dev)
+                y = torch.empty_like(x)
+                fn = lambda: api.columnwise_op_appinv(y, x, op)   # noqa: E731
+            elif kind == "tri":
+                t0 = torch.from_numpy((4 + rng.random(u)).astype(np.float32)).to(dev)
+                tp = torch.from_numpy((-0.5 - 0.1 * rng.random(u)).astype(np.float32)).to(dev)
+                tm = torch.from_numpy((-0.5 - 0.1 * rng.random(u)).astype(np.float32)).to(dev)
+                x = torch.from_numpy(rng.random(u, dtype=np.float32)).to(dev)
+                y = torch.empty_like(x)
+                fn = lambda: api.tri_solve(y, x, t0, tp, tm, sp)   # noqa: E731
+            else:
+                # reads 4 u floats, writes u: the appinv byte count
+                src = torch.ones(4 * u, dtype=torch.float32, device=dev)
+                dst = torch.empty(u, dtype=torch.float32, device=dev)
+                fn = lambda: torch.sum(src.view(4, u), dim=0, out=dst)   # noqa: E731
+            for _ in range(reps):
+                flush.sum()
+                fn()
+            torch.cuda.synchronize()
+            log(f"profile {kind} C{nc}L{nl}: {reps} calls")
+    elif kind == "cg":
+        from lfric_core_amd.configs import precision_config
+        from lfric_core_amd.solvers import ConjugateGradient, RefinedCG
+        cfg = precision_config(96, nl, with_div_grad=False)
+        w2 = api.Space(ctx, cfg.w2)
+        M64 = torch.from_numpy(cfg.mass).to(dev)
+        op64 = api.LMAOperator(M64, w2, w2, cfg.colouring)
+        op32 = api.LMAOperator(M64.float().contiguous(), w2, w2, cfg.colouring)
+        op64.packed = api.PackedOperator(op64)
+        op32.packed = api.PackedOperator(op32)
+        b = torch.from_numpy(cfg.b).to(dev)
+        for _ in range(2):
+            x = torch.zeros_like(b)
+            RefinedCG(op32, op64, r_tol=1e-10).apply(x, b)
+        b32 = b.float()
+        xb = torch.zeros_like(b32)
+        cg = ConjugateGradient(op32, max_iter=50, monitor_convergence=False)
+        for _ in range(2):
+            cg.apply(xb, b32, x_is_zero=True)
+        torch.cuda.synchronize()
+        log("profile cg: 2 refined solves, 2 x CG50 FP32")
+    else:
+        raise SystemExit(f"unknown --profile workload {kind!r}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -762,7 +834,12 @@ def main():
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a plain copy of 1 GiB (FP32) 5 times, for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes")
+    ap.add_argument("--profile", choices=["appinv", "tri", "cg", "copy"],
+                    help="run one short fixed workload for a rocprofv3 run and exit (no JSON line)")
     args = ap.parse_args()
+    if args.profile:
+        profile_workload(args.profile)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not built)"}))

@@ -863,6 +863,102 @@ k_thomas(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ t0, c
   }
 }
 
+// tri_solve, FP32, every load issued at entry (the scheme of
+// k_appinv_deep): a wave owns COLS columns, loads the KCH-row stages of
+// tri_minus, tri_0, tri_plus and x of all its columns into registers up
+// front (slot it of stage s, array v: row s KCH + qs of column cs with
+// lane + it kCols = cs KCH + qs; a column's W3 dofs are contiguous), moves
+// one stage at a time to LDS for the recurrence, and writes the solution
+// back through LDS in flat order (64 consecutive dofs per instruction when
+// the columns are contiguous) with streaming stores.  Same arithmetic as
+// k_tri_solve.
+template <int NROW, int KCH, int COLS>
+__global__ void __launch_bounds__(kCols)
+k_thomas_deep(float* __restrict__ y, const float* __restrict__ x, const float* __restrict__ t0,
+              const float* __restrict__ tp, const float* __restrict__ tm, const int32_t* __restrict__ map0,
+              int ndf, int ncell) {
+  static_assert(NROW % KCH == 0 && COLS <= kCols, "stage shape");
+  constexpr int NST = NROW / KCH;
+  constexpr int SLOTS = (COLS * KCH + kCols - 1) / kCols;
+  constexpr int PB = 4 * KCH + 1;                    // odd pitches
+  constexpr int PX = NROW | 1;
+  __shared__ float sb[COLS * PB];
+  __shared__ float sxa[COLS * PX];
+  __shared__ int32_t sbase[COLS];
+  const int lane = threadIdx.x;
+  const int c0 = blockIdx.x * COLS;
+  const int ncols = min(COLS, ncell - c0);
+  if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
+  __syncthreads();
+  const float* src[4] = {tm, t0, tp, x};
+  float rv[NST][4][SLOTS];
+#pragma unroll
+  for (int s = 0; s < NST; ++s)
+#pragma unroll
+    for (int it = 0; it < SLOTS; ++it) {
+      const int e = min(lane + it * kCols, COLS * KCH - 1);
+      const int cs = e / KCH, qs = e - cs * KCH;
+      const int64_t a = (int64_t)sbase[min(cs, ncols - 1)] + s * KCH + qs;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) rv[s][v][it] = src[v][a];
+    }
+  float cp[NROW], dp[NROW];
+#pragma unroll
+  for (int s = 0; s < NST; ++s) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SLOTS; ++it) {
+      const int e = lane + it * kCols;
+      if (e < COLS * KCH) {
+        const int cs = e / KCH, qs = e - cs * KCH;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sb[cs * PB + 4 * qs + v] = rv[s][v][it];
+      }
+    }
+    __syncthreads();
+    if (lane < COLS) {
+#pragma unroll
+      for (int q = 0; q < KCH; ++q) {
+        const int i = s * KCH + q;
+        const float* w = sb + lane * PB + 4 * q;
+        const float a = w[0], dg = w[1], u = w[2], xv = w[3];
+        if (i == 0) {
+          const float inv = 1.0f / dg;
+          cp[i] = u * inv;
+          dp[i] = xv * inv;
+        } else {
+          const float ac = a * cp[i - 1];
+          const float inv = 1.0f / (dg - ac);
+          const float ad = a * dp[i - 1];
+          if (i < NROW - 1) cp[i] = u * inv;
+          dp[i] = (xv - ad) * inv;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = NROW - 2; i >= 0; --i) {
+    const float cd = cp[i] * dp[i + 1];
+    dp[i] = dp[i] - cd;
+  }
+  __syncthreads();
+  if (lane < COLS) {
+#pragma unroll
+    for (int i = 0; i < NROW; ++i) sxa[lane * PX + i] = dp[i];
+  }
+  __syncthreads();
+  const int nflat = ncols * NROW;
+  constexpr int NXF = (COLS * NROW + kCols - 1) / kCols;
+#pragma unroll
+  for (int it = 0; it < NXF; ++it) {
+    const int e = lane + it * kCols;
+    if (e < nflat) {
+      const int c = e / NROW, r = e - c * NROW;
+      __builtin_nontemporal_store(sxa[c * PX + r], y + (int64_t)sbase[c] + r);
+    }
+  }
+}
+
 // operator_tri_solve with LDS staging: the 2x2 local matrices of the 64
 // columns of a workgroup come in KCH-row stages with coalesced, branch-free
 // loads (clamped addresses; slots past the last column or row hold
@@ -992,6 +1088,23 @@ template <typename T, int MODE>
 bool launch_thomas(hipStream_t s, T* y, const T* x, const T* b0, const T* b1, const T* b2, const lfr_space* sp) {
   const unsigned nb = (unsigned)((sp->ncell + kCols - 1) / kCols);
   if constexpr (MODE == 0) {   // tri_solve: b0 = tri_0, b1 = tri_plus, b2 = tri_minus
+    if constexpr (sizeof(T) == 4) {
+      // every load at entry: 32 columns a wave while the grid fits two waves
+      // per SIMD, 16 beyond (LFR_TRI_DEEP=0 keeps k_thomas)
+      static const bool deep = [] {
+        const char* e = getenv("LFR_TRI_DEEP");
+        return !(e && e[0] == '0');
+      }();
+      if (deep && sp->nlayers == 70) {
+        if ((sp->ncell + 31) / 32 <= 2048)
+          hipLaunchKernelGGL((k_thomas_deep<70, 14, 32>), dim3((sp->ncell + 31) / 32), dim3(kCols), 0, s, y, x, b0,
+                             b1, b2, sp->map0_d, sp->ndf, sp->ncell);
+        else
+          hipLaunchKernelGGL((k_thomas_deep<70, 10, 16>), dim3((sp->ncell + 15) / 16), dim3(kCols), 0, s, y, x, b0,
+                             b1, b2, sp->map0_d, sp->ndf, sp->ncell);
+        return true;
+      }
+    }
     switch (sp->nlayers) {
 #define CASE(NR)                                                                                          \
   case NR:                                                                                                \
