@@ -750,6 +750,30 @@ int lfr_iterative_solve(lfr_ctx* ctx, const lfr_iterative_solver* s, const lfr_l
 int lfr_mm_diagonal(lfr_ctx* ctx, int prec, void* diag_d, const void* matrix_d, const lfr_space* space,
                     const lfr_plan* plan);
 
+/* ---------------------------------------------------------------------------
+ * Autotuner (no counterpart in the reference, whose kernels have one fixed
+ * loop form).  Kernels with several result-identical launch shapes (column
+ * solves: columns a wave and rows a stage; packed apply: output staging and
+ * cache policy; multigrid smoother: workgroup size) time every shape on the
+ * first call of a problem size and keep the fastest under a key naming the
+ * kernel family and the shape ("appinv f32 nrow=70 ncell=55296", ...).
+ * LFR_TUNE=0 disables timing (defaults are used); LFR_TUNE_FILE names a
+ * file loaded by lfr_init and written by lfr_finalise.  Choices never change
+ * results, only speed.
+ * ------------------------------------------------------------------------- */
+/* Store `choice` for `key` (used from then on, also by graph captures). */
+int lfr_tune_set(lfr_ctx* ctx, const char* key, int choice);
+/* The stored choice for `key` (LFR_ERR_ARG if none). */
+int lfr_tune_get(lfr_ctx* ctx, const char* key, int* choice);
+/* Number of stored choices, and entry i (0-based, key order). */
+int lfr_tune_count(const lfr_ctx* ctx, int* n);
+int lfr_tune_entry(lfr_ctx* ctx, int i, char* key, int keylen, int* choice);
+/* Forget every stored choice. */
+int lfr_tune_clear(lfr_ctx* ctx);
+/* Write / read the stored choices as "choice key" lines. */
+int lfr_tune_save(lfr_ctx* ctx, const char* path);
+int lfr_tune_load(lfr_ctx* ctx, const char* path);
+
 #ifdef __cplusplus
 }
 #endif

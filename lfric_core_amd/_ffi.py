@@ -79,6 +79,13 @@ SIGNATURES = {
     "lfr_vec_dot": [_P, _I, _I64, _P, _P, C.POINTER(_D)],
     "lfr_vec_dot_dev": [_P, _I, _I64, _P, _P, _P],
     "lfr_vec_dot_dev_noalloc": [_P, _I, _I64, _P, _P, _P],
+    "lfr_tune_set": [_P, C.c_char_p, _I],
+    "lfr_tune_get": [_P, C.c_char_p, C.POINTER(_I)],
+    "lfr_tune_count": [_P, C.POINTER(_I)],
+    "lfr_tune_entry": [_P, _I, C.c_char_p, _I, C.POINTER(_I)],
+    "lfr_tune_clear": [_P],
+    "lfr_tune_save": [_P, C.c_char_p],
+    "lfr_tune_load": [_P, C.c_char_p],
     # solver entry points take structure pointers; lfric_core_amd.solvers
     # installs the typed signatures before calling them
     "lfr_columnwise_op_appinv": [_P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I],

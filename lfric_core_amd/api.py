@@ -58,6 +58,36 @@ class Context:
     def synchronize(self) -> None:
         check(lib().lfr_synchronize(self.h), self.h)
 
+    # -- autotuner: launch-shape choices per (kernel family, shape) key ------
+    def tune_entries(self) -> dict:
+        """Every stored launch-shape choice, key -> variant index."""
+        n = C.c_int()
+        check(lib().lfr_tune_count(self.h, C.byref(n)), self.h)
+        out = {}
+        buf = C.create_string_buffer(256)
+        choice = C.c_int()
+        for i in range(n.value):
+            check(lib().lfr_tune_entry(self.h, i, buf, 256, C.byref(choice)), self.h)
+            out[buf.value.decode()] = choice.value
+        return out
+
+    def tune_set(self, key: str, choice: int) -> None:
+        check(lib().lfr_tune_set(self.h, key.encode(), int(choice)), self.h)
+
+    def tune_get(self, key: str) -> int:
+        choice = C.c_int()
+        check(lib().lfr_tune_get(self.h, key.encode(), C.byref(choice)), self.h)
+        return choice.value
+
+    def tune_clear(self) -> None:
+        check(lib().lfr_tune_clear(self.h), self.h)
+
+    def tune_save(self, path) -> None:
+        check(lib().lfr_tune_save(self.h, str(path).encode()), self.h)
+
+    def tune_load(self, path) -> None:
+        check(lib().lfr_tune_load(self.h, str(path).encode()), self.h)
+
     def close(self) -> None:
         if getattr(self, "h", None):
             lib().lfr_finalise(self.h)

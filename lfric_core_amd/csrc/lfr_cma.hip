@@ -416,16 +416,43 @@ int launch_appinv(lfr_ctx* ctx, T* lhs, const T* x, const T* cm, const lfr_space
     }
     // 8-byte band pairs need an 8-byte aligned matrix (columns are 840 bytes apart)
     if (v == 2 && ((uintptr_t)cm & 7) == 0) {
-      // 32 columns a wave while the grid fits two waves per SIMD (all resident
-      // at once); beyond that, 16 columns a wave (three waves per SIMD)
-      if ((int64_t)(ncell + 31) / 32 <= 2048) {
-        hipLaunchKernelGGL((k_appinv_deep<70, 14, 32>), dim3((ncell + 31) / 32), dim3(kCols), 0, ctx->stream, lhs,
-                           x, cm, sp->map0_d, sp->ndf, ind, ncell);
-      } else {
-        hipLaunchKernelGGL((k_appinv_deep<70, 10, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, ctx->stream, lhs,
-                           x, cm, sp->map0_d, sp->ndf, ind, ncell);
+      // result-identical variants: columns a wave x rows a stage, and the
+      // staged kernel.  Default: 32 columns a wave while the grid fits two
+      // waves per SIMD (all resident at once), 16 beyond.  The autotuner
+      // times them on the first call of a shape when lhs does not alias x
+      // (every variant then rewrites the same lhs from the same inputs).
+      const auto launch = [&](int c) -> int {
+        switch (c) {
+          case 0:
+            hipLaunchKernelGGL((k_appinv_deep<70, 14, 32>), dim3((ncell + 31) / 32), dim3(kCols), 0, ctx->stream,
+                               lhs, x, cm, sp->map0_d, sp->ndf, ind, ncell);
+            break;
+          case 1:
+            hipLaunchKernelGGL((k_appinv_deep<70, 10, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, ctx->stream,
+                               lhs, x, cm, sp->map0_d, sp->ndf, ind, ncell);
+            break;
+          case 2:
+            hipLaunchKernelGGL((k_appinv_deep<70, 14, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, ctx->stream,
+                               lhs, x, cm, sp->map0_d, sp->ndf, ind, ncell);
+            break;
+          case 3:
+            hipLaunchKernelGGL((k_appinv_deep<70, 10, 32>), dim3((ncell + 31) / 32), dim3(kCols), 0, ctx->stream,
+                               lhs, x, cm, sp->map0_d, sp->ndf, ind, ncell);
+            break;
+          default:
+            hipLaunchKernelGGL((k_appinv_staged<T, 70>), dim3(nb), dim3(kCols), 0, ctx->stream, lhs, x, cm,
+                               sp->map0_d, sp->ndf, ind, ncell);
+        }
+        return hipGetLastError() == hipSuccess ? LFR_OK : LFR_ERR_HIP;
+      };
+      const int dflt = (ncell + 31) / 32 <= 2048 ? 0 : 1;
+      int c = dflt;
+      if ((const void*)lhs != (const void*)x) {
+        char key[96];
+        snprintf(key, sizeof key, "appinv f32 nrow=70 ncell=%d", ncell);
+        c = lfr_tune_pick(ctx, key, 5, dflt, launch);
       }
-      LFR_CHECK_LAUNCH(ctx);
+      if (launch(c) != LFR_OK) return lfr_set_error(ctx, LFR_ERR_HIP, "appinv: launch failed");
       return LFR_OK;
     }
   }
@@ -1085,23 +1112,45 @@ k_optri_staged(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__
 }
 
 template <typename T, int MODE>
-bool launch_thomas(hipStream_t s, T* y, const T* x, const T* b0, const T* b1, const T* b2, const lfr_space* sp) {
+bool launch_thomas(lfr_ctx* ctx, hipStream_t s, T* y, const T* x, const T* b0, const T* b1, const T* b2,
+                   const lfr_space* sp) {
   const unsigned nb = (unsigned)((sp->ncell + kCols - 1) / kCols);
   if constexpr (MODE == 0) {   // tri_solve: b0 = tri_0, b1 = tri_plus, b2 = tri_minus
     if constexpr (sizeof(T) == 4) {
-      // every load at entry: 32 columns a wave while the grid fits two waves
-      // per SIMD, 16 beyond (LFR_TRI_DEEP=0 keeps k_thomas)
+      // every load at entry (LFR_TRI_DEEP=0 keeps k_thomas); result-identical
+      // variants (columns a wave x rows a stage, k_thomas) chosen by the
+      // autotuner on the first call of a shape (y never aliases an input)
       static const bool deep = [] {
         const char* e = getenv("LFR_TRI_DEEP");
         return !(e && e[0] == '0');
       }();
       if (deep && sp->nlayers == 70) {
-        if ((sp->ncell + 31) / 32 <= 2048)
-          hipLaunchKernelGGL((k_thomas_deep<70, 14, 32>), dim3((sp->ncell + 31) / 32), dim3(kCols), 0, s, y, x, b0,
-                             b1, b2, sp->map0_d, sp->ndf, sp->ncell);
-        else
-          hipLaunchKernelGGL((k_thomas_deep<70, 10, 16>), dim3((sp->ncell + 15) / 16), dim3(kCols), 0, s, y, x, b0,
-                             b1, b2, sp->map0_d, sp->ndf, sp->ncell);
+        const int ncell = sp->ncell;
+        const auto launch = [&](int c) -> int {
+          switch (c) {
+            case 0:
+              hipLaunchKernelGGL((k_thomas_deep<70, 14, 32>), dim3((ncell + 31) / 32), dim3(kCols), 0, s, y, x, b0,
+                                 b1, b2, sp->map0_d, sp->ndf, ncell);
+              break;
+            case 1:
+              hipLaunchKernelGGL((k_thomas_deep<70, 10, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y, x, b0,
+                                 b1, b2, sp->map0_d, sp->ndf, ncell);
+              break;
+            case 2:
+              hipLaunchKernelGGL((k_thomas_deep<70, 14, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y, x, b0,
+                                 b1, b2, sp->map0_d, sp->ndf, ncell);
+              break;
+            default:
+              hipLaunchKernelGGL((k_thomas<T, 70>), dim3(nb), dim3(kCols), 0, s, y, x, b0, b1, b2, sp->map0_d,
+                                 sp->ndf, ncell);
+          }
+          return hipGetLastError() == hipSuccess ? LFR_OK : LFR_ERR_HIP;
+        };
+        const int dflt = (ncell + 31) / 32 <= 2048 ? 0 : 1;
+        char key[96];
+        snprintf(key, sizeof key, "tri_solve f32 nl=70 ncell=%d", ncell);
+        const int c = ctx ? lfr_tune_pick(ctx, key, 4, dflt, launch) : dflt;
+        (void)launch(c);
         return true;
       }
     }
@@ -1136,9 +1185,9 @@ int lfr_tri_solve(lfr_ctx* ctx, int prec, void* y, const void* x, const void* tr
   LFR_REQUIRE(ctx, y != x && y != tri_0 && y != tri_plus && y != tri_minus,
               "lfr_tri_solve: y must not alias an input");
   const bool staged =
-      prec == 4 ? launch_thomas<float, 0>(ctx->stream, (float*)y, (const float*)x, (const float*)tri_0,
+      prec == 4 ? launch_thomas<float, 0>(ctx, ctx->stream, (float*)y, (const float*)x, (const float*)tri_0,
                                           (const float*)tri_plus, (const float*)tri_minus, space)
-                : launch_thomas<double, 0>(ctx->stream, (double*)y, (const double*)x, (const double*)tri_0,
+                : launch_thomas<double, 0>(ctx, ctx->stream, (double*)y, (const double*)x, (const double*)tri_0,
                                            (const double*)tri_plus, (const double*)tri_minus, space);
   if (staged) {
     LFR_CHECK_LAUNCH(ctx);
@@ -1177,9 +1226,9 @@ int lfr_operator_tri_solve(lfr_ctx* ctx, int prec, void* y, const void* x, const
   LFR_REQUIRE(ctx, space->vstack == 1,
               "lfr_operator_tri_solve: the top dof of a cell does not follow its bottom dof");
   const bool staged =
-      prec == 4 ? launch_thomas<float, 1>(ctx->stream, (float*)y, (const float*)x, (const float*)matrix, nullptr,
+      prec == 4 ? launch_thomas<float, 1>(ctx, ctx->stream, (float*)y, (const float*)x, (const float*)matrix, nullptr,
                                           nullptr, space)
-                : launch_thomas<double, 1>(ctx->stream, (double*)y, (const double*)x, (const double*)matrix,
+                : launch_thomas<double, 1>(ctx, ctx->stream, (double*)y, (const double*)x, (const double*)matrix,
                                            nullptr, nullptr, space);
   if (staged) {
     LFR_CHECK_LAUNCH(ctx);

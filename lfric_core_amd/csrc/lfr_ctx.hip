@@ -6,6 +6,8 @@
 #include <mutex>
 #include <unordered_set>
 
+#include <cstdlib>
+
 #include "lfr_internal.h"
 
 int lfr_set_error(lfr_ctx* ctx, int code, const char* fmt, ...) {
@@ -121,6 +123,13 @@ int lfr_init(int device, lfr_ctx** out) {
     std::lock_guard<std::mutex> lk(g_live_mu);
     g_live.insert(ctx);
   }
+  // launch-shape choices of an earlier run (a missing file is not an error)
+  if (const char* tf = getenv("LFR_TUNE_FILE")) {
+    if (FILE* f = fopen(tf, "r")) {
+      fclose(f);
+      (void)lfr_tune_load_file(ctx, tf);
+    }
+  }
   *out = ctx;
   return LFR_OK;
 }
@@ -132,8 +141,12 @@ int lfr_finalise(lfr_ctx* ctx) {
     g_live.erase(ctx);
   }
   (void)hipStreamSynchronize(ctx->stream);
+  if (const char* tf = getenv("LFR_TUNE_FILE"))
+    if (!ctx->tune.empty()) (void)lfr_tune_save_file(ctx, tf);
   for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
   for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->tev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->work) (void)hipFree(ctx->work);
   if (ctx->kwork) (void)hipFree(ctx->kwork);

@@ -5,6 +5,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
+#include <map>
 #include <string>
 #include <utility>
 #include <vector>
@@ -46,6 +48,9 @@ struct lfr_ctx {
   // operator or preconditioner applied inside a solve may use `work`
   void* kwork = nullptr;
   size_t kwork_bytes = 0;
+  // autotuner: kept variant per (kernel family, shape) key, and its events
+  std::map<std::string, int> tune;
+  hipEvent_t tev[2] = {nullptr, nullptr};
 };
 
 struct lfr_space {
@@ -138,9 +143,24 @@ struct lfr_lma_packed {
   // unified layout: every column in memory order, 2 entries x ndf_in rows
   // of nl+1 levels (the unified gather's packed form); off/pcs have one slot
   bool unified = false;
+  // unified gather variant kept by the autotuner at pack time (-1: the
+  // global default): output staged through LDS, non-temporal matrix loads
+  int stage = -1, nt = -1;
 };
 
 int lfr_set_error(lfr_ctx* ctx, int code, const char* fmt, ...);
+
+// Autotuner (lfr_tune.hip): the stored choice for `key`, else (tuning on,
+// stream not capturing) the fastest of ncand result-identical variants,
+// timed by calling run(c) on ctx->stream, which is stored; else dflt.
+int lfr_tune_pick(lfr_ctx* ctx, const std::string& key, int ncand, int dflt, const std::function<int(int)>& run);
+bool lfr_tune_enabled();
+int lfr_tune_load_file(lfr_ctx* ctx, const char* path);
+int lfr_tune_save_file(const lfr_ctx* ctx, const char* path);
+// Unified-gather variant flags of a packed operator (its tuned choice or the
+// global default).
+bool lfr_pk_stage(const lfr_lma_packed* pk);
+bool lfr_pk_nt(const lfr_lma_packed* pk);
 
 // Validate a square solver operator (kind-0 plan of its space, packed copy
 // of the same plan) and build its column records outside any graph capture.

@@ -213,13 +213,12 @@ void launch_rec_shape(hipStream_t s, int kind, unsigned nb, const T* M, const T*
 }
 
 template <typename T, int NDF_IN, int KIND, int LPT>
-void launch_uni_t(hipStream_t s, bool nt, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur,
+void launch_uni_t(hipStream_t s, bool nt, bool st, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur,
                   int ncol, int nl, int64_t n3, int ndf_out, int overwrite, int64_t pcs) {
 #define LUNI(NTV, STV)                                                                                        \
   hipLaunchKernelGGL((lfr::k_gather_uni<T, NDF_IN, KIND, false, LPT, NTV, STV>), dim3(nbp), dim3(kThreads), 0, s, \
                      M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs, nbp, (const T*)nullptr, (double*)nullptr, \
                      (const double*)nullptr)
-  const bool st = lfr_gather_stage();
   if (nt) {
     if (st) LUNI(true, true); else LUNI(true, false);
   } else {
@@ -229,14 +228,14 @@ void launch_uni_t(hipStream_t s, bool nt, unsigned nbp, const T* M, const T* x, 
 }
 
 template <typename T, int NDF_IN>
-void launch_uni_k(hipStream_t s, int kind, int lpt, bool nt, unsigned nbp, const T* M, const T* x, T* y,
+void launch_uni_k(hipStream_t s, int kind, int lpt, bool nt, bool st, unsigned nbp, const T* M, const T* x, T* y,
                   const int32_t* ur, int ncol, int nl, int64_t n3, int ndf_out, int overwrite, int64_t pcs) {
   if (kind == 0) {
-    if (lpt == 4) launch_uni_t<T, NDF_IN, 0, 4>(s, nt, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
-    else launch_uni_t<T, NDF_IN, 0, 2>(s, nt, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+    if (lpt == 4) launch_uni_t<T, NDF_IN, 0, 4>(s, nt, st, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+    else launch_uni_t<T, NDF_IN, 0, 2>(s, nt, st, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
   } else {
-    if (lpt == 4) launch_uni_t<T, NDF_IN, 1, 4>(s, nt, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
-    else launch_uni_t<T, NDF_IN, 1, 2>(s, nt, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+    if (lpt == 4) launch_uni_t<T, NDF_IN, 1, 4>(s, nt, st, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+    else launch_uni_t<T, NDF_IN, 1, 2>(s, nt, st, nbp, M, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
   }
 }
 
@@ -252,12 +251,13 @@ int run_gather(lfr_ctx* ctx, const lfr_plan* plan, int kind, const T* M, const T
     const unsigned nbp = pad8(lfr_unified_blocks(plan, lptu));
     const T* Mu = pk ? (const T*)pk->data_d : M;
     const int64_t pcs = pk ? pk->pcs[0] : 0;
-    const bool nt = lfr_gather_vec_nt(pk != nullptr);
+    const bool nt = pk ? lfr_pk_nt(pk) : lfr_gather_vec_nt(false);
+    const bool st = pk ? lfr_pk_stage(pk) : lfr_gather_stage();
     const int ncol = lfr_plan_ncolumns_total(plan);
     if (in->ndf == 6)
-      launch_uni_k<T, 6>(ctx->stream, kind, lptu, nt, nbp, Mu, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+      launch_uni_k<T, 6>(ctx->stream, kind, lptu, nt, st, nbp, Mu, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
     else
-      launch_uni_k<T, 1>(ctx->stream, kind, lptu, nt, nbp, Mu, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
+      launch_uni_k<T, 1>(ctx->stream, kind, lptu, nt, st, nbp, Mu, x, y, ur, ncol, nl, n3, ndf_out, overwrite, pcs);
     LFR_CHECK_LAUNCH(ctx);
     return LFR_OK;
   }
@@ -602,6 +602,9 @@ bool lfr_gather_stage() {
   return on;
 }
 
+bool lfr_pk_stage(const lfr_lma_packed* pk) { return pk && pk->stage >= 0 ? pk->stage != 0 : lfr_gather_stage(); }
+bool lfr_pk_nt(const lfr_lma_packed* pk) { return pk && pk->nt >= 0 ? pk->nt != 0 : lfr_gather_vec_nt(true); }
+
 int lfr_plan_ncolumns_total(const lfr_plan* plan) {
   int n = 0;
   for (const auto& c : plan->classes) n += c.ncol;
@@ -866,6 +869,36 @@ extern "C" int lfr_lma_pack(lfr_ctx* ctx, int prec, const void* M, const lfr_spa
       delete pk;
       return LFR_ERR_NOMEM;  // message set by lfr_plan_records
     }
+  }
+  // autotuner: time the four unified-gather variants (output through LDS or
+  // not, non-temporal matrix loads or not; identical results) on scratch
+  // vectors and keep the fastest on the packed operator, unless the
+  // variant is forced through LFR_GATHER_STAGE / LFR_GATHER_VEC_NT
+  if (pk->unified && lfr_tune_enabled() && !getenv("LFR_GATHER_STAGE") && !getenv("LFR_GATHER_VEC_NT")) {
+    void* xs = nullptr;
+    void* ys = nullptr;
+    if (hipMalloc(&xs, (size_t)in->undf * prec) == hipSuccess && hipMalloc(&ys, (size_t)sp->undf * prec) == hipSuccess &&
+        hipMemsetAsync(xs, 0, (size_t)in->undf * prec, ctx->stream) == hipSuccess) {
+      char key[128];
+      snprintf(key, sizeof key, "packed gather prec=%d ndf=%d kind=%d ncol=%d nl=%d", prec, in->ndf, pk->kind,
+               lfr_plan_ncolumns_total(plan), nl);
+      const int dflt = (lfr_gather_stage() ? 1 : 0) | (lfr_gather_vec_nt(true) ? 2 : 0);
+      const auto run = [&](int c) -> int {
+        pk->stage = c & 1;
+        pk->nt = (c >> 1) & 1;
+        return prec == 4 ? run_gather<float>(ctx, plan, pk->kind, nullptr, (const float*)xs, (float*)ys, in, sp->ndf,
+                                              n3, 1, pk)
+                         : run_gather<double>(ctx, plan, pk->kind, nullptr, (const double*)xs, (double*)ys, in,
+                                              sp->ndf, n3, 1, pk);
+      };
+      const int c = lfr_tune_pick(ctx, key, 4, dflt, run);
+      pk->stage = c & 1;
+      pk->nt = (c >> 1) & 1;
+      (void)hipStreamSynchronize(ctx->stream);
+    }
+    (void)hipGetLastError();
+    if (xs) (void)hipFree(xs);
+    if (ys) (void)hipFree(ys);
   }
   // matrix_d may be released as soon as this returns
   const hipError_t e = hipStreamSynchronize(ctx->stream);

@@ -33,6 +33,7 @@
 // use and replayed; each level's kernels use the workgroup size chosen by a
 // short timing sweep at set-up (the autotuner).
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <cmath>
 #include <vector>
@@ -467,36 +468,25 @@ void mg_free(lfr_mg* mg) {
 }
 
 // Pick the workgroup size of the smoother (the dominant kernel) for each
-// level by timing one sweep with each candidate.
+// level through the autotuner (one sweep per candidate size, identical
+// results), keyed by the level's shape so that levels of the same size and
+// later hierarchies reuse the choice.
 template <typename T>
 int autotune(lfr_mg* mg) {
   lfr_ctx* ctx = mg->ctx;
-  hipEvent_t e0, e1;
-  LFR_HIP(ctx, hipEventCreate(&e0));
-  LFR_HIP(ctx, hipEventCreate(&e1));
   const int cand[3] = {64, 128, 256};   // launch bound of the smoother is 256
   for (auto& L : mg->lev) {
-    float best = 1e30f;
-    int bestb = 256;
-    for (int b : cand) {
-      L.block_smooth = b;
-      launch_smooth<T>(ctx->stream, mg, L, (T*)L.t_d, (const T*)L.x_d);  // warm
-      LFR_HIP(ctx, hipEventRecord(e0, ctx->stream));
-      for (int r = 0; r < 3; ++r) launch_smooth<T>(ctx->stream, mg, L, (T*)L.t_d, (const T*)L.x_d);
-      LFR_HIP(ctx, hipEventRecord(e1, ctx->stream));
-      LFR_HIP(ctx, hipEventSynchronize(e1));
-      float ms = 0;
-      LFR_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        bestb = b;
-      }
-    }
-    L.block_smooth = bestb;
+    char key[128];
+    snprintf(key, sizeof key, "mg smooth prec=%d nl=%d ncell=%d rhs=%d", (int)sizeof(T), mg->nl, L.ncell,
+             mg->nrhs);
+    const auto run = [&](int c) -> int {
+      L.block_smooth = cand[c];
+      launch_smooth<T>(ctx->stream, mg, L, (T*)L.t_d, (const T*)L.x_d);
+      return hipGetLastError() == hipSuccess ? LFR_OK : LFR_ERR_HIP;
+    };
+    L.block_smooth = cand[lfr_tune_pick(ctx, key, 3, 2, run)];
     L.block_other = 256;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return LFR_OK;
 }
 

@@ -163,6 +163,28 @@ k_residual(T* __restrict__ r, const T* __restrict__ b, int64_t n, double* __rest
   if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
 
+// The start of a solve from x = 0 in one pass: x = 0, r = b - A 0 = b
+// (the reference's r%scale(-1); r%axpy(1, b) on A x = 0 gives b exactly),
+// p = r when p is given (no preconditioner), and the partials of <r, r> in
+// k_residual's grid and order.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_cg_start_zero(T* __restrict__ x, T* __restrict__ r, T* __restrict__ p, const T* __restrict__ b, int64_t n,
+                double* __restrict__ part) {
+  __shared__ double lds[kThreads / 64];
+  double rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    const T v = b[i];
+    x[i] = T(0);
+    r[i] = v;
+    if (p) p[i] = v;
+    rr = rr + (double)v * (double)v;
+  }
+  rr = lfr::block_sum(rr, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
 // ||r0|| < epsilon: cg converged in 0 iterations
 // (sci_iterative_solver_mod.f90:523-529); every later kernel returns at entry.
 __global__ void k_cg_start(double* __restrict__ S, double eps) {
@@ -410,19 +432,19 @@ void launch_apply_uni_st(hipStream_t s, unsigned nbp, const T* M, const T* x, T*
 }
 
 template <typename T, int NDF_IN, int LPT, bool NT>
-void launch_apply_uni_nt(hipStream_t s, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur, int ncol,
-                         int nl, int64_t n3, int64_t pcs, const T* dotv, double* part, const double* stop) {
-  if (lfr_gather_stage())
+void launch_apply_uni_nt(hipStream_t s, bool st, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur,
+                         int ncol, int nl, int64_t n3, int64_t pcs, const T* dotv, double* part, const double* stop) {
+  if (st)
     launch_apply_uni_st<T, NDF_IN, LPT, NT, true>(s, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
   else
     launch_apply_uni_st<T, NDF_IN, LPT, NT, false>(s, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
 }
 
 template <typename T, int NDF_IN, int LPT>
-void launch_apply_uni(hipStream_t s, bool nt, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur,
+void launch_apply_uni(hipStream_t s, bool nt, bool st, unsigned nbp, const T* M, const T* x, T* y, const int32_t* ur,
                       int ncol, int nl, int64_t n3, int64_t pcs, const T* dotv, double* part, const double* stop) {
-  if (nt) launch_apply_uni_nt<T, NDF_IN, LPT, true>(s, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
-  else launch_apply_uni_nt<T, NDF_IN, LPT, false>(s, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+  if (nt) launch_apply_uni_nt<T, NDF_IN, LPT, true>(s, st, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+  else launch_apply_uni_nt<T, NDF_IN, LPT, false>(s, st, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
 }
 
 // y = A x (overwrite) by the gather kernels; with dotv: partials of
@@ -444,13 +466,14 @@ int apply_gather(lfr_ctx* ctx, const lfr_lma_op* A, const T* x, T* y, const T* d
     const T* M = A->packed ? (const T*)A->packed->data_d : (const T*)A->matrix_d;
     const int64_t pcs = A->packed ? A->packed->pcs[0] : 0;
     const int ncol = lfr_plan_ncolumns_total(A->plan);
-    const bool nt = lfr_gather_vec_nt(A->packed != nullptr);
+    const bool nt = A->packed ? lfr_pk_nt(A->packed) : lfr_gather_vec_nt(false);
+    const bool st = A->packed ? lfr_pk_stage(A->packed) : lfr_gather_stage();
     if (sp->ndf == 6) {
-      if (lpt == 4) launch_apply_uni<T, 6, 4>(ctx->stream, nt, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
-      else launch_apply_uni<T, 6, 2>(ctx->stream, nt, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+      if (lpt == 4) launch_apply_uni<T, 6, 4>(ctx->stream, nt, st, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+      else launch_apply_uni<T, 6, 2>(ctx->stream, nt, st, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
     } else {
-      if (lpt == 4) launch_apply_uni<T, 1, 4>(ctx->stream, nt, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
-      else launch_apply_uni<T, 1, 2>(ctx->stream, nt, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+      if (lpt == 4) launch_apply_uni<T, 1, 4>(ctx->stream, nt, st, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
+      else launch_apply_uni<T, 1, 2>(ctx->stream, nt, st, nbp, M, x, y, ur, ncol, nl, n3, pcs, dotv, part, stop);
     }
     LFR_CHECK_LAUNCH(ctx);
     return LFR_OK;
@@ -732,12 +755,10 @@ int cg_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x
   hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(kThreads), 0, ctx->stream, w.S, (int64_t)S_NSLOT, 0.0);
   // r = b - A x
   if (P->x_is_zero) {
-    // x = 0, so A x = 0 exactly: skip the apply
-    hipLaunchKernelGGL(k_fill<T>, dim3(G), dim3(kThreads), 0, ctx->stream, x, n, T(0));
-    hipLaunchKernelGGL(k_fill<T>, dim3(G), dim3(kThreads), 0, ctx->stream, r, n, T(0));
-    // r = b - 0 evaluated as the reference does: (-1)*0 + b == b exactly
-    hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kThreads), 0, ctx->stream, r, b, n,
-                       w.part_rr, (float*)nullptr);
+    // x = 0, so A x = 0 exactly: no apply; r = b, and p = r right away
+    // without a preconditioner
+    hipLaunchKernelGGL(k_cg_start_zero<T>, dim3(G), dim3(kThreads), 0, ctx->stream, x, r, diag ? (T*)nullptr : p,
+                       b, n, w.part_rr);
   } else {
     int rc = apply_op<T>(ctx, A, x, r, (const T*)nullptr, nullptr, nullptr);
     if (rc) return rc;
@@ -755,12 +776,21 @@ int cg_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x
     hipLaunchKernelGGL(k_prec_diag<T>, dim3(G), dim3(kThreads), 0, ctx->stream, z, r,
                        (const T*)P->diag_d, n, w.part_rz);
     hipLaunchKernelGGL(k_copy_v<T>, dim3(G), dim3(kThreads), 0, ctx->stream, p, (const T*)z, n);
-  } else {
+  } else if (!P->x_is_zero) {
     hipLaunchKernelGGL(k_copy_v<T>, dim3(G), dim3(kThreads), 0, ctx->stream, p, (const T*)r, n);
   }
   LFR_CHECK_LAUNCH(ctx);
   if (P->max_iter > 0) {
-    const int chunk = P->max_iter < 10 ? P->max_iter : 10;
+    // iterations per captured chunk: a monitored solve keeps one chunk queued
+    // while the host checks the previous one, so after convergence up to
+    // ~1.5 chunks of iterations return at entry; 4 keeps that to a few
+    // launches while a chunk still outlasts the host's check
+    int cmax = P->monitor_convergence ? 4 : 10;
+    if (const char* e = getenv("LFR_CG_CHUNK")) {   // experiments: iterations per captured chunk
+      const int v = atoi(e);
+      if (v >= 1 && v <= 64) cmax = v;
+    }
+    const int chunk = P->max_iter < cmax ? P->max_iter : cmax;
     int rc = cg_iterations<T>(ctx, A, P, x, w, n, chunk);  // ends with S in Sh
     if (rc) return rc;
   } else {

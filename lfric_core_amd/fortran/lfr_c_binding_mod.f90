@@ -115,7 +115,8 @@ module lfr_c_binding_mod
             lfr_vector_field_norm, lfr_linop_lma, lfr_linop_block, lfr_linop_callback,    &
             lfr_linop_destroy, lfr_linop_apply, lfr_precon_null, lfr_precon_diagonal,    &
             lfr_precon_columnwise, lfr_precon_callback, lfr_precon_destroy,            &
-            lfr_precon_apply, lfr_iterative_solve, lfr_apply_fn, lfr_error_message
+            lfr_precon_apply, lfr_iterative_solve, lfr_apply_fn, lfr_error_message,     &
+            lfr_tune_set, lfr_tune_get, lfr_tune_clear, lfr_tune_save, lfr_tune_load
 
   interface
 
@@ -532,6 +533,39 @@ module lfr_c_binding_mod
       type(lfr_iterative_solver), intent(in) :: solver
       type(lfr_solver_stats), intent(out) :: stats
     end function lfr_iterative_solve
+
+    ! autotuner: force / read a launch-shape choice, forget all, save / load
+    ! them (keys and paths are NUL-terminated: trim(s)//c_null_char)
+    integer(c_int) function lfr_tune_set(ctx, key, choice) bind(C, name="lfr_tune_set")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: key
+      integer(c_int), value :: choice
+    end function lfr_tune_set
+
+    integer(c_int) function lfr_tune_get(ctx, key, choice) bind(C, name="lfr_tune_get")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: key
+      integer(c_int), intent(out) :: choice
+    end function lfr_tune_get
+
+    integer(c_int) function lfr_tune_clear(ctx) bind(C, name="lfr_tune_clear")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function lfr_tune_clear
+
+    integer(c_int) function lfr_tune_save(ctx, path) bind(C, name="lfr_tune_save")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: path
+    end function lfr_tune_save
+
+    integer(c_int) function lfr_tune_load(ctx, path) bind(C, name="lfr_tune_load")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: path
+    end function lfr_tune_load
 
   end interface
 
