@@ -521,6 +521,16 @@ def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
     rec("a11_axpy_fp32", timed(lambda: api.axpy(ctx, 0.5, xa, ya)), 3 * u2 * 4)
     rec("a14_dot_fp32_fp64acc", timed(lambda: api.dot(ctx, xa, ya)), 2 * u2 * 4,
         note="includes the host read-back of the FP64 result")
+    # the same inner product left on the device (what the solvers use):
+    # partials pass + one-block final pass, no host synchronisation
+    import ctypes as C
+    from lfric_core_amd._ffi import lib
+    outd = torch.zeros(1, dtype=torch.float64, device=dev)
+    ctx.sync_stream()
+    rec("a14_dot_fp32_fp64acc_device",
+        timed(lambda: lib().lfr_vec_dot_dev(ctx.h, 4, u2, C.c_void_p(xa.data_ptr()), C.c_void_p(ya.data_ptr()),
+                                            C.c_void_p(outd.data_ptr()))), 2 * u2 * 4,
+        note="result left in device memory (lfr_vec_dot_dev)")
     del xa, ya, flush
     torch.cuda.empty_cache()
     return out

@@ -1111,6 +1111,117 @@ k_optri_staged(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__
   }
 }
 
+// operator_tri_solve, FP32, 70 layers (71 dofs a column), every load issued
+// at entry (the scheme of k_thomas_deep): a wave owns COLS columns and
+// loads, for every KCH-row stage, the four planes of the 2x2 local matrices
+// m(ik, i, j) at ik + n3 (i + 2 j) and x for the stage's rows into
+// registers up front (slot it of stage s: row s KCH + qs of column cs with
+// lane + it kCols = cs KCH + qs, rows past the column clamped; layer r of
+// planes (0,0), (0,1) and layer r - 1 of planes (1,0), (1,1), clamped into
+// the column, masked to zero where the reference has no term), moves one
+// stage at a time to LDS for the recurrence and writes the solution back in
+// flat order with streaming stores.  Same arithmetic as k_operator_tri_solve.
+template <int NROW, int KCH, int COLS>
+__global__ void __launch_bounds__(kCols)
+k_optri_deep(float* __restrict__ y, const float* __restrict__ x, const float* __restrict__ m,
+             const int32_t* __restrict__ map0, int ndf, int ncell) {
+  static_assert(COLS <= kCols, "stage shape");
+  constexpr int NL = NROW - 1;                      // layers
+  constexpr int NST = (NROW + KCH - 1) / KCH;
+  constexpr int SLOTS = (COLS * KCH + kCols - 1) / kCols;
+  constexpr int PB = 4 * KCH + 1;                   // odd pitches
+  constexpr int PX = NROW | 1;
+  __shared__ float sb[COLS * PB];
+  __shared__ float sxa[COLS * PX];
+  __shared__ int32_t sbase[COLS];
+  const int lane = threadIdx.x;
+  const int c0 = blockIdx.x * COLS;
+  const int ncols = min(COLS, ncell - c0);
+  const int64_t n3 = (int64_t)ncell * NL;
+  if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
+  __syncthreads();
+  // per slot: a = m(r-1;1,0), d0 = m(r;0,0), d1 = m(r-1;1,1), u = m(r;0,1), x(r)
+  float rv[NST][5][SLOTS];
+#pragma unroll
+  for (int s = 0; s < NST; ++s)
+#pragma unroll
+    for (int it = 0; it < SLOTS; ++it) {
+      const int e = min(lane + it * kCols, COLS * KCH - 1);
+      const int cs = min(e / KCH, ncols - 1);
+      const int r = min(s * KCH + (e % KCH), NROW - 1);
+      const int64_t ik = (int64_t)(c0 + cs) * NL + r;
+      const int64_t ia = r < NL ? ik : ik - 1;          // layer r (clamped at the top)
+      const int64_t ib = r >= 1 ? ik - 1 : ik;          // layer r - 1 (clamped at the bottom)
+      rv[s][0][it] = m[ib + n3];
+      rv[s][1][it] = m[ia];
+      rv[s][2][it] = m[ib + 3 * n3];
+      rv[s][3][it] = m[ia + 2 * n3];
+      rv[s][4][it] = x[(int64_t)sbase[cs] + r];
+    }
+  float cp[NROW], dp[NROW];
+#pragma unroll
+  for (int s = 0; s < NST; ++s) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SLOTS; ++it) {
+      const int e = lane + it * kCols;
+      if (e < COLS * KCH) {
+        const int cs = e / KCH, qs = e - cs * KCH;
+        const int r = min(s * KCH + qs, NROW - 1);
+        float* o = sb + cs * PB + 4 * qs;
+        o[0] = r >= 1 ? rv[s][0][it] : 0.0f;                                   // sub
+        o[1] = r == 0 ? rv[s][1][it] + 0.0f
+                      : (r < NL ? rv[s][1][it] + rv[s][2][it] : rv[s][2][it]);  // diag
+        o[2] = r < NL ? rv[s][3][it] : 0.0f;                                   // super
+        o[3] = rv[s][4][it];                                                   // x
+      }
+    }
+    __syncthreads();
+    if (lane < COLS) {
+#pragma unroll
+      for (int q = 0; q < KCH; ++q) {
+        const int i = s * KCH + q;
+        if (i < NROW) {
+          const float* w = sb + lane * PB + 4 * q;
+          const float a = w[0], dg = w[1], u = w[2], xv = w[3];
+          if (i == 0) {
+            const float inv = 1.0f / dg;
+            cp[i] = u * inv;
+            dp[i] = xv * inv;
+          } else {
+            const float ac = a * cp[i - 1];
+            const float inv = 1.0f / (dg - ac);
+            const float ad = a * dp[i - 1];
+            if (i < NROW - 1) cp[i] = u * inv;
+            dp[i] = (xv - ad) * inv;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = NROW - 2; i >= 0; --i) {
+    const float cd = cp[i] * dp[i + 1];
+    dp[i] = dp[i] - cd;
+  }
+  __syncthreads();
+  if (lane < COLS) {
+#pragma unroll
+    for (int i = 0; i < NROW; ++i) sxa[lane * PX + i] = dp[i];
+  }
+  __syncthreads();
+  const int nflat = ncols * NROW;
+  constexpr int NXF = (COLS * NROW + kCols - 1) / kCols;
+#pragma unroll
+  for (int it = 0; it < NXF; ++it) {
+    const int e = lane + it * kCols;
+    if (e < nflat) {
+      const int c = e / NROW, r = e - c * NROW;
+      __builtin_nontemporal_store(sxa[c * PX + r], y + (int64_t)sbase[c] + r);
+    }
+  }
+}
+
 template <typename T, int MODE>
 bool launch_thomas(lfr_ctx* ctx, hipStream_t s, T* y, const T* x, const T* b0, const T* b1, const T* b2,
                    const lfr_space* sp) {
@@ -1166,6 +1277,35 @@ bool launch_thomas(lfr_ctx* ctx, hipStream_t s, T* y, const T* x, const T* b0, c
     }
   } else {                     // operator_tri_solve: b0 = the 2x2 local matrices
     if (sp->nlayers != 70) return false;
+    if constexpr (sizeof(T) == 4) {
+      // result-identical variants (columns a wave x rows a stage, the staged
+      // kernel) chosen by the autotuner on the first call of a shape
+      const int ncell = sp->ncell;
+      const auto launch = [&](int c) -> int {
+        switch (c) {
+          case 0:
+            hipLaunchKernelGGL((k_optri_deep<71, 12, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y, x, b0,
+                               sp->map0_d, sp->ndf, ncell);
+            break;
+          case 1:
+            hipLaunchKernelGGL((k_optri_deep<71, 8, 32>), dim3((ncell + 31) / 32), dim3(kCols), 0, s, y, x, b0,
+                               sp->map0_d, sp->ndf, ncell);
+            break;
+          default:
+            hipLaunchKernelGGL((k_optri_staged<T, 71, kCh>), dim3(nb), dim3(kCols), 0, s, y, x, b0, sp->map0_d,
+                               sp->ndf, ncell, sp->nlayers);
+        }
+        return hipGetLastError() == hipSuccess ? LFR_OK : LFR_ERR_HIP;
+      };
+      char key[96];
+      snprintf(key, sizeof key, "operator_tri_solve f32 nl=70 ncell=%d", ncell);
+      // untuned default: 32 columns a wave while that still gives few
+      // enough waves for one pass over the chip, else 16
+      const int dflt = (ncell + 31) / 32 <= 2048 ? 1 : 0;
+      const int c = ctx ? lfr_tune_pick(ctx, key, 3, dflt, launch) : dflt;
+      (void)launch(c);
+      return true;
+    }
     hipLaunchKernelGGL((k_optri_staged<T, 71, kCh>), dim3(nb), dim3(kCols), 0, s, y, x, b0, sp->map0_d,
                        sp->ndf, sp->ncell, sp->nlayers);
     return true;

@@ -38,7 +38,7 @@ def test_tri_solve_bitexact(ctx, dtype, nc, nl):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("space,nc,nl", [("WTHETA", 3, 8), ("W2V", 6, 70)])
+@pytest.mark.parametrize("space,nc,nl", [("WTHETA", 3, 8), ("W2V", 6, 70), ("WTHETA", 5, 70)])
 def test_operator_tri_solve_bitexact(ctx, dtype, space, nc, nl):
     from lfric_core_amd import api
     print(f"case operator_tri_solve {space} nc={nc} nl={nl} {dtype.__name__}", flush=True)

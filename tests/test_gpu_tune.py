@@ -84,6 +84,32 @@ def test_tri_solve_every_variant_identical(ctx):
     assert ctx.tune_get(key) == 3
 
 
+def test_operator_tri_solve_every_variant_identical(ctx):
+    from lfric_core_amd import api
+    print("case tune operator_tri_solve WTHETA C9L70", flush=True)
+    fs = build_dofmap(cubed_sphere(9), "WTHETA", 70)
+    st = Stream(47)
+    n3 = fs.mesh.ncells * 70
+    m = np.empty((2, 2, n3))
+    m[0, 0] = 2.0 + st.uniform(n3)
+    m[0, 1] = 0.5 * st.uniform(n3)
+    m[1, 0] = 0.5 * st.uniform(n3)
+    m[1, 1] = 2.0 + st.uniform(n3)
+    m = m.astype(np.float32)
+    x = st.uniform(fs.undf).astype(np.float32)
+    ref = oracle.operator_tri_solve(np.zeros(fs.undf, np.float32), x, m, fs.dofmap, 70)
+    sp = api.Space(ctx, fs)
+    key = f"operator_tri_solve f32 nl=70 ncell={fs.mesh.ncells}"
+    ctx.tune_clear()
+    for c in (None, 0, 1, 2):
+        if c is not None:
+            ctx.tune_set(key, c)
+        y = torch.full((fs.undf,), float("nan"), dtype=torch.float32, device=ctx.device)
+        api.operator_tri_solve(y, _t(ctx, x), _t(ctx, m), sp)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(y.cpu().numpy(), ref, err_msg=f"variant {c}")
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_packed_every_variant_identical(ctx, dtype):
     """The packed apply's variant is tuned when the operator is packed; each
