@@ -581,6 +581,24 @@ def bench_multi(ctx, torch, edge_cells: int = 48, nl: int = 70, steps: int = 20)
         for _ in range(3):
             mop.apply(y, x32)
         ms = time_events(lambda: mop.apply(y, x32), steps, torch)
+        # the colour-by-colour form of the same apply (one launch per colour,
+        # later colours add onto y): same bits in the overwrite form
+        import os
+        prev = os.environ.get("LFR_MULTI_OWNER")
+        os.environ["LFR_MULTI_OWNER"] = "0"
+        cop = api.MultiOperator(op32, mprec)
+        if prev is None:
+            del os.environ["LFR_MULTI_OWNER"]
+        else:
+            os.environ["LFR_MULTI_OWNER"] = prev
+        yc = torch.empty_like(y)
+        for _ in range(3):
+            cop.apply(yc, x32)
+        ms_col = time_events(lambda: cop.apply(yc, x32), steps, torch)
+        col_same = bool(torch.equal(yc, y))
+        launches = (mop.info()["launches"], cop.info()["launches"])
+        cop.close()
+        del yc
         err = torch.linalg.vector_norm(y.double() - Y64, dim=0) / ref_n
         nbytes = n3 * 1296 * sm + 2 * 16 * undf * 4 + cfg.mesh.ncells * 36 * 4
         out[name] = {"ms": round(ms, 4), "gdofs": round(16 * undf / (ms * 1e-3) / 1e9, 2),
@@ -588,10 +606,14 @@ def bench_multi(ctx, torch, edge_cells: int = 48, nl: int = 70, steps: int = 20)
                      "roofline_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
                      "algorithmic_bytes": nbytes,
-                     "max_rel_err_vs_fp64": float(err.max()), "mean_rel_err_vs_fp64": float(err.mean())}
+                     "max_rel_err_vs_fp64": float(err.max()), "mean_rel_err_vs_fp64": float(err.mean()),
+                     "launches": launches[0],
+                     "colour_form": {"ms": round(ms_col, 4), "launches": launches[1],
+                                     "roofline_frac": round(nbytes / (ms_col * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                     "same_bits": col_same}}
         log(f"multi {name}: {ms*1e3:.0f} us, {16*undf/(ms*1e-3)/1e9:.1f} GDOF/s, "
             f"{nbytes/(ms*1e-3)/1e9:.0f} GB/s, {flops/(ms*1e-3)/1e12:.1f} TFLOP/s, "
-            f"max rel err vs FP64 {float(err.max()):.2e}")
+            f"max rel err vs FP64 {float(err.max()):.2e}; colour form {ms_col*1e3:.0f} us, same bits {col_same}")
         mop.close()
         del y
     # 16 simultaneous refined solves (FP32 MFMA CG inside FP64 residuals) to 1e-10,
@@ -756,7 +778,9 @@ def profile_workload(kind: str, reps: int = 5) -> None:
     (columnwise_op_appinv FP32 on W3 columns, C96L70 then C192L70, each call
     after a 1 GiB cache-evicting read), `tri` (tri_solve FP32, same shapes),
     `cg` (refined CG to 1e-10 and CG50 FP32 on the C96L70 W2 mass matrix,
-    packed operators), `copy` (a plain FP32 copy of the appinv C96 and C192
+    packed operators), `multi` (the 16-field W2 order-1 apply at C48L70,
+    owner-computes and colour-by-colour forms, FP32 and BF16 matrices),
+    `copy` (a plain FP32 copy of the appinv C96 and C192
     byte counts, for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
     import torch
     from lfric_core_amd import api
@@ -819,6 +843,28 @@ def profile_workload(kind: str, reps: int = 5) -> None:
             cg.apply(xb, b32, x_is_zero=True)
         torch.cuda.synchronize()
         log("profile cg: 2 refined solves, 2 x CG50 FP32")
+    elif kind == "multi":
+        import os
+        from lfric_core_amd.configs import multi_config
+        cfg = multi_config(dev, 48, nl)
+        w2 = api.Space(ctx, cfg.w2)
+        op32 = api.LMAOperator(cfg.matrix.float(), w2, w2, cfg.colouring)
+        x32 = cfg.x.float().contiguous()
+        y = torch.empty(cfg.w2.undf, 16, dtype=torch.float32, device=dev)
+        prev = os.environ.get("LFR_MULTI_OWNER")
+        for owner in ("1", "0"):
+            os.environ["LFR_MULTI_OWNER"] = owner
+            for mprec in (4, 2):
+                mop = api.MultiOperator(op32, mprec)
+                for _ in range(reps):
+                    mop.apply(y, x32)
+                torch.cuda.synchronize()
+                mop.close()
+        if prev is None:
+            del os.environ["LFR_MULTI_OWNER"]
+        else:
+            os.environ["LFR_MULTI_OWNER"] = prev
+        log(f"profile multi C48L{nl}: {reps} applies each of owner/colour x FP32/BF16")
     else:
         raise SystemExit(f"unknown --profile workload {kind!r}")
 
@@ -844,7 +890,7 @@ def main():
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a plain copy of 1 GiB (FP32) 5 times, for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes")
-    ap.add_argument("--profile", choices=["appinv", "tri", "cg", "copy"],
+    ap.add_argument("--profile", choices=["appinv", "tri", "cg", "multi", "copy"],
                     help="run one short fixed workload for a rocprofv3 run and exit (no JSON line)")
     args = ap.parse_args()
     if args.profile:

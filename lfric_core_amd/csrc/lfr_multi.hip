@@ -33,6 +33,19 @@
 // registers (lane shift by one layer).  Across columns a dof is written by
 // its first toucher in colour order (overwrite form) and incremented by the
 // later ones, so results are deterministic.
+//
+// Owner-computes form (the W2 order-1 shape on a mesh whose side faces are
+// each shared by exactly two cells): every cell owns two of its four side
+// faces (an Euler orientation of the face graph gives each cell exactly two)
+// and computes both cells' contributions to the dofs of the faces it owns,
+// its own rows with its x and the neighbour's rows (packed into this cell's
+// block) with the neighbour's x.  Every output value is then written once,
+// in one launch over all cells, with nothing read back; a shared dof is
+// own + neighbour in that order, so results are deterministic.  Cells are
+// launched so that each XCD walks one contiguous range of the cell order,
+// which keeps a neighbour's x warm in that XCD's L2 or the memory-side cache.
+#include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -55,6 +68,9 @@ struct lfr_multi {
   int32_t* cells_d = nullptr;     // cells in launch order, 0-based
   std::vector<int> launch_off;    // launches: [launch_off[i], launch_off[i+1])
   int64_t mp_bytes = 0;
+  bool owner = false;             // owner-computes form (one launch, each output written once)
+  int32_t* slotc_d = nullptr;     // owner form: (ncell, 28) output row | kind << 16 of tiles 0-6
+  int32_t* nbr_d = nullptr;       // owner form: (ncell, 2) neighbours whose rows fill tiles 7, 8
 };
 
 namespace {
@@ -96,6 +112,45 @@ __global__ void k_multi_pack(TO* __restrict__ mp, const TI* __restrict__ m, cons
   }
 }
 
+// Owner form, block-major layout: the layers of a cell go in blocks of 16
+// (the last block holds the remaining L = nl - 16 kb layers), and inside a
+// block Mp[(jq*ntp + p)*L + kl][jr] = row of slot p, column 4 jq + jr, layer
+// 16 kb + kl.  One 16-layer step of a workgroup then reads one contiguous
+// run of the block (no layer padding): slot p of cell c holds row
+// pk[2(c*ntp+p)+1] of cell pk[2(c*ntp+p)].
+template <typename TI, typename TO>
+__global__ void k_multi_pack_own(TO* __restrict__ mp, const TI* __restrict__ m, const int32_t* __restrict__ pk,
+                                 int ncell, int nl, int nt, int nf, int ntp, int jqn) {
+  const int64_t n3 = (int64_t)ncell * nl;
+  const int64_t per_layer = (int64_t)jqn * ntp * 4;
+  const int64_t per_cell = per_layer * nl;
+  const int64_t total = per_cell * ncell;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = idx / per_cell;
+    const int64_t rem = idx - c * per_cell;
+    const int kb = (int)(rem / (16 * per_layer));
+    const int64_t r2 = rem - (int64_t)kb * 16 * per_layer;
+    const int L = min(16, nl - 16 * kb);
+    const int jr = (int)(r2 & 3);
+    const int64_t r3 = r2 >> 2;
+    const int kl = (int)(r3 % L);
+    const int64_t r4 = r3 / L;
+    const int p = (int)(r4 % ntp);
+    const int q = (int)(r4 / ntp);
+    const int k = 16 * kb + kl;
+    const int j = 4 * q + jr;
+    const int64_t src = pk[2 * (c * ntp + p)];
+    const int row = pk[2 * (c * ntp + p) + 1];
+    float v = 0.f;
+    if (j < nf) v = (float)m[(src * nl + k) + n3 * ((int64_t)row + (int64_t)nt * j)];
+    if constexpr (sizeof(TO) == 2)
+      mp[idx] = f2bf(v);
+    else
+      mp[idx] = v;
+  }
+}
+
 // One row tile's matrix operand for one jq step: 4 FP32 (16 B) or 4 BF16 (8 B)
 template <bool BF>
 struct MTile;
@@ -106,6 +161,10 @@ struct MTile<false> {
   __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
     m = __builtin_bit_cast(lfr_v4f, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
   }
+  // non-temporal: the line is streamed through L2 and evicted first
+  __device__ __forceinline__ void load_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    m = __builtin_bit_cast(lfr_v4f, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 2));
+  }
 };
 template <>
 struct MTile<true> {
@@ -113,6 +172,9 @@ struct MTile<true> {
   __device__ __forceinline__ void load(const void* p) { m = *(const lfr_v4s*)p; }
   __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
     m = __builtin_bit_cast(lfr_v4s, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  }
+  __device__ __forceinline__ void load_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    m = __builtin_bit_cast(lfr_v4s, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
   }
 };
 
@@ -333,6 +395,196 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
   }
 }
 
+// Owner form, W2 order 1 (36 rows and columns, 9 row tiles).  The packed
+// block of cell c: tile 0 the 4 bottom rows, 1 their top partners, 2-4 the
+// 12 private rows, 5 and 6 the own rows of the owned faces A and B, 7 and 8
+// the neighbours' rows for A and B, lane for lane on the same dofs.  Wave 0
+// runs tiles {0, 1, 2} (bottoms paired with the tops of the layer below, as
+// in k_multi16), wave 1 {3, 4}, wave 2 {5, 7} and wave 3 {6, 8}: waves 2 and
+// 3 read x of this cell and of one neighbour and add the two products
+// before their single store.  Raw buffer loads with wave-uniform scalar
+// offsets and two steps of operand prefetch (ring of three), as k_multi16;
+// the matrix (block-major, read once) with the non-temporal hint, so that
+// it is evicted from L2 before the x lines other cells read again.
+template <bool BF, int PF, int NTL, bool TWO, bool PAIR>
+__device__ __forceinline__ void own_wave(float* __restrict__ y, __amdgpu_buffer_rsrc_t rx, __amdgpu_buffer_rsrc_t rm,
+                                         const int32_t* __restrict__ mt, const int32_t* __restrict__ mfa,
+                                         const int32_t* __restrict__ mfb, const int (&tiles)[NTL],
+                                         const int (&sinfo)[NTL], int nl, int overwrite) {
+  constexpr int NTP = 36, JQ = 9, R = PF == 2 ? 3 : 2;
+  constexpr int ME = BF ? 2 : 4;
+  static_assert(!TWO || NTL == 2, "a two-source wave runs one own and one neighbour tile");
+  static_assert(PF == 1 || PF == 2, "one or two steps of operand prefetch");
+  const int l = threadIdx.x & 63;
+  const int b = l >> 2, q = l & 3;
+  auto load_x = [&](const int32_t* mf, int k, int jq, lfr_v4f (&xa)[4]) {
+    const unsigned vx = (unsigned)k * (kRhs * 4u) + 16u * q;
+#pragma unroll
+    for (int jr = 0; jr < 4; ++jr)
+      xa[jr] = __builtin_bit_cast(
+          lfr_v4f, __builtin_amdgcn_raw_buffer_load_b128(rx, vx, (unsigned)mf[4 * jq + jr] * (kRhs * 4u), 0));
+  };
+  // step (block starting at layer kbase, jq): lane layer k clamped to the
+  // top; the block holds L layers of every slot, contiguous
+  auto load_step = [&](int kbase, int jq, lfr_v4f (&xa)[4], lfr_v4f (&xb)[4], MTile<BF> (&tl)[NTL]) {
+    const int k = min(kbase + b, nl - 1);
+    const int L = min(16, nl - kbase);
+    load_x(mfa, k, jq, xa);
+    if constexpr (TWO) load_x(mfb, k, jq, xb);
+    const unsigned vm = ((unsigned)q * L + (k - kbase)) * (4u * ME);
+#pragma unroll
+    for (int i = 0; i < NTL; ++i) {
+      const unsigned so = (unsigned)(kbase * NTP * NTP + (jq * NTP + 4 * tiles[i]) * L * 4) * ME;
+      tl[i].load_nt(rm, vm, so);
+    }
+  };
+  lfr_v4f xr[R][4], xbr[TWO ? R : 1][4];
+  MTile<BF> tr[R][NTL];
+  lfr_v4f carry[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) carry[v] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
+  load_step(0, 0, xr[0], xbr[0], tr[0]);
+  if constexpr (PF == 2) load_step(0, 1, xr[1], xbr[TWO ? 1 : 0], tr[1]);
+
+  for (int k0 = 0; k0 < nl; k0 += 16) {
+    lfr_v4f acc[NTL][4];
+#pragma unroll
+    for (int i = 0; i < NTL; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[i][v] = lfr_v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int jq = 0; jq < JQ; ++jq) {
+      // PF = 2: ring of three (JQ a multiple of 3, so the ring continues
+      // across blocks); PF = 1: the next step's operands go to buffer 1 and
+      // move to buffer 0 after this step's products
+      const int cur = PF == 2 ? jq % R : 0;
+      const int sn = jq + PF;
+      const int nb = PF == 2 ? sn % R : 1;
+      if (sn < JQ) load_step(k0, sn, xr[nb], xbr[TWO ? nb : 0], tr[nb]);
+      else if (k0 + 16 < nl) load_step(k0 + 16, sn - JQ, xr[nb], xbr[TWO ? nb : 0], tr[nb]);
+      if constexpr (TWO) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          acc[0][v] = mfma_step<BF>(xr[cur], v, tr[cur][0], acc[0][v]);
+          acc[1][v] = mfma_step<BF>(xbr[cur], v, tr[cur][1], acc[1][v]);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < NTL; ++i)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[i][v] = mfma_step<BF>(xr[cur], v, tr[cur][i], acc[i][v]);
+      }
+      if constexpr (PF == 1) {
+#pragma unroll
+        for (int jr = 0; jr < 4; ++jr) {
+          xr[0][jr] = xr[1][jr];
+          if constexpr (TWO) xbr[0][jr] = xbr[1][jr];
+        }
+#pragma unroll
+        for (int i = 0; i < NTL; ++i) tr[0][i] = tr[1][i];
+      }
+    }
+
+    // epilogue: lane (b, q) of a tile holds its row q at layer k0 + b,
+    // acc[i][v][m] = rhs 4m + v
+    const int kk = k0 + b;
+    if constexpr (TWO) {
+      if (kk < nl) {
+        float* yp = y + ((int64_t)mt[sinfo[0] & 0xffff] + kk) * kRhs;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          lfr_v4f o = {acc[0][0][m] + acc[1][0][m], acc[0][1][m] + acc[1][1][m], acc[0][2][m] + acc[1][2][m],
+                       acc[0][3][m] + acc[1][3][m]};
+          if (!overwrite) o = o + *(const lfr_v4f*)(yp + 4 * m);
+          *(lfr_v4f*)(yp + 4 * m) = o;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NTL; ++i) {
+        const int kind = sinfo[i] >> 16;
+        lfr_v4f val[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) val[v] = acc[i][v];
+        if constexpr (PAIR) {
+          if (i == 0) {
+            // bottom rows: add the partner top row of the layer below
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              lfr_v4f t;
+#pragma unroll
+              for (int m = 0; m < 4; ++m) {
+                const float cur = __shfl(acc[1][v][m], (l - 4) & 63);
+                const float prv = __shfl(carry[v][m], (l + 60) & 63);
+                t[m] = b == 0 ? prv : cur;
+              }
+              if (kind == 2 && kk > 0) val[v] = val[v] + t;
+            }
+          }
+        }
+        const bool skip = kind == 0 || kk >= nl || (kind == 3 && kk < nl - 1);
+        if (!skip) {
+          float* yp = y + ((int64_t)mt[sinfo[i] & 0xffff] + kk) * kRhs;
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            lfr_v4f o = {val[0][m], val[1][m], val[2][m], val[3][m]};
+            if (!overwrite) o = o + *(const lfr_v4f*)(yp + 4 * m);
+            *(lfr_v4f*)(yp + 4 * m) = o;
+          }
+        }
+      }
+      if constexpr (PAIR) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) carry[v] = acc[1][v];
+      }
+    }
+  }
+}
+
+// PF = 1 keeps fewer operands in flight per wave and fits in 168 VGPRs
+// (WPE = 3 waves share each SIMD); PF = 2 runs two waves per SIMD.
+template <bool BF, int PF, int WPE>
+__global__ void __launch_bounds__(256, WPE)
+k_multi16_own(float* __restrict__ y, const float* __restrict__ x, const void* __restrict__ mp_,
+              const int32_t* __restrict__ cells, const int32_t* __restrict__ map_to,
+              const int32_t* __restrict__ map_from, const int32_t* __restrict__ slotc,
+              const int32_t* __restrict__ nbr, int nl, int overwrite, uint32_t x_bytes) {
+  constexpr int NTP = 36, NF = 36;
+  constexpr int ME = BF ? 2 : 4;
+  const int c = cells[blockIdx.x];
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int q = threadIdx.x & 3;
+  const int64_t cb = (int64_t)nl * NTP * NTP * ME;   // bytes of one cell's packed block
+  const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc((void*)x, (short)0, (int)x_bytes, kBufWord3);
+  const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)((const char*)mp_ + (int64_t)c * cb), (short)0, (int)cb, kBufWord3);
+  const int32_t* mt = map_to + (int64_t)c * NTP;
+  const int32_t* mf = map_from + (int64_t)c * NF;
+  const int32_t* sc = slotc + (int64_t)c * 28;
+  if (w == 0) {
+    const int tiles[3] = {0, 1, 2};
+    const int si[3] = {sc[q], sc[4 + q], sc[8 + q]};
+    own_wave<BF, PF, 3, false, true>(y, rx, rm, mt, mf, mf, tiles, si, nl, overwrite);
+  } else if (w == 1) {
+    const int tiles[2] = {3, 4};
+    const int si[2] = {sc[12 + q], sc[16 + q]};
+    own_wave<BF, PF, 2, false, false>(y, rx, rm, mt, mf, mf, tiles, si, nl, overwrite);
+  } else {
+    const int f = w - 2;
+    const int nb = nbr[2 * c + f];
+    const int tiles[2] = {5 + f, 7 + f};
+    const int si[2] = {sc[20 + 4 * f + q], 0};
+    own_wave<BF, PF, 2, true, false>(y, rx, rm, mt, mf, map_from + (int64_t)nb * NF, tiles, si, nl, overwrite);
+  }
+}
+
+template <bool BF, int PF, int WPE>
+void launch_multi_own(hipStream_t s, lfr_multi* h, float* y, const float* x, int overwrite) {
+  hipLaunchKernelGGL((k_multi16_own<BF, PF, WPE>), dim3(h->ncell), dim3(256), 0, s, y, x, (const void*)h->mp_d, h->cells_d,
+                     h->to->map0_d, h->from->map0_d, h->slotc_d, h->nbr_d, h->nl, overwrite,
+                     (uint32_t)((int64_t)h->from->undf * kRhs * 4));
+}
+
 template <bool BF, int NTP, int NF, int PIG, int IGW, int UNR = 1, int PF = 1, bool BUF = false>
 void launch_multi(hipStream_t s, lfr_multi* h, float* y, const float* x, int overwrite) {
   constexpr int WS = (NTP / 4 + IGW - 1) / IGW;
@@ -346,9 +598,154 @@ void launch_multi(hipStream_t s, lfr_multi* h, float* y, const float* x, int ove
   }
 }
 
+// Tables of the owner form for a W2 order-1 output space with 4 vertical
+// pairs (bottoms/tops, the first 8 slots of the shared slot table): every
+// other row is either private to its cell or on a side face of 4 dofs shared
+// with exactly one neighbour, 4 such faces per cell.  Faces are oriented
+// along closed trails of the face graph (every cell has 4 faces, so each
+// trail through a cell enters once and leaves once): each cell owns exactly
+// the 2 faces it leaves by.  Returns false (and the colour form is used)
+// when the mesh or space does not have this structure.
+//   slotc (ncell, 28): output row | kind << 16 of tiles 0-6
+//   pk (ncell, 36, 2): source cell and row of every packed slot
+//   nbr (ncell, 2):    the neighbours across the owned faces
+//   order (ncell):     launch order, a contiguous range of cells per XCD
+bool owner_tables(const lfr_space* to, const std::vector<int>& bottoms, const std::vector<int>& tops,
+                  std::vector<int32_t>& slotc, std::vector<int32_t>& pk, std::vector<int32_t>& nbr,
+                  std::vector<int32_t>& order) {
+  const int nt = to->ndf, ncell = to->ncell;
+  if (nt != 36 || bottoms.size() != 4 || ncell < 2) return false;
+  std::vector<char> paired(nt, 0);
+  for (int i = 0; i < 4; ++i) paired[bottoms[i]] = paired[tops[i]] = 1;
+  std::vector<int> plain;
+  for (int r = 0; r < nt; ++r)
+    if (!paired[r]) plain.push_back(r);
+  const int64_t ncol = (int64_t)to->undf + 2;
+  std::vector<int> cnt(ncol, 0), t1(ncol, -1), t2(ncol, -1);
+  auto mapv = [&](int c, int r) { return to->map_h[(size_t)c * nt + r]; };
+  for (int c = 0; c < ncell; ++c)
+    for (int r : plain) {
+      const int d = mapv(c, r);
+      if (d < 1 || d > to->undf) return false;
+      if (cnt[d] == 0) t1[d] = c;
+      else if (cnt[d] == 1) t2[d] = c;
+      if (++cnt[d] > 2) return false;
+    }
+  // per cell: 12 private rows, 4 groups of 4 rows shared with one neighbour each
+  std::vector<int32_t> priv((size_t)ncell * 12), gnb((size_t)ncell * 4), grow((size_t)ncell * 16);
+  for (int c = 0; c < ncell; ++c) {
+    int np = 0, ng = 0, gsz[4] = {0, 0, 0, 0};
+    for (int r : plain) {
+      const int d = mapv(c, r);
+      if (cnt[d] == 1) {
+        if (np == 12) return false;
+        priv[(size_t)c * 12 + np++] = r;
+        continue;
+      }
+      const int n = t1[d] == c ? t2[d] : t1[d];
+      if (n == c || n < 0) return false;
+      int g = 0;
+      while (g < ng && gnb[(size_t)c * 4 + g] != n) ++g;
+      if (g == ng) {
+        if (ng == 4) return false;
+        gnb[(size_t)c * 4 + ng++] = n;
+      }
+      if (gsz[g] == 4) return false;
+      grow[(size_t)c * 16 + 4 * g + gsz[g]++] = r;   // ascending row order
+    }
+    if (np != 12 || ng != 4 || gsz[0] != 4 || gsz[1] != 4 || gsz[2] != 4 || gsz[3] != 4) return false;
+  }
+  auto group_of = [&](int c, int n) {
+    for (int g = 0; g < 4; ++g)
+      if (gnb[(size_t)c * 4 + g] == n) return g;
+    return -1;
+  };
+  // orient faces along closed trails
+  std::vector<char> used((size_t)ncell * 4, 0);
+  std::vector<int> nown(ncell, 0), own((size_t)ncell * 2, -1), ptr(ncell, 0);
+  for (int v = 0; v < ncell; ++v) {
+    for (;;) {
+      while (ptr[v] < 4 && used[(size_t)v * 4 + ptr[v]]) ++ptr[v];
+      if (ptr[v] == 4) break;
+      int cur = v;
+      for (;;) {
+        while (ptr[cur] < 4 && used[(size_t)cur * 4 + ptr[cur]]) ++ptr[cur];
+        if (ptr[cur] == 4) break;
+        const int g = ptr[cur];
+        const int w = gnb[(size_t)cur * 4 + g];
+        const int gw = group_of(w, cur);
+        if (gw < 0 || used[(size_t)w * 4 + gw]) return false;
+        used[(size_t)cur * 4 + g] = used[(size_t)w * 4 + gw] = 1;
+        if (nown[cur] == 2) return false;
+        own[(size_t)cur * 2 + nown[cur]++] = g;
+        cur = w;
+      }
+      if (cur != v) return false;
+    }
+  }
+  for (int c = 0; c < ncell; ++c)
+    if (nown[c] != 2) return false;
+  slotc.assign((size_t)ncell * 28, 0);
+  pk.assign((size_t)ncell * 72, 0);
+  nbr.assign((size_t)ncell * 2, 0);
+  for (int c = 0; c < ncell; ++c) {
+    int32_t* sc = &slotc[(size_t)c * 28];
+    int32_t* pc = &pk[(size_t)c * 72];
+    for (int i = 0; i < 4; ++i) {
+      sc[i] = bottoms[i] | (2 << 16);
+      sc[4 + i] = tops[i] | (3 << 16);
+    }
+    for (int i = 0; i < 12; ++i) sc[8 + i] = priv[(size_t)c * 12 + i] | (1 << 16);
+    for (int f = 0; f < 2; ++f) {
+      const int g = own[(size_t)c * 2 + f];
+      const int n = gnb[(size_t)c * 4 + g];
+      const int gn = group_of(n, c);
+      nbr[(size_t)c * 2 + f] = n;
+      for (int i = 0; i < 4; ++i) {
+        const int r = grow[(size_t)c * 16 + 4 * g + i];
+        sc[20 + 4 * f + i] = r | (1 << 16);
+        int rn = -1;
+        for (int e = 0; e < 4; ++e)
+          if (mapv(n, grow[(size_t)n * 16 + 4 * gn + e]) == mapv(c, r)) rn = grow[(size_t)n * 16 + 4 * gn + e];
+        if (rn < 0) return false;
+        pc[2 * (28 + 4 * f + i)] = n;
+        pc[2 * (28 + 4 * f + i) + 1] = rn;
+      }
+    }
+    for (int p = 0; p < 28; ++p) {
+      pc[2 * p] = c;
+      pc[2 * p + 1] = sc[p] & 0xffff;
+    }
+  }
+  // every plain dof column is written by exactly one (cell, slot)
+  std::vector<int> wr(ncol, 0);
+  for (int c = 0; c < ncell; ++c)
+    for (int p = 8; p < 28; ++p) wr[mapv(c, slotc[(size_t)c * 28 + p] & 0xffff)]++;
+  for (int c = 0; c < ncell; ++c)
+    for (int r : plain)
+      if (wr[mapv(c, r)] != 1) return false;
+  // launch order: block i runs on XCD i % 8; XCD x walks cells
+  // [x*ncell/8, (x+1)*ncell/8) in order, so that the neighbours whose x an
+  // owner reads are mostly processed on the same XCD at about the same time
+  constexpr int kXcd = 8;
+  std::vector<int> next(kXcd), end(kXcd);
+  for (int x = 0; x < kXcd; ++x) {
+    next[x] = (int)((int64_t)x * ncell / kXcd);
+    end[x] = (int)((int64_t)(x + 1) * ncell / kXcd);
+  }
+  order.assign(ncell, 0);
+  for (int i = 0; i < ncell; ++i) {
+    int x = i % kXcd;
+    while (next[x] == end[x]) x = (x + 1) % kXcd;
+    order[i] = next[x]++;
+  }
+  return true;
+}
+
 void multi_free(lfr_multi* h) {
   if (!h) return;
-  for (void* p : {h->mp_d, (void*)h->slot_d, (void*)h->first_d, (void*)h->cells_d})
+  for (void* p : {h->mp_d, (void*)h->slot_d, (void*)h->first_d, (void*)h->cells_d, (void*)h->slotc_d,
+                  (void*)h->nbr_d})
     if (p) (void)hipFree(p);
   delete h;
 }
@@ -459,8 +856,23 @@ int lfr_multi_create(lfr_ctx* ctx, int mprec, int prec_in, const void* matrix_d,
       firstm[c] = m;
     }
   }
+  // owner-computes form: W2 order 1 (36 x 36), x within reach of 32-bit buffer offsets
+  std::vector<int32_t> oslot, opk, onbr, oorder;
+  bool owner = false;
+  {
+    const char* e = getenv("LFR_MULTI_OWNER");
+    const bool want = !(e && e[0] == '0');
+    if (want && !discontinuous && ntp == 36 && nf == 36 && pig == 1 &&
+        (int64_t)from->undf * kRhs * 4 < (int64_t)1 << 32)
+      owner = owner_tables(to, bottoms, tops, oslot, opk, onbr, oorder);
+  }
+  if (owner) {
+    order = oorder;
+    off = {0, ncell};
+  }
   lfr_multi* h = new lfr_multi();
   h->ctx = ctx;
+  h->owner = owner;
   h->mprec = mprec;
   h->nt = nt;
   h->nf = nf;
@@ -477,7 +889,8 @@ int lfr_multi_create(lfr_ctx* ctx, int mprec, int prec_in, const void* matrix_d,
   h->to = to;
   h->from = from;
   h->launch_off = off;
-  const int64_t nel = (int64_t)ncell * h->jq * ntp * h->nlp * 4;
+  // owner form: block-major, no layer padding
+  const int64_t nel = owner ? (int64_t)ncell * h->jq * ntp * nl * 4 : (int64_t)ncell * h->jq * ntp * h->nlp * 4;
   h->mp_bytes = nel * (mprec == 4 ? 4 : 2);
   if (hipMalloc(&h->mp_d, h->mp_bytes) != hipSuccess || hipMalloc(&h->slot_d, ntp * 4) != hipSuccess ||
       hipMalloc(&h->first_d, (size_t)ncell * 8) != hipSuccess ||
@@ -486,12 +899,44 @@ int lfr_multi_create(lfr_ctx* ctx, int mprec, int prec_in, const void* matrix_d,
     return lfr_set_error(ctx, LFR_ERR_NOMEM, "lfr_multi_create: allocation of %lld bytes failed",
                          (long long)h->mp_bytes);
   }
+  int32_t* pk_d = nullptr;
+  if (owner && (hipMalloc(&h->slotc_d, oslot.size() * 4) != hipSuccess ||
+                hipMalloc(&h->nbr_d, onbr.size() * 4) != hipSuccess || hipMalloc(&pk_d, opk.size() * 4) != hipSuccess)) {
+    if (pk_d) (void)hipFree(pk_d);
+    multi_free(h);
+    return lfr_set_error(ctx, LFR_ERR_NOMEM, "lfr_multi_create: allocation of the owner tables failed");
+  }
   hipStream_t s = ctx->stream;
+  if (owner) {
+    hipError_t e = hipMemcpyAsync(h->slotc_d, oslot.data(), oslot.size() * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->nbr_d, onbr.data(), onbr.size() * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(pk_d, opk.data(), opk.size() * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      (void)hipFree(pk_d);
+      multi_free(h);
+      return lfr_set_error(ctx, LFR_ERR_HIP, "lfr_multi_create: %s", hipGetErrorString(e));
+    }
+  }
   LFR_HIP(ctx, hipMemcpyAsync(h->slot_d, slot.data(), ntp * 4, hipMemcpyHostToDevice, s));
   LFR_HIP(ctx, hipMemcpyAsync(h->first_d, firstm.data(), (size_t)ncell * 8, hipMemcpyHostToDevice, s));
   LFR_HIP(ctx, hipMemcpyAsync(h->cells_d, order.data(), (size_t)ncell * 4, hipMemcpyHostToDevice, s));
   const unsigned g = lfr_blocks(nel, 256, 8192);
-  if (prec_in == 4 && mprec == 4)
+  if (owner) {
+    if (prec_in == 4 && mprec == 4)
+      hipLaunchKernelGGL((k_multi_pack_own<float, float>), dim3(g), dim3(256), 0, s, (float*)h->mp_d,
+                         (const float*)matrix_d, pk_d, ncell, nl, nt, nf, ntp, h->jq);
+    else if (prec_in == 8 && mprec == 4)
+      hipLaunchKernelGGL((k_multi_pack_own<double, float>), dim3(g), dim3(256), 0, s, (float*)h->mp_d,
+                         (const double*)matrix_d, pk_d, ncell, nl, nt, nf, ntp, h->jq);
+    else if (prec_in == 4)
+      hipLaunchKernelGGL((k_multi_pack_own<float, unsigned short>), dim3(g), dim3(256), 0, s,
+                         (unsigned short*)h->mp_d, (const float*)matrix_d, pk_d, ncell, nl, nt, nf, ntp,
+                         h->jq);
+    else
+      hipLaunchKernelGGL((k_multi_pack_own<double, unsigned short>), dim3(g), dim3(256), 0, s,
+                         (unsigned short*)h->mp_d, (const double*)matrix_d, pk_d, ncell, nl, nt, nf, ntp,
+                         h->jq);
+  } else if (prec_in == 4 && mprec == 4)
     hipLaunchKernelGGL((k_multi_pack<float, float>), dim3(g), dim3(256), 0, s, (float*)h->mp_d,
                        (const float*)matrix_d, h->slot_d, ncell, nl, h->nlp, nt, nf, ntp, h->jq);
   else if (prec_in == 8 && mprec == 4)
@@ -506,6 +951,7 @@ int lfr_multi_create(lfr_ctx* ctx, int mprec, int prec_in, const void* matrix_d,
                        h->jq);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (pk_d) (void)hipFree(pk_d);
   if (e != hipSuccess) {
     multi_free(h);
     return lfr_set_error(ctx, LFR_ERR_HIP, "lfr_multi_create: %s", hipGetErrorString(e));
@@ -546,6 +992,34 @@ int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void*
   float* y = (float*)lhs_d;
   const float* x = (const float*)x_d;
   const bool bf = h->mprec == 2;
+  if (h->owner) {
+    // prefetch depth chosen by the autotuner on the first overwrite apply of
+    // a shape (the overwrite form can be repeated on the caller's fields);
+    // an increment apply uses a stored choice or the default
+    // variants (prefetch steps, waves per SIMD): 0 (2, 2), 1 (1, 3)
+    const auto launch = [&](int c) -> int {
+      switch (c) {
+        case 1:
+          bf ? launch_multi_own<true, 1, 3>(s, h, y, x, overwrite) : launch_multi_own<false, 1, 3>(s, h, y, x, overwrite);
+          break;
+        default:
+          bf ? launch_multi_own<true, 2, 2>(s, h, y, x, overwrite) : launch_multi_own<false, 2, 2>(s, h, y, x, overwrite);
+      }
+      return hipGetLastError() == hipSuccess ? LFR_OK : LFR_ERR_HIP;
+    };
+    char key[96];
+    snprintf(key, sizeof key, "multi16 owner prec=%d nl=%d ncell=%d", h->mprec, h->nl, h->ncell);
+    int c = 0;
+    if (overwrite) {
+      c = lfr_tune_pick(ctx, key, 2, 0, launch);
+    } else {
+      auto it = ctx->tune.find(key);
+      if (it != ctx->tune.end() && it->second >= 0 && it->second < 2) c = it->second;
+    }
+    (void)launch(c);
+    LFR_CHECK_LAUNCH(ctx);
+    return LFR_OK;
+  }
   const int key = h->ntp * 1000 + h->nf * 10 + h->pig;
 #define LFR_MULTI_CASE(NTP, NF, PIG, IGW)                               \
   case NTP * 1000 + NF * 10 + PIG:                                      \

@@ -94,11 +94,44 @@ def test_multi_info_and_errors(ctx):
     op = api.LMAOperator(torch.from_numpy(M.astype(np.float32)).to(ctx.device), s, s, col)
     mop = api.MultiOperator(op, 4)
     info = mop.info()
-    assert info["slots"] == 36 and info["vertical_pairs"] == 4 and info["launches"] == col.ncolours
+    # owner-computes form: one launch over all cells
+    assert info["slots"] == 36 and info["vertical_pairs"] == 4 and info["launches"] == 1
     with pytest.raises(LfrError):
         mop.apply(torch.zeros(fs_to.undf, 8, device=ctx.device),
                   torch.zeros(fs_to.undf, 16, device=ctx.device))
     mop.close()
+
+
+@pytest.mark.parametrize("mprec", [4, 2])
+@pytest.mark.parametrize("edge,nl", [(4, 20), (6, 70), (2, 16)])
+def test_multi_owner_form_matches_colour_form(ctx, monkeypatch, mprec, edge, nl):
+    """The owner-computes form (one launch, each output written once) and the
+    colour-by-colour form (LFR_MULTI_OWNER=0) give the same bits in the
+    overwrite form: each cell's contribution is the same MFMA sequence and a
+    shared dof is the sum of exactly two contributions, a + b == b + a."""
+    from lfric_core_amd import api
+    print(f"multi owner vs colour W2k1 C{edge}L{nl} mprec={mprec}", flush=True)
+    mesh, fs, _, col, M, X = _problem("W2", 1, "W2", 1, "spd", nl, edge=edge, seed=11)
+    s = api.Space(ctx, fs)
+    op = api.LMAOperator(torch.from_numpy(M.astype(np.float32)).to(ctx.device), s, s, col)
+    xd = torch.from_numpy(X.astype(np.float32)).to(ctx.device)
+    own = api.MultiOperator(op, mprec)
+    monkeypatch.setenv("LFR_MULTI_OWNER", "0")
+    clr = api.MultiOperator(op, mprec)
+    assert own.info()["launches"] == 1 and clr.info()["launches"] == col.ncolours
+    ya = torch.full((fs.undf, 16), np.nan, dtype=torch.float32, device=ctx.device)
+    yb = torch.full_like(ya, np.nan)
+    own.apply(ya, xd, overwrite=True)
+    clr.apply(yb, xd, overwrite=True)
+    assert torch.isfinite(ya).all()
+    assert torch.equal(ya, yb)
+    # increment form: y + (a + b) against (y + a) + b, equal to rounding
+    za, zb = ya.clone(), ya.clone()
+    own.apply(za, xd, overwrite=False)
+    clr.apply(zb, xd, overwrite=False)
+    torch.testing.assert_close(za, zb, rtol=1e-6, atol=1e-6 * float(ya.abs().max()))
+    own.close()
+    clr.close()
 
 
 def test_refined_cg_multi_16_fields(ctx):

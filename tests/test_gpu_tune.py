@@ -139,6 +139,48 @@ def test_packed_every_variant_identical(ctx, dtype):
         pk.close()
 
 
+@pytest.mark.parametrize("mprec", [4, 2])
+def test_multi16_owner_every_variant_identical(ctx, mprec):
+    """The 16-field owner-computes apply: its prefetch-depth variants give the
+    same bits; the first overwrite apply stores the choice, an increment
+    apply uses it without timing."""
+    from lfric_core_amd import api
+    from lfric_core_amd import function_space as fsm
+    from lfric_core_amd.configs import spd_lma
+    print(f"case tune multi16 owner W2k1 C4L20 mprec={mprec}", flush=True)
+    mesh = cubed_sphere(4)
+    nl = 20
+    fs = fsm.build_dofmap(mesh, "W2", nl, k=1)
+    col = fsm.colour_cells(mesh)
+    st = Stream(53)
+    M = spd_lma(st, mesh.ncells * nl, 36, 36.0, std=1.0 / 6.0).astype(np.float32)
+    X = st.uniform(fs.undf * 16).reshape(fs.undf, 16).astype(np.float32)
+    sp = api.Space(ctx, fs)
+    op = api.LMAOperator(_t(ctx, M), sp, sp, col)
+    mop = api.MultiOperator(op, mprec)
+    assert mop.info()["launches"] == 1
+    key = f"multi16 owner prec={mprec} nl={nl} ncell={mesh.ncells}"
+    ctx.tune_clear()
+    x = _t(ctx, X)
+    y0 = torch.full((fs.undf, 16), float("nan"), dtype=torch.float32, device=ctx.device)
+    mop.apply(y0, x, overwrite=True)
+    torch.cuda.synchronize()
+    assert key in ctx.tune_entries()
+    for c in (0, 1):
+        ctx.tune_set(key, c)
+        y = torch.full_like(y0, float("nan"))
+        mop.apply(y, x, overwrite=True)
+        assert torch.equal(y, y0), f"variant {c}"
+        y.copy_(y0)
+        mop.apply(y, x, overwrite=False)
+        torch.testing.assert_close(y, 2 * y0, rtol=1e-6, atol=1e-6 * float(y0.abs().max()))
+    ctx.tune_clear()
+    y = y0.clone()
+    mop.apply(y, x, overwrite=False)           # increment form: no timing
+    assert ctx.tune_entries() == {}
+    mop.close()
+
+
 def test_multigrid_levels_tuned_and_no_tuning_switch(ctx, monkeypatch):
     from lfric_core_amd.multigrid import Multigrid, build_hierarchy
     print("case tune multigrid keys, LFR_TUNE=0", flush=True)
