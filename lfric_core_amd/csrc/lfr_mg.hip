@@ -34,6 +34,7 @@
 // short timing sweep at set-up (the autotuner).
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <cmath>
 #include <vector>
@@ -51,6 +52,7 @@ struct lfr_mg_level {
   void* u_d = nullptr;
   void* cp_d = nullptr;             // Thomas factors of D, [k][ncell]
   void* iv_d = nullptr;
+  void* adui_d = nullptr;           // (a, d, u, iv) interleaved, [k][ncell] records of 4 values
   void* x_d = nullptr;              // [k][ncell][rhs]
   void* b_d = nullptr;
   void* t_d = nullptr;              // smoother ping-pong buffer
@@ -84,6 +86,20 @@ __device__ __forceinline__ T horiz(const T* __restrict__ xk, const int32_t* __re
   const T sn = s5 + xk[(int64_t)nb[4 * c + 3] * R + r];
   const T four = (T)4 * xc;
   return sigma * (four - sn);
+}
+
+// The forward sweep's four per-cell coefficients of one layer in one record,
+// so that a wave reads them with one vector load instead of four scalar ones
+template <typename T>
+struct alignas(4 * sizeof(T)) Band4 {
+  T a, d, u, iv;
+};
+
+template <typename T>
+__global__ void k_mg_interleave(Band4<T>* __restrict__ out, const T* __restrict__ ba, const T* __restrict__ bd,
+                                const T* __restrict__ bu, const T* __restrict__ iv, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = Band4<T>{ba[i], bd[i], bu[i], iv[i]};
 }
 
 // Thomas factors of D = tridiag(a, d + 4 sigma, u), per column
@@ -146,12 +162,14 @@ __global__ void __launch_bounds__(256) k_mg_smooth(T* __restrict__ xo, const T* 
 // r = b - A x on the fine level.  Also zeroes the coarse solution.
 template <typename T>
 __global__ void __launch_bounds__(256) k_mg_restrict(T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ xf,
-                                                     const T* __restrict__ bf, const T* __restrict__ ba,
-                                                     const T* __restrict__ bd, const T* __restrict__ bu,
+                                                     const T* __restrict__ bf, const Band4<T>* __restrict__ adui,
                                                      const int32_t* __restrict__ nb, const int32_t* __restrict__ children,
-                                                     int ncell_c, int ncell_f, int nl, int lgR, T sigma) {
+                                                     int ncell_c, int ncell_f, int nl, int lgR, T sigma,
+                                                     unsigned nb_pad) {
   const int R = 1 << lgR;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-contiguous blocks, as the smoother: the fine neighbours across the
+  // children's N/S faces are then read on the same XCD
+  const int64_t t = (int64_t)lfr_xcd_block(blockIdx.x, nb_pad) * blockDim.x + threadIdx.x;
   if (t >= (int64_t)ncell_c * R) return;
   const int C = (int)(t >> lgR), r = (int)(t & (R - 1));
   const int64_t pf = (int64_t)ncell_f * R, pc = (int64_t)ncell_c * R;
@@ -176,9 +194,10 @@ __global__ void __launch_bounds__(256) k_mg_restrict(T* __restrict__ bc, T* __re
       const T x0 = x0v[q];
       const T xm = xmv[q];
       const T xp = k + 1 < nl ? xf[i + pf] : T(0);
-      T ax = ba[ik] * xm;
-      ax = ax + bd[ik] * x0;
-      ax = ax + bu[ik] * xp;
+      const Band4<T> bv = adui[ik];
+      T ax = bv.a * xm;
+      ax = ax + bv.d * x0;
+      ax = ax + bv.u * xp;
       ax = ax + horiz(xf + (int64_t)k * pf, nb, f, R, r, x0, sigma);
       const T res = bf[i] - ax;
       acc = acc + res * denom;
@@ -331,15 +350,15 @@ __global__ void __launch_bounds__(256) k_from_mg(T* __restrict__ out, const T* _
 // layers before them), and d' stays in registers, so xo is written once
 // instead of written, re-read and rewritten.  Same operations in the same
 // order as k_mg_smooth.
-template <typename T, int NL, int PF = 4>
+template <typename T, int NL, int PF = 4, bool KX = false>
 __global__ void __launch_bounds__(256) k_mg_smooth_nl(T* __restrict__ xo, const T* __restrict__ xi,
-                                                      const T* __restrict__ b, const T* __restrict__ ba,
-                                                      const T* __restrict__ bd, const T* __restrict__ bu,
-                                                      const T* __restrict__ cp, const T* __restrict__ iv,
-                                                      const int32_t* __restrict__ nb, int ncell, int lgR, T sigma,
-                                                      T omega) {
+                                                      const T* __restrict__ b, const Band4<T>* __restrict__ adui,
+                                                      const T* __restrict__ cp, const int32_t* __restrict__ nb,
+                                                      int ncell, int lgR, T sigma, T omega, unsigned nb_pad) {
   const int R = 1 << lgR;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-contiguous blocks: the N and S neighbours of a cell (a panel row
+  // away) are then read on the same XCD as the cell, mostly from its L2
+  const int64_t t = (int64_t)lfr_xcd_block(blockIdx.x, nb_pad) * blockDim.x + threadIdx.x;
   if (t >= (int64_t)ncell * R) return;
   const int c = (int)(t >> lgR), r = (int)(t & (R - 1));
   const int64_t plane = (int64_t)ncell * R;
@@ -356,10 +375,11 @@ __global__ void __launch_bounds__(256) k_mg_smooth_nl(T* __restrict__ xo, const 
       const int64_t ik = (int64_t)k * ncell + c;
       lx[k] = xk[t];
       lb[k] = b[(int64_t)k * plane + t];
-      la[k] = ba[ik];
-      ld[k] = bd[ik];
-      lu[k] = bu[ik];
-      li[k] = iv[ik];
+      const Band4<T> bv = adui[ik];
+      la[k] = bv.a;
+      ld[k] = bv.d;
+      lu[k] = bv.u;
+      li[k] = bv.iv;
       h0[k] = xk[o0];
       h1[k] = xk[o1];
       h2[k] = xk[o2];
@@ -392,7 +412,7 @@ __global__ void __launch_bounds__(256) k_mg_smooth_nl(T* __restrict__ xo, const 
     const int64_t i = (int64_t)k * plane + t;
     T d = dp[k];
     if (k < NL - 1) d = d - cp[(int64_t)k * ncell + c] * dnext;
-    xo[i] = xi[i] + omega * d;
+    xo[i] = (KX ? lx[k] : xi[i]) + omega * d;
     dnext = d;
   }
 }
@@ -409,10 +429,26 @@ void launch_smooth(hipStream_t s, lfr_mg* mg, lfr_mg_level& L, T* xo, const T* x
   const int64_t n = (int64_t)L.ncell * mg->nrhs;
   if (mg->nl == 70) {
     // FP64: two layers ahead (the registers of four would halve occupancy)
-    hipLaunchKernelGGL((k_mg_smooth_nl<T, 70, sizeof(T) == 4 ? 4 : 2>), dim3((unsigned)((n + bs - 1) / bs)), dim3(bs),
-                       0, s, xo, xi,
-                       (const T*)L.b_d, (const T*)L.a_d, (const T*)L.d_d, (const T*)L.u_d, (const T*)L.cp_d,
-                       (const T*)L.iv_d, L.cell_next_d, L.ncell, lg2(mg->nrhs), (T)L.sigma, (T)mg->omega);
+    // and x read again in the backward sweep
+    const unsigned nbp = ((unsigned)((n + bs - 1) / bs) + 7u) & ~7u;
+    // FP32 keeps every layer's x in registers for the backward sweep
+    // instead of reading it again (228 VGPRs; measured 4.19 -> 4.09 ms per
+    // C192 V-cycle); LFR_MG_KEEPX=0 reads it again
+    static const bool kx = [] {
+      const char* e = getenv("LFR_MG_KEEPX");
+      return !(e && e[0] == '0');
+    }();
+    if constexpr (sizeof(T) == 4) {
+      if (kx) {
+        hipLaunchKernelGGL((k_mg_smooth_nl<T, 70, 4, true>), dim3(nbp), dim3(bs), 0, s, xo, xi, (const T*)L.b_d,
+                           (const Band4<T>*)L.adui_d, (const T*)L.cp_d, L.cell_next_d, L.ncell, lg2(mg->nrhs),
+                           (T)L.sigma, (T)mg->omega, nbp);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_mg_smooth_nl<T, 70, sizeof(T) == 4 ? 4 : 2>), dim3(nbp), dim3(bs), 0, s, xo, xi,
+                         (const T*)L.b_d, (const Band4<T>*)L.adui_d, (const T*)L.cp_d, L.cell_next_d, L.ncell,
+                         lg2(mg->nrhs), (T)L.sigma, (T)mg->omega, nbp);
     return;
   }
   hipLaunchKernelGGL(k_mg_smooth<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, xo, xi, (const T*)L.b_d,
@@ -440,9 +476,10 @@ void vcycle(hipStream_t s, lfr_mg* mg, int l) {
   {
     const int bs = L.block_other;
     const int64_t n = (int64_t)C.ncell * mg->nrhs;
-    hipLaunchKernelGGL(k_mg_restrict<T>, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, (T*)C.b_d,
-                       (T*)C.x_d, (const T*)L.x_d, (const T*)L.b_d, (const T*)L.a_d, (const T*)L.d_d,
-                       (const T*)L.u_d, L.cell_next_d, C.children_d, C.ncell, L.ncell, mg->nl, lgR, (T)L.sigma);
+    const unsigned nbp = ((unsigned)((n + bs - 1) / bs) + 7u) & ~7u;
+    hipLaunchKernelGGL(k_mg_restrict<T>, dim3(nbp), dim3(bs), 0, s, (T*)C.b_d, (T*)C.x_d, (const T*)L.x_d,
+                       (const T*)L.b_d, (const Band4<T>*)L.adui_d, L.cell_next_d, C.children_d, C.ncell, L.ncell,
+                       mg->nl, lgR, (T)L.sigma, nbp);
   }
   vcycle<T>(s, mg, l + 1);
   {
@@ -461,7 +498,7 @@ void mg_free(lfr_mg* mg) {
   if (mg->graph) (void)hipGraphExecDestroy(mg->graph);
   for (auto& L : mg->lev)
     for (void* p : {(void*)L.cell_next_d, (void*)L.children_d, (void*)L.parent_d, L.a_d, L.d_d, L.u_d,
-                    L.cp_d, L.iv_d, L.x_d, L.b_d, L.t_d})
+                    L.cp_d, L.iv_d, L.adui_d, L.x_d, L.b_d, L.t_d})
       if (p) (void)hipFree(p);
   if (mg->part_d) (void)hipFree(mg->part_d);
   delete mg;
@@ -533,7 +570,7 @@ int lfr_mg_create(lfr_ctx* ctx, int prec, int nlev, int nlayers, int nrhs, const
     bool ok = hipMalloc(&L.cell_next_d, (size_t)nc * 4 * 4) == hipSuccess &&
               hipMalloc(&L.a_d, nb * s) == hipSuccess && hipMalloc(&L.d_d, nb * s) == hipSuccess &&
               hipMalloc(&L.u_d, nb * s) == hipSuccess && hipMalloc(&L.cp_d, nb * s) == hipSuccess &&
-              hipMalloc(&L.iv_d, nb * s) == hipSuccess &&
+              hipMalloc(&L.iv_d, nb * s) == hipSuccess && hipMalloc(&L.adui_d, nb * 4 * s) == hipSuccess &&
               hipMalloc(&L.x_d, nb * nrhs * s) == hipSuccess &&
               hipMalloc(&L.b_d, nb * nrhs * s) == hipSuccess &&
               hipMalloc(&L.t_d, nb * nrhs * s) == hipSuccess;
@@ -563,6 +600,15 @@ int lfr_mg_create(lfr_ctx* ctx, int prec, int nlev, int nlayers, int nrhs, const
         hipLaunchKernelGGL(k_mg_factor<double>, dim3(g), dim3(256), 0, ctx->stream, (double*)L.cp_d,
                            (double*)L.iv_d, (const double*)L.a_d, (const double*)L.d_d, (const double*)L.u_d, nc,
                            nlayers, L.sigma);
+      const unsigned gi = (unsigned)std::min<size_t>((nb + 255) / 256, 8192);
+      if (prec == 4)
+        hipLaunchKernelGGL(k_mg_interleave<float>, dim3(gi), dim3(256), 0, ctx->stream, (Band4<float>*)L.adui_d,
+                           (const float*)L.a_d, (const float*)L.d_d, (const float*)L.u_d, (const float*)L.iv_d,
+                           (int64_t)nb);
+      else
+        hipLaunchKernelGGL(k_mg_interleave<double>, dim3(gi), dim3(256), 0, ctx->stream, (Band4<double>*)L.adui_d,
+                           (const double*)L.a_d, (const double*)L.d_d, (const double*)L.u_d, (const double*)L.iv_d,
+                           (int64_t)nb);
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
         mg_free(mg);
         return lfr_set_error(ctx, LFR_ERR_HIP, "lfr_mg_create: factor kernel failed");
