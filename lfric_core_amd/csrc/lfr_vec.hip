@@ -107,8 +107,24 @@ __global__ void __launch_bounds__(lfr::kRedThreads)
 k_dot_part(const T* __restrict__ x, const T* __restrict__ y, int64_t n, double* __restrict__ part) {
   __shared__ double lds[lfr::kRedThreads / 64];
   double s = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * lfr::kRedThreads + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * lfr::kRedThreads) {
+  const int64_t st = (int64_t)gridDim.x * lfr::kRedThreads;
+  int64_t i = (int64_t)blockIdx.x * lfr::kRedThreads + threadIdx.x;
+  // eight elements' loads issued before their products are added, in the
+  // same order as one at a time
+  for (; i + 7 * st < n; i += 8 * st) {
+    T a[8], b[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      a[q] = x[i + q * st];
+      b[q] = y[i + q * st];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double p = (double)a[q] * (double)b[q];
+      s = s + p;
+    }
+  }
+  for (; i < n; i += st) {
     const double p = (double)x[i] * (double)y[i];
     s = s + p;
   }
