@@ -145,18 +145,21 @@ k_restrict_w2(T* __restrict__ coarse, const T* __restrict__ fine, const int32_t*
 // (1/n) (w a + (1 - w) b) of the coarse W and E (N and S) fluxes with
 // w = (n + 1 - x) / n, a fine B/T face 1/n^2 of the coarse one.  Square
 // children only (nfx = nfy = n): a fine face shared by two coarse cells then
-// gets the same weight, and the same value, from both.  One thread per
-// coarse cell and LPT of its nl + 1 levels (strided by S).
+// gets the same weight, and the same value, from both, so only the coarse
+// cell that owns the shared coarse side (wmask, as for the restriction)
+// writes it: 12 fine face columns per coarse cell instead of 16.  One thread
+// per coarse cell and LPT of its nl + 1 levels (strided by S).
 template <typename T, int LPT>
 __global__ void __launch_bounds__(256)
 k_prolong_w2(T* __restrict__ fine, const T* __restrict__ coarse, const int32_t* __restrict__ map_c,
-             const int32_t* __restrict__ fface, int ncc, int nl, int S, int n) {
+             const int32_t* __restrict__ fface, const uint8_t* __restrict__ wmask, int ncc, int nl, int S, int n) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= (int64_t)ncc * S) return;
   const int c = (int)(t / S);
   const int j = (int)(t - (int64_t)c * S);
   const int32_t* ff = fface + (int64_t)c * n * n * 5;
   const int32_t* mc = map_c + (int64_t)c * 6;
+  const unsigned m = wmask[c];   // bits W, S, E, N
   // fine face column of child (x, y) (0-based), face df
   auto fcol = [&](int x, int y, int df) { return fine + ff[(x + n * y) * 5 + df]; };
   auto side = [&](T* f, const T* val) {
@@ -175,18 +178,26 @@ k_prolong_w2(T* __restrict__ fine, const T* __restrict__ coarse, const int32_t* 
     ce[q] = in ? coarse[mc[2] + k] : T(0);
     cn[q] = in ? coarse[mc[3] + k] : T(0);
   }
+  if (m & 8u) {
 #pragma unroll
-  for (int q = 0; q < LPT; ++q) tmp[q] = w1 * cn[q];
-  for (int x = 0; x < n; ++x) side(fcol(x, 0, 3), tmp);         // N faces of the first row
+    for (int q = 0; q < LPT; ++q) tmp[q] = w1 * cn[q];
+    for (int x = 0; x < n; ++x) side(fcol(x, 0, 3), tmp);       // N faces of the first row
+  }
+  if (m & 2u) {
 #pragma unroll
-  for (int q = 0; q < LPT; ++q) tmp[q] = w1 * cs[q];
-  for (int x = 0; x < n; ++x) side(fcol(x, n - 1, 1), tmp);     // S faces of the last row
+    for (int q = 0; q < LPT; ++q) tmp[q] = w1 * cs[q];
+    for (int x = 0; x < n; ++x) side(fcol(x, n - 1, 1), tmp);   // S faces of the last row
+  }
+  if (m & 1u) {
 #pragma unroll
-  for (int q = 0; q < LPT; ++q) tmp[q] = w1 * cw[q];
-  for (int y = 0; y < n; ++y) side(fcol(0, y, 0), tmp);         // W faces of the first column
+    for (int q = 0; q < LPT; ++q) tmp[q] = w1 * cw[q];
+    for (int y = 0; y < n; ++y) side(fcol(0, y, 0), tmp);       // W faces of the first column
+  }
+  if (m & 4u) {
 #pragma unroll
-  for (int q = 0; q < LPT; ++q) tmp[q] = w1 * ce[q];
-  for (int y = 0; y < n; ++y) side(fcol(n - 1, y, 2), tmp);     // E faces of the last column
+    for (int q = 0; q < LPT; ++q) tmp[q] = w1 * ce[q];
+    for (int y = 0; y < n; ++y) side(fcol(n - 1, y, 2), tmp);   // E faces of the last column
+  }
   for (int x = 1; x < n; ++x) {                                  // interior W faces
     const T iw = T(n - x) / T(n);                                // (n + 1 - x_1based) / n
 #pragma unroll
@@ -470,10 +481,10 @@ extern "C" int lfr_prolong_w2(lfr_ctx* ctx, int prec, void* fine_d, const void* 
     constexpr int LPT = decltype(L)::value;
     if (prec == 4)
       hipLaunchKernelGGL((k_prolong_w2<float, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (float*)fine_d,
-                         (const float*)coarse_d, cs->map0_d, ig->fbase_d, cs->ncell, nl, S, ig->nfx);
+                         (const float*)coarse_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, S, ig->nfx);
     else
       hipLaunchKernelGGL((k_prolong_w2<double, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (double*)fine_d,
-                         (const double*)coarse_d, cs->map0_d, ig->fbase_d, cs->ncell, nl, S, ig->nfx);
+                         (const double*)coarse_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, S, ig->nfx);
   });
   LFR_CHECK_LAUNCH(ctx);
   return LFR_OK;
