@@ -246,7 +246,11 @@ k_appinv_staged(T* __restrict__ lhs, const T* __restrict__ x, const T* __restric
 // reads), the solution goes back through LDS and out in the same flat order
 // with streaming stores.  The recurrence is k_appinv's term for term, so the
 // bits are the same.
-template <int NROW, int KCH, int COLS>
+// DPL: d' lives in the column's right-hand-side slots in LDS (each row reads
+// its right-hand side once, then holds d'), so a lane keeps only c' in
+// registers (about 70 fewer VGPRs, more waves per SIMD) and the backward
+// sweep leaves the solution in place for the store.
+template <int NROW, int KCH, int COLS, bool DPL = false>
 __global__ void __launch_bounds__(kCols)
 k_appinv_deep(float* __restrict__ lhs, const float* __restrict__ x, const float* __restrict__ cm,
               const int32_t* __restrict__ map0, int ndf, const int32_t* __restrict__ ind, int ncell) {
@@ -299,7 +303,8 @@ k_appinv_deep(float* __restrict__ lhs, const float* __restrict__ x, const float*
       }
     }
   }
-  float cp[NROW], dp[NROW];
+  float cp[NROW], dp[DPL ? 1 : NROW];
+  float dprev = 0.f;   // DPL: d' of the previous row
 #pragma unroll
   for (int s = 0; s < NST; ++s) {
     __syncthreads();   // the previous stage's reads are done (and, first time, the rhs is in)
@@ -319,31 +324,55 @@ k_appinv_deep(float* __restrict__ lhs, const float* __restrict__ x, const float*
         const int i = s * KCH + q;
         const float* v = sb + lane * PB + 3 * q;
         const float a = v[0], d = v[1], u = v[2], xv = sxa[lane * PX + i];
+        float di;
         if (i == 0) {
           const float inv = 1.0f / d;
           cp[i] = inv * u;
-          dp[i] = inv * xv;
+          di = inv * xv;
         } else {
           const float ac = a * cp[i - 1];
           const float inv = 1.0f / (d - ac);
           if (i < NROW - 1) cp[i] = inv * u;
-          const float ad = a * dp[i - 1];
-          dp[i] = inv * (xv - ad);
+          float dprv;
+          if constexpr (DPL) dprv = dprev;
+          else dprv = dp[i - 1];
+          const float ad = a * dprv;
+          di = inv * (xv - ad);
+        }
+        if constexpr (DPL) {
+          sxa[lane * PX + i] = di;
+          dprev = di;
+        } else {
+          dp[i] = di;
         }
       }
     }
   }
+  if constexpr (DPL) {
+    if (lane < COLS) {
+      float dnext = dprev;   // d'(NROW-1), already the solution there
 #pragma unroll
-  for (int i = NROW - 2; i >= 0; --i) {
-    const float cd = cp[i] * dp[i + 1];
-    dp[i] = dp[i] - cd;
-  }
-  __syncthreads();
-  if (lane < COLS) {
+      for (int i = NROW - 2; i >= 0; --i) {
+        const float cd = cp[i] * dnext;
+        const float di = sxa[lane * PX + i] - cd;
+        sxa[lane * PX + i] = di;
+        dnext = di;
+      }
+    }
+    __syncthreads();
+  } else {
 #pragma unroll
-    for (int i = 0; i < NROW; ++i) sxa[lane * PX + i] = dp[i];
+    for (int i = NROW - 2; i >= 0; --i) {
+      const float cd = cp[i] * dp[i + 1];
+      dp[i] = dp[i] - cd;
+    }
+    __syncthreads();
+    if (lane < COLS) {
+#pragma unroll
+      for (int i = 0; i < NROW; ++i) sxa[lane * PX + i] = dp[i];
+    }
+    __syncthreads();
   }
-  __syncthreads();
 #pragma unroll
   for (int it = 0; it < NXF; ++it) {
     const int e = lane + it * kCols;
@@ -440,18 +469,30 @@ int launch_appinv(lfr_ctx* ctx, T* lhs, const T* x, const T* cm, const lfr_space
             hipLaunchKernelGGL((k_appinv_deep<70, 10, 32>), dim3((ncell + 31) / 32), dim3(kCols), 0, ctx->stream,
                                lhs, x, cm, sp->map0_d, sp->ndf, ind, ncell);
             break;
+          case 5:   // d' in LDS
+            hipLaunchKernelGGL((k_appinv_deep<70, 10, 16, true>), dim3((ncell + 15) / 16), dim3(kCols), 0,
+                               ctx->stream, lhs, x, cm, sp->map0_d, sp->ndf, ind, ncell);
+            break;
+          case 6:
+            hipLaunchKernelGGL((k_appinv_deep<70, 14, 32, true>), dim3((ncell + 31) / 32), dim3(kCols), 0,
+                               ctx->stream, lhs, x, cm, sp->map0_d, sp->ndf, ind, ncell);
+            break;
+          case 7:
+            hipLaunchKernelGGL((k_appinv_deep<70, 14, 16, true>), dim3((ncell + 15) / 16), dim3(kCols), 0,
+                               ctx->stream, lhs, x, cm, sp->map0_d, sp->ndf, ind, ncell);
+            break;
           default:
             hipLaunchKernelGGL((k_appinv_staged<T, 70>), dim3(nb), dim3(kCols), 0, ctx->stream, lhs, x, cm,
                                sp->map0_d, sp->ndf, ind, ncell);
         }
         return hipGetLastError() == hipSuccess ? LFR_OK : LFR_ERR_HIP;
       };
-      const int dflt = (ncell + 31) / 32 <= 2048 ? 0 : 1;
+      const int dflt = (ncell + 31) / 32 <= 2048 ? 0 : 7;
       int c = dflt;
       if ((const void*)lhs != (const void*)x) {
         char key[96];
         snprintf(key, sizeof key, "appinv f32 nrow=70 ncell=%d", ncell);
-        c = lfr_tune_pick(ctx, key, 5, dflt, launch);
+        c = lfr_tune_pick(ctx, key, 8, dflt, launch);
       }
       if (launch(c) != LFR_OK) return lfr_set_error(ctx, LFR_ERR_HIP, "appinv: launch failed");
       return LFR_OK;
@@ -1012,7 +1053,7 @@ k_thomas(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__ t0, c
 // back through LDS in flat order (64 consecutive dofs per instruction when
 // the columns are contiguous) with streaming stores.  Same arithmetic as
 // k_tri_solve.
-template <int NROW, int KCH, int COLS>
+template <int NROW, int KCH, int COLS, bool DPL = false>
 __global__ void __launch_bounds__(kCols)
 k_thomas_deep(float* __restrict__ y, const float* __restrict__ x, const float* __restrict__ t0,
               const float* __restrict__ tp, const float* __restrict__ tm, const int32_t* __restrict__ map0,
@@ -1042,7 +1083,8 @@ k_thomas_deep(float* __restrict__ y, const float* __restrict__ x, const float* _
 #pragma unroll
       for (int v = 0; v < 4; ++v) rv[s][v][it] = src[v][a];
     }
-  float cp[NROW], dp[NROW];
+  float cp[NROW], dp[DPL ? 1 : NROW];
+  float dprev = 0.f;   // DPL: d' of the previous row
 #pragma unroll
   for (int s = 0; s < NST; ++s) {
     __syncthreads();
@@ -1062,31 +1104,55 @@ k_thomas_deep(float* __restrict__ y, const float* __restrict__ x, const float* _
         const int i = s * KCH + q;
         const float* w = sb + lane * PB + 4 * q;
         const float a = w[0], dg = w[1], u = w[2], xv = w[3];
+        float di;
         if (i == 0) {
           const float inv = 1.0f / dg;
           cp[i] = u * inv;
-          dp[i] = xv * inv;
+          di = xv * inv;
         } else {
           const float ac = a * cp[i - 1];
           const float inv = 1.0f / (dg - ac);
-          const float ad = a * dp[i - 1];
+          float dprv;
+          if constexpr (DPL) dprv = dprev;
+          else dprv = dp[i - 1];
+          const float ad = a * dprv;
           if (i < NROW - 1) cp[i] = u * inv;
-          dp[i] = (xv - ad) * inv;
+          di = (xv - ad) * inv;
+        }
+        if constexpr (DPL) {
+          sxa[lane * PX + i] = di;
+          dprev = di;
+        } else {
+          dp[i] = di;
         }
       }
     }
   }
+  if constexpr (DPL) {
+    if (lane < COLS) {
+      float dnext = dprev;   // d'(NROW-1), already the solution there
 #pragma unroll
-  for (int i = NROW - 2; i >= 0; --i) {
-    const float cd = cp[i] * dp[i + 1];
-    dp[i] = dp[i] - cd;
-  }
-  __syncthreads();
-  if (lane < COLS) {
+      for (int i = NROW - 2; i >= 0; --i) {
+        const float cd = cp[i] * dnext;
+        const float di = sxa[lane * PX + i] - cd;
+        sxa[lane * PX + i] = di;
+        dnext = di;
+      }
+    }
+    __syncthreads();
+  } else {
 #pragma unroll
-    for (int i = 0; i < NROW; ++i) sxa[lane * PX + i] = dp[i];
+    for (int i = NROW - 2; i >= 0; --i) {
+      const float cd = cp[i] * dp[i + 1];
+      dp[i] = dp[i] - cd;
+    }
+    __syncthreads();
+    if (lane < COLS) {
+#pragma unroll
+      for (int i = 0; i < NROW; ++i) sxa[lane * PX + i] = dp[i];
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const int nflat = ncols * NROW;
   constexpr int NXF = (COLS * NROW + kCols - 1) / kCols;
 #pragma unroll
@@ -1364,16 +1430,24 @@ bool launch_thomas(lfr_ctx* ctx, hipStream_t s, T* y, const T* x, const T* b0, c
               hipLaunchKernelGGL((k_thomas_deep<70, 14, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y, x, b0,
                                  b1, b2, sp->map0_d, sp->ndf, ncell);
               break;
+            case 4:   // d' in LDS
+              hipLaunchKernelGGL((k_thomas_deep<70, 10, 16, true>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y,
+                                 x, b0, b1, b2, sp->map0_d, sp->ndf, ncell);
+              break;
+            case 5:
+              hipLaunchKernelGGL((k_thomas_deep<70, 14, 16, true>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y,
+                                 x, b0, b1, b2, sp->map0_d, sp->ndf, ncell);
+              break;
             default:
               hipLaunchKernelGGL((k_thomas<T, 70>), dim3(nb), dim3(kCols), 0, s, y, x, b0, b1, b2, sp->map0_d,
                                  sp->ndf, ncell);
           }
           return hipGetLastError() == hipSuccess ? LFR_OK : LFR_ERR_HIP;
         };
-        const int dflt = (ncell + 31) / 32 <= 2048 ? 0 : 1;
+        const int dflt = (ncell + 31) / 32 <= 2048 ? 0 : 5;
         char key[96];
         snprintf(key, sizeof key, "tri_solve f32 nl=70 ncell=%d", ncell);
-        const int c = ctx ? lfr_tune_pick(ctx, key, 4, dflt, launch) : dflt;
+        const int c = ctx ? lfr_tune_pick(ctx, key, 6, dflt, launch) : dflt;
         (void)launch(c);
         return true;
       }

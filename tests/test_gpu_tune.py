@@ -50,7 +50,7 @@ def test_appinv_every_variant_identical(ctx):
     torch.cuda.synchronize()
     assert key in ctx.tune_entries()
     np.testing.assert_array_equal(y.cpu().numpy(), ref)
-    for c in range(5):
+    for c in range(8):
         ctx.tune_set(key, c)
         y = torch.full_like(x, float("nan"))
         api.columnwise_op_appinv(y, x, op)
@@ -74,14 +74,14 @@ def test_tri_solve_every_variant_identical(ctx):
     key = f"tri_solve f32 nl=70 ncell={fs.mesh.ncells}"
     ctx.tune_clear()
     args = [_t(ctx, a) for a in (x, t0, tp, tm)]
-    for c in (None, 0, 1, 2, 3):
+    for c in (None, 0, 1, 2, 3, 4, 5):
         if c is not None:
             ctx.tune_set(key, c)
         y = torch.full((u,), float("nan"), dtype=torch.float32, device=ctx.device)
         api.tri_solve(y, *args, sp)
         torch.cuda.synchronize()
         np.testing.assert_array_equal(y.cpu().numpy(), ref, err_msg=f"variant {c}")
-    assert ctx.tune_get(key) == 3
+    assert ctx.tune_get(key) == 5
 
 
 def test_operator_tri_solve_every_variant_identical(ctx):
