@@ -513,6 +513,21 @@ def bench_rows(ctx, cfg, w2, w3, torch, steps: int = 50):
     step_bytes = n3 * 36 * 8 + 2 * u2 * 8
     vec = sum(2 + 4 * iv + 3 + 2 for iv in range(10)) * u2 * 8 / 10
     rec("a10_gmres10_per_arnoldi_step_fp64", ms / 20, step_bytes + vec)
+    # FGMRES(10) and GCR(10) through the generic solver engine (same operator,
+    # null preconditioner, two restarts of ten steps): time per Krylov step
+    # only, their vector passes differing from GMRES's
+    from lfric_core_amd.solvers import FGMRES, GCR
+    for name, cls in (("a10_fgmres10_per_step_fp64", FGMRES), ("a10_gcr10_per_step_fp64", GCR)):
+        # monitor off: exactly 2 restarts x 10 steps per solve
+        sv = cls(op64, gcrk=10, r_tol=1e-30, max_iter=2, monitor_convergence=False)
+        xg.zero_()
+        sv.apply(xg, b)
+        torch.cuda.synchronize()
+        ms_s = timed(lambda: sv.apply(xg, b), 5)
+        per = ms_s / 20
+        out[name] = {"us": round(per * 1e3, 2), "steps_per_solve": 20,
+                     "note": "time per step of a 2-restart solve; no byte count"}
+        log(f"row {name}: {per*1e3:.1f} us per step (20 steps a solve)")
     op64.packed.close()
     del M64, op64, b, xg
     # a11-a14: vector kernels on W2-sized FP32 fields
