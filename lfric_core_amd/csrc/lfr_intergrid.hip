@@ -24,55 +24,111 @@
 namespace {
 
 // fbase(child, coarse cell): first (0-based) dof of the child's column, in
-// the map's order (x fastest, then y)
+// the map's order (x fastest, then y).  Levels past the column are clamped
+// to its top for the loads (always in bounds, so every load issues without a
+// branch) and masked out of the sums and stores.  NCH > 0: that many
+// children, unrolled, every child's loads issued before the first sum.
 template <typename T, int LPT>
+__device__ __forceinline__ void level_slots(int j, int S, int nlev, int (&kk)[LPT], bool (&ok)[LPT]) {
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) {
+    const int k = j + q * S;
+    ok[q] = k < nlev;
+    kk[q] = ok[q] ? k : nlev - 1;
+  }
+}
+
+template <typename T, int LPT, int NCH>
 __global__ void __launch_bounds__(256)
 k_restrict_scalar(T* __restrict__ coarse, const T* __restrict__ fine, const int32_t* __restrict__ map_c,
-                  const int32_t* __restrict__ fbase, int ncc, int ndf, int nlev, int S, int nch, T denom) {
+                  const int32_t* __restrict__ fbase, int ncc, int ndf, int nlev, int S, int nch_rt, T denom) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= (int64_t)ncc * S) return;
   const int c = (int)(t / S);
   const int j = (int)(t - (int64_t)c * S);
+  const int nch = NCH > 0 ? NCH : nch_rt;
   const int32_t* fb = fbase + (int64_t)c * nch;
+  int kk[LPT];
+  bool ok[LPT];
+  level_slots<T, LPT>(j, S, nlev, kk, ok);
   T v[LPT];
 #pragma unroll
   for (int q = 0; q < LPT; ++q) v[q] = T(0);
-  for (int i = 0; i < nch; ++i) {   // y outer, x inner
-    const T* f = fine + fb[i];
+  if constexpr (NCH > 0) {
+    int fbv[NCH];
 #pragma unroll
-    for (int q = 0; q < LPT; ++q) {
-      const int k = j + q * S;
-      if (k < nlev) {
-        const T p = f[k] * denom;
-        v[q] = v[q] + p;
+    for (int i = 0; i < NCH; ++i) fbv[i] = fb[i];
+    T val[NCH][LPT];
+#pragma unroll
+    for (int i = 0; i < NCH; ++i)
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) val[i][q] = fine[fbv[i] + kk[q]];
+#pragma unroll
+    for (int i = 0; i < NCH; ++i)   // y outer, x inner
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) {
+        const T p = val[i][q] * denom;
+        if (ok[q]) v[q] = v[q] + p;
+      }
+  } else {
+    for (int i = 0; i < nch; ++i) {   // y outer, x inner
+      const T* f = fine + fb[i];
+      T val[LPT];
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) val[q] = f[kk[q]];
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) {
+        const T p = val[q] * denom;
+        if (ok[q]) v[q] = v[q] + p;
       }
     }
   }
   T* o = coarse + map_c[(int64_t)c * ndf];
 #pragma unroll
   for (int q = 0; q < LPT; ++q)
-    if (j + q * S < nlev) o[j + q * S] = v[q];
+    if (ok[q]) o[kk[q]] = v[q];
 }
 
-template <typename T, int LPT>
+template <typename T, int LPT, int NCH>
 __global__ void __launch_bounds__(256)
 k_prolong_scalar(T* __restrict__ fine, const T* __restrict__ coarse, const int32_t* __restrict__ map_c,
-                 const int32_t* __restrict__ fbase, int ncc, int ndf, int nlev, int S, int nch) {
+                 const int32_t* __restrict__ fbase, int ncc, int ndf, int nlev, int S, int nch_rt) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= (int64_t)ncc * S) return;
   const int c = (int)(t / S);
   const int j = (int)(t - (int64_t)c * S);
+  const int nch = NCH > 0 ? NCH : nch_rt;
   const int32_t* fb = fbase + (int64_t)c * nch;
   const T* cc = coarse + map_c[(int64_t)c * ndf];
+  int kk[LPT];
+  bool ok[LPT];
+  level_slots<T, LPT>(j, S, nlev, kk, ok);
   T cv[LPT];
 #pragma unroll
-  for (int q = 0; q < LPT; ++q) cv[q] = (j + q * S < nlev) ? cc[j + q * S] : T(0);
-  for (int i = 0; i < nch; ++i) {
-    T* f = fine + fb[i];
+  for (int q = 0; q < LPT; ++q) cv[q] = cc[kk[q]];
+  if constexpr (NCH > 0) {
+    int fbv[NCH];
 #pragma unroll
-    for (int q = 0; q < LPT; ++q) {
-      const int k = j + q * S;
-      if (k < nlev) f[k] = f[k] + cv[q];
+    for (int i = 0; i < NCH; ++i) fbv[i] = fb[i];
+    T val[NCH][LPT];
+#pragma unroll
+    for (int i = 0; i < NCH; ++i)
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) val[i][q] = fine[fbv[i] + kk[q]];
+#pragma unroll
+    for (int i = 0; i < NCH; ++i)
+#pragma unroll
+      for (int q = 0; q < LPT; ++q)
+        if (ok[q]) fine[fbv[i] + kk[q]] = val[i][q] + cv[q];
+  } else {
+    for (int i = 0; i < nch; ++i) {
+      T* f = fine + fb[i];
+      T val[LPT];
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) val[q] = f[kk[q]];
+#pragma unroll
+      for (int q = 0; q < LPT; ++q)
+        if (ok[q]) f[kk[q]] = val[q] + cv[q];
     }
   }
 }
@@ -86,7 +142,7 @@ k_prolong_scalar(T* __restrict__ fine, const T* __restrict__ coarse, const int32
 // writer in cell order of the reference's loop, whose faces come from the
 // face selectors), and fface(child, face, coarse cell) is the first dof
 // (0-based) of the fine face column.
-template <typename T, int LPT>
+template <typename T, int LPT, int N>
 __global__ void __launch_bounds__(256)
 k_restrict_w2(T* __restrict__ coarse, const T* __restrict__ fine, const int32_t* __restrict__ map_c,
               const int32_t* __restrict__ fface, const uint8_t* __restrict__ wmask, int ncc, int nl, int S,
@@ -99,6 +155,12 @@ k_restrict_w2(T* __restrict__ coarse, const T* __restrict__ fine, const int32_t*
   const int32_t* ff = fface + (int64_t)c * nch * 5;
   const int32_t* mc = map_c + (int64_t)c * 6;
   const unsigned m = wmask[c];
+  // side faces have nl levels, the B face column nl + 1 (loads clamped into
+  // the column, masked out of sums and stores)
+  int ks[LPT], kb[LPT];
+  bool oks[LPT], okb[LPT];
+  level_slots<T, LPT>(j, S, nl, ks, oks);
+  level_slots<T, LPT>(j, S, nl + 1, kb, okb);
   T v[LPT];
 #pragma unroll
   for (int df = 0; df < 4; ++df) {   // W, S, E, N
@@ -110,30 +172,72 @@ k_restrict_w2(T* __restrict__ coarse, const T* __restrict__ fine, const int32_t*
     else x0 = nfx - 1;                // E: last column
 #pragma unroll
     for (int q = 0; q < LPT; ++q) v[q] = T(0);
-    for (int y = y0; y <= y1; ++y)
-      for (int x = x0; x <= x1; ++x) {
-        const T* f = fine + ff[(x + nfx * y) * 5 + df];
+    if constexpr (N > 0) {
+      // the N fine faces on this side, y outer and x inner
+      const T* fs[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int x = x0 == x1 ? x0 : x0 + i, y = y0 == y1 ? y0 : y0 + i;
+        fs[i] = fine + ff[(x + N * y) * 5 + df];
+      }
+      T val[N][LPT];
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int q = 0; q < LPT; ++q) val[i][q] = fs[i][ks[q]];
+#pragma unroll
+      for (int i = 0; i < N; ++i)
 #pragma unroll
         for (int q = 0; q < LPT; ++q)
-          if (j + q * S < nl) v[q] = v[q] + f[j + q * S];
-      }
+          if (oks[q]) v[q] = v[q] + val[i][q];
+    } else {
+      for (int y = y0; y <= y1; ++y)
+        for (int x = x0; x <= x1; ++x) {
+          const T* f = fine + ff[(x + nfx * y) * 5 + df];
+          T val[LPT];
+#pragma unroll
+          for (int q = 0; q < LPT; ++q) val[q] = f[ks[q]];
+#pragma unroll
+          for (int q = 0; q < LPT; ++q)
+            if (oks[q]) v[q] = v[q] + val[q];
+        }
+    }
     T* o = coarse + mc[df];
 #pragma unroll
     for (int q = 0; q < LPT; ++q)
-      if (j + q * S < nl) o[j + q * S] = v[q];
+      if (oks[q]) o[ks[q]] = v[q];
   }
 #pragma unroll
   for (int q = 0; q < LPT; ++q) v[q] = T(0);
-  for (int i = 0; i < nch; ++i) {
-    const T* f = fine + ff[i * 5 + 4];
+  if constexpr (N > 0) {
+    const T* fs[N * N];
 #pragma unroll
-    for (int q = 0; q < LPT; ++q)
-      if (j + q * S <= nl) v[q] = v[q] + f[j + q * S];
+    for (int i = 0; i < N * N; ++i) fs[i] = fine + ff[i * 5 + 4];
+    T val[N * N][LPT];
+#pragma unroll
+    for (int i = 0; i < N * N; ++i)
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) val[i][q] = fs[i][kb[q]];
+#pragma unroll
+    for (int i = 0; i < N * N; ++i)
+#pragma unroll
+      for (int q = 0; q < LPT; ++q)
+        if (okb[q]) v[q] = v[q] + val[i][q];
+  } else {
+    for (int i = 0; i < nch; ++i) {
+      const T* f = fine + ff[i * 5 + 4];
+      T val[LPT];
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) val[q] = f[kb[q]];
+#pragma unroll
+      for (int q = 0; q < LPT; ++q)
+        if (okb[q]) v[q] = v[q] + val[q];
+    }
   }
   T* o = coarse + mc[4];
 #pragma unroll
   for (int q = 0; q < LPT; ++q)
-    if (j + q * S <= nl) o[j + q * S] = v[q];
+    if (okb[q]) o[kb[q]] = v[q];
 }
 
 // prolong_w2_kernel_code
@@ -168,15 +272,20 @@ k_prolong_w2(T* __restrict__ fine, const T* __restrict__ coarse, const int32_t* 
       if (j + q * S < nl) f[j + q * S] = val[q];
   };
   const T w1 = T(1) / T(n);
-  T cw[LPT], cs[LPT], ce[LPT], cn[LPT], tmp[LPT];
+  // loads clamped into the columns (side faces nl levels, B/T nl + 1), so
+  // they issue without branches; the stores are masked
+  int ks[LPT], kb[LPT];
+  bool oks[LPT], okb[LPT];
+  level_slots<T, LPT>(j, S, nl, ks, oks);
+  level_slots<T, LPT>(j, S, nl + 1, kb, okb);
+  T cw[LPT], cs[LPT], ce[LPT], cn[LPT], cb[LPT], tmp[LPT];
 #pragma unroll
   for (int q = 0; q < LPT; ++q) {
-    const int k = j + q * S;
-    const bool in = k < nl;
-    cw[q] = in ? coarse[mc[0] + k] : T(0);
-    cs[q] = in ? coarse[mc[1] + k] : T(0);
-    ce[q] = in ? coarse[mc[2] + k] : T(0);
-    cn[q] = in ? coarse[mc[3] + k] : T(0);
+    cw[q] = coarse[mc[0] + ks[q]];
+    cs[q] = coarse[mc[1] + ks[q]];
+    ce[q] = coarse[mc[2] + ks[q]];
+    cn[q] = coarse[mc[3] + ks[q]];
+    cb[q] = coarse[mc[4] + kb[q]];
   }
   if (m & 8u) {
 #pragma unroll
@@ -220,10 +329,7 @@ k_prolong_w2(T* __restrict__ fine, const T* __restrict__ coarse, const int32_t* 
   }
   const T wb = T(1) / T(n * n);
 #pragma unroll
-  for (int q = 0; q < LPT; ++q) {
-    const int k = j + q * S;
-    tmp[q] = (k <= nl) ? wb * coarse[mc[4] + k] : T(0);
-  }
+  for (int q = 0; q < LPT; ++q) tmp[q] = wb * cb[q];
   for (int i = 0; i < n * n; ++i) {
     T* f = fine + ff[i * 5 + 4];
 #pragma unroll
@@ -259,18 +365,19 @@ struct lfr_intergrid {
 
 namespace {
 
-// Levels per thread of the transfer kernels: 4 for the restrictions (more
-// loads in flight per map entry), 2 for the prolongations (their strided
-// stores touch more lines per instruction as LPT grows; measured 42.6 µs
-// against 44.8 µs at 4 for the C96 to C192 W3 prolongation, 88.4 against
-// 105.7 µs for W2); LFR_TRANSFER_LPT=1, 2 or 4 forces one value.
-int transfer_lpt(bool restrict_) {
+// Levels per thread of the transfer kernels (C192 <-> C96 L70 FP32, loads
+// issued without branches): the W3 restriction 4 (19.8 µs; 20.9 at 2, 22.0
+// at 1), the W2 restriction 2 (48.7 µs; 51.6 at 4), the prolongations 2
+// (W3 32.3 µs, 34.8 at 4; W2 59.0 µs, 78.3 at 4: their strided stores touch
+// more lines per instruction as LPT grows); LFR_TRANSFER_LPT=1, 2 or 4
+// forces one value for all.
+int transfer_lpt(bool restrict_, bool w2 = false) {
   static const int forced = [] {
     const char* e = getenv("LFR_TRANSFER_LPT");
     if (e && (e[0] == '1' || e[0] == '2' || e[0] == '4')) return e[0] - '0';
     return 0;
   }();
-  return forced ? forced : (restrict_ ? 4 : 2);
+  return forced ? forced : (restrict_ && !w2 ? 4 : 2);
 }
 
 // f(std::integral_constant<int, LPT>) for lpt 1, 2 or 4
@@ -299,20 +406,39 @@ int transfer(lfr_ctx* ctx, bool restrict_, int prec, void* out, const void* in, 
   with_lpt(lpt, [&](auto L) {
     constexpr int LPT = decltype(L)::value;
     if (restrict_) {
-      if (prec == 4)
-        hipLaunchKernelGGL((k_restrict_scalar<float, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (float*)out,
-                           (const float*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch, 1.0f / (float)nch);
-      else
-        hipLaunchKernelGGL((k_restrict_scalar<double, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (double*)out,
-                           (const double*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch,
-                           1.0 / (double)nch);
+      if (prec == 4) {
+        if (nch == 4)
+          hipLaunchKernelGGL((k_restrict_scalar<float, LPT, 4>), dim3(nb), dim3(256), 0, ctx->stream, (float*)out,
+                             (const float*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch, 1.0f / (float)nch);
+        else
+          hipLaunchKernelGGL((k_restrict_scalar<float, LPT, 0>), dim3(nb), dim3(256), 0, ctx->stream, (float*)out,
+                             (const float*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch, 1.0f / (float)nch);
+      } else {
+        if (nch == 4)
+          hipLaunchKernelGGL((k_restrict_scalar<double, LPT, 4>), dim3(nb), dim3(256), 0, ctx->stream,
+                             (double*)out, (const double*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch,
+                             1.0 / (double)nch);
+        else
+          hipLaunchKernelGGL((k_restrict_scalar<double, LPT, 0>), dim3(nb), dim3(256), 0, ctx->stream,
+                             (double*)out, (const double*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch,
+                             1.0 / (double)nch);
+      }
     } else {
-      if (prec == 4)
-        hipLaunchKernelGGL((k_prolong_scalar<float, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (float*)out,
-                           (const float*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch);
-      else
-        hipLaunchKernelGGL((k_prolong_scalar<double, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (double*)out,
-                           (const double*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch);
+      if (prec == 4) {
+        if (nch == 4)
+          hipLaunchKernelGGL((k_prolong_scalar<float, LPT, 4>), dim3(nb), dim3(256), 0, ctx->stream, (float*)out,
+                             (const float*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch);
+        else
+          hipLaunchKernelGGL((k_prolong_scalar<float, LPT, 0>), dim3(nb), dim3(256), 0, ctx->stream, (float*)out,
+                             (const float*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch);
+      } else {
+        if (nch == 4)
+          hipLaunchKernelGGL((k_prolong_scalar<double, LPT, 4>), dim3(nb), dim3(256), 0, ctx->stream,
+                             (double*)out, (const double*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch);
+        else
+          hipLaunchKernelGGL((k_prolong_scalar<double, LPT, 0>), dim3(nb), dim3(256), 0, ctx->stream,
+                             (double*)out, (const double*)in, cs->map0_d, fb, cs->ncell, cs->ndf, nlev, S, nch);
+      }
     }
   });
   LFR_CHECK_LAUNCH(ctx);
@@ -446,19 +572,31 @@ extern "C" int lfr_restrict_w2(lfr_ctx* ctx, int prec, void* coarse_d, const voi
   LFR_REQUIRE(ctx, ig->coarse->ndf == 6, "%s: the intergrid map is not between W2 spaces", who);
   const lfr_space* cs = ig->coarse;
   const int nl = cs->nlayers;
-  const int lpt = transfer_lpt(true);
+  const int lpt = transfer_lpt(true, true);
   const int S = (nl + 1 + lpt - 1) / lpt;
   const unsigned nb = (unsigned)(((int64_t)cs->ncell * S + 255) / 256);
   with_lpt(lpt, [&](auto L) {
     constexpr int LPT = decltype(L)::value;
-    if (prec == 4)
-      hipLaunchKernelGGL((k_restrict_w2<float, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (float*)coarse_d,
-                         (const float*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, S, ig->nfx,
-                         ig->nfy);
-    else
-      hipLaunchKernelGGL((k_restrict_w2<double, LPT>), dim3(nb), dim3(256), 0, ctx->stream, (double*)coarse_d,
-                         (const double*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, S, ig->nfx,
-                         ig->nfy);
+    const bool two = ig->nfx == 2 && ig->nfy == 2;
+    if (prec == 4) {
+      if (two)
+        hipLaunchKernelGGL((k_restrict_w2<float, LPT, 2>), dim3(nb), dim3(256), 0, ctx->stream, (float*)coarse_d,
+                           (const float*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, S, ig->nfx,
+                           ig->nfy);
+      else
+        hipLaunchKernelGGL((k_restrict_w2<float, LPT, 0>), dim3(nb), dim3(256), 0, ctx->stream, (float*)coarse_d,
+                           (const float*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell, nl, S, ig->nfx,
+                           ig->nfy);
+    } else {
+      if (two)
+        hipLaunchKernelGGL((k_restrict_w2<double, LPT, 2>), dim3(nb), dim3(256), 0, ctx->stream,
+                           (double*)coarse_d, (const double*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell,
+                           nl, S, ig->nfx, ig->nfy);
+      else
+        hipLaunchKernelGGL((k_restrict_w2<double, LPT, 0>), dim3(nb), dim3(256), 0, ctx->stream,
+                           (double*)coarse_d, (const double*)fine_d, cs->map0_d, ig->fbase_d, ig->wmask_d, cs->ncell,
+                           nl, S, ig->nfx, ig->nfy);
+    }
   });
   LFR_CHECK_LAUNCH(ctx);
   return LFR_OK;
