@@ -800,33 +800,39 @@ k_cells(const T* __restrict__ M, const T* __restrict__ x, T* __restrict__ y,
     const int j = (int)(t - (int64_t)c * S);
     const int32_t* mo = map_out + (int64_t)c * ndf_out;
     const int32_t* mi = map_in + (int64_t)c * ndf_in;
-    const T* Mc = M + (int64_t)c * nl + j;
+    const T* Mc = M + (int64_t)c * nl;
+    // levels past the column are clamped to its top for the loads (always in
+    // bounds, so they issue without branches) and dropped at the store
+    int kk[LPT];
+    bool ok[LPT];
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+      const int k = j + q * S;
+      ok[q] = k < nl;
+      kk[q] = ok[q] ? k : nl - 1;
+    }
     T acc[LPT][NDF_OUT];
 #pragma unroll
     for (int q = 0; q < LPT; ++q)
 #pragma unroll
-      for (int i = 0; i < NDF_OUT; ++i)
-        acc[q][i] = (inc && j + q * S < nl) ? y[mo[i] + j + q * S] : T(0);
+      for (int i = 0; i < NDF_OUT; ++i) acc[q][i] = inc ? y[mo[i] + kk[q]] : T(0);
 #pragma unroll
     for (int jj = 0; jj < NDF_IN; ++jj) {
 #pragma unroll
       for (int q = 0; q < LPT; ++q) {
-        const int k = j + q * S;
-        if (k < nl) {
-          const T xv = x[mi[jj] + k];
+        const T xv = x[mi[jj] + kk[q]];
 #pragma unroll
-          for (int i = 0; i < NDF_OUT; ++i) {
-            const T p = Mc[q * S + n3 * ((int64_t)i + (int64_t)NDF_OUT * jj)] * xv;
-            acc[q][i] = acc[q][i] + p;
-          }
+        for (int i = 0; i < NDF_OUT; ++i) {
+          const T p = Mc[kk[q] + n3 * ((int64_t)i + (int64_t)NDF_OUT * jj)] * xv;
+          acc[q][i] = acc[q][i] + p;
         }
       }
     }
 #pragma unroll
     for (int q = 0; q < LPT; ++q)
-      if (j + q * S < nl)
+      if (ok[q])
 #pragma unroll
-        for (int i = 0; i < NDF_OUT; ++i) y[mo[i] + j + q * S] = acc[q][i];
+        for (int i = 0; i < NDF_OUT; ++i) y[mo[i] + kk[q]] = acc[q][i];
   } else {
     if (t >= (int64_t)ncell * nl) return;
     const int c = (int)(t / nl);
