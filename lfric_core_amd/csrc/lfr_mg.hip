@@ -174,32 +174,60 @@ __global__ void __launch_bounds__(256) k_mg_restrict(T* __restrict__ bc, T* __re
   const int C = (int)(t >> lgR), r = (int)(t & (R - 1));
   const int64_t pf = (int64_t)ncell_f * R, pc = (int64_t)ncell_c * R;
   const T denom = T(1) / T(4);
-  int ch[4];
-  // each child's x at levels k - 1 and k, carried from level to level (one
-  // new load per child and level instead of three)
-  T xmv[4], x0v[4];
+  // in-plane offsets of each child and of its four neighbours, loaded once
+  // (ncell * R < 2^31 is checked at set-up)
+  int ch[4], of[4], on[4][4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     ch[q] = children[4 * C + q];
-    xmv[q] = T(0);
-    x0v[q] = xf[(int64_t)ch[q] * R + r];
+    of[q] = ch[q] * R + r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) on[q][j] = nb[4 * ch[q] + j] * R + r;
   }
-  auto level = [&](int k) {
+  // one level's operands for the four children; two sets in turn, so that
+  // the loads of level k + 1 are in flight while level k is summed.  The
+  // load of x above the top layer is clamped to the top layer (in bounds)
+  // and its value replaced by 0.
+  struct Lv {
+    T xp[4], bf[4], h[4][4];
+    Band4<T> bv[4];
+  };
+  auto load = [&](int k, Lv& v) {
+    const int kp = min(k + 1, nl - 1);
+    const T* xk = xf + (int64_t)k * pf;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      v.xp[q] = xf[(int64_t)kp * pf + of[q]];
+      v.bv[q] = adui[(int64_t)k * ncell_f + ch[q]];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v.h[q][j] = xk[on[q][j]];
+      v.bf[q] = bf[(int64_t)k * pf + of[q]];
+    }
+  };
+  // each child's x at levels k - 1 and k, carried from level to level
+  T xmv[4], x0v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    xmv[q] = T(0);
+    x0v[q] = xf[of[q]];
+  }
+  auto level = [&](int k, const Lv& v) {
     T acc = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int f = ch[q];
-      const int64_t i = (int64_t)k * pf + (int64_t)f * R + r;
-      const int64_t ik = (int64_t)k * ncell_f + f;
       const T x0 = x0v[q];
       const T xm = xmv[q];
-      const T xp = k + 1 < nl ? xf[i + pf] : T(0);
-      const Band4<T> bv = adui[ik];
+      const T xp = k + 1 < nl ? v.xp[q] : T(0);
+      const Band4<T> bv = v.bv[q];
       T ax = bv.a * xm;
       ax = ax + bv.d * x0;
       ax = ax + bv.u * xp;
-      ax = ax + horiz(xf + (int64_t)k * pf, nb, f, R, r, x0, sigma);
-      const T res = bf[i] - ax;
+      const T s4 = v.h[q][0] + v.h[q][1];
+      const T s5 = s4 + v.h[q][2];
+      const T sn = s5 + v.h[q][3];
+      const T four = (T)4 * x0;
+      ax = ax + sigma * (four - sn);
+      const T res = v.bf[q] - ax;
       acc = acc + res * denom;
       xmv[q] = x0;
       x0v[q] = xp;
@@ -207,14 +235,88 @@ __global__ void __launch_bounds__(256) k_mg_restrict(T* __restrict__ bc, T* __re
     bc[(int64_t)k * pc + t] = acc;
     xc[(int64_t)k * pc + t] = T(0);
   };
-  // two levels per loop trip in FP32 (measured 18 % faster); FP64 keeps one
-  // (two take all 256 VGPRs and one wave per SIMD, measured 3-5 % slower)
-  if constexpr (sizeof(T) == 4) {
-#pragma unroll 2
-    for (int k = 0; k < nl; ++k) level(k);
-  } else {
-#pragma unroll 1
-    for (int k = 0; k < nl; ++k) level(k);
+  Lv A, B;
+  load(0, A);
+  for (int k = 0; k < nl; k += 2) {
+    load(min(k + 1, nl - 1), B);
+    level(k, A);
+    load(min(k + 2, nl - 1), A);
+    if (k + 1 < nl) level(k + 1, B);
+  }
+}
+
+// The same restriction for a fixed number of layers, unrolled: the loads of
+// level k + PF are issued before level k is summed (PF + 1 operand sets in
+// registers, scheduling barriers keep that order), which a loop over a
+// run-time layer count does not achieve (the compiler waits for every load
+// before each loop trip's stores).  Same operations in the same order.
+template <typename T, int NL, int PF>
+__global__ void __launch_bounds__(256) k_mg_restrict_nl(T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ xf,
+                                                        const T* __restrict__ bf, const Band4<T>* __restrict__ adui,
+                                                        const int32_t* __restrict__ nb,
+                                                        const int32_t* __restrict__ children, int ncell_c, int ncell_f,
+                                                        int lgR, T sigma, unsigned nb_pad) {
+  const int R = 1 << lgR;
+  const int64_t t = (int64_t)lfr_xcd_block(blockIdx.x, nb_pad) * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ncell_c * R) return;
+  const int C = (int)(t >> lgR), r = (int)(t & (R - 1));
+  const int64_t pf = (int64_t)ncell_f * R, pc = (int64_t)ncell_c * R;
+  const T denom = T(1) / T(4);
+  int ch[4], of[4], on[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ch[q] = children[4 * C + q];
+    of[q] = ch[q] * R + r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) on[q][j] = nb[4 * ch[q] + j] * R + r;
+  }
+  T xp[PF + 1][4], bfv[PF + 1][4], h[PF + 1][4][4];
+  Band4<T> bv[PF + 1][4];
+  auto load = [&](int k, int s) {
+    const T* xk = xf + (int64_t)k * pf;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (k + 1 < NL) xp[s][q] = xf[(int64_t)(k + 1) * pf + of[q]];
+      bv[s][q] = adui[(int64_t)k * ncell_f + ch[q]];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) h[s][q][j] = xk[on[q][j]];
+      bfv[s][q] = bf[(int64_t)k * pf + of[q]];
+    }
+  };
+  T xmv[4], x0v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    xmv[q] = T(0);
+    x0v[q] = xf[of[q]];
+  }
+#pragma unroll
+  for (int k = 0; k < PF && k < NL; ++k) load(k, k);
+#pragma clang loop unroll(full)
+  for (int k = 0; k < NL; ++k) {
+    if (k + PF < NL) load(k + PF, (k + PF) % (PF + 1));
+    const int s = k % (PF + 1);
+    T acc = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const T x0 = x0v[q];
+      const T xm = xmv[q];
+      const T xpv = k + 1 < NL ? xp[s][q] : T(0);
+      T ax = bv[s][q].a * xm;
+      ax = ax + bv[s][q].d * x0;
+      ax = ax + bv[s][q].u * xpv;
+      const T s4 = h[s][q][0] + h[s][q][1];
+      const T s5 = s4 + h[s][q][2];
+      const T sn = s5 + h[s][q][3];
+      const T four = (T)4 * x0;
+      ax = ax + sigma * (four - sn);
+      const T res = bfv[s][q] - ax;
+      acc = acc + res * denom;
+      xmv[q] = x0;
+      x0v[q] = xpv;
+    }
+    bc[(int64_t)k * pc + t] = acc;
+    xc[(int64_t)k * pc + t] = T(0);
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -406,13 +508,32 @@ __global__ void __launch_bounds__(256) k_mg_smooth_nl(T* __restrict__ xo, const 
     }
     if ((q + 1) % PF == 0) __builtin_amdgcn_sched_barrier(0);
   }
+  // Backward sweep.  cp (and x when it is not kept) of a chunk of CH layers
+  // is loaded one chunk ahead of its use: loaded just in time, each layer's
+  // load would wait for the store issued before it (the vector memory
+  // counter covers both), one memory round trip per layer.
+  constexpr int CH = sizeof(T) == 4 ? 8 : 4;
+  T cpv[NL], xv[NL];
+  auto load_chunk = [&](int hi, int lo) {   // layers hi down to lo (inclusive, clipped to >= 0)
+#pragma unroll
+    for (int kk = hi; kk >= lo && kk >= 0; --kk) {
+      if (kk < NL - 1) cpv[kk] = cp[(int64_t)kk * ncell + c];
+      if (!KX) xv[kk] = xi[(int64_t)kk * plane + t];
+    }
+  };
+  load_chunk(NL - 1, NL - CH);
   T dnext = 0;
 #pragma unroll
   for (int k = NL - 1; k >= 0; --k) {
+    if ((NL - 1 - k) % CH == 0) {
+      // this chunk is layers k .. k - CH + 1; fetch the next one
+      load_chunk(k - CH, k - 2 * CH + 1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     const int64_t i = (int64_t)k * plane + t;
     T d = dp[k];
-    if (k < NL - 1) d = d - cp[(int64_t)k * ncell + c] * dnext;
-    xo[i] = (KX ? lx[k] : xi[i]) + omega * d;
+    if (k < NL - 1) d = d - cpv[k] * dnext;
+    xo[i] = (KX ? lx[k] : xv[k]) + omega * d;
     dnext = d;
   }
 }
@@ -477,9 +598,14 @@ void vcycle(hipStream_t s, lfr_mg* mg, int l) {
     const int bs = L.block_other;
     const int64_t n = (int64_t)C.ncell * mg->nrhs;
     const unsigned nbp = ((unsigned)((n + bs - 1) / bs) + 7u) & ~7u;
-    hipLaunchKernelGGL(k_mg_restrict<T>, dim3(nbp), dim3(bs), 0, s, (T*)C.b_d, (T*)C.x_d, (const T*)L.x_d,
-                       (const T*)L.b_d, (const Band4<T>*)L.adui_d, L.cell_next_d, C.children_d, C.ncell, L.ncell,
-                       mg->nl, lgR, (T)L.sigma, nbp);
+    if (mg->nl == 70)
+      hipLaunchKernelGGL((k_mg_restrict_nl<T, 70, sizeof(T) == 4 ? 2 : 1>), dim3(nbp), dim3(bs), 0, s, (T*)C.b_d,
+                         (T*)C.x_d, (const T*)L.x_d, (const T*)L.b_d, (const Band4<T>*)L.adui_d, L.cell_next_d,
+                         C.children_d, C.ncell, L.ncell, lgR, (T)L.sigma, nbp);
+    else
+      hipLaunchKernelGGL(k_mg_restrict<T>, dim3(nbp), dim3(bs), 0, s, (T*)C.b_d, (T*)C.x_d, (const T*)L.x_d,
+                         (const T*)L.b_d, (const Band4<T>*)L.adui_d, L.cell_next_d, C.children_d, C.ncell, L.ncell,
+                         mg->nl, lgR, (T)L.sigma, nbp);
   }
   vcycle<T>(s, mg, l + 1);
   {
@@ -541,6 +667,9 @@ int lfr_mg_create(lfr_ctx* ctx, int prec, int nlev, int nlayers, int nrhs, const
                        nrhs >= 1 && nrhs <= kMaxRhs && (nrhs & (nrhs - 1)) == 0,
               "lfr_mg_create: bad sizes (prec %d nlev %d nlayers %d nrhs %d; nrhs a power of 2 <= %d)", prec,
               nlev, nlayers, nrhs, kMaxRhs);
+  for (int l = 0; l < nlev; ++l)
+    LFR_REQUIRE(ctx, ncell[l] > 0 && (int64_t)ncell[l] * nrhs < ((int64_t)1 << 31),
+                "lfr_mg_create: level %d has %d cells (ncell * nrhs must be below 2^31)", l, ncell[l]);
   LFR_REQUIRE(ctx, nu1 % 2 == 0 && nu2 % 2 == 0 && ncoarse % 2 == 0 && ncoarse > 0,
               "lfr_mg_create: smoothing counts must be even (ping-pong sweeps)");
   lfr_mg* mg = new lfr_mg();
