@@ -245,77 +245,79 @@ __global__ void __launch_bounds__(256) k_mg_restrict(T* __restrict__ bc, T* __re
   }
 }
 
-// The same restriction for a fixed number of layers, unrolled: the loads of
-// level k + PF are issued before level k is summed (PF + 1 operand sets in
-// registers, scheduling barriers keep that order), which a loop over a
-// run-time layer count does not achieve (the compiler waits for every load
-// before each loop trip's stores).  Same operations in the same order.
+// The same restriction with one lane per fine child and right-hand side
+// (4R consecutive lanes per coarse cell, R <= 16): each lane forms its
+// child's scaled residual, the child-0 lane gathers the other three by lane
+// shuffles and adds them in child order, acc = (((0 + v0) + v1) + v2) + v3,
+// the statement order of the cell-at-a-time form, so the bits are the same.
+// Four times the lanes of that form, each with a quarter of its loads; for
+// a fixed layer count, unrolled with the loads PF levels ahead.
 template <typename T, int NL, int PF>
-__global__ void __launch_bounds__(256) k_mg_restrict_nl(T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ xf,
-                                                        const T* __restrict__ bf, const Band4<T>* __restrict__ adui,
-                                                        const int32_t* __restrict__ nb,
-                                                        const int32_t* __restrict__ children, int ncell_c, int ncell_f,
-                                                        int lgR, T sigma, unsigned nb_pad) {
+__global__ void __launch_bounds__(256) k_mg_restrict_q(T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ xf,
+                                                       const T* __restrict__ bf, const Band4<T>* __restrict__ adui,
+                                                       const int32_t* __restrict__ nb,
+                                                       const int32_t* __restrict__ children, int ncell_c, int ncell_f,
+                                                       int lgR, T sigma, unsigned nb_pad) {
   const int R = 1 << lgR;
-  const int64_t t = (int64_t)lfr_xcd_block(blockIdx.x, nb_pad) * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)ncell_c * R) return;
-  const int C = (int)(t >> lgR), r = (int)(t & (R - 1));
+  const int64_t g = (int64_t)lfr_xcd_block(blockIdx.x, nb_pad) * blockDim.x + threadIdx.x;
+  // every lane stays for the shuffles; lanes past the last coarse cell work
+  // on cell 0 and do not store
+  const bool valid = g < (int64_t)ncell_c * 4 * R;
+  const int64_t gg = valid ? g : 0;
+  const int C = (int)(gg >> (lgR + 2));
+  const int q = (int)(gg >> lgR) & 3;
+  const int r = (int)gg & (R - 1);
   const int64_t pf = (int64_t)ncell_f * R, pc = (int64_t)ncell_c * R;
   const T denom = T(1) / T(4);
-  int ch[4], of[4], on[4][4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    ch[q] = children[4 * C + q];
-    of[q] = ch[q] * R + r;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) on[q][j] = nb[4 * ch[q] + j] * R + r;
-  }
-  T xp[PF + 1][4], bfv[PF + 1][4], h[PF + 1][4][4];
-  Band4<T> bv[PF + 1][4];
+  const int f = children[4 * C + q];
+  const int of = f * R + r;   // ncell * R < 2^31 is checked at set-up
+  const int o0 = nb[4 * f + 0] * R + r, o1 = nb[4 * f + 1] * R + r;
+  const int o2 = nb[4 * f + 2] * R + r, o3 = nb[4 * f + 3] * R + r;
+  T xp[PF + 1], bfv[PF + 1], h0[PF + 1], h1[PF + 1], h2[PF + 1], h3[PF + 1];
+  Band4<T> bv[PF + 1];
   auto load = [&](int k, int s) {
     const T* xk = xf + (int64_t)k * pf;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (k + 1 < NL) xp[s][q] = xf[(int64_t)(k + 1) * pf + of[q]];
-      bv[s][q] = adui[(int64_t)k * ncell_f + ch[q]];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) h[s][q][j] = xk[on[q][j]];
-      bfv[s][q] = bf[(int64_t)k * pf + of[q]];
-    }
+    if (k + 1 < NL) xp[s] = xf[(int64_t)(k + 1) * pf + of];
+    bv[s] = adui[(int64_t)k * ncell_f + f];
+    h0[s] = xk[o0];
+    h1[s] = xk[o1];
+    h2[s] = xk[o2];
+    h3[s] = xk[o3];
+    bfv[s] = bf[(int64_t)k * pf + of];
   };
-  T xmv[4], x0v[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    xmv[q] = T(0);
-    x0v[q] = xf[of[q]];
-  }
+  T xm = T(0), x0 = xf[of];
 #pragma unroll
   for (int k = 0; k < PF && k < NL; ++k) load(k, k);
 #pragma clang loop unroll(full)
   for (int k = 0; k < NL; ++k) {
     if (k + PF < NL) load(k + PF, (k + PF) % (PF + 1));
     const int s = k % (PF + 1);
-    T acc = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const T x0 = x0v[q];
-      const T xm = xmv[q];
-      const T xpv = k + 1 < NL ? xp[s][q] : T(0);
-      T ax = bv[s][q].a * xm;
-      ax = ax + bv[s][q].d * x0;
-      ax = ax + bv[s][q].u * xpv;
-      const T s4 = h[s][q][0] + h[s][q][1];
-      const T s5 = s4 + h[s][q][2];
-      const T sn = s5 + h[s][q][3];
-      const T four = (T)4 * x0;
-      ax = ax + sigma * (four - sn);
-      const T res = bfv[s][q] - ax;
-      acc = acc + res * denom;
-      xmv[q] = x0;
-      x0v[q] = xpv;
+    const T xpv = k + 1 < NL ? xp[s] : T(0);
+    T ax = bv[s].a * xm;
+    ax = ax + bv[s].d * x0;
+    ax = ax + bv[s].u * xpv;
+    const T s4 = h0[s] + h1[s];
+    const T s5 = s4 + h2[s];
+    const T sn = s5 + h3[s];
+    const T four = (T)4 * x0;
+    ax = ax + sigma * (four - sn);
+    const T res = bfv[s] - ax;
+    const T v = res * denom;
+    const T v1 = __shfl_down(v, R);
+    const T v2 = __shfl_down(v, 2 * R);
+    const T v3 = __shfl_down(v, 3 * R);
+    if (valid && q == 0) {
+      T acc = 0;
+      acc = acc + v;
+      acc = acc + v1;
+      acc = acc + v2;
+      acc = acc + v3;
+      const int64_t o = (int64_t)k * pc + (int64_t)C * R + r;
+      bc[o] = acc;
+      xc[o] = T(0);
     }
-    bc[(int64_t)k * pc + t] = acc;
-    xc[(int64_t)k * pc + t] = T(0);
+    xm = x0;
+    x0 = xpv;
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -598,11 +600,20 @@ void vcycle(hipStream_t s, lfr_mg* mg, int l) {
     const int bs = L.block_other;
     const int64_t n = (int64_t)C.ncell * mg->nrhs;
     const unsigned nbp = ((unsigned)((n + bs - 1) / bs) + 7u) & ~7u;
-    if (mg->nl == 70)
-      hipLaunchKernelGGL((k_mg_restrict_nl<T, 70, sizeof(T) == 4 ? 2 : 1>), dim3(nbp), dim3(bs), 0, s, (T*)C.b_d,
-                         (T*)C.x_d, (const T*)L.x_d, (const T*)L.b_d, (const Band4<T>*)L.adui_d, L.cell_next_d,
-                         C.children_d, C.ncell, L.ncell, lgR, (T)L.sigma, nbp);
-    else
+    // 70 layers and at most 16 right-hand sides: one lane per child (C192
+    // FP32 V-cycle 3.03 against 3.05 ms, FP64 7.36 against 7.43 ms for the
+    // cell-at-a-time form unrolled the same way); LFR_MG_RESTRICT=0 selects
+    // the cell-at-a-time loop
+    static const int rq = [] {
+      const char* e = getenv("LFR_MG_RESTRICT");
+      return e ? atoi(e) : 1;
+    }();
+    if (mg->nl == 70 && rq == 1 && mg->nrhs <= 16) {
+      const unsigned nbq = ((unsigned)((4 * n + bs - 1) / bs) + 7u) & ~7u;
+      hipLaunchKernelGGL((k_mg_restrict_q<T, 70, 4>), dim3(nbq), dim3(bs), 0, s, (T*)C.b_d, (T*)C.x_d,
+                         (const T*)L.x_d, (const T*)L.b_d, (const Band4<T>*)L.adui_d, L.cell_next_d, C.children_d,
+                         C.ncell, L.ncell, lgR, (T)L.sigma, nbq);
+    } else
       hipLaunchKernelGGL(k_mg_restrict<T>, dim3(nbp), dim3(bs), 0, s, (T*)C.b_d, (T*)C.x_d, (const T*)L.x_d,
                          (const T*)L.b_d, (const Band4<T>*)L.adui_d, L.cell_next_d, C.children_d, C.ncell, L.ncell,
                          mg->nl, lgR, (T)L.sigma, nbp);
