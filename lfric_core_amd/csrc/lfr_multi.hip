@@ -406,11 +406,11 @@ k_multi16(float* __restrict__ y, const float* __restrict__ x, const void* __rest
 // offsets and two steps of operand prefetch (ring of three), as k_multi16;
 // the matrix (block-major, read once) with the non-temporal hint, so that
 // it is evicted from L2 before the x lines other cells read again.
-template <bool BF, int PF, int NTL, bool TWO, bool PAIR>
+template <bool BF, int PF, int NTL, bool TWO, bool PAIR, bool INC>
 __device__ __forceinline__ void own_wave(float* __restrict__ y, __amdgpu_buffer_rsrc_t rx, __amdgpu_buffer_rsrc_t rm,
                                          const int32_t* __restrict__ mt, const int32_t* __restrict__ mfa,
                                          const int32_t* __restrict__ mfb, const int (&tiles)[NTL],
-                                         const int (&sinfo)[NTL], int nl, int overwrite) {
+                                         const int (&sinfo)[NTL], int nl) {
   constexpr int NTP = 36, JQ = 9, R = PF == 2 ? 3 : 2;
   constexpr int ME = BF ? 2 : 4;
   static_assert(!TWO || NTL == 2, "a two-source wave runs one own and one neighbour tile");
@@ -487,19 +487,38 @@ __device__ __forceinline__ void own_wave(float* __restrict__ y, __amdgpu_buffer_
 
     // epilogue: lane (b, q) of a tile holds its row q at layer k0 + b,
     // acc[i][v][m] = rhs 4m + v
+    // increment form (INC): every lane loads the y it adds to before any
+    // store, at its layer clamped to the top (in bounds: every slot names a
+    // real row).  Loaded under the per-lane store condition, each load would
+    // wait for all earlier stores, one memory round trip per row.
     const int kk = k0 + b;
     if constexpr (TWO) {
+      lfr_v4f yo[4];
+      if constexpr (INC) {
+        const float* yq = y + ((int64_t)mt[sinfo[0] & 0xffff] + min(kk, nl - 1)) * kRhs;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) yo[m] = *(const lfr_v4f*)(yq + 4 * m);
+      }
       if (kk < nl) {
         float* yp = y + ((int64_t)mt[sinfo[0] & 0xffff] + kk) * kRhs;
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
           lfr_v4f o = {acc[0][0][m] + acc[1][0][m], acc[0][1][m] + acc[1][1][m], acc[0][2][m] + acc[1][2][m],
                        acc[0][3][m] + acc[1][3][m]};
-          if (!overwrite) o = o + *(const lfr_v4f*)(yp + 4 * m);
+          if constexpr (INC) o = o + yo[m];
           *(lfr_v4f*)(yp + 4 * m) = o;
         }
       }
     } else {
+      lfr_v4f yo[INC ? NTL : 1][4];
+      if constexpr (INC) {
+#pragma unroll
+        for (int i = 0; i < NTL; ++i) {
+          const float* yq = y + ((int64_t)mt[sinfo[i] & 0xffff] + min(kk, nl - 1)) * kRhs;
+#pragma unroll
+          for (int m = 0; m < 4; ++m) yo[i][m] = *(const lfr_v4f*)(yq + 4 * m);
+        }
+      }
 #pragma unroll
       for (int i = 0; i < NTL; ++i) {
         const int kind = sinfo[i] >> 16;
@@ -528,7 +547,7 @@ __device__ __forceinline__ void own_wave(float* __restrict__ y, __amdgpu_buffer_
 #pragma unroll
           for (int m = 0; m < 4; ++m) {
             lfr_v4f o = {val[0][m], val[1][m], val[2][m], val[3][m]};
-            if (!overwrite) o = o + *(const lfr_v4f*)(yp + 4 * m);
+            if constexpr (INC) o = o + yo[INC ? i : 0][m];
             *(lfr_v4f*)(yp + 4 * m) = o;
           }
         }
@@ -543,12 +562,12 @@ __device__ __forceinline__ void own_wave(float* __restrict__ y, __amdgpu_buffer_
 
 // PF = 1 keeps fewer operands in flight per wave and fits in 168 VGPRs
 // (WPE = 3 waves share each SIMD); PF = 2 runs two waves per SIMD.
-template <bool BF, int PF, int WPE>
+template <bool BF, int PF, int WPE, bool INC>
 __global__ void __launch_bounds__(256, WPE)
 k_multi16_own(float* __restrict__ y, const float* __restrict__ x, const void* __restrict__ mp_,
               const int32_t* __restrict__ cells, const int32_t* __restrict__ map_to,
               const int32_t* __restrict__ map_from, const int32_t* __restrict__ slotc,
-              const int32_t* __restrict__ nbr, int nl, int overwrite, uint32_t x_bytes) {
+              const int32_t* __restrict__ nbr, int nl, uint32_t x_bytes) {
   constexpr int NTP = 36, NF = 36;
   constexpr int ME = BF ? 2 : 4;
   const int c = cells[blockIdx.x];
@@ -564,25 +583,33 @@ k_multi16_own(float* __restrict__ y, const float* __restrict__ x, const void* __
   if (w == 0) {
     const int tiles[3] = {0, 1, 2};
     const int si[3] = {sc[q], sc[4 + q], sc[8 + q]};
-    own_wave<BF, PF, 3, false, true>(y, rx, rm, mt, mf, mf, tiles, si, nl, overwrite);
+    own_wave<BF, PF, 3, false, true, INC>(y, rx, rm, mt, mf, mf, tiles, si, nl);
   } else if (w == 1) {
     const int tiles[2] = {3, 4};
     const int si[2] = {sc[12 + q], sc[16 + q]};
-    own_wave<BF, PF, 2, false, false>(y, rx, rm, mt, mf, mf, tiles, si, nl, overwrite);
+    own_wave<BF, PF, 2, false, false, INC>(y, rx, rm, mt, mf, mf, tiles, si, nl);
   } else {
     const int f = w - 2;
     const int nb = nbr[2 * c + f];
     const int tiles[2] = {5 + f, 7 + f};
     const int si[2] = {sc[20 + 4 * f + q], 0};
-    own_wave<BF, PF, 2, true, false>(y, rx, rm, mt, mf, map_from + (int64_t)nb * NF, tiles, si, nl, overwrite);
+    own_wave<BF, PF, 2, true, false, INC>(y, rx, rm, mt, mf, map_from + (int64_t)nb * NF, tiles, si, nl);
   }
 }
 
+// The increment form always runs the two-step variant: it has the register
+// room to load every tile's y before the first store (the one-step variant
+// would spill); all variants give the same bits.
 template <bool BF, int PF, int WPE>
 void launch_multi_own(hipStream_t s, lfr_multi* h, float* y, const float* x, int overwrite) {
-  hipLaunchKernelGGL((k_multi16_own<BF, PF, WPE>), dim3(h->ncell), dim3(256), 0, s, y, x, (const void*)h->mp_d, h->cells_d,
-                     h->to->map0_d, h->from->map0_d, h->slotc_d, h->nbr_d, h->nl, overwrite,
-                     (uint32_t)((int64_t)h->from->undf * kRhs * 4));
+  const uint32_t xb = (uint32_t)((int64_t)h->from->undf * kRhs * 4);
+  if (overwrite)
+    hipLaunchKernelGGL((k_multi16_own<BF, PF, WPE, false>), dim3(h->ncell), dim3(256), 0, s, y, x,
+                       (const void*)h->mp_d, h->cells_d, h->to->map0_d, h->from->map0_d, h->slotc_d, h->nbr_d, h->nl,
+                       xb);
+  else
+    hipLaunchKernelGGL((k_multi16_own<BF, 2, 2, true>), dim3(h->ncell), dim3(256), 0, s, y, x, (const void*)h->mp_d,
+                       h->cells_d, h->to->map0_d, h->from->map0_d, h->slotc_d, h->nbr_d, h->nl, xb);
 }
 
 template <bool BF, int NTP, int NF, int PIG, int IGW, int UNR = 1, int PF = 1, bool BUF = false>
@@ -995,7 +1022,7 @@ int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void*
   if (h->owner) {
     // prefetch depth chosen by the autotuner on the first overwrite apply of
     // a shape (the overwrite form can be repeated on the caller's fields);
-    // an increment apply uses a stored choice or the default
+    // an increment apply always runs the two-step kernel (launch_multi_own)
     // variants (prefetch steps, waves per SIMD): 0 (2, 2), 1 (1, 3)
     const auto launch = [&](int c) -> int {
       switch (c) {
@@ -1009,13 +1036,7 @@ int lfr_matrix_vector_multi(lfr_ctx* ctx, lfr_multi* h, void* lhs_d, const void*
     };
     char key[96];
     snprintf(key, sizeof key, "multi16 owner prec=%d nl=%d ncell=%d", h->mprec, h->nl, h->ncell);
-    int c = 0;
-    if (overwrite) {
-      c = lfr_tune_pick(ctx, key, 2, 0, launch);
-    } else {
-      auto it = ctx->tune.find(key);
-      if (it != ctx->tune.end() && it->second >= 0 && it->second < 2) c = it->second;
-    }
+    const int c = overwrite ? lfr_tune_pick(ctx, key, 2, 0, launch) : 0;
     (void)launch(c);
     LFR_CHECK_LAUNCH(ctx);
     return LFR_OK;
