@@ -559,6 +559,7 @@ struct lfr_cma {
   int32_t* banded_from_d = nullptr; // column_banded_dofmap_from(ndf_from, nlayers)
   int32_t* row_ptr_d = nullptr;     // per row i: its (k, df1) entries, in (k, df1) order
   int32_t* row_ent_d = nullptr;     // k * ndf_to + df1
+  int32_t* asm_tab_d = nullptr;     // per row: count, then (k, df1 + ndf_to df2, band slot) per contribution
   int32_t* app_tab_d = nullptr;     // per row: output offset, then bw input offsets (-1 outside [lo, hi])
 };
 
@@ -606,6 +607,55 @@ k_cma_asm(T* __restrict__ cm, const T* __restrict__ ls, const int32_t* __restric
   for (int b = 0; b < BW; ++b) out[b] = acc[b];
 }
 
+// The same assembly from a per-row table (built at creation) of at most
+// kAsmMc contributions in the reference loop order (entries (k, df1), then
+// df2): one table load, then every stencil load at once (unused slots read
+// the first contribution's value and are not added), then the sums in table
+// order.  The loop form above waits for an index load before each stencil
+// load.
+constexpr int kAsmMc = 4;
+constexpr int kAsmStride = 16;   // 1 + 3 kAsmMc, padded to whole int4
+template <typename T, int BW>
+__global__ void __launch_bounds__(256)
+k_cma_asm_tab(T* __restrict__ cm, const T* __restrict__ ls, const int32_t* __restrict__ tab, int ncell, int nl,
+              int nrow) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ncell * nrow) return;
+  const int cell = (int)(t / nrow), i0 = (int)(t - (int64_t)cell * nrow);
+  const int64_t n3 = (int64_t)ncell * nl;
+  int r[kAsmStride];
+  const int4* r4 = reinterpret_cast<const int4*>(tab + (int64_t)i0 * kAsmStride);
+#pragma unroll
+  for (int q = 0; q < kAsmStride / 4; ++q) {
+    const int4 v = r4[q];
+    r[4 * q] = v.x;
+    r[4 * q + 1] = v.y;
+    r[4 * q + 2] = v.z;
+    r[4 * q + 3] = v.w;
+  }
+  const int cnt = r[0];
+  T v[kAsmMc];
+#pragma unroll
+  for (int m = 0; m < kAsmMc; ++m) {
+    const int mm = m < cnt ? m : 0;
+    v[m] = ls[(int64_t)cell * nl + r[1 + 3 * mm] + n3 * r[2 + 3 * mm]];
+  }
+  T acc[BW];
+#pragma unroll
+  for (int b = 0; b < BW; ++b) acc[b] = T(0);
+#pragma unroll
+  for (int m = 0; m < kAsmMc; ++m)
+    if (m < cnt) {
+      const int b = r[3 + 3 * m];
+#pragma unroll
+      for (int q = 0; q < BW; ++q)
+        if (q == b) acc[q] = acc[q] + v[m];
+    }
+  T* out = cm + ((int64_t)cell * nrow + i0) * BW;
+#pragma unroll
+  for (int b = 0; b < BW; ++b) out[b] = acc[b];
+}
+
 // columnwise_op_app_kernel_code with per-row tables: row i0 of every column
 // adds its band entries b = j - jm for the columns j in [max(1, jm),
 // min(ncol, jp)] in ascending order, as the reference loop does; the band
@@ -636,13 +686,16 @@ k_cma_app_tab(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict_
       mu[u] = (int64_t)map_to[(int64_t)cell * ndf_to] + r[0];
       const int64_t bf = map_from[(int64_t)cell * ndf_from];
       const T* c = cm + ((int64_t)cell * nrow + i0) * BW;
-      acc[u] = ok[u] ? lhs[mu[u]] : T(0);
+      // every load is unconditional at an in-bounds address (item 0 past
+      // the end, input offset 0 outside the loop range); the values that
+      // the reference loop does not reach are never added.  Conditional
+      // loads would each wait for all loads before them.
+      acc[u] = lhs[mu[u]];
 #pragma unroll
       for (int b = 0; b < BW; ++b) {
         xo[u][b] = r[1 + b];
-        const bool v = ok[u] && xo[u][b] >= 0;
-        cv[u][b] = v ? c[b] : T(0);
-        xv[u][b] = v ? x[bf + xo[u][b]] : T(0);
+        cv[u][b] = c[b];
+        xv[u][b] = x[bf + max(xo[u][b], 0)];
       }
     }
 #pragma unroll
@@ -688,7 +741,7 @@ k_cma_app(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm
 void cma_free(lfr_cma* h) {
   if (!h) return;
   for (void* p : {(void*)h->ind_to_d, (void*)h->ind_from_d, (void*)h->banded_from_d, (void*)h->row_ptr_d,
-                  (void*)h->row_ent_d, (void*)h->app_tab_d})
+                  (void*)h->row_ent_d, (void*)h->app_tab_d, (void*)h->asm_tab_d})
     if (p) (void)hipFree(p);
   delete h;
 }
@@ -758,6 +811,34 @@ int lfr_cma_create(lfr_ctx* ctx, int nlayers, int nrow, int ncol, int bandwidth,
     cma_free(h);
     return lfr_set_error(ctx, LFR_ERR_HIP, "lfr_cma_create: %s", hipGetErrorString(e));
   }
+  // per-row table of the assembly (k_cma_asm_tab) when no row has more than
+  // kAsmMc contributions; unused slots name layer 0, plane 0 (in bounds)
+  {
+    bool fits = true;
+    for (int i0 = 0; i0 < nrow && fits; ++i0) fits = (int)rows[i0].size() * ndf_from <= kAsmMc;
+    if (fits) {
+      std::vector<int32_t> at((size_t)nrow * kAsmStride, 0);
+      for (int i0 = 0; i0 < nrow; ++i0) {
+        int32_t* r = &at[(size_t)i0 * kAsmStride];
+        const int jm = (int)std::ceil((double)(alpha * (i0 + 1) - gamma_p) / (double)beta);
+        int m = 0;
+        for (int ent_v : rows[i0]) {
+          const int k = ent_v / ndf_to, df1 = ent_v - k * ndf_to;
+          for (int df2 = 0; df2 < ndf_from; ++df2, ++m) {
+            r[1 + 3 * m] = k;
+            r[2 + 3 * m] = df1 + ndf_to * df2;
+            r[3 + 3 * m] = banded_from_h[df2 + ndf_from * k] - jm;
+          }
+        }
+        r[0] = m;
+      }
+      if (hipMalloc(&h->asm_tab_d, at.size() * 4) != hipSuccess ||
+          hipMemcpy(h->asm_tab_d, at.data(), at.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        cma_free(h);
+        return lfr_set_error(ctx, LFR_ERR_NOMEM, "lfr_cma_create: assembly table allocation failed");
+      }
+    }
+  }
   // per-row table of the apply: band limits formed as the reference does
   // (exact for these small integers), usable when every column of a row's
   // loop range falls inside the band
@@ -804,10 +885,14 @@ int lfr_columnwise_op_asm(lfr_ctx* ctx, int prec, const lfr_cma* h, const void* 
   const int64_t n = (int64_t)ncell_2d * h->nrow;
   const unsigned g = (unsigned)((n + 255) / 256);
   hipStream_t s = ctx->stream;
-#define LFR_ASM(T, BWV)                                                                                 \
-  hipLaunchKernelGGL((k_cma_asm<T, BWV>), dim3(g), dim3(256), 0, s, (T*)cm_d, (const T*)local_stencil_d, \
-                     h->row_ptr_d, h->row_ent_d, h->banded_from_d, ncell_2d, h->nlayers, h->nrow, h->ndf_to, \
-                     h->ndf_from, h->alpha, h->beta, h->gamma_p)
+#define LFR_ASM(T, BWV)                                                                                   \
+  if (h->asm_tab_d)                                                                                         \
+    hipLaunchKernelGGL((k_cma_asm_tab<T, BWV>), dim3(g), dim3(256), 0, s, (T*)cm_d,                         \
+                       (const T*)local_stencil_d, h->asm_tab_d, ncell_2d, h->nlayers, h->nrow);             \
+  else                                                                                                      \
+    hipLaunchKernelGGL((k_cma_asm<T, BWV>), dim3(g), dim3(256), 0, s, (T*)cm_d, (const T*)local_stencil_d,   \
+                       h->row_ptr_d, h->row_ent_d, h->banded_from_d, ncell_2d, h->nlayers, h->nrow, h->ndf_to, \
+                       h->ndf_from, h->alpha, h->beta, h->gamma_p)
 #define LFR_ASM_BW(T)                      \
   switch (h->bw) {                         \
     case 1: LFR_ASM(T, 1); break;          \
