@@ -130,6 +130,10 @@ def test_multi_owner_form_matches_colour_form(ctx, monkeypatch, mprec, edge, nl)
     own.apply(za, xd, overwrite=False)
     clr.apply(zb, xd, overwrite=False)
     torch.testing.assert_close(za, zb, rtol=1e-6, atol=1e-6 * float(ya.abs().max()))
+    # the owner increment adds y to the very sum the overwrite form stores
+    # (y loaded ahead of the stores, layers past the top of a partial last
+    # block clamped): with y = that sum the result is y + y exactly
+    assert torch.equal(za, ya + ya)
     own.close()
     clr.close()
 

@@ -43,6 +43,20 @@ def test_vcycle_bit_exact(ctx, prec, nl):
     np.testing.assert_allclose(res, reso, rtol=1e-12 if prec == 8 else 1e-5)
 
 
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("nrhs", [8, 32])
+def test_vcycle_70_layers_bit_exact(ctx, prec, nrhs):
+    """70 layers: the unrolled smoother (backward-sweep loads a chunk ahead)
+    and, for up to 16 right-hand sides, the residual-and-restriction with one
+    lane per fine child (children summed on the child-0 lane); for 32 the
+    loop form.  Every right-hand side bit-exact against the oracle."""
+    H = build_hierarchy((8, 4), nlayers=70)
+    print(f"mg L70 prec={prec} nrhs={nrhs}", flush=True)
+    x, res, xo, reso = _run(ctx, H, prec, nrhs=nrhs, ncycles=2)
+    np.testing.assert_array_equal(x, xo)
+    np.testing.assert_allclose(res, reso, rtol=1e-12 if prec == 8 else 1e-5)
+
+
 def test_vcycle_options_and_repeat(ctx):
     """Other smoothing counts; a second solve replays the captured cycle."""
     H = build_hierarchy((8, 4, 2), nlayers=10)
