@@ -607,53 +607,90 @@ k_cma_asm(T* __restrict__ cm, const T* __restrict__ ls, const int32_t* __restric
   for (int b = 0; b < BW; ++b) out[b] = acc[b];
 }
 
+// (cell, row) of item b + threadIdx.x, items numbered row-fastest, with b
+// uniform across the workgroup: one division per workgroup (scalar) and,
+// for the lane's offset, a multiply by minv = ceil(2^20 / nrow) and a shift,
+// exact for offsets below 256 + nrow when (256 + nrow) nrow < 2^20 (checked
+// for every nrow <= kSplitMaxRow).  A division per lane (64-bit for the
+// assembly) cost more time than the kernel's memory traffic.
+constexpr int kSplitMaxRow = 768;
+__device__ __forceinline__ void split_item(unsigned b, unsigned nrow, unsigned minv, int& cell, int& row) {
+  const unsigned c0 = b / nrow;
+  const unsigned local = threadIdx.x + (b - c0 * nrow);
+  const unsigned dq = (local * minv) >> 20;
+  cell = (int)(c0 + dq);
+  row = (int)(local - dq * nrow);
+}
+unsigned split_minv(int nrow) { return (unsigned)(((1u << 20) + (unsigned)nrow - 1) / (unsigned)nrow); }
+
 // The same assembly from a per-row table (built at creation) of at most
 // kAsmMc contributions in the reference loop order (entries (k, df1), then
 // df2): one table load, then every stencil load at once (unused slots read
-// the first contribution's value and are not added), then the sums in table
-// order.  The loop form above waits for an index load before each stencil
+// layer 0, plane 0 and are not added), then the sums in table order.  The loop form above waits for an index load before each stencil
 // load.
 constexpr int kAsmMc = 4;
 constexpr int kAsmStride = 16;   // 1 + 3 kAsmMc, padded to whole int4
-template <typename T, int BW>
+template <typename T, int BW, int IT>
 __global__ void __launch_bounds__(256)
 k_cma_asm_tab(T* __restrict__ cm, const T* __restrict__ ls, const int32_t* __restrict__ tab, int ncell, int nl,
-              int nrow) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)ncell * nrow) return;
-  const int cell = (int)(t / nrow), i0 = (int)(t - (int64_t)cell * nrow);
-  const int64_t n3 = (int64_t)ncell * nl;
-  int r[kAsmStride];
-  const int4* r4 = reinterpret_cast<const int4*>(tab + (int64_t)i0 * kAsmStride);
+              int nrow, unsigned minv) {
+  // IT items a grid stride apart per thread: both table records, then all
+  // their stencil loads, are in flight together (each item's stencil loads
+  // wait for its record)
+  const unsigned n = (unsigned)ncell * (unsigned)nrow;
+  const unsigned stride = gridDim.x * 256u;
+  const unsigned n3u = (unsigned)ncell * (unsigned)nl;
+  int cell[IT], i0[IT];
+  bool ok[IT];
+  int r[IT][kAsmStride];
 #pragma unroll
-  for (int q = 0; q < kAsmStride / 4; ++q) {
-    const int4 v = r4[q];
-    r[4 * q] = v.x;
-    r[4 * q + 1] = v.y;
-    r[4 * q + 2] = v.z;
-    r[4 * q + 3] = v.w;
-  }
-  const int cnt = r[0];
-  T v[kAsmMc];
+  for (int u = 0; u < IT; ++u) {
+    const unsigned b = blockIdx.x * 256u + u * stride;
+    ok[u] = b + threadIdx.x < n;
+    split_item(b, (unsigned)nrow, minv, cell[u], i0[u]);
+    if (!ok[u]) cell[u] = i0[u] = 0;
+    const int4* r4 = reinterpret_cast<const int4*>(tab + (int64_t)i0[u] * kAsmStride);
 #pragma unroll
-  for (int m = 0; m < kAsmMc; ++m) {
-    const int mm = m < cnt ? m : 0;
-    v[m] = ls[(int64_t)cell * nl + r[1 + 3 * mm] + n3 * r[2 + 3 * mm]];
-  }
-  T acc[BW];
-#pragma unroll
-  for (int b = 0; b < BW; ++b) acc[b] = T(0);
-#pragma unroll
-  for (int m = 0; m < kAsmMc; ++m)
-    if (m < cnt) {
-      const int b = r[3 + 3 * m];
-#pragma unroll
-      for (int q = 0; q < BW; ++q)
-        if (q == b) acc[q] = acc[q] + v[m];
+    for (int q = 0; q < kAsmStride / 4; ++q) {
+      const int4 v = r4[q];
+      r[u][4 * q] = v.x;
+      r[u][4 * q + 1] = v.y;
+      r[u][4 * q + 2] = v.z;
+      r[u][4 * q + 3] = v.w;
     }
-  T* out = cm + ((int64_t)cell * nrow + i0) * BW;
+  }
+  // 32-bit offsets: the launch checks that the stencil and band arrays have
+  // fewer than 2^31 entries; unused slots hold layer 0, plane 0 (in bounds;
+  // their values are not added)
+  T v[IT][kAsmMc];
 #pragma unroll
-  for (int b = 0; b < BW; ++b) out[b] = acc[b];
+  for (int u = 0; u < IT; ++u) {
+    const unsigned base = (unsigned)cell[u] * (unsigned)nl;
+#pragma unroll
+    for (int m = 0; m < kAsmMc; ++m)
+      v[u][m] = ls[base + (unsigned)r[u][1 + 3 * m] + n3u * (unsigned)r[u][2 + 3 * m]];
+  }
+  // branch-free sums: a slot that the reference loop does not reach adds
+  // +0, which leaves the sum's bits unchanged (it starts at +0 and, rounding
+  // to nearest, never becomes -0)
+#pragma unroll
+  for (int u = 0; u < IT; ++u) {
+    const int cnt = r[u][0];
+    T acc[BW];
+#pragma unroll
+    for (int b = 0; b < BW; ++b) acc[b] = T(0);
+#pragma unroll
+    for (int m = 0; m < kAsmMc; ++m) {
+      const int b = m < cnt ? r[u][3 + 3 * m] : -1;
+#pragma unroll
+      for (int q = 0; q < BW; ++q) acc[q] = acc[q] + (q == b ? v[u][m] : T(0));
+    }
+    if (ok[u]) {
+      T* out = cm + ((unsigned)cell[u] * (unsigned)nrow + (unsigned)i0[u]) * (unsigned)BW;
+#pragma unroll
+      for (int b = 0; b < BW; ++b) out[b] = acc[b];
+    }
+  }
 }
 
 // columnwise_op_app_kernel_code with per-row tables: row i0 of every column
@@ -668,20 +705,21 @@ template <typename T, int BW, int IT>
 __global__ void __launch_bounds__(256)
 k_cma_app_tab(T* __restrict__ lhs, const T* __restrict__ x, const T* __restrict__ cm,
               const int32_t* __restrict__ map_to, const int32_t* __restrict__ map_from,
-              const int32_t* __restrict__ tab, int ncell, int nrow, int ndf_to, int ndf_from) {
+              const int32_t* __restrict__ tab, int ncell, int nrow, int ndf_to, int ndf_from, unsigned minv) {
   const int n = ncell * nrow;
   const int stride = gridDim.x * 256;
-  for (int b0 = blockIdx.x * 256 + threadIdx.x; b0 < n; b0 += stride * IT) {
+  for (int base = blockIdx.x * 256; base < n; base += stride * IT) {
     T acc[IT], cv[IT][BW], xv[IT][BW];
     int64_t mu[IT];
     int xo[IT][BW];
     bool ok[IT];
 #pragma unroll
     for (int u = 0; u < IT; ++u) {
-      const int idx = b0 + u * stride;
+      const int idx = base + u * stride + (int)threadIdx.x;
       ok[u] = idx < n;
-      const int q = ok[u] ? idx : 0;
-      const int cell = q / nrow, i0 = q - cell * nrow;
+      int cell, i0;
+      split_item((unsigned)(base + u * stride), (unsigned)nrow, minv, cell, i0);
+      if (!ok[u]) cell = i0 = 0;
       const int32_t* r = tab + i0 * (BW + 1);
       mu[u] = (int64_t)map_to[(int64_t)cell * ndf_to] + r[0];
       const int64_t bf = map_from[(int64_t)cell * ndf_from];
@@ -884,11 +922,15 @@ int lfr_columnwise_op_asm(lfr_ctx* ctx, int prec, const lfr_cma* h, const void* 
               "lfr_columnwise_op_asm: bad arguments");
   const int64_t n = (int64_t)ncell_2d * h->nrow;
   const unsigned g = (unsigned)((n + 255) / 256);
+  constexpr int kAsmIt = 2;   // 4 measured the same (32.9 against 32.8 µs)
+  const unsigned gi = (unsigned)std::max<int64_t>(1, (n + 256 * kAsmIt - 1) / (256 * kAsmIt));
   hipStream_t s = ctx->stream;
 #define LFR_ASM(T, BWV)                                                                                   \
-  if (h->asm_tab_d)                                                                                         \
-    hipLaunchKernelGGL((k_cma_asm_tab<T, BWV>), dim3(g), dim3(256), 0, s, (T*)cm_d,                         \
-                       (const T*)local_stencil_d, h->asm_tab_d, ncell_2d, h->nlayers, h->nrow);             \
+  if (h->asm_tab_d && h->nrow <= kSplitMaxRow && n * 8 < ((int64_t)1 << 31) &&                             \
+      (int64_t)ncell_2d * h->nlayers * h->ndf_to * h->ndf_from < ((int64_t)1 << 31))                          \
+    hipLaunchKernelGGL((k_cma_asm_tab<T, BWV, kAsmIt>), dim3(gi), dim3(256), 0, s, (T*)cm_d,                \
+                       (const T*)local_stencil_d, h->asm_tab_d, ncell_2d, h->nlayers, h->nrow,              \
+                       split_minv(h->nrow));                                                                \
   else                                                                                                      \
     hipLaunchKernelGGL((k_cma_asm<T, BWV>), dim3(g), dim3(256), 0, s, (T*)cm_d, (const T*)local_stencil_d,   \
                        h->row_ptr_d, h->row_ent_d, h->banded_from_d, ncell_2d, h->nlayers, h->nrow, h->ndf_to, \
@@ -924,7 +966,7 @@ int lfr_columnwise_op_app(lfr_ctx* ctx, int prec, const lfr_cma* h, void* lhs_d,
               "lfr_columnwise_op_app: spaces do not match the operator geometry");
   const int ncell = to_space->ncell;
   const int64_t n = (int64_t)ncell * h->nrow;
-  if (h->app_tab_d && n < ((int64_t)1 << 31)) {
+  if (h->app_tab_d && h->nrow <= kSplitMaxRow && n < ((int64_t)1 << 31)) {
     // two items per thread (C96L70 Wtheta FP32: 37.1 µs at one, 32.1 at
     // two, 33.2 at four; 38.3 with the per-thread band arithmetic)
     constexpr int IT = 2;
@@ -932,7 +974,7 @@ int lfr_columnwise_op_app(lfr_ctx* ctx, int prec, const lfr_cma* h, void* lhs_d,
 #define LFR_APP(T, BWV)                                                                                             \
   hipLaunchKernelGGL((k_cma_app_tab<T, BWV, IT>), dim3(gt), dim3(256), 0, ctx->stream, (T*)lhs_d, (const T*)x_d, \
                      (const T*)cm_d, to_space->map0_d, from_space->map0_d, h->app_tab_d, ncell, h->nrow, h->ndf_to, \
-                     h->ndf_from)
+                     h->ndf_from, split_minv(h->nrow))
 #define LFR_APP_BW(T)                 \
   switch (h->bw) {                    \
     case 1: LFR_APP(T, 1); break;     \
