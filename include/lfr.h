@@ -742,6 +742,15 @@ typedef struct lfr_iterative_solver {
 int lfr_iterative_solve(lfr_ctx* ctx, const lfr_iterative_solver* s, const lfr_linop* A,
                         const lfr_precon* P, void* x_d, const void* b_d, lfr_solver_stats* stats);
 
+/* Measurement aid, no counterpart in the reference: the algorithmic HBM bytes
+ * of the last lfr_iterative_solve on this context (every vector pass of the
+ * method counted once per array it reads or writes, each operator apply as
+ * its matrices plus its input and output fields, each preconditioner apply
+ * as its copy, division or column solve), for rating a solve against the
+ * memory bandwidth; -1 when a caller-written operator or preconditioner took
+ * part (its traffic is unknown) or no solve has run. */
+int lfr_iterative_solve_bytes(lfr_ctx* ctx, double* bytes);
+
 /* invoke(mm_diagonal_kernel_type(diag, mass_matrix)): diag(map(df)+k) +=
  * mass_matrix(ik, df, df) over all cells (components/science/source/kernel/
  * algebra/sci_mm_diagonal_kernel_mod.F90), in the colour order of `plan`

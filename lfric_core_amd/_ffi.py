@@ -126,6 +126,7 @@ SIGNATURES = {
     "lfr_precon_destroy": [_P],
     "lfr_precon_apply": [_P, _P, _P, _P],
     "lfr_iterative_solve": None,
+    "lfr_iterative_solve_bytes": [_P, C.POINTER(C.c_double)],
     "lfr_mm_diagonal": [_P, _I, _P, _P, _P, _P],
     "lfr_cg_tape_create": None,
     "lfr_cg_tape_destroy": [_P],

@@ -48,6 +48,7 @@ struct lfr_ctx {
   // operator or preconditioner applied inside a solve may use `work`
   void* kwork = nullptr;
   size_t kwork_bytes = 0;
+  double solve_bytes = -1.0;        // algorithmic bytes of the last lfr_iterative_solve (-1: unknown)
   // autotuner: kept variant per (kernel family, shape) key, and its events
   std::map<std::string, int> tune;
   hipEvent_t tev[2] = {nullptr, nullptr};

@@ -293,6 +293,46 @@ enum Slots {
   K_SC, K_T1, K_T2, K_C1, K_C2, K_HN, K_GAM, K_NSLOT = 32
 };
 
+// Algorithmic bytes of one operator apply: per LMA matrix its entries and
+// the input field read, the output written (and read when incremented);
+// -1 for a caller-written operator.
+double lma_pass_bytes(const lfr_space* to, const lfr_space* from, int s, bool inc) {
+  const double m = (double)to->ncell * to->nlayers * to->ndf * from->ndf * s;
+  return m + (double)from->undf * s + (double)to->undf * s * (inc ? 2 : 1);
+}
+double linop_bytes(const lfr_linop* A) {
+  const int s = A->layout.prec;
+  switch (A->kind) {
+    case 0:
+      return lma_pass_bytes(A->lma.space, A->lma.space, s, false);
+    case 1: {
+      double b = 0.0;
+      const lfr_vector_layout& L = A->layout;
+      for (int f = 0; f < L.nfield; ++f)
+        if (A->single[f] < 0) b += (double)L.undf[f] * s;   // zeroed first
+      for (size_t q = 0; q < A->blocks.size(); ++q) {
+        const lfr_block_entry& e = A->blocks[q];
+        b += lma_pass_bytes(e.to_space, e.from_space, s, A->single[e.row] != (int)q);
+      }
+      return b;
+    }
+    default:
+      return -1.0;
+  }
+}
+// ... and of one preconditioner apply: copy, divide, the column solve (three
+// bands, right-hand side and solution); -1 for a caller-written one
+double precon_bytes(const lfr_precon* P) {
+  const int s = P->layout.prec;
+  const double n = (double)layout_len(P->layout);
+  switch (P->kind) {
+    case 0: return 2 * n * s;
+    case 1: return 3 * n * s;
+    case 2: return (3.0 * P->nrow * P->space->ncell + 2 * n) * s;
+    default: return -1.0;
+  }
+}
+
 template <typename T>
 struct Eng {
   lfr_ctx* ctx = nullptr;
@@ -307,6 +347,12 @@ struct Eng {
   size_t vstride = 0;
   FieldNb nb{};
   int64_t off[LFR_MAX_FIELDS] = {};
+  // algorithmic HBM bytes of the solve: every vector pass (each array read
+  // or written once), operator and preconditioner applies (their matrix and
+  // vector bytes); unknown when a caller-written operator or preconditioner
+  // takes part
+  double bytes = 0.0, apply_b = 0.0, precon_b = 0.0;
+  bool bytes_known = true;
 
   int init(lfr_ctx* c, const lfr_linop* a, const lfr_precon* p, int nvec, int m) {
     ctx = c;
@@ -314,6 +360,9 @@ struct Eng {
     P = p;
     L = a->layout;
     n = layout_len(L);
+    apply_b = linop_bytes(a);
+    precon_b = precon_bytes(p);
+    bytes_known = apply_b >= 0.0 && precon_b >= 0.0;
     nb.nfield = L.nfield;
     for (int f = 0; f < L.nfield; ++f) {
       off[f] = layout_off(L, f);
@@ -347,12 +396,18 @@ struct Eng {
 
   template <int OP>
   int vop(T* z, const T* x, const T* y, Coef a, Coef b = cst(0.0)) {
+    constexpr int reads = OP == V_SET ? 0
+                          : (OP == V_COPY || OP == V_SCALE || OP == V_COPY_SCALE) ? 1
+                          : OP == V_AXPY2 ? 3
+                                          : 2;
+    bytes += (double)n * sizeof(T) * (reads + 1);
     hipLaunchKernelGGL((k_vop<T, OP>), dim3(vgrid(n)), dim3(kT), 0, st(), z, x, y, n, (const double*)S, a, b);
     LFR_CHECK_LAUNCH(ctx);
     return LFR_OK;
   }
   // S[slot] = <x, y> (or its square root)
   int dot(const T* x, const T* y, int slot, bool take_sqrt = false) {
+    bytes += (double)n * sizeof(T) * (x == y ? 1 : 2);
     for (int f = 0; f < L.nfield; ++f)
       hipLaunchKernelGGL((k_fdot<T, 0>), dim3(nb.nb[f]), dim3(kRedThreads), 0, st(), (T*)nullptr, x + off[f],
                          y + off[f], (int64_t)L.undf[f], (const double*)S, cst(0.0), part + (int64_t)f * kRedMaxBlocks);
@@ -364,6 +419,7 @@ struct Eng {
   int norm(const T* x, int slot) { return dot(x, x, slot, true); }
   // z = z + a w, then S[slot] = <y, z> (y == nullptr: <z, z>, sqrt if asked)
   int axpy_dot(T* z, const T* w, Coef a, const T* y, int slot, bool take_sqrt = false) {
+    bytes += (double)n * sizeof(T) * (3 + (y && y != z && y != w ? 1 : 0));
     for (int f = 0; f < L.nfield; ++f)
       hipLaunchKernelGGL((k_fdot<T, 1>), dim3(nb.nb[f]), dim3(kRedThreads), 0, st(), z + off[f], w + off[f],
                          y ? y + off[f] : (const T*)nullptr, (int64_t)L.undf[f], (const double*)S, a,
@@ -455,10 +511,12 @@ int precon_apply(lfr_ctx* ctx, const lfr_precon* P, const void* x, void* y) {
 
 template <typename T>
 int Eng<T>::apply(const T* x, T* y) {
+  bytes += apply_b;
   return linop_apply(ctx, A, x, y);
 }
 template <typename T>
 int Eng<T>::precon(const T* x, T* y) {
+  bytes += precon_b;
   return precon_apply(ctx, P, x, y);
 }
 
@@ -943,17 +1001,21 @@ int solve_t(lfr_ctx* ctx, const lfr_iterative_solver* s, const lfr_linop* A, con
             const T* b, lfr_solver_stats* st) {
   Eng<T> e;
   const bool kry = s->method == LFR_SOLVER_GMRES || s->method == LFR_SOLVER_FGMRES;
+  ctx->solve_bytes = -1.0;
   KTRY(e.init(ctx, A, P, nvec_for(s), kry ? s->gcrk : 0));
+  int rc;
   switch (s->method) {
-    case LFR_SOLVER_CG: return cg(e, s, x, b, st);
-    case LFR_SOLVER_BICGSTAB: return bicgstab(e, s, x, b, st);
-    case LFR_SOLVER_GMRES: return gmres(e, s, x, b, st);
-    case LFR_SOLVER_FGMRES: return fgmres(e, s, x, b, st);
-    case LFR_SOLVER_GCR: return gcr(e, s, x, b, st);
-    case LFR_SOLVER_PRECONDITION_ONLY: return precondition_only(e, s, x, b, st);
-    case LFR_SOLVER_JACOBI: return jacobi(e, s, x, b, st);
-    default: return chebyshev(e, s, x, b, st);
+    case LFR_SOLVER_CG: rc = cg(e, s, x, b, st); break;
+    case LFR_SOLVER_BICGSTAB: rc = bicgstab(e, s, x, b, st); break;
+    case LFR_SOLVER_GMRES: rc = gmres(e, s, x, b, st); break;
+    case LFR_SOLVER_FGMRES: rc = fgmres(e, s, x, b, st); break;
+    case LFR_SOLVER_GCR: rc = gcr(e, s, x, b, st); break;
+    case LFR_SOLVER_PRECONDITION_ONLY: rc = precondition_only(e, s, x, b, st); break;
+    case LFR_SOLVER_JACOBI: rc = jacobi(e, s, x, b, st); break;
+    default: rc = chebyshev(e, s, x, b, st);
   }
+  ctx->solve_bytes = e.bytes_known ? e.bytes : -1.0;
+  return rc;
 }
 
 // field reductions through the engine's kernels on the context workspace
@@ -1191,6 +1253,13 @@ int lfr_iterative_solve(lfr_ctx* ctx, const lfr_iterative_solver* s, const lfr_l
   st->r_nrm0 = st->r_nrm = 0.0;
   if (A->layout.prec == 4) return solve_t<float>(ctx, s, A, P, (float*)x, (const float*)b, st);
   return solve_t<double>(ctx, s, A, P, (double*)x, (const double*)b, st);
+}
+
+int lfr_iterative_solve_bytes(lfr_ctx* ctx, double* bytes) {
+  if (!ctx) return LFR_ERR_ARG;
+  LFR_REQUIRE(ctx, bytes, "lfr_iterative_solve_bytes: null argument");
+  *bytes = ctx->solve_bytes;
+  return LFR_OK;
 }
 
 int lfr_mm_diagonal(lfr_ctx* ctx, int prec, void* diag, const void* M, const lfr_space* sp, const lfr_plan* plan) {

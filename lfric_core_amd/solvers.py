@@ -490,6 +490,7 @@ class _IterativeSolver:
         self.fail_on_non_converged = fail_on_non_converged
         self.gcrk, self.rho_relax, self.lmin, self.lmax = 0, 0.0, 0.0, 0.0
         self.stats = None
+        self.bytes = None
 
     def _generic(self, x, b) -> SolverStats:
         op = _linop(self.lin_op)
@@ -506,6 +507,11 @@ class _IterativeSolver:
                                                 _P(bd.data_ptr()), C.byref(st))
         self.stats = st
         check(rc, ctx.h)
+        nb = C.c_double(-1.0)
+        check(lib().lfr_iterative_solve_bytes(ctx.h, C.byref(nb)), ctx.h)
+        # algorithmic HBM bytes of this solve (None: a caller-written operator
+        # or preconditioner took part)
+        self.bytes = nb.value if nb.value >= 0 else None
         return st
 
     def apply(self, x, b) -> SolverStats:

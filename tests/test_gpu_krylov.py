@@ -171,6 +171,30 @@ def test_callback_operator_with_column_preconditioner(ctx, prob, name):
     np.testing.assert_allclose(apply_orc(xo), b, atol=1e-7 * np.abs(b).max())
 
 
+def test_solve_bytes_closed_form(ctx, prob):
+    """lfr_iterative_solve_bytes: Jacobi with monitoring off makes, per
+    iteration, one apply (matrix, x read, y written), two axpys (3 passes
+    each) and one null-preconditioner copy (2 passes): 3 iterations count
+    3 (M + 2 undf s + 8 undf s) bytes exactly.  A caller-written operator
+    makes the count unknown."""
+    from lfric_core_amd import api, solvers
+    cfg, w2, w3 = prob
+    print("case solve bytes Jacobi W2 C4L6 fp64", flush=True)
+    op = api.LMAOperator(_t(ctx, cfg.mass), w2, w2, cfg.colouring)
+    s = solvers.Jacobi(solvers.MassMatrixOperator(op), None, max_iter=3, monitor_convergence=False,
+                       rho_relax=0.5)
+    x = torch.zeros(cfg.w2.undf, dtype=torch.float64, device=ctx.device)
+    s.apply(x, _t(ctx, cfg.b))
+    u, n3 = cfg.w2.undf, cfg.mesh.ncells * cfg.nlayers
+    assert s.bytes == 3 * (n3 * 36 * 8 + 2 * u * 8 + 8 * u * 8)
+    apply_gpu, _, _, _ = _w3_problem(ctx, cfg, w2, w3)
+    cb = solvers.CallbackOperator(ctx, [cfg.w3.undf], torch.float64, apply_gpu)
+    s2 = solvers.Jacobi(cb, None, max_iter=2, monitor_convergence=False)
+    s2.apply(torch.zeros(cfg.w3.undf, dtype=torch.float64, device=ctx.device),
+             torch.ones(cfg.w3.undf, dtype=torch.float64, device=ctx.device))
+    assert s2.bytes is None
+
+
 def test_chebyshev_and_precondition_only(ctx, prob):
     from lfric_core_amd import api, solvers
     cfg, w2, _ = prob
