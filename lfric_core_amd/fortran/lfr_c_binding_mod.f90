@@ -115,7 +115,8 @@ module lfr_c_binding_mod
             lfr_vector_field_norm, lfr_linop_lma, lfr_linop_block, lfr_linop_callback,    &
             lfr_linop_destroy, lfr_linop_apply, lfr_precon_null, lfr_precon_diagonal,    &
             lfr_precon_columnwise, lfr_precon_callback, lfr_precon_destroy,            &
-            lfr_precon_apply, lfr_iterative_solve, lfr_apply_fn, lfr_error_message,     &
+            lfr_precon_apply, lfr_iterative_solve, lfr_iterative_solve_bytes,          &
+            lfr_apply_fn, lfr_error_message,                                           &
             lfr_tune_set, lfr_tune_get, lfr_tune_clear, lfr_tune_save, lfr_tune_load
 
   interface
@@ -533,6 +534,14 @@ module lfr_c_binding_mod
       type(lfr_iterative_solver), intent(in) :: solver
       type(lfr_solver_stats), intent(out) :: stats
     end function lfr_iterative_solve
+
+    ! measurement aid: algorithmic HBM bytes of the last iterative solve on
+    ! ctx (-1 when a caller-written operator or preconditioner took part)
+    integer(c_int) function lfr_iterative_solve_bytes(ctx, bytes) bind(C, name="lfr_iterative_solve_bytes")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: bytes
+    end function lfr_iterative_solve_bytes
 
     ! autotuner: force / read a launch-shape choice, forget all, save / load
     ! them (keys and paths are NUL-terminated: trim(s)//c_null_char)
