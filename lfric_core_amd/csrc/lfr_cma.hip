@@ -269,6 +269,14 @@ k_appinv_deep(float* __restrict__ lhs, const float* __restrict__ x, const float*
   const int lane = threadIdx.x;
   const int c0 = blockIdx.x * COLS;
   const int ncols = min(COLS, ncell - c0);
+  // the column bases and the row indirection first, unconditionally at
+  // clamped (in-bounds) indices: the right-hand side's addresses depend on
+  // them, and loaded after the bands (or behind a branch) their wait would
+  // also wait for every band load
+  static_assert(NROW <= 2 * kCols, "two indirection entries a lane");
+  const int32_t mbv = map0[(int64_t)(c0 + min(lane, ncols - 1)) * ndf];
+  const int32_t ir0 = ind[min(lane, NROW - 1)];
+  const int32_t ir1 = ind[min(lane + kCols, NROW - 1)];
   // bands: pair slot it of stage s is pair qv of column cs (lane + it kCols =
   // cs NBV + qv), clamped into the workgroup's columns
   const float* cmw = cm + (int64_t)c0 * 3 * NROW;
@@ -281,8 +289,13 @@ k_appinv_deep(float* __restrict__ lhs, const float* __restrict__ x, const float*
       const int cs = e / NBV, qv = e - cs * NBV;
       rb[s][it] = *reinterpret_cast<const float2*>(cmw + min(cs, ncols - 1) * 3 * NROW + s * NB + 2 * qv);
     }
-  if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
-  for (int i = lane; i < NROW; i += kCols) sind[i] = ind0(ind, i, NROW);
+  // an unconditional use of the three values after the band loads, so that
+  // the compiler does not move their loads down into the branches below
+  // (behind the band loads); the statement emits no instruction
+  asm volatile("" ::"v"(mbv), "v"(ir0), "v"(ir1));
+  if (lane < ncols) sbase[lane] = mbv;
+  if (lane < NROW) sind[lane] = min(max(ir0 - 1, 0), NROW - 1);   // as ind0
+  if (lane + kCols < NROW) sind[lane + kCols] = min(max(ir1 - 1, 0), NROW - 1);
   __syncthreads();
   // right-hand side, flat slot e = column c row r (e = c NROW + r)
   const int nflat = ncols * NROW;
@@ -1444,8 +1457,11 @@ k_optri_deep(float* __restrict__ y, const float* __restrict__ x, const float* __
   const int c0 = blockIdx.x * COLS;
   const int ncols = min(COLS, ncell - c0);
   const int64_t n3 = (int64_t)ncell * NL;
-  if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
-  __syncthreads();
+  // the column bases first (unconditionally, clamped), then the matrix
+  // planes, which are addressed by cell and do not wait for them; x, whose
+  // addresses do, after (a base loaded behind a branch and waited for before
+  // the matrix loads cost a whole round trip)
+  const int32_t mbv = map0[(int64_t)(c0 + min(lane, ncols - 1)) * ndf];
   // per slot: a = m(r-1;1,0), d0 = m(r;0,0), d1 = m(r-1;1,1), u = m(r;0,1), x(r)
   float rv[NST][5][SLOTS];
 #pragma unroll
@@ -1462,6 +1478,19 @@ k_optri_deep(float* __restrict__ y, const float* __restrict__ x, const float* __
       rv[s][1][it] = m[ia];
       rv[s][2][it] = m[ib + 3 * n3];
       rv[s][3][it] = m[ia + 2 * n3];
+    }
+  // an unconditional use after the matrix loads, so that the base's load is
+  // not moved down into the branch below; the statement emits no instruction
+  asm volatile("" ::"v"(mbv));
+  if (lane < ncols) sbase[lane] = mbv;
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < NST; ++s)
+#pragma unroll
+    for (int it = 0; it < SLOTS; ++it) {
+      const int e = min(lane + it * kCols, COLS * KCH - 1);
+      const int cs = min(e / KCH, ncols - 1);
+      const int r = min(s * KCH + (e % KCH), NROW - 1);
       rv[s][4][it] = x[(int64_t)sbase[cs] + r];
     }
   float cp[NROW], dp[NROW];
