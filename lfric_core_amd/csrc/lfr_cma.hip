@@ -1197,7 +1197,7 @@ template <int NROW, int KCH, int COLS, bool DPL = false>
 __global__ void __launch_bounds__(kCols)
 k_thomas_deep(float* __restrict__ y, const float* __restrict__ x, const float* __restrict__ t0,
               const float* __restrict__ tp, const float* __restrict__ tm, const int32_t* __restrict__ map0,
-              int ndf, int ncell) {
+              int ndf, int ncell, int cstride) {
   static_assert(NROW % KCH == 0 && COLS <= kCols, "stage shape");
   constexpr int NST = NROW / KCH;
   constexpr int SLOTS = (COLS * KCH + kCols - 1) / kCols;
@@ -1209,7 +1209,12 @@ k_thomas_deep(float* __restrict__ y, const float* __restrict__ x, const float* _
   const int lane = threadIdx.x;
   const int c0 = blockIdx.x * COLS;
   const int ncols = min(COLS, ncell - c0);
-  if (lane < ncols) sbase[lane] = map0[(int64_t)(c0 + lane) * ndf];
+  // every load addresses through the column base: formed from the column
+  // stride when the map has one (no round trip before the loads), read
+  // from the map otherwise
+  const int cl = c0 + min(lane, ncols - 1);
+  const int32_t mybase = cstride >= 0 ? cl * cstride : map0[(int64_t)cl * ndf];
+  if (lane < ncols) sbase[lane] = mybase;
   __syncthreads();
   const float* src[4] = {tm, t0, tp, x};
   float rv[NST][4][SLOTS];
@@ -1576,23 +1581,23 @@ bool launch_thomas(lfr_ctx* ctx, hipStream_t s, T* y, const T* x, const T* b0, c
           switch (c) {
             case 0:
               hipLaunchKernelGGL((k_thomas_deep<70, 14, 32>), dim3((ncell + 31) / 32), dim3(kCols), 0, s, y, x, b0,
-                                 b1, b2, sp->map0_d, sp->ndf, ncell);
+                                 b1, b2, sp->map0_d, sp->ndf, ncell, sp->col_stride);
               break;
             case 1:
               hipLaunchKernelGGL((k_thomas_deep<70, 10, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y, x, b0,
-                                 b1, b2, sp->map0_d, sp->ndf, ncell);
+                                 b1, b2, sp->map0_d, sp->ndf, ncell, sp->col_stride);
               break;
             case 2:
               hipLaunchKernelGGL((k_thomas_deep<70, 14, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y, x, b0,
-                                 b1, b2, sp->map0_d, sp->ndf, ncell);
+                                 b1, b2, sp->map0_d, sp->ndf, ncell, sp->col_stride);
               break;
             case 4:   // d' in LDS
               hipLaunchKernelGGL((k_thomas_deep<70, 10, 16, true>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y,
-                                 x, b0, b1, b2, sp->map0_d, sp->ndf, ncell);
+                                 x, b0, b1, b2, sp->map0_d, sp->ndf, ncell, sp->col_stride);
               break;
             case 5:
               hipLaunchKernelGGL((k_thomas_deep<70, 14, 16, true>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y,
-                                 x, b0, b1, b2, sp->map0_d, sp->ndf, ncell);
+                                 x, b0, b1, b2, sp->map0_d, sp->ndf, ncell, sp->col_stride);
               break;
             default:
               hipLaunchKernelGGL((k_thomas<T, 70>), dim3(nb), dim3(kCols), 0, s, y, x, b0, b1, b2, sp->map0_d,

@@ -226,6 +226,14 @@ int lfr_space_create(lfr_ctx* ctx, const int32_t* map, int ndf, int ncell, int n
   s->undf = undf;
   s->map_h.assign(map, map + n);
   for (int c = 0; c < ncell; ++c) s->max_first = std::max(s->max_first, (int)map[(int64_t)c * ndf]);
+  {
+    // columns at a constant stride (W3, Wtheta, ...): kernels then form a
+    // column's base instead of reading it first
+    const int64_t st = ncell > 1 ? (int64_t)m0[ndf] - m0[0] : 0;
+    bool aff = m0[0] == 0 && st >= 0;
+    for (int c = 1; c < ncell && aff; ++c) aff = (int64_t)m0[(int64_t)c * ndf] == c * st;
+    s->col_stride = aff ? (int)st : -1;
+  }
   if (hipMalloc(&s->map0_d, n * sizeof(int32_t)) != hipSuccess) {
     delete s;
     return lfr_set_error(ctx, LFR_ERR_NOMEM, "dofmap allocation failed");

@@ -63,6 +63,7 @@ struct lfr_space {
   std::vector<int32_t> map_h;       // 1-based host copy (plan construction)
   std::vector<int32_t> boundary_h;  // boundary_value(ndf, 2) for enforce_bc (empty: not set)
   mutable int vstack = -1;          // every cell's dof j = its dof 1 + j - 1 (one column; -1: not checked)
+  int col_stride = -1;              // map(1, c) - 1 == c * col_stride for every cell (-1: not so)
 };
 
 // Queue enforce_bc over all cells of `sp` on `field` (sp must have its

@@ -37,6 +37,29 @@ def test_tri_solve_bitexact(ctx, dtype, nc, nl):
     np.testing.assert_array_equal(y.cpu().numpy(), ref)
 
 
+def test_tri_solve_columns_not_at_a_constant_stride(ctx):
+    """A W3 dofmap whose cells point at a permutation of the column blocks:
+    the kernel reads each column's base from the map (a generated W3 map has
+    its columns at a constant stride, and the base is formed instead)."""
+    import dataclasses
+    from lfric_core_amd import api
+    print("case tri_solve permuted W3 columns C8L70 float32", flush=True)
+    fs0 = build_dofmap(cubed_sphere(8), "W3", 70)
+    perm = np.random.default_rng(5).permutation(fs0.dofmap.shape[0])
+    fs = dataclasses.replace(fs0, dofmap=np.ascontiguousarray(fs0.dofmap[perm]))
+    st = Stream(53)
+    t0 = (4.0 + st.uniform(fs.undf)).astype(np.float32)
+    tp = (0.5 * st.uniform(fs.undf)).astype(np.float32)
+    tm = (0.5 * st.uniform(fs.undf)).astype(np.float32)
+    x = st.uniform(fs.undf).astype(np.float32)
+    ref = oracle.tri_solve(np.zeros(fs.undf, np.float32), x, t0, tp, tm, fs.dofmap, 70)
+    sp = api.Space(ctx, fs)
+    y = torch.full((fs.undf,), float("nan"), dtype=torch.float32, device=ctx.device)
+    api.tri_solve(y, _t(x, ctx), _t(t0, ctx), _t(tp, ctx), _t(tm, ctx), sp)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(y.cpu().numpy(), ref)
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("space,nc,nl", [("WTHETA", 3, 8), ("W2V", 6, 70), ("WTHETA", 5, 70)])
 def test_operator_tri_solve_bitexact(ctx, dtype, space, nc, nl):
