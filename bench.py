@@ -912,8 +912,8 @@ def main():
         profile_workload(args.profile)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this library runs on one GPU; --gpus must be 1 "
+                                   "(sharding across GPUs is designed, not built)"}))
         sys.exit(2)
 
     import torch

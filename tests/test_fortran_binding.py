@@ -1,6 +1,6 @@
 """The Fortran iso_c_binding interface (lfric_core_amd/fortran) compiles with
 flang, links against liblfr.so and calls through the C ABI: the Fortran
-snippets of INTEGRATION.md compile as written against the module, and a
+snippets of the integration guide compile as written against the module, and a
 Fortran program drives a W2 mass apply (reference layout and packed), a CG
 solve and a generic BiCGstab solve on the GPU, checked against the oracle."""
 import os
@@ -55,7 +55,7 @@ def _build(tmp_path):
 
 
 def test_integration_snippets_compile(tmp_path):
-    """Every ```fortran block of INTEGRATION.md is a complete program unit that
+    """Every ```fortran block of the integration guide is a complete program unit that
     compiles against lfr_c_binding_mod (with a stub of LFRic's log_mod)."""
     _module(tmp_path)
     text = (ROOT / "INTEGRATION.md").read_text()
