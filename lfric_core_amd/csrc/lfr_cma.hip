@@ -1448,7 +1448,7 @@ k_optri_staged(T* __restrict__ y, const T* __restrict__ x, const T* __restrict__
 template <int NROW, int KCH, int COLS>
 __global__ void __launch_bounds__(kCols)
 k_optri_deep(float* __restrict__ y, const float* __restrict__ x, const float* __restrict__ m,
-             const int32_t* __restrict__ map0, int ndf, int ncell) {
+             const int32_t* __restrict__ map0, int ndf, int ncell, int cstride) {
   static_assert(COLS <= kCols, "stage shape");
   constexpr int NL = NROW - 1;                      // layers
   constexpr int NST = (NROW + KCH - 1) / KCH;
@@ -1462,11 +1462,13 @@ k_optri_deep(float* __restrict__ y, const float* __restrict__ x, const float* __
   const int c0 = blockIdx.x * COLS;
   const int ncols = min(COLS, ncell - c0);
   const int64_t n3 = (int64_t)ncell * NL;
-  // the column bases first (unconditionally, clamped), then the matrix
+  // the column bases first (formed from the column stride when the map has
+  // one, else read unconditionally at a clamped index), then the matrix
   // planes, which are addressed by cell and do not wait for them; x, whose
   // addresses do, after (a base loaded behind a branch and waited for before
   // the matrix loads cost a whole round trip)
-  const int32_t mbv = map0[(int64_t)(c0 + min(lane, ncols - 1)) * ndf];
+  const int cl = c0 + min(lane, ncols - 1);
+  const int32_t mbv = cstride >= 0 ? cl * cstride : map0[(int64_t)cl * ndf];
   // per slot: a = m(r-1;1,0), d0 = m(r;0,0), d1 = m(r-1;1,1), u = m(r;0,1), x(r)
   float rv[NST][5][SLOTS];
 #pragma unroll
@@ -1633,11 +1635,11 @@ bool launch_thomas(lfr_ctx* ctx, hipStream_t s, T* y, const T* x, const T* b0, c
         switch (c) {
           case 0:
             hipLaunchKernelGGL((k_optri_deep<71, 12, 16>), dim3((ncell + 15) / 16), dim3(kCols), 0, s, y, x, b0,
-                               sp->map0_d, sp->ndf, ncell);
+                               sp->map0_d, sp->ndf, ncell, sp->col_stride);
             break;
           case 1:
             hipLaunchKernelGGL((k_optri_deep<71, 8, 32>), dim3((ncell + 31) / 32), dim3(kCols), 0, s, y, x, b0,
-                               sp->map0_d, sp->ndf, ncell);
+                               sp->map0_d, sp->ndf, ncell, sp->col_stride);
             break;
           default:
             hipLaunchKernelGGL((k_optri_staged<T, 71, kCh>), dim3(nb), dim3(kCols), 0, s, y, x, b0, sp->map0_d,

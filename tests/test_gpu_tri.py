@@ -96,3 +96,29 @@ def test_tri_solve_bad_args(ctx):
     with pytest.raises(LfrError):   # not a 2-dof column space
         api.operator_tri_solve(z.clone(), z, torch.zeros((2, 2, fs.mesh.ncells * 3), dtype=torch.float64,
                                                          device=ctx.device), sp)
+
+
+def test_operator_tri_solve_columns_not_at_a_constant_stride(ctx):
+    """Wtheta with its cells pointing at a permutation of the column blocks:
+    the column bases come from the map instead of the column stride."""
+    import dataclasses
+    from lfric_core_amd import api
+    print("case operator_tri_solve permuted WTHETA columns C5L70 float32", flush=True)
+    fs0 = build_dofmap(cubed_sphere(5), "WTHETA", 70)
+    perm = np.random.default_rng(7).permutation(fs0.dofmap.shape[0])
+    fs = dataclasses.replace(fs0, dofmap=np.ascontiguousarray(fs0.dofmap[perm]))
+    st = Stream(54)
+    n3 = fs.mesh.ncells * 70
+    m = np.empty((2, 2, n3))
+    m[0, 0] = 2.0 + st.uniform(n3)
+    m[0, 1] = 0.5 * st.uniform(n3)
+    m[1, 0] = 0.5 * st.uniform(n3)
+    m[1, 1] = 2.0 + st.uniform(n3)
+    m = m.astype(np.float32)
+    x = st.uniform(fs.undf).astype(np.float32)
+    ref = oracle.operator_tri_solve(np.zeros(fs.undf, np.float32), x, m, fs.dofmap, 70)
+    sp = api.Space(ctx, fs)
+    y = torch.full((fs.undf,), float("nan"), dtype=torch.float32, device=ctx.device)
+    api.operator_tri_solve(y, _t(x, ctx), _t(m, ctx), sp)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(y.cpu().numpy(), ref)
