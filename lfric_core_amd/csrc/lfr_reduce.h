@@ -44,7 +44,11 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 
 // sum of p[threadIdx.x + k*BS] over k in increasing order (the thread's
 // share of a block-wide sum of n values over BS threads); eight loads are
-// issued before they are added, the addition order is unchanged.
+// issued before they are added, the addition order is unchanged.  The last
+// (at most seven) values are loaded together too, at clamped addresses, and
+// a slot past the end adds +0: s starts at +0 and, rounding to nearest,
+// never becomes -0, so adding +0 leaves it bitwise unchanged (added one a
+// trip, each value waited for its own load: a round trip per value).
 template <int BS = kRedThreads>
 __device__ __forceinline__ double strided_sum(const double* __restrict__ p, int n) {
   double s = 0.0;
@@ -56,7 +60,17 @@ __device__ __forceinline__ double strided_sum(const double* __restrict__ p, int 
 #pragma unroll
     for (int q = 0; q < 8; ++q) s += v[q];
   }
-  for (; b < n; b += BS) s += p[b];
+  if (n > 0) {
+    double v[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      const int i = b + q * BS;
+      const double w = p[i < n ? i : n - 1];
+      v[q] = i < n ? w : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 7; ++q) s += v[q];
+  }
   return s;
 }
 

@@ -57,11 +57,14 @@ k_cg_alpha(const double* __restrict__ part, int np, const double* __restrict__ p
     return;
   }
   __shared__ double lds[kAlphaThreads / 64];
-  const double s = lfr::block_sum<kAlphaThreads>(lfr::strided_sum<kAlphaThreads>(part, np), lds);
+  // both threads' shares first, so that their loads are in flight together
+  // (each sum's order is unchanged)
+  const double sp = lfr::strided_sum<kAlphaThreads>(part, np);
   // rz in the order k_cg_beta_direction sums the same partials (256-thread
   // stride; the extra waves add zeros), so both see the same bits
-  const double rz = lfr::block_sum<kAlphaThreads>(
-      threadIdx.x < kThreads ? lfr::strided_sum<kThreads>(part_rz, nrz) : 0.0, lds);
+  const double sz = threadIdx.x < kThreads ? lfr::strided_sum<kThreads>(part_rz, nrz) : 0.0;
+  const double s = lfr::block_sum<kAlphaThreads>(sp, lds);
+  const double rz = lfr::block_sum<kAlphaThreads>(sz, lds);
   if (threadIdx.x == 0) {
     S[S_RZ] = rz;
     S[S_ALPHA] = rz / s;
@@ -113,8 +116,10 @@ k_cg_beta_direction(T* __restrict__ p, const T* __restrict__ z, int64_t n, const
   if (S[S_STOP] != 0.0) return;
   __shared__ double lds[kThreads / 64];
   __shared__ double bc[2];
-  const double rr = lfr::block_sum(lfr::strided_sum(part_rr, np), lds);
-  const double rzs = diag ? lfr::block_sum(lfr::strided_sum(part_rz, np), lds) : 0.0;
+  const double srr = lfr::strided_sum(part_rr, np);
+  const double srz = diag ? lfr::strided_sum(part_rz, np) : 0.0;
+  const double rr = lfr::block_sum(srr, lds);
+  const double rzs = diag ? lfr::block_sum(srz, lds) : 0.0;
   if (threadIdx.x == 0) {
     const double rz = diag ? rzs : rr;
     const double rn = sqrt(rr);
