@@ -124,9 +124,23 @@ k_dot_part(const T* __restrict__ x, const T* __restrict__ y, int64_t n, double* 
       s = s + p;
     }
   }
-  for (; i < n; i += st) {
-    const double p = (double)x[i] * (double)y[i];
-    s = s + p;
+  // the last (at most seven) elements loaded together at clamped indices; a
+  // slot past the end adds +0 (selected, not 0 * x, which may be NaN), which
+  // leaves the sum bitwise unchanged: it starts at +0 and, rounding to
+  // nearest, never becomes -0
+  if (n > 0) {
+    T a[7], b[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      const int64_t k = i + q * st < n ? i + q * st : n - 1;
+      a[q] = x[k];
+      b[q] = y[k];
+    }
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      const double p = (double)a[q] * (double)b[q];
+      s = s + (i + q * st < n ? p : 0.0);
+    }
   }
   s = lfr::block_sum(s, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
