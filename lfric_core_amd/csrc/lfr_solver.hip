@@ -20,6 +20,7 @@
 //
 // With the null preconditioner z = P^{-1} r is a copy of r
 // (sci_null_preconditioner_alg_mod.x90:84); r is used in its place.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -623,7 +624,7 @@ bool use_graphs() {
 
 template <typename T>
 int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x, Ws& w,
-                  int64_t n, int chunk) {
+                  int64_t n, int chunk, bool sync_end) {
   const bool diag = P->preconditioner == 1;
   T* r = (T*)w.r;
   T* p = (T*)w.p;
@@ -725,22 +726,54 @@ int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P,
       if (rc) return rc;
     }
   } else {
-    // one chunk stays queued while the host checks the previous one; a
-    // chunk that starts after convergence returns at entry
-    int rc = run_chunk();
-    if (rc) return rc;
-    LFR_HIP(ctx, hipMemcpyAsync(Sh + 16, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    LFR_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    for (int k = 0;; ++k) {
-      rc = run_chunk();
+    // While the host checks chunk k, chunk k+1 is normally queued behind it
+    // so that the device never waits for the host.  A chunk that starts
+    // after convergence returns at entry, but each of its launches still
+    // costs a few microseconds, so chunk k+1 is queued ahead only while the
+    // residual history (the geometric rate over the last checked chunk)
+    // says chunk k will not converge; a wrong guess costs one host round
+    // trip instead of a chunk of empty launches.  The decision changes only
+    // when work is queued, never which iterations run.
+    auto queue = [&](int slot) -> int {
+      int rc = run_chunk();
       if (rc) return rc;
-      const int nxt = (k + 1) & 1;
-      LFR_HIP(ctx, hipMemcpyAsync(Sh + 16 + 16 * nxt, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost,
+      LFR_HIP(ctx, hipMemcpyAsync(Sh + 16 + 16 * slot, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost,
                                   ctx->stream));
-      LFR_HIP(ctx, hipEventRecord(ctx->ev[nxt], ctx->stream));
-      LFR_HIP(ctx, hipEventSynchronize(ctx->ev[k & 1]));
-      if (Sh[16 + 16 * (k & 1) + S_STOP] != 0.0) break;
+      LFR_HIP(ctx, hipEventRecord(ctx->ev[slot], ctx->stream));
+      return LFR_OK;
+    };
+    int rc = queue(0);
+    if (rc) return rc;
+    double it_a = 0.0, rn_a = -1.0, it_b = 0.0, rn_b = -1.0, target = 0.0;
+    for (int k = 0;; ++k) {
+      const int cur = k & 1, nxt = (k + 1) & 1;
+      bool ahead = true;
+      if (rn_a > 0.0 && rn_b > 0.0 && it_b > it_a) {
+        const double rho = pow(rn_b / rn_a, 1.0 / (it_b - it_a));
+        if (rho < 1.0 && rn_b * pow(rho, (double)chunk) <= target) ahead = false;
+      }
+      if (ahead && (rc = queue(nxt))) return rc;
+      LFR_HIP(ctx, hipEventSynchronize(ctx->ev[cur]));
+      const double* s = Sh + 16 + 16 * cur;
+      if (s[S_STOP] != 0.0) {
+        // iterations after the stop return at entry and leave S as it is,
+        // so this copy is the final state
+        memcpy(Sh, s, S_NSLOT * sizeof(double));
+        break;
+      }
+      if (!ahead && (rc = queue(nxt))) return rc;
+      if (rn_b < 0.0) {
+        rn_a = s[S_RNRM0];
+        target = std::max(P->r_tol * rn_a, P->a_tol);
+      } else {
+        it_a = it_b;
+        rn_a = rn_b;
+      }
+      it_b = s[S_ITER];
+      rn_b = s[S_RNRM];
     }
+    if (sync_end) LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LFR_OK;
   }
   LFR_HIP(ctx, hipMemcpyAsync(Sh, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   LFR_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -749,7 +782,7 @@ int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P,
 
 template <typename T>
 int cg_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x, const T* b,
-             lfr_solver_stats* st, Ws& w) {
+             lfr_solver_stats* st, Ws& w, bool sync_end = true) {
   const int64_t n = A->space->undf;
   const int G = lfr::red_blocks(n);
   T* r = (T*)w.r;
@@ -796,7 +829,7 @@ int cg_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x
       if (v >= 1 && v <= 64) cmax = v;
     }
     const int chunk = P->max_iter < cmax ? P->max_iter : cmax;
-    int rc = cg_iterations<T>(ctx, A, P, x, w, n, chunk);  // ends with S in Sh
+    int rc = cg_iterations<T>(ctx, A, P, x, w, n, chunk, sync_end);  // ends with S in Sh
     if (rc) return rc;
   } else {
     LFR_HIP(ctx, hipMemcpyAsync(Sh, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -894,7 +927,9 @@ extern "C" int lfr_refined_cg(lfr_ctx* ctx, const lfr_lma_op* A32, const lfr_lma
     lfr_solver_params ip = *inner;
     ip.x_is_zero = 1;
     ip.fail_on_non_converged = 0;
-    rc = cg_solve<float>(ctx, A32, &ip, d32, rhs32, &ist, w);
+    // no wait at its end: the correction and the next residual queue behind
+    // it, and the residual's read-back waits for all of them
+    rc = cg_solve<float>(ctx, A32, &ip, d32, rhs32, &ist, w, false);
     if (rc) return rc;
     st->inner_total += ist.iterations;
     st->outer = it + 1;
