@@ -150,11 +150,9 @@ k_cg_beta_direction(T* __restrict__ p, const T* __restrict__ z, int64_t n, const
 
 // r = b - r  (r holds A x): the reference's r%scale(-1); r%axpy(1, b)
 // (sci_iterative_solver_mod.f90:518-520), with FP64 partials of <r, r>.
-// If out32 is given, also store (float) r there (refinement residual).
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_residual(T* __restrict__ r, const T* __restrict__ b, int64_t n, double* __restrict__ part,
-           float* __restrict__ out32) {
+k_residual(T* __restrict__ r, const T* __restrict__ b, int64_t n, double* __restrict__ part) {
   __shared__ double lds[kThreads / 64];
   double rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
@@ -162,7 +160,6 @@ k_residual(T* __restrict__ r, const T* __restrict__ b, int64_t n, double* __rest
     const T m = (T)(-1) * r[i];
     const T v = m + b[i];
     r[i] = v;
-    if (out32) out32[i] = (float)v;
     rr = rr + (double)v * (double)v;
   }
   rr = lfr::block_sum(rr, lds);
@@ -189,6 +186,37 @@ k_cg_start_zero(T* __restrict__ x, T* __restrict__ r, T* __restrict__ p, const T
   }
   rr = lfr::block_sum(rr, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// Refinement residual and the start of its inner FP32 solve in one pass:
+// r = b - A x in FP64 from A x (in ax; r itself is not stored), FP64
+// partials of <r, r> in part64, and the inner solve's start from zero,
+// d = 0 and r32 = p32 = (float) r, with FP64 partials of <r32, r32> in
+// part32.  The sums are k_residual's and k_cg_start_zero's (on the FP32
+// right-hand side (float) r), in their grid and order.
+__global__ void __launch_bounds__(kThreads)
+k_residual_start32(const double* __restrict__ ax, const double* __restrict__ b, int64_t n,
+                   double* __restrict__ part64, float* __restrict__ d, float* __restrict__ r32,
+                   float* __restrict__ p32, double* __restrict__ part32) {
+  __shared__ double lds[kThreads / 64];
+  double rr = 0.0, ff = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads) {
+    const double m = -1.0 * ax[i];
+    const double v = m + b[i];
+    const float f = (float)v;
+    d[i] = 0.0f;
+    r32[i] = f;
+    p32[i] = f;
+    rr = rr + v * v;
+    ff = ff + (double)f * (double)f;
+  }
+  rr = lfr::block_sum(rr, lds);
+  ff = lfr::block_sum(ff, lds);
+  if (threadIdx.x == 0) {
+    part64[blockIdx.x] = rr;
+    part32[blockIdx.x] = ff;
+  }
 }
 
 // ||r0|| < epsilon: cg converged in 0 iterations
@@ -251,9 +279,8 @@ struct Ws {
   void* r = nullptr;
   void* p = nullptr;
   void* z = nullptr;
-  void* aux = nullptr;       // refinement: fp64 residual, fp32 correction, fp32 rhs
+  void* aux = nullptr;       // refinement: fp64 A x, fp32 correction
   void* aux2 = nullptr;
-  void* aux3 = nullptr;
   double* part_apply = nullptr;
   double* part_rr = nullptr;
   double* part_rz = nullptr;
@@ -308,7 +335,6 @@ int ws_get(lfr_ctx* ctx, int64_t n, int prec, const lfr_plan* plan, int naux, Ws
   w.z = b; b += vb;
   if (naux > 0) { w.aux = b; b += vb; }
   if (naux > 1) { w.aux2 = b; b += vb; }
-  if (naux > 2) { w.aux3 = b; b += vb; }
   w.part_apply = (double*)b; b += align_up(napply * sizeof(double));
   w.part_rr = (double*)b; b += align_up(lfr::kRedMaxBlocks * sizeof(double));
   w.part_rz = (double*)b; b += align_up(lfr::kRedMaxBlocks * sizeof(double));
@@ -782,7 +808,7 @@ int cg_iterations(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P,
 
 template <typename T>
 int cg_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x, const T* b,
-             lfr_solver_stats* st, Ws& w, bool sync_end = true) {
+             lfr_solver_stats* st, Ws& w, bool sync_end = true, bool started = false) {
   const int64_t n = A->space->undf;
   const int G = lfr::red_blocks(n);
   T* r = (T*)w.r;
@@ -794,14 +820,16 @@ int cg_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T* x
   // r = b - A x
   if (P->x_is_zero) {
     // x = 0, so A x = 0 exactly: no apply; r = b, and p = r right away
-    // without a preconditioner
-    hipLaunchKernelGGL(k_cg_start_zero<T>, dim3(G), dim3(kThreads), 0, ctx->stream, x, r, diag ? (T*)nullptr : p,
+    // without a preconditioner ('started': the caller's pass formed x, r, p
+    // and the partials already)
+    if (!started)
+      hipLaunchKernelGGL(k_cg_start_zero<T>, dim3(G), dim3(kThreads), 0, ctx->stream, x, r, diag ? (T*)nullptr : p,
                        b, n, w.part_rr);
   } else {
     int rc = apply_op<T>(ctx, A, x, r, (const T*)nullptr, nullptr, nullptr);
     if (rc) return rc;
     hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kThreads), 0, ctx->stream, r, b, n,
-                       w.part_rr, (float*)nullptr);
+                       w.part_rr);
   }
   hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rr, G, w.S,
                      (int)S_RNRM0, 1);
@@ -885,11 +913,10 @@ extern "C" int lfr_refined_cg(lfr_ctx* ctx, const lfr_lma_op* A32, const lfr_lma
   const double* b = (const double*)b64_v;
   // workspace: fp64-sized slots; the inner fp32 CG uses r, p, z as fp32 views
   Ws w;
-  rc = ws_get(ctx, n, 8, A32->plan, 3, w);
+  rc = ws_get(ctx, n, 8, A32->plan, 2, w);
   if (rc) return rc;
-  double* r64 = (double*)w.aux;    // fp64 residual b - A x
+  double* r64 = (double*)w.aux;    // fp64 A x (the residual pass does not store b - A x)
   float* d32 = (float*)w.aux2;     // fp32 correction
-  float* rhs32 = (float*)w.aux3;   // fp32 right-hand side of the inner solve
   const int G = lfr::red_blocks(n);
   double* Sh = ctx->host_scalars;
   st->outer = 0;
@@ -904,12 +931,13 @@ extern "C" int lfr_refined_cg(lfr_ctx* ctx, const lfr_lma_op* A32, const lfr_lma
     st->b_nrm = sqrt(bn);
   }
   for (int it = 0; it <= max_outer; ++it) {
-    // r64 = b - A64 x ; rhs32 = (float) r64 ; ||r||
+    // r = b - A64 x in FP64 and ||r||, with the inner solve's start from
+    // zero on (float) r in the same pass (r64 holds A64 x)
     rc = apply_op<double>(ctx, A64, x, r64, (const double*)nullptr, nullptr, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_residual<double>, dim3(G), dim3(kThreads), 0, ctx->stream, r64, b, n,
-                       w.part_rr, rhs32);
-    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rr, G, w.S,
+    hipLaunchKernelGGL(k_residual_start32, dim3(G), dim3(kThreads), 0, ctx->stream, (const double*)r64, b, n,
+                       w.part_rz, d32, (float*)w.r, (float*)w.p, w.part_rr);
+    hipLaunchKernelGGL(k_store_sum, dim3(1), dim3(kThreads), 0, ctx->stream, w.part_rz, G, w.S,
                        (int)S_RNRM, 1);
     LFR_CHECK_LAUNCH(ctx);
     LFR_HIP(ctx, hipMemcpyAsync(Sh, w.S, S_NSLOT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -929,7 +957,7 @@ extern "C" int lfr_refined_cg(lfr_ctx* ctx, const lfr_lma_op* A32, const lfr_lma
     ip.fail_on_non_converged = 0;
     // no wait at its end: the correction and the next residual queue behind
     // it, and the residual's read-back waits for all of them
-    rc = cg_solve<float>(ctx, A32, &ip, d32, rhs32, &ist, w, false);
+    rc = cg_solve<float>(ctx, A32, &ip, d32, (const float*)nullptr, &ist, w, false, true);
     if (rc) return rc;
     st->inner_total += ist.iterations;
     st->outer = it + 1;
@@ -1060,7 +1088,7 @@ int gmres_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T
   // res = b - A x  (res%copy(b); res%axpy(-1, Ax))
   rc = apply_op<T>(ctx, A, x, res, (const T*)nullptr, nullptr, nullptr);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kThreads), 0, ctx->stream, res, b, n, part, (float*)nullptr);
+  hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kThreads), 0, ctx->stream, res, b, n, part);
   double sc_err = 0;
   rc = norm_host(res, &sc_err);
   if (rc) return rc;
@@ -1148,7 +1176,7 @@ int gmres_solve(lfr_ctx* ctx, const lfr_lma_op* A, const lfr_solver_params* P, T
     // res = -1 * (A x) + b   (res%copy(Ax); res%aypx(-1, b))
     rc = apply_op<T>(ctx, A, x, res, (const T*)nullptr, nullptr, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kThreads), 0, ctx->stream, res, b, n, part, (float*)nullptr);
+    hipLaunchKernelGGL(k_residual<T>, dim3(G), dim3(kThreads), 0, ctx->stream, res, b, n, part);
     LFR_CHECK_LAUNCH(ctx);
     st->iterations = it;
     if (P->monitor_convergence) {
